@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""SSIMULACRA 2.1 throughput on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1080p|4k|512]
+                    [--planes f32|f16|bf16] [--cpu-sample P | --no-cpu]
+
+A step scores one batch of synthetic pairs already resident in HBM: the 1080p
+workload is 3 distortion strengths x 100 pairs (block step 4/12/24, noise
+sigma 1/3/8), generated deterministically (vqm_amd/synth.py). Every pair of a
+step goes through one prepare, one scale and one reduce launch of the batch
+engine; the norms of all pairs come back to the host before the step ends.
+
+Printed: one JSON line with pairs/s (`value`), the roofline of the dominant
+kernel (algorithmic bytes per launch over its HIP-event duration on the batch
+stream, against 8 TB/s), the CPU baseline (the oracle's FP32 device form on
+the host's cores, one pair per thread), and per-mode extras (streaming and
+single-pair latency).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+RTX5070_1080P_RESIDENT = 3836.7  # 1080p resident, 3 workers, RTX 5070
+WORKLOADS = {"1080p": (1920, 1080, 1), "4k": (3840, 2160, 2), "512": (512, 512, 0)}
+PLANES = {"f32": 0, "f16": 1, "bf16": 2}
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def plane_elements(w: int, h: int) -> int:
+    """Stored plane elements of one image (pruned planes excluded)."""
+    from vqm_amd import Size, channel_pruned, plan_scales
+    sizes = plan_scales(Size(w, h))
+    n = len(sizes)
+    return sum(s.width * s.height for l, s in enumerate(sizes) for c in range(3)
+               if not channel_pruned(c, l, n))
+
+
+def make_pairs(w, h, cfg, per_strength):
+    from vqm_amd import synth
+    pairs = []
+    for k, strength in enumerate(synth.STRENGTHS):
+        for i in range(per_strength):
+            pairs.append(synth.pair(cfg, k * per_strength + i, w, h, strength))
+    return pairs
+
+
+def cpu_baseline(w, h, pairs, sample):
+    import oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    chosen = [pairs[(i * 37) % len(pairs)] for i in range(sample)]
+    oracle.compute_many(oracle.F32_DEV, w, h, chosen[:threads], threads)  # warm
+    t0 = time.perf_counter()
+    oracle.compute_many(oracle.F32_DEV, w, h, chosen, threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(sample / dt, 3), "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": f"{sample} pairs of the {w}x{h} workload, oracle FP32 device form, one pair "
+                      f"per host thread ({dt:.1f} s wall)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="1080p")
+    ap.add_argument("--pairs-per-strength", type=int, default=100)
+    ap.add_argument("--planes", choices=sorted(PLANES), default="f32")
+    ap.add_argument("--cpu-sample", type=int, default=48)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-modes", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        log(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (pairs are independent; "
+            "several GPUs would each score their own share, see DESIGN.md)")
+        return 2
+
+    from vqm_amd import (DeviceBuffer, Image, ImageFormat, Size, Ssimulacra2Batch,
+                         Ssimulacra2Config, Ssimulacra2Pipeline, device_info)
+
+    w, h, cfg = WORKLOADS[args.workload]
+    fmt = PLANES[args.planes]
+    elem = 4 if fmt == 0 else 2
+    info = device_info()
+    log(f"device {info['name']} {info['arch']} cus={info['cus']}")
+
+    t0 = time.perf_counter()
+    pairs = make_pairs(w, h, cfg, args.pairs_per_strength)
+    log(f"generated {len(pairs)} pairs of {w}x{h} in {time.perf_counter() - t0:.1f} s")
+    img_bytes = pairs[0][0].nbytes
+    slot = (img_bytes + 255) // 256 * 256
+    dev = DeviceBuffer(2 * slot * len(pairs))
+    refs = []
+    for i, (a, b) in enumerate(pairs):
+        dev.upload(a, 2 * i * slot)
+        dev.upload(b, (2 * i + 1) * slot)
+        refs.append((dev.ptr + 2 * i * slot, dev.ptr + (2 * i + 1) * slot, w, h))
+    log(f"uploaded {2 * len(pairs)} images ({2 * slot * len(pairs) / 1e9:.2f} GB)")
+
+    batch = Ssimulacra2Batch(max_pairs=len(refs), plane_format=fmt)
+    for _ in range(args.warmup):
+        scales, norms = batch.Finish(batch.Submit(refs), raw=True)
+    step_ms, stages = [], []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        scales, norms = batch.Finish(batch.Submit(refs), raw=True)
+        step_ms.append((time.perf_counter() - t) * 1e3)
+        stages.append(batch.StageTimes())
+    total = time.perf_counter() - t_start
+    if not np.all(np.isfinite(norms)):
+        log("non-finite norms")
+        return 1
+    value = len(refs) * args.steps / total
+    log(f"batch: {value:.1f} pairs/s, {total / args.steps * 1e3:.2f} ms/step")
+
+    # Roofline of the dominant kernel: algorithmic bytes per launch over the
+    # average HIP-event duration of that launch on the batch stream.
+    n_px = w * h
+    planes_bytes = plane_elements(w, h) * elem
+    launch_bytes = {
+        "prepare": len(refs) * (2 * 3 * n_px + 2 * planes_bytes),
+        "scale": len(refs) * (2 * planes_bytes),
+    }
+    avg = {"prepare": statistics.mean(s[0] for s in stages),
+           "scale": statistics.mean(s[1] for s in stages),
+           "reduce": statistics.mean(s[2] for s in stages)}
+    dominant = max(("prepare", "scale"), key=lambda k: avg[k])
+    achieved = launch_bytes[dominant] / (avg[dominant] * 1e-3) / 1e9
+    pair_bytes = 2 * 3 * n_px + 4 * planes_bytes
+    roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                "traffic": None,
+                "bytes_per_launch": launch_bytes[dominant],
+                "stage_ms": {k: round(v, 4) for k, v in avg.items()},
+                "pair_algorithmic_bytes": pair_bytes,
+                "whole_path_gbps": round(value * pair_bytes / 1e9, 1)}
+
+    out = {
+        "metric": "SSIMULACRA2 pairs/s at 1080p & 4K (% HBM peak); p99 latency at fixed QPS",
+        "value": round(value, 2),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(value / RTX5070_1080P_RESIDENT, 3) if args.workload == "1080p" else None,
+        "dtype": {0: "f32", 1: "f32 (f16 planes)", 2: "f32 (bf16 planes)"}[fmt],
+        "data": "synthetic (vqm_amd/synth.py: sinusoid field + block quantisation + Gaussian noise)",
+        "config": {"workload": f"{w}x{h} 8-bit sRGB RGB, 6 scales, {len(refs)} pairs per step "
+                               f"(3 strengths x {args.pairs_per_strength}), resident in HBM, "
+                               f"{args.planes} planes, one batched launch per kernel",
+                   "pairs_per_step": len(refs), "width": w, "height": h, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "vs_baseline_note": "value / 3836.7 pairs/s (vqm 1080p resident, 3 workers, RTX 5070, BASELINE.md)",
+    }
+
+    if not args.no_modes:
+        modes = {}
+        # Single pair at a time, resident inputs: per-pair latency.
+        pipe = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(w, h), plane_format=fmt))
+        from vqm_amd import DeviceImage
+        lat = []
+        for i in range(min(40, len(refs))):
+            a, b = refs[i][0], refs[i][1]
+            t = time.perf_counter()
+            pipe.Compute(DeviceImage(a, Size(w, h)), DeviceImage(b, Size(w, h)))
+            lat.append((time.perf_counter() - t) * 1e3)
+        lat = lat[5:]
+        modes["resident_single_pair"] = {"pairs_per_s": round(1e3 / statistics.mean(lat), 1),
+                                         "p50_ms": round(float(np.percentile(lat, 50)), 4),
+                                         "p99_ms": round(float(np.percentile(lat, 99)), 4)}
+        # Streaming: host images uploaded per call (PCIe-inclusive).
+        imgs = [(Image(Size(w, h), ImageFormat.srgb8(), a), Image(Size(w, h), ImageFormat.srgb8(), b))
+                for a, b in pairs[:30]]
+        lat = []
+        for a, b in imgs:
+            t = time.perf_counter()
+            pipe.Compute(a, b)
+            lat.append((time.perf_counter() - t) * 1e3)
+        lat = lat[5:]
+        modes["streaming_single_pair"] = {"pairs_per_s": round(1e3 / statistics.mean(lat), 1),
+                                          "p50_ms": round(float(np.percentile(lat, 50)), 4),
+                                          "p99_ms": round(float(np.percentile(lat, 99)), 4)}
+        pipe.close()
+        out["modes"] = modes
+        log(f"modes: {modes}")
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, h, pairs, args.cpu_sample)
+        log(f"cpu baseline: {out['cpu_baseline']}")
+
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,289 @@
+/*
+ * vqm_ss2.h -- C ABI of the MI355X-native SSIMULACRA 2.1 scoring path
+ * (libvqm_hip.so). Plain pointers and sizes only; every entry point that can
+ * fail returns a status code and leaves a thread-local message for
+ * vqm_last_error().
+ *
+ * This is the operator boundary of the reference's backend seam:
+ * internal::Ssimulacra2GpuPipeline (src/metrics/ssimulacra2/
+ * ssimulacra2_internal.h:267-317), created by CreateVulkanSsimulacra2Pipeline
+ * (src/metrics/ssimulacra2/vulkan.h:19-30, vulkan.cc:751-774). The C++ facade
+ * in include/ssimulacra2.h rebuilds the reference's public API
+ * (include/ssimulacra2.h:17-215) on top of it, mapping status codes to the
+ * reference's exceptions:
+ *   VQM_INVALID_ARGUMENT -> std::invalid_argument
+ *   VQM_LOGIC_ERROR      -> std::logic_error   (call while a pair is in flight)
+ *   VQM_RUNTIME_ERROR    -> std::runtime_error (no device, HIP failure, allocation)
+ *   VQM_OVERFLOW         -> std::overflow_error (layout does not fit)
+ *
+ * Inputs are planar 8-bit sRGB RGB images in the reference's Image layout
+ * (include/vqm_image.h:122-133): planes R, G, B of width*height bytes, each
+ * plane starting on a 4-byte boundary. Norm arrays hold 108 doubles, index
+ * level * 18 + k with k = 0..5 the Ssimulacra2Scale::avg_ssim entries
+ * [channel * 2 + norm] and k = 6..17 the avg_edgediff entries
+ * [channel * 4 + n] (include/ssimulacra2.h:17-25); pruned (channel, level)
+ * pairs and levels beyond `scales` are 0.0.
+ *
+ * Threading (include/ssimulacra2.h:66-70, 135-139): a pipeline is not
+ * thread-safe and holds one submission in flight; separate pipelines on
+ * separate threads run concurrently, each on its own HIP stream.
+ */
+#ifndef VQM_SS2_H
+#define VQM_SS2_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    VQM_OK = 0,
+    VQM_INVALID_ARGUMENT = 1,
+    VQM_LOGIC_ERROR = 2,
+    VQM_RUNTIME_ERROR = 3,
+    VQM_OVERFLOW = 4
+};
+
+/* Stored plane precision. FP16 is the reference's half_planes
+ * (include/ssimulacra2.h:41-46); BF16 is an addition. Arithmetic is FP32 in
+ * every format. */
+enum { VQM_PLANES_F32 = 0, VQM_PLANES_F16 = 1, VQM_PLANES_BF16 = 2 };
+
+/* ImageFormat, include/vqm_image.h:100-120; numeric enum values are zimg's
+ * as in the reference (vqm_image.h:25-96). This path decodes type Byte (0),
+ * family Rgb (1), alpha None (0), range Full (1), matrix Rgb (0) or
+ * Unspecified (2), transfer Srgb (13), primaries Bt709 (1), no subsampling,
+ * depth 0 or 8; anything else is VQM_INVALID_ARGUMENT. */
+typedef struct vqm_image_format {
+    uint8_t type;
+    uint8_t family;
+    uint8_t alpha;
+    uint8_t range;
+    uint8_t matrix;
+    uint8_t transfer;
+    uint8_t primaries;
+    uint8_t chroma_location;
+    uint8_t subsample_w;
+    uint8_t subsample_h;
+    uint8_t depth;
+    uint8_t reserved[5];
+} vqm_image_format;
+
+/* A host image: geometry, format and the byte length of its data
+ * (Image::data.size(), include/vqm_image.h:134-141). */
+typedef struct vqm_image_desc {
+    uint32_t width;
+    uint32_t height;
+    vqm_image_format format;
+    uint64_t bytes;
+} vqm_image_desc;
+
+/* A device-resident image owned by the caller: replaces VulkanBufferView
+ * (include/vqm_vulkan.h:74-101). `data` is a device pointer to the packed
+ * planes and must stay alive and unmodified until vqm_ss2_finish returns. */
+typedef struct vqm_device_image {
+    const void* data;
+    uint32_t width;
+    uint32_t height;
+    vqm_image_format format;
+    uint64_t bytes;
+} vqm_device_image;
+
+/* Ssimulacra2Config, include/ssimulacra2.h:35-64. */
+typedef struct vqm_ss2_config {
+    uint32_t width;
+    uint32_t height;
+    uint32_t resident_images;   /* gpu_resident_images, default 2 */
+    uint32_t plane_format;      /* VQM_PLANES_*; half_planes = VQM_PLANES_F16 */
+    uint32_t streaming_inputs;  /* default 1 */
+    uint32_t strip_rows;        /* output rows per scale work item, 0 = default */
+    uint64_t input_bytes;       /* caller-owned image pool (AllocateImage), default 0 */
+} vqm_ss2_config;
+
+/* PipelineMemoryUsage, include/vqm_memory_types.h:29-38: device bytes the
+ * pipeline holds, by region. */
+typedef struct vqm_ss2_memory {
+    uint64_t workspace_bytes;   /* planes + partials + norms */
+    uint64_t resident_bytes;    /* resident image slots */
+    uint64_t streaming_bytes;   /* streaming input buffers on the device */
+    uint64_t input_pool_bytes;  /* caller-owned image pool */
+    uint64_t pinned_host_bytes; /* pinned staging on the host */
+    uint64_t total_device_bytes;
+} vqm_ss2_memory;
+
+typedef struct vqm_ss2_pipeline vqm_ss2_pipeline;
+
+/* ---- host-only helpers (no GPU) ---------------------------------------- */
+
+/* Thread-local message of the last failing call on this thread. */
+const char* vqm_last_error(void);
+
+/* Defaults of Ssimulacra2Config for a geometry. */
+void vqm_ss2_config_init(vqm_ss2_config* config, uint32_t width, uint32_t height);
+
+/* The default image format: Byte, Rgb, opaque, full range, sRGB, BT.709. */
+void vqm_image_format_srgb8(vqm_image_format* format);
+
+/* RequiredBytes, include/vqm_image.h:143-146 (input.cc:244-255). */
+int vqm_required_bytes(const vqm_image_format* format, uint32_t width, uint32_t height,
+                       uint64_t* bytes);
+
+/* PlanSsimulacra2Scales, pipeline.cc:28-48: writes `count` and width,height
+ * pairs of each level. INVALID_ARGUMENT below 8x8. */
+int vqm_ss2_plan_scales(uint32_t width, uint32_t height, uint32_t sizes[12],
+                        uint32_t* count);
+
+/* MakeGaussian(1.5), pipeline.cc:274-291. */
+void vqm_ss2_gaussian_taps(float taps[13]);
+
+/* Ssimulacra2ChannelPruned, score.cc:103-114: 1 when pruned. */
+int vqm_ss2_channel_pruned(uint32_t channel, uint32_t scale, uint32_t scales);
+
+/* Ssimulacra2Result::Score, score.cc:118-163. Writes the score to *score;
+ * INVALID_ARGUMENT for more than six scales, nonfinite norms or overflow. */
+int vqm_ss2_score(uint32_t scales, const double norms[108], double* score);
+
+/* The u8 sRGB decode table the kernels use (color.glsl:94-113,135-137). */
+void vqm_ss2_srgb_table(float table[256]);
+
+/* Device bytes a pipeline with `config` allocates (layout.cc:106-325). */
+int vqm_ss2_memory_plan(const vqm_ss2_config* config, vqm_ss2_memory* plan);
+
+/* Algorithmic HBM bytes of one resident pair of this geometry: both u8
+ * inputs read, every stored plane element written and read once. */
+int vqm_ss2_algorithmic_bytes(uint32_t width, uint32_t height, uint32_t plane_format,
+                              uint64_t* bytes);
+
+/* ---- the pipeline (Ssimulacra2Pipeline, include/ssimulacra2.h:83-215) ---- */
+
+/* Ssimulacra2Pipeline(Ssimulacra2Config), pipeline.cc:76-106.
+ * INVALID_ARGUMENT below 8x8 or for a bad config; RUNTIME_ERROR without a
+ * usable gfx950 device or on allocation failure. */
+int vqm_ss2_create(const vqm_ss2_config* config, vqm_ss2_pipeline** out);
+
+/* Waits for in-flight work and frees everything; never fails. */
+void vqm_ss2_destroy(vqm_ss2_pipeline* pipeline);
+
+/* Upload (include/ssimulacra2.h:115-118, pipeline.cc:108-114): copies a host
+ * image into a resident slot. INVALID_ARGUMENT on a mismatched image or when
+ * every slot is in use; LOGIC_ERROR while a pair is in flight. The handle is
+ * process-unique and never reused. */
+int vqm_ss2_upload(vqm_ss2_pipeline* pipeline, const vqm_image_desc* image,
+                   const void* data, uint64_t* handle);
+
+/* Release (include/ssimulacra2.h:119-121): unknown handles are ignored;
+ * LOGIC_ERROR while a pair is in flight. */
+int vqm_ss2_release(vqm_ss2_pipeline* pipeline, uint64_t handle);
+
+/* Submit(Image, Image) (include/ssimulacra2.h:145-147): streaming; the host
+ * data is copied before this returns. `background` is accepted for API
+ * parity (the alpha blend target; opaque inputs ignore it). */
+int vqm_ss2_submit_host(vqm_ss2_pipeline* pipeline, const vqm_image_desc* original,
+                        const void* original_data, const vqm_image_desc* distorted,
+                        const void* distorted_data, float background, uint64_t* token);
+
+/* Submit(UploadedImage, UploadedImage) (include/ssimulacra2.h:152-155).
+ * INVALID_ARGUMENT for a stale or unknown handle. */
+int vqm_ss2_submit_resident(vqm_ss2_pipeline* pipeline, uint64_t original,
+                            uint64_t distorted, float background, uint64_t* token);
+
+/* Submit(UploadedImage, Image) (include/ssimulacra2.h:156-159). */
+int vqm_ss2_submit_resident_host(vqm_ss2_pipeline* pipeline, uint64_t original,
+                                 const vqm_image_desc* distorted, const void* distorted_data,
+                                 float background, uint64_t* token);
+
+/* Submit(VulkanBufferView, VulkanBufferView[, VulkanSubmitSync])
+ * (include/ssimulacra2.h:186-201): caller-owned device images. When
+ * `wait_stream` is not NULL (a hipStream_t), the pair's work is ordered
+ * after everything already queued on it, the HIP analogue of the wait
+ * semaphores of VulkanSubmitSync. */
+int vqm_ss2_submit_device(vqm_ss2_pipeline* pipeline, const vqm_device_image* original,
+                          const vqm_device_image* distorted, void* wait_stream,
+                          float background, uint64_t* token);
+
+/* Finish (include/ssimulacra2.h:203, vulkan.cc:644-650): waits for the token
+ * and writes the level count and 108 norms. INVALID_ARGUMENT for an unknown
+ * or already finished token. */
+int vqm_ss2_finish(vqm_ss2_pipeline* pipeline, uint64_t token, uint32_t* scales,
+                   double norms[108]);
+
+/* GetMemoryUsage (include/ssimulacra2.h:204-205). */
+int vqm_ss2_memory_usage(const vqm_ss2_pipeline* pipeline, vqm_ss2_memory* usage);
+
+/* AllocateImage / MapImage / Release(ImageBuffer) (include/ssimulacra2.h:
+ * 161-184) over a device pool of config.input_bytes. MapImage returns a
+ * device pointer: fill it with hipMemcpy or a kernel before the Submit that
+ * scores it. INVALID_ARGUMENT when the pool is zero or full; LOGIC_ERROR
+ * while a pair is in flight. */
+int vqm_ss2_allocate_image(vqm_ss2_pipeline* pipeline, const vqm_image_format* format,
+                           uint64_t* buffer, uint64_t* bytes);
+int vqm_ss2_map_image(vqm_ss2_pipeline* pipeline, uint64_t buffer, void** device_ptr);
+int vqm_ss2_release_image(vqm_ss2_pipeline* pipeline, uint64_t buffer);
+int vqm_ss2_submit_buffers(vqm_ss2_pipeline* pipeline, uint64_t original, uint64_t distorted,
+                           float background, uint64_t* token);
+
+/* Stage times of the last finished pair in milliseconds (prepare, scale,
+ * reduce), measured with HIP events on the pipeline's stream. The reference's
+ * VQM_STAGE_TIMING stages (src/timing.h:18-21). */
+int vqm_ss2_stage_times(const vqm_ss2_pipeline* pipeline, float ms[3]);
+
+/* ---- batched scoring (no counterpart in the reference: it overlaps pairs
+ * only by holding several pipelines, vqm_pool.h:35-50) -------------------- */
+
+typedef struct vqm_ss2_batch vqm_ss2_batch;
+
+typedef struct vqm_ss2_batch_config {
+    uint32_t plane_format;   /* VQM_PLANES_* */
+    uint32_t strip_rows;     /* 0 = default */
+    uint32_t max_pairs;      /* pairs per submission */
+    uint32_t reserved;
+    uint64_t hbm_budget;     /* device bytes the workspace pool may hold, 0 = unlimited */
+} vqm_ss2_batch_config;
+
+/* One pair of caller-owned device images (packed u8 sRGB RGB). */
+typedef struct vqm_ss2_device_pair {
+    const void* original;
+    const void* distorted;
+    uint32_t width;
+    uint32_t height;
+} vqm_ss2_device_pair;
+
+int vqm_ss2_batch_create(const vqm_ss2_batch_config* config, vqm_ss2_batch** out);
+void vqm_ss2_batch_destroy(vqm_ss2_batch* batch);
+
+/* Scores `count` pairs of any geometries (each >= 8x8) in one launch of each
+ * kernel. Workspaces come from a per-geometry device pool that grows on
+ * demand within hbm_budget (INVALID_ARGUMENT when a submission cannot fit).
+ * One submission may be in flight (LOGIC_ERROR otherwise). */
+int vqm_ss2_batch_submit(vqm_ss2_batch* batch, const vqm_ss2_device_pair* pairs,
+                         uint32_t count, uint64_t* token);
+
+/* Waits; writes scales[count] and norms[count * 108]. */
+int vqm_ss2_batch_finish(vqm_ss2_batch* batch, uint64_t token, uint32_t* scales,
+                         double* norms);
+
+/* Stage times of the last finished submission (prepare, scale, reduce), ms. */
+int vqm_ss2_batch_stage_times(const vqm_ss2_batch* batch, float ms[3]);
+
+/* Device bytes held by the batch's workspace pool. */
+int vqm_ss2_batch_pool_bytes(const vqm_ss2_batch* batch, uint64_t* bytes);
+
+/* ---- device plumbing for callers without a HIP toolchain ---------------- */
+
+int vqm_device_count(int* count);
+/* Name, gfx architecture name, CU count, LDS bytes per block, HBM bytes. */
+int vqm_device_info(char name[256], char arch[64], int* cus, int* lds_per_block,
+                    uint64_t* hbm_bytes);
+int vqm_device_malloc(uint64_t bytes, void** ptr);
+int vqm_device_free(void* ptr);
+int vqm_memcpy_htod(void* dst, const void* src, uint64_t bytes);
+int vqm_memcpy_dtoh(void* dst, const void* src, uint64_t bytes);
+int vqm_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VQM_SS2_H */

@@ -1,0 +1,113 @@
+/*
+ * CPU oracle for the SSIMULACRA 2.1 scoring path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This library is the checker the parity tests compare the HIP path against.
+ * It is never linked into, loaded by or called from the product library
+ * (vqm_amd/csrc); only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.
+ *
+ * It restates the reference's algorithm (GreatValueCreamSoda/vqm,
+ * src/metrics/ssimulacra2/ and its shaders) in three arithmetic forms:
+ *
+ *   FP64      everything in double: the "true" value of the metric as the
+ *             reference defines it (13-tap zero-padded FIR, pruned channels).
+ *   FP32 ref  the reference shaders' FP32 operation order (horizontal blur
+ *             first, scale.comp:149-301), correctly rounded cube root.
+ *   FP32 dev  the HIP kernels' FP32 operations: the reference's blur order
+ *             (horizontal pass first), the kernels' own reproducible cube
+ *             root, per-pixel map values summed in FP64, optional FP16/BF16
+ *             rounding of stored planes. The HIP path matches this form
+ *             per pixel bit for bit; only the order of the FP64 sums
+ *             differs.
+ *
+ * Parity pins (checked in tests/test_oracle.py):
+ *   - end to end: the reference's 32x32 goldens 93.17505696 / 92.12553218
+ *     (tests/metrics/ssimulacra2/pipeline_test.cc:146-174);
+ *   - identity scores exactly 100 (pipeline_test.cc:120-144);
+ *   - scale counts (pipeline_test.cc:126-128, 767-776);
+ *   - pruning mask (docs/ssimulacra2.md:96-104, pipeline_test.cc:163-169);
+ *   - score mapping contract (tests/metrics/ssimulacra2/score_test.cc);
+ *   - Gaussian taps, sRGB decode, XYB and box filter against closed forms;
+ *   - blur against a brute-force 2D convolution on tiny images.
+ */
+#ifndef VQM_SS2_ORACLE_H
+#define VQM_SS2_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Arithmetic form of one oracle run. */
+typedef struct ss2o_mode {
+    int32_t arith;        /* 0 = FP64, 1 = FP32 */
+    int32_t blur_order;   /* 0 = horizontal then vertical, 1 = vertical then horizontal */
+    int32_t cbrt_kind;    /* 0 = libm cbrt (correctly rounded to FP32), 1 = device Newton */
+    int32_t plane_format; /* FP32 arith only: 0 = FP32, 1 = FP16, 2 = BF16 stored planes */
+    uint32_t plane_round_levels; /* levels whose stored planes use plane_format */
+    int32_t sum_quads;    /* 1 = FP32 sums of 4 adjacent pixels, then FP64; 0 = FP64 per pixel */
+} ss2o_mode;
+
+/* Scale plan: returns the level count (1..6) and writes width,height pairs,
+ * or -1 below 8x8. pipeline.cc:28-48. */
+int32_t ss2o_plan_scales(uint32_t width, uint32_t height, uint32_t sizes[12]);
+
+/* The 13 taps of the sigma 1.5 sampled Gaussian. pipeline.cc:274-291. */
+void ss2o_gaussian_taps(float taps[13]);
+
+/* 1 if every one of the six weights of (channel, scale) is <= 0.01 at a
+ * pyramid of `scales` levels. score.cc:103-114. */
+int32_t ss2o_channel_pruned(int32_t channel, int32_t scale, int32_t scales);
+
+/* Weighted sum and score mapping. score.cc:118-163. status: 0 ok,
+ * 1 invalid (too many scales, nonfinite norms, mapping overflow). */
+double ss2o_score(int32_t scales, const double norms[108], int32_t* status);
+
+/* u8 sRGB code -> linear FP32 / FP64 (color.glsl:94-113,135-137). */
+void ss2o_srgb_lut_f32(float lut[256]);
+void ss2o_srgb_lut_f64(double lut[256]);
+
+/* The HIP kernels' cube root (bit-exact restatement). */
+float ss2o_cbrt_dev(float x);
+
+/* linear RGB -> XYB in FP32 (fused multiply-adds as the kernels issue them)
+ * and FP64. prepare.comp:81-98. */
+void ss2o_xyb_f32(float r, float g, float b, int32_t cbrt_kind, float out[3]);
+void ss2o_xyb_f64(double r, double g, double b, double out[3]);
+
+/* Blur one zero-padded plane with the 13-tap Gaussian, in the given order,
+ * FP32 (arith 1) or FP64 (arith 0); in/out are float or double arrays. */
+void ss2o_blur_f32(const float* in, float* out, uint32_t w, uint32_t h, int32_t order);
+void ss2o_blur_f64(const double* in, double* out, uint32_t w, uint32_t h);
+
+/* The linear RGB pyramid of one packed u8 image: level by level, channel
+ * by channel, w_l*h_l values each, FP32 (device and FP32 forms) or FP64.
+ * prepare.comp:100-111,149-168. Returns the level count or -1. */
+int32_t ss2o_pyramid_f32(uint32_t width, uint32_t height, const uint8_t* image, float* out);
+int32_t ss2o_pyramid_f64(uint32_t width, uint32_t height, const uint8_t* image, double* out);
+
+/* Bytes of one packed u8 RGB image (planes 4-byte aligned). vqm_image.h:122-133,
+ * input.cc:114-145,244-255. */
+uint64_t ss2o_image_bytes(uint32_t width, uint32_t height);
+
+/* Score one pair of packed planar u8 sRGB RGB images. norms: 108 doubles,
+ * index level*18 + k, k 0..5 = avg_ssim[c*2+n], 6..17 = avg_edgediff[c*4+m];
+ * pruned and absent levels are 0. sums (may be NULL): the raw per-level map
+ * sums in the same layout. Returns the level count or -1. */
+int32_t ss2o_compute(const ss2o_mode* mode, uint32_t width, uint32_t height,
+                     const uint8_t* image0, const uint8_t* image1,
+                     double norms[108], double sums[108]);
+
+/* Several pairs of one geometry on `threads` host threads, one pair per
+ * thread at a time; images are arrays of pointers. Used as the CPU baseline. */
+int32_t ss2o_compute_many(const ss2o_mode* mode, uint32_t width, uint32_t height,
+                          uint32_t pairs, const uint8_t* const* images0,
+                          const uint8_t* const* images1, double* norms,
+                          uint32_t threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

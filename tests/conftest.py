@@ -1,0 +1,94 @@
+"""Shared fixtures of the test suite.
+
+Tests marked ``gpu`` need a gfx950 device and run the HIP path through the C
+ABI; everything else runs on the CPU (oracle pins, host logic, ABI exports).
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+# Feature and score tolerances of the parity contract. Features: relative 1e-4
+# with an absolute floor of 2e-6, the FP32 cancellation noise of the variance
+# terms (s11 - mu1^2) at the coarsest active level, where the SSIM' map
+# averages ~2e-4; the reference's own FP32 operation order is up to 1.2e-6
+# away from FP64 there on the synthetic workloads.
+FEATURE_REL = 1e-4
+FEATURE_ABS = 2e-6
+SCORE_ABS = 1e-3
+
+# The reference's 32x32 goldens (tests/metrics/ssimulacra2/pipeline_test.cc:172-173).
+GOLDEN_B = 93.17505696
+GOLDEN_C = 92.12553218
+GOLDEN_TOL = 5e-3
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a gfx950 GPU (runs the HIP path)")
+
+
+def gradient_image(variant: int, side: int = 32) -> np.ndarray:
+    """tests/support.h:93-103,259-289: 0 = gradient, 1 = gradient_B (checkerboard
+    +16 on red), 2 = gradient_C (low three bits cleared); packed u8 planes."""
+    x = np.arange(side)[None, :].repeat(side, 0)
+    y = np.arange(side)[:, None].repeat(side, 1)
+    r = (x * 255) // (side - 1)
+    g = (y * 255) // (side - 1)
+    b = ((x * 7 + y * 11) * 13) & 255
+    if variant == 1:
+        r = np.where(((x + y) & 1) == 0, np.minimum(r + 16, 255), r)
+    if variant == 2:
+        r, g, b = r & 0xF8, g & 0xF8, b & 0xF8
+    return pack_planes(r, g, b)
+
+
+def pack_planes(r, g, b) -> np.ndarray:
+    h, w = np.asarray(r).shape
+    stride = (w * h + 3) // 4 * 4
+    out = np.zeros(3 * stride, np.uint8)
+    for i, p in enumerate((r, g, b)):
+        out[i * stride:i * stride + w * h] = np.asarray(p, np.uint8).reshape(-1)
+    return out
+
+
+def ramp_image(width: int, height: int, seed: int = 0) -> np.ndarray:
+    """An 8-bit ramp in the spirit of tests/support.h:305-331 (MakeRampImage)."""
+    x = np.arange(width)[None, :].repeat(height, 0)
+    y = np.arange(height)[:, None].repeat(width, 1)
+    r = np.round(255.0 * x / max(width - 1, 1))
+    g = np.round(255.0 * y / max(height - 1, 1))
+    b = (x * 7 + y * 11 + seed) % 256
+    return pack_planes(r.astype(np.uint8), g.astype(np.uint8), b.astype(np.uint8))
+
+
+def random_pair(width: int, height: int, seed: int, noise: float = 6.0):
+    rng = np.random.default_rng(seed)
+    n = (width * height + 3) // 4 * 4 * 3
+    a = rng.integers(0, 256, n, dtype=np.uint8)
+    # Smooth the original a little so the maps are not pure noise.
+    b = np.clip(a.astype(np.int32) + rng.normal(0, noise, n).round().astype(np.int32), 0, 255)
+    return a, b.astype(np.uint8)
+
+
+def assert_features_close(got, want, rel=FEATURE_REL, abs_=FEATURE_ABS, what=""):
+    got = np.asarray(got, np.float64)
+    want = np.asarray(want, np.float64)
+    err = np.abs(got - want)
+    bound = rel * np.abs(want) + abs_
+    bad = np.nonzero(err > bound)[0]
+    assert bad.size == 0, (f"{what}: {bad.size} features out of tolerance, first "
+                           f"index {bad[0]}: got {got[bad[0]]!r} want {want[bad[0]]!r}")
+
+
+@pytest.fixture(scope="session")
+def oracle_lib():
+    import oracle
+    oracle.build()
+    return oracle

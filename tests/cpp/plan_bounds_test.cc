@@ -1,0 +1,126 @@
+// Host-only check of the work plans the kernels consume (vqm_amd/csrc/
+// ss2_host.cc): for every geometry and strip height, every address the scale
+// and prepare kernels compute lies inside its plane, planes do not overlap,
+// and the work items cover each (level, channel) exactly once.
+//
+// Built and run by tests/test_plan_bounds.py (g++, no GPU).
+
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "ss2_common.h"
+#include "ss2_host.h"
+
+using namespace vqm::ss2;
+
+static int failures = 0;
+#define EXPECT(cond, ...)                                      \
+    do {                                                       \
+        if (!(cond)) {                                         \
+            if (failures < 20) {                               \
+                std::fprintf(stderr, "FAIL %s:%d: ", __FILE__, __LINE__); \
+                std::fprintf(stderr, __VA_ARGS__);             \
+                std::fprintf(stderr, "\n");                    \
+            }                                                  \
+            ++failures;                                        \
+        }                                                      \
+    } while (0)
+
+static void check(uint32_t width, uint32_t height, PlaneFormat fmt, uint32_t strip_rows) {
+    const GeometryPlan g = BuildPlan(width, height, fmt, strip_rows);
+    const Plan& p = g.plan;
+    const uint32_t sh = p.strip_h;
+    // Planes: aligned, disjoint, inside image_elems.
+    uint64_t prev_end = 0;
+    for (uint32_t l = 0; l < p.levels; ++l) {
+        EXPECT(p.pitch[l] % 4 == 0, "pitch %u not a multiple of 4", p.pitch[l]);
+        const uint32_t strips = (p.w[l] + kStripCols - 1) / kStripCols;
+        EXPECT(p.pitch[l] >= strips * kStripCols + 14, "pitch too small at level %u", l);
+        EXPECT(p.rows[l] >= (p.h[l] + sh - 1) / sh * sh + 2 * kRadius, "rows too small");
+        for (int c = 0; c < 3; ++c) {
+            const bool active = p.mask[l] & (1u << c);
+            EXPECT(active == !ChannelPruned(c, l, p.levels), "mask mismatch");
+            if (!active) {
+                EXPECT(p.plane_off[l][c] == ~0ull, "pruned plane has storage");
+                continue;
+            }
+            EXPECT(p.plane_off[l][c] % 64 == 0, "plane offset not 64-aligned");
+            EXPECT(p.plane_off[l][c] >= prev_end, "planes overlap");
+            prev_end = p.plane_off[l][c] + static_cast<uint64_t>(p.pitch[l]) * p.rows[l];
+        }
+    }
+    EXPECT(prev_end <= p.image_elems, "planes beyond image_elems");
+
+    // Scale items: reads of rows y0 .. y0+sh+11 and columns x0+2 .. x0+141 of
+    // the padded plane; the map covers each output pixel once.
+    std::vector<std::vector<std::vector<uint8_t>>> cover(p.levels);
+    for (uint32_t l = 0; l < p.levels; ++l) cover[l].assign(3, std::vector<uint8_t>(static_cast<size_t>(p.w[l]) * p.h[l], 0));
+    for (uint32_t i = 0; i < p.items; ++i) {
+        const ScaleItem& it = g.items[i];
+        EXPECT(it.level < p.levels && it.channel < 3, "item out of range");
+        EXPECT(p.mask[it.level] & (1u << it.channel), "item on a pruned plane");
+        EXPECT(i >= p.item_begin[it.level][it.channel] &&
+                   i < p.item_begin[it.level][it.channel] + p.item_count[it.level][it.channel],
+               "item outside its range");
+        const uint64_t last_row = it.y0 + sh + 2 * kRadius - 1;
+        const uint64_t last_col = it.x0 + 2 + kSrcCols - 1;
+        EXPECT(last_row < p.rows[it.level], "scale reads past the last row");
+        EXPECT(last_col < p.pitch[it.level], "scale reads past the pitch");
+        for (uint32_t y = it.y0; y < it.y0 + sh && y < p.h[it.level]; ++y)
+            for (uint32_t x = it.x0; x < it.x0 + kStripCols && x < p.w[it.level]; ++x)
+                cover[it.level][it.channel][static_cast<size_t>(y) * p.w[it.level] + x]++;
+    }
+    for (uint32_t l = 0; l < p.levels; ++l)
+        for (int c = 0; c < 3; ++c) {
+            if (!(p.mask[l] & (1u << c))) continue;
+            for (uint8_t v : cover[l][c]) EXPECT(v == 1, "pixel covered %u times", v);
+        }
+
+    // Prepare: tiles cover the base image; level-l stores of in-plane pixels
+    // stay inside the plane's interior; the tile nests levels up to 5.
+    EXPECT(p.prep_tiles_x * kPrepTileW >= width && p.prep_tiles_y * kPrepTileH >= height, "tiles");
+    for (uint32_t l = 0; l < p.levels; ++l) {
+        EXPECT((kPrepTileW >> l) >= 1 && (kPrepTileH >> l) >= 1, "tile does not nest level %u", l);
+        const uint64_t at = static_cast<uint64_t>(p.h[l] - 1 + kPadTop) * p.pitch[l] + p.w[l] - 1 + kPadLeft;
+        EXPECT(at < static_cast<uint64_t>(p.pitch[l]) * p.rows[l], "prepare store out of plane");
+        if (l > 0) {
+            EXPECT(p.w[l] == (p.w[l - 1] + 1) / 2 && p.h[l] == (p.h[l - 1] + 1) / 2, "level sizes");
+        }
+    }
+    EXPECT(p.plane_stride == (static_cast<uint64_t>(width) * height + 3) / 4 * 4, "plane stride");
+}
+
+int main() {
+    const PlaneFormat fmts[] = {kPlaneF32, kPlaneF16, kPlaneBF16};
+    std::vector<std::pair<uint32_t, uint32_t>> sizes = {
+        {8, 8}, {13, 9}, {32, 32}, {64, 64}, {65, 65}, {129, 97}, {512, 512}, {1920, 1080},
+        {3840, 2160}, {640, 360}, {1280, 720}, {2560, 1440}, {127, 8}, {8, 300}, {255, 257}};
+    for (uint32_t w = 8; w < 80; w += 7)
+        for (uint32_t h = 8; h < 80; h += 9) sizes.push_back({w, h});
+    for (auto [w, h] : sizes)
+        for (PlaneFormat f : fmts)
+            for (uint32_t sh : {0u, 4u, 16u, 64u, 100u, 128u}) check(w, h, f, sh);
+    // Invalid geometry and strip heights are rejected.
+    bool threw = false;
+    try {
+        BuildPlan(7, 8, kPlaneF32, 0);
+    } catch (const Error& e) {
+        threw = e.status == VQM_INVALID_ARGUMENT;
+    }
+    EXPECT(threw, "7x8 accepted");
+    threw = false;
+    try {
+        BuildPlan(64, 64, kPlaneF32, 6);
+    } catch (const Error& e) {
+        threw = e.status == VQM_INVALID_ARGUMENT;
+    }
+    EXPECT(threw, "strip_rows 6 accepted");
+    if (failures) {
+        std::fprintf(stderr, "%d failures\n", failures);
+        return 1;
+    }
+    std::printf("plan bounds ok over %zu geometries\n", sizes.size());
+    return 0;
+}

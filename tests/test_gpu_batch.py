@@ -1,0 +1,81 @@
+"""Batched scoring: pairs of uneven sizes in one launch give the same result
+as each pair alone (the pool_test.cc:27-62 pattern, here bit for bit), and
+the workspace pool respects its HBM budget."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import random_pair
+import vqm_amd
+from vqm_amd import (DeviceBuffer, Image, ImageFormat, InvalidArgument, LogicError, Size,
+                     Ssimulacra2Batch, Ssimulacra2Config, Ssimulacra2Pipeline)
+
+pytestmark = pytest.mark.gpu
+
+
+def upload_pairs(shapes, seed=0):
+    pairs, keep, host = [], [], []
+    for i, (w, h) in enumerate(shapes):
+        a, b = random_pair(w, h, seed=seed + i)
+        da, db = DeviceBuffer(a.nbytes), DeviceBuffer(b.nbytes)
+        da.upload(a)
+        db.upload(b)
+        keep += [da, db]
+        host.append((a, b))
+        pairs.append((da.ptr, db.ptr, w, h))
+    return pairs, keep, host
+
+
+def alone(w, h, a, b, fmt=0):
+    p = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(w, h), plane_format=fmt))
+    r = p.Compute(Image(Size(w, h), ImageFormat.srgb8(), a), Image(Size(w, h), ImageFormat.srgb8(), b))
+    p.close()
+    return r.norms()
+
+
+@pytest.mark.parametrize("fmt", [vqm_amd.PLANES_F32, vqm_amd.PLANES_F16])
+def test_mixed_batch_equals_each_pair_alone(fmt):
+    shapes = [(64, 64), (200, 130), (33, 17), (640, 360), (64, 64), (129, 97), (8, 8)]
+    pairs, keep, host = upload_pairs(shapes, seed=11)
+    batch = Ssimulacra2Batch(max_pairs=16, plane_format=fmt)
+    scales, norms = batch.Finish(batch.Submit(pairs), raw=True)
+    for i, ((w, h), (a, b)) in enumerate(zip(shapes, host)):
+        assert np.array_equal(norms[i], alone(w, h, a, b, fmt)), f"pair {i} {w}x{h}"
+        assert scales[i] == len(vqm_amd.plan_scales(Size(w, h)))
+    # Again, in another order: workspaces are reused from the pool.
+    before = batch.PoolBytes()
+    order = [3, 0, 6, 2, 5, 1, 4]
+    scales2, norms2 = batch.Finish(batch.Submit([pairs[i] for i in order]), raw=True)
+    for k, i in enumerate(order):
+        assert np.array_equal(norms2[k], norms[i])
+    assert batch.PoolBytes() == before
+
+
+def test_uniform_batch_equals_each_pair_alone():
+    shapes = [(320, 180)] * 5
+    pairs, keep, host = upload_pairs(shapes, seed=21)
+    batch = Ssimulacra2Batch(max_pairs=8)
+    _, norms = batch.Finish(batch.Submit(pairs), raw=True)
+    for i, (a, b) in enumerate(host):
+        assert np.array_equal(norms[i], alone(320, 180, a, b))
+    t = batch.StageTimes()
+    assert all(v > 0 for v in t)
+
+
+def test_batch_budget_and_misuse():
+    pairs, keep, _ = upload_pairs([(256, 256)] * 3, seed=31)
+    batch = Ssimulacra2Batch(max_pairs=2, hbm_budget=1)
+    with pytest.raises(InvalidArgument):
+        batch.Submit(pairs[:1])           # one workspace exceeds a 1-byte budget
+    big = Ssimulacra2Batch(max_pairs=2)
+    with pytest.raises(InvalidArgument):
+        big.Submit(pairs)                 # more than max_pairs
+    token = big.Submit(pairs[:2])
+    with pytest.raises(LogicError):
+        big.Submit(pairs[:1])
+    big.Finish(token)
+    with pytest.raises(InvalidArgument):
+        big.Finish(token)
+    with pytest.raises(InvalidArgument):
+        big.Submit([(pairs[0][0], pairs[0][1], 4, 4)])  # below 8x8

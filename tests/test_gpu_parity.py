@@ -1,0 +1,148 @@
+"""Parity of the HIP path with the oracle, through the C ABI.
+
+Bars:
+  * against the oracle's FP32 device form: the kernels compute each pixel's
+    values bit for bit as it does, so the 108 norms agree to FP64 summation
+    order (relative 1e-10);
+  * against the FP32 form in the reference shaders' operation order and
+    against FP64: |dscore| <= 1e-3, features within 1e-4 relative + 2e-6;
+  * identity scores exactly 100; results repeat bit for bit.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import (GOLDEN_B, GOLDEN_C, GOLDEN_TOL, SCORE_ABS, assert_features_close,
+                      gradient_image, ramp_image, random_pair)
+import vqm_amd
+from vqm_amd import Image, ImageFormat, Size, Ssimulacra2Config, Ssimulacra2Pipeline, synth
+
+pytestmark = pytest.mark.gpu
+
+DEV_REL = 1e-10
+
+
+def image(w, h, data) -> Image:
+    return Image(Size(w, h), ImageFormat.srgb8(), data)
+
+
+def gpu(w, h, a, b, **cfg):
+    p = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(w, h), **cfg))
+    r = p.Compute(image(w, h, a), image(w, h, b))
+    p.close()
+    return len(r.scales), r.norms(), r
+
+
+def check_against_oracle(oracle_lib, w, h, a, b, dev_mode=None, **cfg):
+    n, got, res = gpu(w, h, a, b, **cfg)
+    dev = dev_mode if dev_mode is not None else oracle_lib.F32_DEV
+    n_o, want, _ = oracle_lib.compute(dev, w, h, a, b)
+    assert n == n_o
+    np.testing.assert_allclose(got, want, rtol=DEV_REL, atol=1e-300,
+                               err_msg=f"{w}x{h} GPU vs oracle device form")
+    return n, got, res
+
+
+def test_reference_goldens_on_gpu(oracle_lib):
+    """pipeline_test.cc:146-174."""
+    orig = gradient_image(0)
+    for variant, golden in ((1, GOLDEN_B), (2, GOLDEN_C)):
+        dist = gradient_image(variant)
+        n, norms, res = check_against_oracle(oracle_lib, 32, 32, orig, dist)
+        assert n == 4
+        score = res.Score()
+        assert abs(score - golden) <= GOLDEN_TOL
+        _, ref, _ = oracle_lib.compute(oracle_lib.F32_REF, 32, 32, orig, dist)
+        assert abs(score - oracle_lib.score(n, ref)) <= SCORE_ABS
+
+
+@pytest.mark.parametrize("size,scales", [((8, 8), 2), ((13, 9), 2), ((32, 32), 4), ((64, 64), 5),
+                                         ((65, 65), 5), ((129, 97), 5), ((300, 41), 3)])
+def test_identity_and_small_geometries(size, scales):
+    """pipeline_test.cc:120-144, 760-783: identity is exactly 100 and repeats."""
+    w, h = size
+    img = ramp_image(w, h)
+    p = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(w, h)))
+    first = p.Compute(image(w, h, img), image(w, h, img))
+    second = p.Compute(image(w, h, img), image(w, h, img))
+    assert len(first.scales) == scales
+    assert first.Score() == 100.0
+    assert np.array_equal(first.norms(), second.norms())
+
+
+@pytest.mark.parametrize("size", [(32, 32), (97, 61), (200, 130), (257, 129), (1000, 37), (37, 600),
+                                  (128, 64), (129, 65)])
+def test_random_pairs_match_oracle(oracle_lib, size):
+    w, h = size
+    a, b = random_pair(w, h, seed=w * 7 + h)
+    n, got, _ = check_against_oracle(oracle_lib, w, h, a, b)
+    _, f64, _ = oracle_lib.compute(oracle_lib.F64, w, h, a, b)
+    assert abs(oracle_lib.score(n, got) - oracle_lib.score(n, f64)) <= SCORE_ABS
+
+
+@pytest.mark.parametrize("strength", synth.STRENGTHS)
+def test_synthetic_512_matches_oracle(oracle_lib, strength):
+    """The 512x512 workload: bit-level parity with the device form and the
+    parity contract against the reference-order FP32 form and FP64."""
+    a, b = synth.pair(0, 0, 512, 512, strength)
+    n, got, res = check_against_oracle(oracle_lib, 512, 512, a, b)
+    for form in (oracle_lib.F32_REF, oracle_lib.F64):
+        _, want, _ = oracle_lib.compute(form, 512, 512, a, b)
+        assert abs(res.Score() - oracle_lib.score(n, want)) <= SCORE_ABS
+        assert_features_close(got, want, what="GPU vs oracle")
+
+
+@pytest.mark.parametrize("strength", synth.STRENGTHS)
+def test_1080p_matches_oracle(oracle_lib, strength):
+    a, b = synth.pair(1, 0, 1920, 1080, strength)
+    n, got, res = check_against_oracle(oracle_lib, 1920, 1080, a, b)
+    assert n == 6
+    _, ref, _ = oracle_lib.compute(oracle_lib.F32_REF, 1920, 1080, a, b)
+    assert abs(res.Score() - oracle_lib.score(n, ref)) <= SCORE_ABS
+    assert_features_close(got, ref, what="1080p GPU vs reference-order FP32")
+
+
+def test_4k_matches_oracle(oracle_lib):
+    a, b = synth.pair(2, 0, 3840, 2160, synth.STRENGTHS[1])
+    n, got, res = check_against_oracle(oracle_lib, 3840, 2160, a, b)
+    _, f64, _ = oracle_lib.compute(oracle_lib.F64, 3840, 2160, a, b)
+    assert abs(res.Score() - oracle_lib.score(n, f64)) <= SCORE_ABS
+    assert_features_close(got, f64, what="4K GPU vs FP64")
+
+
+@pytest.mark.parametrize("fmt", [vqm_amd.PLANES_F16, vqm_amd.PLANES_BF16])
+def test_narrow_planes_match_oracle(oracle_lib, fmt):
+    """FP16/BF16 stored planes: bit-level parity with the oracle's device form
+    rounding the same planes; FP16 stays within the reference's half-plane
+    bounds of the FP32 result (pipeline_test.cc:785-834: score 0.1, norm
+    2e-4)."""
+    orig = gradient_image(0)
+    for variant in (1, 2):
+        dist = gradient_image(variant)
+        n, got, res = check_against_oracle(oracle_lib, 32, 32, orig, dist,
+                                           dev_mode=oracle_lib.f32_dev_planes(fmt), plane_format=fmt)
+        _, full, _ = oracle_lib.compute(oracle_lib.F32_DEV, 32, 32, orig, dist)
+        if fmt == vqm_amd.PLANES_F16:
+            assert abs(res.Score() - oracle_lib.score(n, full)) <= 0.1
+            assert np.max(np.abs(got - full)) <= 2e-4
+    a, b = synth.pair(0, 0, 512, 512, synth.STRENGTHS[1])
+    check_against_oracle(oracle_lib, 512, 512, a, b, dev_mode=oracle_lib.f32_dev_planes(fmt),
+                         plane_format=fmt)
+
+
+@pytest.mark.parametrize("strip_rows", [4, 16, 128])
+def test_strip_height_does_not_change_results(oracle_lib, strip_rows):
+    a, b = random_pair(300, 200, seed=9)
+    check_against_oracle(oracle_lib, 300, 200, a, b, strip_rows=strip_rows)
+
+
+def test_repeatable_and_interleaved():
+    """pipeline_test.cc:335-357: results repeat bit for bit, also after
+    another pair ran in between."""
+    orig, da, db = gradient_image(0), gradient_image(1), gradient_image(2)
+    p = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(32, 32)))
+    first = p.Compute(image(32, 32, orig), image(32, 32, da)).norms()
+    p.Compute(image(32, 32, orig), image(32, 32, db))
+    again = p.Compute(image(32, 32, orig), image(32, 32, da)).norms()
+    assert np.array_equal(first, again)

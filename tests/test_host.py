@@ -1,0 +1,152 @@
+"""Host logic of the product library through its C ABI, with no GPU: the
+library loads, exports every declared symbol, and its host-side functions
+(scale plan, taps, pruning, score mapping, decode table, validation, memory
+plan) agree with the oracle and the reference's contracts."""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+import vqm_amd
+from vqm_amd import _lib as L
+from vqm_amd import (ImageFormat, InvalidArgument, PixelType, Size, Ssimulacra2Config,
+                     Ssimulacra2Result, Ssimulacra2Scale)
+
+
+def declared_functions(header: str):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vqm_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.lib()
+    declared = declared_functions("vqm_ss2.h")
+    assert len(declared) >= 30
+    for name in declared:
+        assert hasattr(lib, name), f"libvqm_hip.so does not export {name}"
+    assert set(declared) == set(L.SIGNATURES), "ctypes signatures and the header disagree"
+
+
+def test_scale_plan_matches_oracle(oracle_lib):
+    for size in [(8, 8), (13, 9), (32, 32), (64, 64), (65, 65), (129, 97), (1920, 1080),
+                 (3840, 2160), (640, 360), (1280, 720), (2560, 1440), (512, 512), (4096, 17)]:
+        got = [(s.width, s.height) for s in vqm_amd.plan_scales(Size(*size))]
+        assert got == oracle_lib.plan_scales(*size)
+    with pytest.raises(InvalidArgument, match="8x8"):
+        vqm_amd.plan_scales(Size(4, 8))
+
+
+def test_taps_and_pruning_match_oracle(oracle_lib):
+    assert np.array_equal(vqm_amd.gaussian_taps(), oracle_lib.gaussian_taps())
+    for scales in range(1, 7):
+        for s in range(scales):
+            for c in range(3):
+                assert vqm_amd.channel_pruned(c, s, scales) == oracle_lib.channel_pruned(c, s, scales)
+    assert not vqm_amd.channel_pruned(0, 6, 6)  # out of range is never pruned
+    assert not vqm_amd.channel_pruned(3, 0, 6)
+
+
+def test_decode_table_is_the_oracles_bit_for_bit(oracle_lib):
+    """The table the kernels read equals the oracle's FP32 decode exactly."""
+    assert np.array_equal(vqm_amd.srgb_table(), oracle_lib.srgb_lut_f32())
+
+
+def test_score_matches_oracle(oracle_lib):
+    rng = np.random.default_rng(5)
+    for scales in range(0, 7):
+        norms = np.zeros(108)
+        norms[:scales * 18] = rng.random(scales * 18) * 0.05
+        r = Ssimulacra2Result.from_norms(scales, norms)
+        assert r.Score() == oracle_lib.score(scales, norms)
+
+
+def test_score_contract():
+    """score_test.cc:11-46 through the product's Ssimulacra2Result."""
+    assert Ssimulacra2Result().Score() == 100.0
+    r = Ssimulacra2Result([Ssimulacra2Scale() for _ in range(6)])
+    assert r.Score() == 100.0
+    for bad in (math.inf, -math.inf, math.nan):
+        for s in range(6):
+            for n in range(6):
+                r.scales[s].avg_ssim[n] = bad
+                with pytest.raises(InvalidArgument):
+                    r.Score()
+                r.scales[s].avg_ssim[n] = 0.0
+            for n in range(12):
+                r.scales[s].avg_edgediff[n] = bad
+                with pytest.raises(InvalidArgument):
+                    r.Score()
+                r.scales[s].avg_edgediff[n] = 0.0
+    r.scales[0].avg_ssim[3] = np.finfo(np.float64).max
+    with pytest.raises(InvalidArgument):
+        r.Score()
+    r.scales[0].avg_ssim[3] = 1e200
+    with pytest.raises(InvalidArgument):
+        r.Score()
+    r.scales[0].avg_ssim[3] = 100.0
+    assert math.isfinite(r.Score()) and r.Score() < 0.0
+    with pytest.raises(InvalidArgument):
+        Ssimulacra2Result([Ssimulacra2Scale() for _ in range(7)]).Score()
+
+
+def test_required_bytes():
+    """input.cc:114-145,244-255: planes start on 4-byte boundaries."""
+    f = ImageFormat.srgb8()
+    assert vqm_amd.required_bytes(f, Size(32, 32)) == 3 * 1024
+    assert vqm_amd.required_bytes(f, Size(13, 9)) == 3 * 120  # 117 -> 120
+    assert vqm_amd.required_bytes(f, Size(1920, 1080)) == 3 * 1920 * 1080
+    assert vqm_amd.required_bytes(ImageFormat(), Size(3, 3)) == 3 * 36  # Float samples
+    grey = ImageFormat(type=PixelType.Byte, family=0, matrix=1)
+    assert vqm_amd.required_bytes(grey, Size(5, 5)) == 28
+    with pytest.raises(InvalidArgument):
+        vqm_amd.required_bytes(ImageFormat(type=7), Size(8, 8))
+
+
+def test_config_validation_before_device_work():
+    """pipeline_test.cc:318-328: an impossible geometry names the minimum
+    before any device work, so it holds with or without a GPU."""
+    for size in (Size(4, 8), Size(8, 4), Size(0, 0)):
+        with pytest.raises(InvalidArgument, match="8x8"):
+            vqm_amd.Ssimulacra2Pipeline(Ssimulacra2Config(size=size))
+    with pytest.raises(InvalidArgument):
+        vqm_amd.Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(32, 32), plane_format=5))
+    with pytest.raises(InvalidArgument):
+        vqm_amd.Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(32, 32), gpu_resident_images=5000))
+
+
+def test_memory_plan_and_algorithmic_bytes():
+    """The pruned plane set at six scales has 1.684 N active channel pixels;
+    resident pair traffic is 6N + 4 * plane bytes (32.94 B/px FP32,
+    19.47 B/px FP16 at 1080p)."""
+    n = 1920 * 1080
+    b32 = vqm_amd.algorithmic_bytes(Size(1920, 1080), L.PLANES_F32)
+    b16 = vqm_amd.algorithmic_bytes(Size(1920, 1080), L.PLANES_F16)
+    assert b32 / n == pytest.approx(32.94, abs=0.01)
+    assert b16 / n == pytest.approx(19.47, abs=0.01)
+    assert b32 == pytest.approx(68.30e6, rel=1e-3)
+    m = vqm_amd.memory_plan(Ssimulacra2Config(size=Size(1920, 1080)))
+    assert m.resident_bytes == 2 * 3 * n
+    assert m.streaming_bytes == 2 * 3 * n
+    # Padded planes of both images: at least the 1.684 N active pixels each.
+    assert m.workspace_bytes >= 2 * 4 * 1.684 * n
+    assert m.total_device_bytes == m.workspace_bytes + m.resident_bytes + m.streaming_bytes
+    half = vqm_amd.memory_plan(Ssimulacra2Config(size=Size(1920, 1080), half_planes=True))
+    assert half.workspace_bytes < 0.6 * m.workspace_bytes
+
+
+def test_last_error_is_thread_local_and_cleared():
+    lib = L.lib()
+    sizes = (L.u32 * 12)()
+    count = L.u32()
+    assert lib.vqm_ss2_plan_scales(4, 4, sizes, ctypes.byref(count)) == L.VQM_INVALID_ARGUMENT
+    assert b"8x8" in lib.vqm_last_error()
+    assert lib.vqm_ss2_plan_scales(8, 8, sizes, ctypes.byref(count)) == L.VQM_OK
+    assert lib.vqm_last_error() == b""

@@ -1,0 +1,276 @@
+"""Pins of the CPU oracle (oracle/): each part checked against the reference's
+own goldens and contracts or against what the mathematics fixes -- never
+against a retyping of the oracle's own formula."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import (GOLDEN_B, GOLDEN_C, GOLDEN_TOL, SCORE_ABS, assert_features_close,
+                      gradient_image, ramp_image)
+
+
+# ---- end to end: the reference's goldens and identity ---------------------
+
+@pytest.mark.parametrize("form", ["F64", "F32_REF", "F32_DEV"])
+def test_reference_goldens(oracle_lib, form):
+    """pipeline_test.cc:146-174: gradient vs gradient_B / gradient_C, +-5e-3."""
+    mode = getattr(oracle_lib, form)
+    orig = gradient_image(0)
+    n, norms_b, _ = oracle_lib.compute(mode, 32, 32, orig, gradient_image(1))
+    _, norms_c, _ = oracle_lib.compute(mode, 32, 32, orig, gradient_image(2))
+    assert n == 4
+    assert abs(oracle_lib.score(n, norms_b) - GOLDEN_B) <= GOLDEN_TOL
+    assert abs(oracle_lib.score(n, norms_c) - GOLDEN_C) <= GOLDEN_TOL
+
+
+def test_reference_order_tracks_the_reference_goldens(oracle_lib):
+    """The FP32 form in the shaders' operation order lands within 1e-3 of the
+    reference's GPU goldens (its remaining gap is driver transcendentals)."""
+    orig = gradient_image(0)
+    for dist, golden in ((gradient_image(1), GOLDEN_B), (gradient_image(2), GOLDEN_C)):
+        n, norms, _ = oracle_lib.compute(oracle_lib.F32_REF, 32, 32, orig, dist)
+        assert abs(oracle_lib.score(n, norms) - golden) <= SCORE_ABS
+
+
+@pytest.mark.parametrize("form", ["F64", "F32_REF", "F32_DEV"])
+@pytest.mark.parametrize("size", [(32, 32), (8, 8), (13, 9), (64, 64), (65, 65), (129, 97)])
+def test_identity_is_exactly_100(oracle_lib, form, size):
+    """pipeline_test.cc:120-144 and :760-783: identical planes cancel before
+    any division rounds, so every map is exactly zero."""
+    w, h = size
+    img = ramp_image(w, h)
+    n, norms, sums = oracle_lib.compute(getattr(oracle_lib, form), w, h, img, img)
+    assert np.all(sums == 0.0)
+    assert oracle_lib.score(n, norms) == 100.0
+
+
+# ---- scale plan (pipeline.cc:28-48) ---------------------------------------
+
+@pytest.mark.parametrize("size,count", [((32, 32), 4), ((8, 8), 2), ((13, 9), 2), ((64, 64), 5),
+                                        ((65, 65), 5), ((129, 97), 5)])
+def test_scale_counts(oracle_lib, size, count):
+    """pipeline_test.cc:126-128 and :767-776."""
+    assert len(oracle_lib.plan_scales(*size)) == count
+
+
+def test_scale_sizes_of_the_workloads(oracle_lib):
+    """Level sizes halve rounding up; the previous level gates the next."""
+    assert oracle_lib.plan_scales(1920, 1080) == [(1920, 1080), (960, 540), (480, 270), (240, 135),
+                                                  (120, 68), (60, 34)]
+    assert oracle_lib.plan_scales(3840, 2160)[-1] == (120, 68)
+    assert oracle_lib.plan_scales(640, 360)[-1] == (20, 12)
+    assert oracle_lib.plan_scales(1280, 720)[-1] == (40, 23)
+    assert oracle_lib.plan_scales(2560, 1440)[-1] == (80, 45)
+    assert oracle_lib.plan_scales(512, 512)[-1] == (16, 16)
+    with pytest.raises(ValueError):
+        oracle_lib.plan_scales(4, 8)
+    with pytest.raises(ValueError):
+        oracle_lib.plan_scales(8, 7)
+
+
+# ---- pruning (score.cc:91-114) --------------------------------------------
+
+def active(oracle_lib, scales):
+    return ["".join("XYB"[c] for c in range(3) if not oracle_lib.channel_pruned(c, s, scales))
+            for s in range(scales)]
+
+
+def test_pruning_mask_six_scales(oracle_lib):
+    """docs/ssimulacra2.md:96-104: at six scales 1:1 keeps Y, 1:2 and 1:4 X,Y,
+    1:8 and 1:16 all three, 1:32 none -- 7 of 18 pairs pruned."""
+    assert active(oracle_lib, 6) == ["Y", "XY", "XY", "XYB", "XYB", ""]
+
+
+def test_pruning_mask_four_scales(oracle_lib):
+    """pipeline_test.cc:163-169: the 32x32 pair runs four scales and three maps
+    are pruned (the flat weight run puts X at 1:1, Y at 1:2, B at the 4th)."""
+    assert active(oracle_lib, 4) == ["YB", "XB", "XYB", "XY"]
+
+
+# ---- score mapping (score_test.cc) -----------------------------------------
+
+def test_score_rejects_nonfinite_norms(oracle_lib):
+    """score_test.cc:11-32: every one of the 108 entries is checked."""
+    for bad in (math.inf, -math.inf, math.nan):
+        for i in range(108):
+            norms = np.zeros(108)
+            norms[i] = bad
+            with pytest.raises(ValueError):
+                oracle_lib.score(6, norms)
+
+
+def test_score_overflow_and_range(oracle_lib):
+    """score_test.cc:34-46."""
+    assert oracle_lib.score(0, np.zeros(108)) == 100.0
+    assert oracle_lib.score(6, np.zeros(108)) == 100.0
+    norms = np.zeros(108)
+    norms[3] = np.finfo(np.float64).max
+    with pytest.raises(ValueError):
+        oracle_lib.score(6, norms)
+    norms[3] = 1e200
+    with pytest.raises(ValueError):
+        oracle_lib.score(6, norms)
+    norms[3] = 100.0
+    s = oracle_lib.score(6, norms)
+    assert math.isfinite(s) and s < 0.0
+    with pytest.raises(ValueError):
+        oracle_lib.score(7, np.zeros(108))
+
+
+def test_score_mapping_closed_form(oracle_lib):
+    """One weighted norm through the published polynomial and power: the Y
+    1:1 4-norm weight 7.466890328078848 (score.cc:59) at six scales."""
+    norms = np.zeros(108)
+    norms[3] = 0.01  # level 0, avg_ssim[Y * 2 + 1]
+    e = 7.466890328078848 * 0.01 * 0.9562382616834844
+    m = 2.326765642916932 * e - 0.020884521182843837 * e * e + 6.248496625763138e-05 * e ** 3
+    assert oracle_lib.score(6, norms) == pytest.approx(100.0 - 10.0 * m ** 0.6276336467831387, abs=1e-12)
+
+
+# ---- taps, decode, XYB ------------------------------------------------------
+
+def test_gaussian_taps(oracle_lib):
+    """pipeline.cc:274-291 sampled at sigma 1.5, radius ceil(3.5 sigma) = 6."""
+    t = oracle_lib.gaussian_taps()
+    k = np.arange(-6, 7) / 1.5
+    w = np.exp(-0.5 * k * k)
+    assert np.array_equal(t, (w / w.sum()).astype(np.float32))
+    assert np.array_equal(t, t[::-1])
+    np.testing.assert_allclose(t[:7], [8.92210686e-05, 1.02819658e-03, 7.5974022e-03,
+                                       3.59943481e-02, 0.109341175, 0.212967529, 0.265964257],
+                               rtol=1e-7)
+
+
+def test_srgb_decode(oracle_lib):
+    """IEC 61966-2-1 EOTF of code/255 (color.glsl:135-137)."""
+    lut64 = oracle_lib.srgb_lut_f64()
+    lut32 = oracle_lib.srgb_lut_f32()
+    v = np.arange(256) / 255.0
+    want = np.where(v <= 0.04045, v / 12.92, ((v + 0.055) / 1.055) ** 2.4)
+    np.testing.assert_allclose(lut64, want, rtol=1e-14, atol=0)
+    # FP32: the decode product and the two FP32 operations before the power
+    # each round, and the exponent 2.4 multiplies their relative error.
+    np.testing.assert_allclose(lut32, want, rtol=1e-6, atol=1e-12)
+    assert lut32[0] == 0.0 and lut32[255] == 1.0
+    assert np.all(np.diff(lut32) > 0)
+
+
+def test_xyb_closed_forms(oracle_lib):
+    """Black: all three opsin mixes equal the bias, so X and B sit exactly at
+    their offsets and Y at 0.01 plus cbrt(bias) - kCbrtBias (the constant is
+    the FP32-rounded cube root, prepare.comp:50). Grey keeps X at its offset
+    (opsin rows 0 and 1 both sum to 1). FP32 forms agree with FP64 to FP32
+    precision."""
+    bias, cbias = 0.0037930732552754493, 0.15595419704914093
+    black = oracle_lib.xyb_f64(0.0, 0.0, 0.0)
+    assert black[0] == 0.42 and black[2] == 0.55
+    assert black[1] == pytest.approx(0.01 + (np.cbrt(bias) - cbias), abs=1e-16)
+    for v in (0.05, 0.2, 0.5, 1.0):
+        g = oracle_lib.xyb_f64(v, v, v)
+        assert abs(g[0] - 0.42) < 1e-15
+        assert g[1] == pytest.approx(np.cbrt(v + bias) - cbias + 0.01, abs=1e-15)
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        r, g, b = rng.random(3).astype(np.float32)
+        ref = oracle_lib.xyb_f64(float(r), float(g), float(b))
+        for kind in (0, 1):
+            got = oracle_lib.xyb_f32(float(r), float(g), float(b), kind)
+            np.testing.assert_allclose(got, ref, atol=3e-6)
+
+
+def test_device_cube_root(oracle_lib):
+    """The kernels' Newton cube root is faithful (<= 0.75 ulp) on the XYB
+    domain and exact at 0 and at perfect cubes of powers of two."""
+    assert oracle_lib.cbrt_dev(0.0) == 0.0
+    assert oracle_lib.cbrt_dev(0.125) == 0.5
+    assert oracle_lib.cbrt_dev(1.0) == 1.0
+    rng = np.random.default_rng(2)
+    xs = np.concatenate([np.float32(0.0037930732) * (1 + rng.random(2000)),
+                         rng.random(5000).astype(np.float32) + np.float32(0.004)]).astype(np.float32)
+    worst = 0.0
+    for x in xs:
+        y = oracle_lib.cbrt_dev(float(x))
+        exact = np.cbrt(np.float64(x))
+        ulp = np.spacing(np.float32(exact))
+        worst = max(worst, abs(y - exact) / ulp)
+    assert worst <= 0.75
+
+
+# ---- blur and pyramid --------------------------------------------------------
+
+def brute_force_blur(plane: np.ndarray) -> np.ndarray:
+    """Direct 2D convolution with the separable Gaussian, zeros outside."""
+    k = np.arange(-6, 7) / 1.5
+    w = np.exp(-0.5 * k * k)
+    w = (w / w.sum()).astype(np.float32).astype(np.float64)
+    h, wd = plane.shape
+    pad = np.pad(plane.astype(np.float64), 6)
+    out = np.zeros((h, wd))
+    for i in range(13):
+        for j in range(13):
+            out += w[i] * w[j] * pad[i:i + h, j:j + wd]
+    return out
+
+
+@pytest.mark.parametrize("shape", [(8, 8), (9, 13), (20, 31), (1, 17)])
+def test_blur_against_brute_force(oracle_lib, shape):
+    rng = np.random.default_rng(shape[0] * 100 + shape[1])
+    plane = rng.random(shape)
+    want = brute_force_blur(plane)
+    np.testing.assert_allclose(oracle_lib.blur_f64(plane), want, rtol=1e-13, atol=1e-15)
+    p32 = plane.astype(np.float32)
+    want32 = brute_force_blur(p32)
+    for order in (0, 1):
+        np.testing.assert_allclose(oracle_lib.blur_f32(p32, order), want32, rtol=2e-6, atol=1e-7)
+
+
+def box_pyramid(levels0: np.ndarray, sizes):
+    """Clamped 2x2 box pyramid via edge padding and block means (numpy)."""
+    out = [levels0]
+    cur = levels0
+    for (w, h) in sizes[1:]:
+        c, ph, pw = cur.shape
+        padded = np.pad(cur, ((0, 0), (0, 2 * h - ph), (0, 2 * w - pw)), mode="edge")
+        cur = padded.reshape(c, h, 2, w, 2).mean(axis=(2, 4))
+        out.append(cur)
+    return out
+
+
+@pytest.mark.parametrize("size", [(32, 32), (13, 9), (65, 65), (129, 97), (40, 23)])
+def test_pyramid_is_the_clamped_box(oracle_lib, size):
+    """prepare.comp:100-111: each level is the 2x2 mean of the previous
+    level's linear RGB, the last row and column duplicated on odd sizes."""
+    w, h = size
+    img = ramp_image(w, h, seed=3)
+    sizes = oracle_lib.plan_scales(w, h)
+    lut = oracle_lib.srgb_lut_f64()
+    stride = (w * h + 3) // 4 * 4
+    base = np.stack([lut[img[c * stride:c * stride + w * h]].reshape(h, w) for c in range(3)])
+    want = box_pyramid(base, sizes)
+    lut32 = oracle_lib.srgb_lut_f32().astype(np.float64)
+    base32 = np.stack([lut32[img[c * stride:c * stride + w * h]].reshape(h, w) for c in range(3)])
+    want32 = box_pyramid(base32, sizes)
+    got64 = oracle_lib.pyramid(w, h, img, f64=True)
+    got32 = oracle_lib.pyramid(w, h, img, f64=False)
+    for l in range(len(sizes)):
+        np.testing.assert_allclose(got64[l], want[l], rtol=1e-14, atol=1e-16)
+        # Three FP32 additions per level, compounded over the levels.
+        np.testing.assert_allclose(got32[l], want32[l], rtol=2.5e-7 * (l + 1), atol=1e-12)
+
+
+# ---- the FP32 forms against FP64 on the synthetic workload -------------------
+
+@pytest.mark.parametrize("strength", [(4, 1.0), (12, 3.0), (24, 8.0)])
+def test_fp32_forms_within_contract_of_fp64(oracle_lib, strength):
+    """512x512 pairs: both FP32 forms stay within the parity contract of the
+    FP64 form (|dscore| <= 1e-3, features 1e-4 relative + 2e-6)."""
+    from vqm_amd import synth
+    a, b = synth.pair(0, 0, 512, 512, strength)
+    n, f64, _ = oracle_lib.compute(oracle_lib.F64, 512, 512, a, b)
+    for form in (oracle_lib.F32_REF, oracle_lib.F32_DEV):
+        _, f32, _ = oracle_lib.compute(form, 512, 512, a, b)
+        assert abs(oracle_lib.score(n, f32) - oracle_lib.score(n, f64)) <= SCORE_ABS
+        assert_features_close(f32, f64, what="fp32 vs fp64")

@@ -1,0 +1,22 @@
+"""Builds and runs tests/cpp/plan_bounds_test.cc (host only): every address
+the kernels compute from a work plan stays inside its buffer."""
+from __future__ import annotations
+
+import os
+import subprocess
+import tempfile
+
+from conftest import ROOT
+
+
+def test_plan_bounds():
+    with tempfile.TemporaryDirectory() as tmp:
+        exe = os.path.join(tmp, "plan_bounds_test")
+        subprocess.run(["g++", "-std=c++20", "-O1", "-Wall",
+                        "-I", os.path.join(ROOT, "include"),
+                        "-I", os.path.join(ROOT, "vqm_amd", "csrc"),
+                        os.path.join(ROOT, "tests", "cpp", "plan_bounds_test.cc"),
+                        os.path.join(ROOT, "vqm_amd", "csrc", "ss2_host.cc"),
+                        "-o", exe], check=True)
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr

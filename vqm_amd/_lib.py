@@ -1,0 +1,164 @@
+"""ctypes binding of libvqm_hip.so (include/vqm_ss2.h).
+
+The library is built in-tree by ``python -m vqm_amd.build`` (or
+``__graft_entry__.build()``). Loading fails loudly when it is missing: there is
+no CPU fallback for the scoring path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvqm_hip.so")
+SYNTH_PATH = os.path.join(_HERE, "libvqm_synth.so")
+
+VQM_OK = 0
+VQM_INVALID_ARGUMENT = 1
+VQM_LOGIC_ERROR = 2
+VQM_RUNTIME_ERROR = 3
+VQM_OVERFLOW = 4
+
+PLANES_F32 = 0
+PLANES_F16 = 1
+PLANES_BF16 = 2
+
+u8, u32, u64, i32, f32, f64 = (ctypes.c_uint8, ctypes.c_uint32, ctypes.c_uint64,
+                               ctypes.c_int32, ctypes.c_float, ctypes.c_double)
+vp = ctypes.c_void_p
+
+
+class ImageFormatC(ctypes.Structure):
+    _fields_ = [("type", u8), ("family", u8), ("alpha", u8), ("range", u8),
+                ("matrix", u8), ("transfer", u8), ("primaries", u8),
+                ("chroma_location", u8), ("subsample_w", u8), ("subsample_h", u8),
+                ("depth", u8), ("reserved", u8 * 5)]
+
+
+class ImageDescC(ctypes.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("format", ImageFormatC), ("bytes", u64)]
+
+
+class DeviceImageC(ctypes.Structure):
+    _fields_ = [("data", vp), ("width", u32), ("height", u32), ("format", ImageFormatC),
+                ("bytes", u64)]
+
+
+class ConfigC(ctypes.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("resident_images", u32),
+                ("plane_format", u32), ("streaming_inputs", u32), ("strip_rows", u32),
+                ("input_bytes", u64)]
+
+
+class MemoryC(ctypes.Structure):
+    _fields_ = [("workspace_bytes", u64), ("resident_bytes", u64), ("streaming_bytes", u64),
+                ("input_pool_bytes", u64), ("pinned_host_bytes", u64),
+                ("total_device_bytes", u64)]
+
+
+class BatchConfigC(ctypes.Structure):
+    _fields_ = [("plane_format", u32), ("strip_rows", u32), ("max_pairs", u32),
+                ("reserved", u32), ("hbm_budget", u64)]
+
+
+class DevicePairC(ctypes.Structure):
+    _fields_ = [("original", vp), ("distorted", vp), ("width", u32), ("height", u32)]
+
+
+# Every function include/vqm_ss2.h declares: name -> (restype, argtypes).
+SIGNATURES = {
+    "vqm_last_error": (ctypes.c_char_p, []),
+    "vqm_ss2_config_init": (None, [ctypes.POINTER(ConfigC), u32, u32]),
+    "vqm_image_format_srgb8": (None, [ctypes.POINTER(ImageFormatC)]),
+    "vqm_required_bytes": (ctypes.c_int, [ctypes.POINTER(ImageFormatC), u32, u32,
+                                          ctypes.POINTER(u64)]),
+    "vqm_ss2_plan_scales": (ctypes.c_int, [u32, u32, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
+    "vqm_ss2_gaussian_taps": (None, [ctypes.POINTER(f32)]),
+    "vqm_ss2_channel_pruned": (ctypes.c_int, [u32, u32, u32]),
+    "vqm_ss2_score": (ctypes.c_int, [u32, ctypes.POINTER(f64), ctypes.POINTER(f64)]),
+    "vqm_ss2_srgb_table": (None, [ctypes.POINTER(f32)]),
+    "vqm_ss2_memory_plan": (ctypes.c_int, [ctypes.POINTER(ConfigC), ctypes.POINTER(MemoryC)]),
+    "vqm_ss2_algorithmic_bytes": (ctypes.c_int, [u32, u32, u32, ctypes.POINTER(u64)]),
+    "vqm_ss2_create": (ctypes.c_int, [ctypes.POINTER(ConfigC), ctypes.POINTER(vp)]),
+    "vqm_ss2_destroy": (None, [vp]),
+    "vqm_ss2_upload": (ctypes.c_int, [vp, ctypes.POINTER(ImageDescC), vp, ctypes.POINTER(u64)]),
+    "vqm_ss2_release": (ctypes.c_int, [vp, u64]),
+    "vqm_ss2_submit_host": (ctypes.c_int, [vp, ctypes.POINTER(ImageDescC), vp,
+                                           ctypes.POINTER(ImageDescC), vp, f32,
+                                           ctypes.POINTER(u64)]),
+    "vqm_ss2_submit_resident": (ctypes.c_int, [vp, u64, u64, f32, ctypes.POINTER(u64)]),
+    "vqm_ss2_submit_resident_host": (ctypes.c_int, [vp, u64, ctypes.POINTER(ImageDescC), vp, f32,
+                                                    ctypes.POINTER(u64)]),
+    "vqm_ss2_submit_device": (ctypes.c_int, [vp, ctypes.POINTER(DeviceImageC),
+                                             ctypes.POINTER(DeviceImageC), vp, f32,
+                                             ctypes.POINTER(u64)]),
+    "vqm_ss2_finish": (ctypes.c_int, [vp, u64, ctypes.POINTER(u32), ctypes.POINTER(f64)]),
+    "vqm_ss2_memory_usage": (ctypes.c_int, [vp, ctypes.POINTER(MemoryC)]),
+    "vqm_ss2_allocate_image": (ctypes.c_int, [vp, ctypes.POINTER(ImageFormatC),
+                                              ctypes.POINTER(u64), ctypes.POINTER(u64)]),
+    "vqm_ss2_map_image": (ctypes.c_int, [vp, u64, ctypes.POINTER(vp)]),
+    "vqm_ss2_release_image": (ctypes.c_int, [vp, u64]),
+    "vqm_ss2_submit_buffers": (ctypes.c_int, [vp, u64, u64, f32, ctypes.POINTER(u64)]),
+    "vqm_ss2_stage_times": (ctypes.c_int, [vp, ctypes.POINTER(f32)]),
+    "vqm_ss2_batch_create": (ctypes.c_int, [ctypes.POINTER(BatchConfigC), ctypes.POINTER(vp)]),
+    "vqm_ss2_batch_destroy": (None, [vp]),
+    "vqm_ss2_batch_submit": (ctypes.c_int, [vp, ctypes.POINTER(DevicePairC), u32,
+                                            ctypes.POINTER(u64)]),
+    "vqm_ss2_batch_finish": (ctypes.c_int, [vp, u64, ctypes.POINTER(u32), ctypes.POINTER(f64)]),
+    "vqm_ss2_batch_stage_times": (ctypes.c_int, [vp, ctypes.POINTER(f32)]),
+    "vqm_ss2_batch_pool_bytes": (ctypes.c_int, [vp, ctypes.POINTER(u64)]),
+    "vqm_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    "vqm_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p,
+                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(u64)]),
+    "vqm_device_malloc": (ctypes.c_int, [u64, ctypes.POINTER(vp)]),
+    "vqm_device_free": (ctypes.c_int, [vp]),
+    "vqm_memcpy_htod": (ctypes.c_int, [vp, vp, u64]),
+    "vqm_memcpy_dtoh": (ctypes.c_int, [vp, vp, u64]),
+    "vqm_device_synchronize": (ctypes.c_int, []),
+}
+
+_lock = threading.Lock()
+_lib = None
+_synth = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded libvqm_hip.so with typed signatures; raises if not built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"{LIB_PATH} is missing: build it with `python -m vqm_amd.build` "
+                    "(the scoring path has no CPU fallback)")
+            handle = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(handle, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = handle
+        return _lib
+
+
+def synth() -> ctypes.CDLL:
+    """The loaded libvqm_synth.so (synthetic pair generator)."""
+    global _synth
+    with _lock:
+        if _synth is None:
+            if not os.path.exists(SYNTH_PATH):
+                raise RuntimeError(f"{SYNTH_PATH} is missing: build it with `python -m vqm_amd.build`")
+            h = ctypes.CDLL(SYNTH_PATH)
+            h.vqm_synth_image_bytes.restype = u64
+            h.vqm_synth_image_bytes.argtypes = [u32, u32]
+            h.vqm_synth_reference.restype = None
+            h.vqm_synth_reference.argtypes = [u32, u32, u32, u32, f64, vp, u32]
+            h.vqm_synth_distort.restype = None
+            h.vqm_synth_distort.argtypes = [u32, u32, u32, u32, vp, i32, f64, vp, u32]
+            h.vqm_synth_draw.restype = f64
+            h.vqm_synth_draw.argtypes = [u32, u32, u32]
+            h.vqm_synth_hash.restype = u64
+            h.vqm_synth_hash.argtypes = [vp, u64]
+            _synth = h
+        return _synth

@@ -1,0 +1,388 @@
+// C ABI of libvqm_hip.so (include/vqm_ss2.h): status codes, thread-local
+// error messages, and the opaque handles over the C++ runtime.
+
+#include <cstring>
+#include <exception>
+#include <new>
+#include <string>
+
+#include "ss2_host.h"
+#include "ss2_runtime.h"
+#include "vqm_ss2.h"
+
+struct vqm_ss2_pipeline {
+    vqm::ss2::Pipeline impl;
+    explicit vqm_ss2_pipeline(const vqm_ss2_config& c) : impl(c) {}
+};
+
+struct vqm_ss2_batch {
+    vqm::ss2::Batch impl;
+    explicit vqm_ss2_batch(const vqm_ss2_batch_config& c) : impl(c) {}
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int SetError(int status, const std::string& message) {
+    g_last_error = message;
+    return status;
+}
+
+template <typename F>
+int Guard(F&& body) {
+    try {
+        body();
+        g_last_error.clear();
+        return VQM_OK;
+    } catch (const vqm::ss2::Error& e) {
+        return SetError(e.status, e.message);
+    } catch (const std::bad_alloc&) {
+        return SetError(VQM_RUNTIME_ERROR, "host allocation failed");
+    } catch (const std::exception& e) {
+        return SetError(VQM_RUNTIME_ERROR, e.what());
+    } catch (...) {
+        return SetError(VQM_RUNTIME_ERROR, "unknown error");
+    }
+}
+
+void Need(const void* p, const char* what) {
+    if (p == nullptr) vqm::ss2::Fail(VQM_INVALID_ARGUMENT, std::string(what) + " is null");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* vqm_last_error(void) { return g_last_error.c_str(); }
+
+void vqm_ss2_config_init(vqm_ss2_config* config, uint32_t width, uint32_t height) {
+    if (config == nullptr) return;
+    std::memset(config, 0, sizeof(*config));
+    config->width = width;
+    config->height = height;
+    config->resident_images = 2;
+    config->plane_format = VQM_PLANES_F32;
+    config->streaming_inputs = 1;
+}
+
+void vqm_image_format_srgb8(vqm_image_format* f) {
+    if (f == nullptr) return;
+    std::memset(f, 0, sizeof(*f));
+    f->type = 0;
+    f->family = 1;
+    f->alpha = 0;
+    f->range = 1;
+    f->matrix = 0;
+    f->transfer = 13;
+    f->primaries = 1;
+}
+
+int vqm_required_bytes(const vqm_image_format* format, uint32_t width, uint32_t height,
+                       uint64_t* bytes) {
+    return Guard([&] {
+        Need(format, "format");
+        Need(bytes, "bytes");
+        *bytes = vqm::ss2::RequiredBytes(*format, width, height);
+    });
+}
+
+int vqm_ss2_plan_scales(uint32_t width, uint32_t height, uint32_t sizes[12], uint32_t* count) {
+    return Guard([&] {
+        Need(sizes, "sizes");
+        Need(count, "count");
+        const vqm::ss2::ScalePlan p = vqm::ss2::PlanScales(width, height);
+        *count = p.count;
+        for (uint32_t i = 0; i < p.count; ++i) {
+            sizes[2 * i] = p.w[i];
+            sizes[2 * i + 1] = p.h[i];
+        }
+    });
+}
+
+void vqm_ss2_gaussian_taps(float taps[13]) {
+    if (taps == nullptr) return;
+    const auto t = vqm::ss2::GaussianTaps();
+    std::memcpy(taps, t.data(), sizeof(float) * 13);
+}
+
+int vqm_ss2_channel_pruned(uint32_t channel, uint32_t scale, uint32_t scales) {
+    return vqm::ss2::ChannelPruned(channel, scale, scales) ? 1 : 0;
+}
+
+int vqm_ss2_score(uint32_t scales, const double norms[108], double* score) {
+    return Guard([&] {
+        Need(score, "score");
+        if (scales > 0) Need(norms, "norms");
+        *score = vqm::ss2::Score(scales, norms);
+    });
+}
+
+void vqm_ss2_srgb_table(float table[256]) {
+    if (table == nullptr) return;
+    const auto t = vqm::ss2::SrgbTable();
+    std::memcpy(table, t.data(), sizeof(float) * 256);
+}
+
+int vqm_ss2_memory_plan(const vqm_ss2_config* config, vqm_ss2_memory* plan) {
+    return Guard([&] {
+        Need(config, "config");
+        Need(plan, "plan");
+        if (config->plane_format > 2) vqm::ss2::Fail(VQM_INVALID_ARGUMENT, "invalid plane_format");
+        const auto g = vqm::ss2::BuildPlan(config->width, config->height,
+                                           static_cast<vqm::ss2::PlaneFormat>(config->plane_format),
+                                           config->strip_rows);
+        const auto layout = vqm::ss2::LayoutWorkspace(g);
+        vqm_image_format f;
+        vqm_image_format_srgb8(&f);
+        const uint64_t image = vqm::ss2::RequiredBytes(f, config->width, config->height);
+        std::memset(plan, 0, sizeof(*plan));
+        plan->workspace_bytes = layout.total + sizeof(vqm::ss2::PairDesc) + sizeof(vqm::ss2::Plan) +
+                                g.items.size() * sizeof(vqm::ss2::ScaleItem);
+        plan->resident_bytes = static_cast<uint64_t>(config->resident_images) * image;
+        plan->streaming_bytes = config->streaming_inputs ? 2 * image : 0;
+        plan->input_pool_bytes = config->input_bytes;
+        plan->pinned_host_bytes = (config->streaming_inputs ? 2 * image : 0) + 108 * sizeof(double);
+        plan->total_device_bytes = plan->workspace_bytes + plan->resident_bytes +
+                                   plan->streaming_bytes + plan->input_pool_bytes;
+    });
+}
+
+int vqm_ss2_algorithmic_bytes(uint32_t width, uint32_t height, uint32_t plane_format,
+                              uint64_t* bytes) {
+    return Guard([&] {
+        Need(bytes, "bytes");
+        if (plane_format > 2) vqm::ss2::Fail(VQM_INVALID_ARGUMENT, "invalid plane_format");
+        const auto g = vqm::ss2::BuildPlan(width, height,
+                                           static_cast<vqm::ss2::PlaneFormat>(plane_format), 0);
+        *bytes = vqm::ss2::AlgorithmicBytes(g);
+    });
+}
+
+int vqm_ss2_create(const vqm_ss2_config* config, vqm_ss2_pipeline** out) {
+    return Guard([&] {
+        Need(config, "config");
+        Need(out, "out");
+        *out = nullptr;
+        *out = new vqm_ss2_pipeline(*config);
+    });
+}
+
+void vqm_ss2_destroy(vqm_ss2_pipeline* pipeline) { delete pipeline; }
+
+int vqm_ss2_upload(vqm_ss2_pipeline* p, const vqm_image_desc* image, const void* data,
+                   uint64_t* handle) {
+    return Guard([&] {
+        Need(p, "pipeline");
+        Need(image, "image");
+        Need(handle, "handle");
+        *handle = p->impl.Upload(*image, data);
+    });
+}
+
+int vqm_ss2_release(vqm_ss2_pipeline* p, uint64_t handle) {
+    return Guard([&] {
+        Need(p, "pipeline");
+        p->impl.Release(handle);
+    });
+}
+
+int vqm_ss2_submit_host(vqm_ss2_pipeline* p, const vqm_image_desc* a, const void* da,
+                        const vqm_image_desc* b, const void* db, float /*background*/,
+                        uint64_t* token) {
+    return Guard([&] {
+        Need(p, "pipeline");
+        Need(a, "original");
+        Need(b, "distorted");
+        Need(token, "token");
+        *token = p->impl.SubmitHost(*a, da, *b, db);
+    });
+}
+
+int vqm_ss2_submit_resident(vqm_ss2_pipeline* p, uint64_t a, uint64_t b, float /*background*/,
+                            uint64_t* token) {
+    return Guard([&] {
+        Need(p, "pipeline");
+        Need(token, "token");
+        *token = p->impl.SubmitResident(a, b);
+    });
+}
+
+int vqm_ss2_submit_resident_host(vqm_ss2_pipeline* p, uint64_t a, const vqm_image_desc* b,
+                                 const void* db, float /*background*/, uint64_t* token) {
+    return Guard([&] {
+        Need(p, "pipeline");
+        Need(b, "distorted");
+        Need(token, "token");
+        *token = p->impl.SubmitResidentHost(a, *b, db);
+    });
+}
+
+int vqm_ss2_submit_device(vqm_ss2_pipeline* p, const vqm_device_image* a,
+                          const vqm_device_image* b, void* wait_stream, float /*background*/,
+                          uint64_t* token) {
+    return Guard([&] {
+        Need(p, "pipeline");
+        Need(a, "original");
+        Need(b, "distorted");
+        Need(token, "token");
+        *token = p->impl.SubmitDevice(*a, *b, static_cast<hipStream_t>(wait_stream));
+    });
+}
+
+int vqm_ss2_finish(vqm_ss2_pipeline* p, uint64_t token, uint32_t* scales, double norms[108]) {
+    return Guard([&] {
+        Need(p, "pipeline");
+        Need(norms, "norms");
+        const uint32_t n = p->impl.Finish(token, norms);
+        if (scales) *scales = n;
+    });
+}
+
+int vqm_ss2_memory_usage(const vqm_ss2_pipeline* p, vqm_ss2_memory* usage) {
+    return Guard([&] {
+        Need(p, "pipeline");
+        Need(usage, "usage");
+        *usage = p->impl.Memory();
+    });
+}
+
+int vqm_ss2_allocate_image(vqm_ss2_pipeline* p, const vqm_image_format* format, uint64_t* buffer,
+                           uint64_t* bytes) {
+    return Guard([&] {
+        Need(p, "pipeline");
+        Need(format, "format");
+        Need(buffer, "buffer");
+        *buffer = p->impl.AllocateImage(*format, bytes);
+    });
+}
+
+int vqm_ss2_map_image(vqm_ss2_pipeline* p, uint64_t buffer, void** device_ptr) {
+    return Guard([&] {
+        Need(p, "pipeline");
+        Need(device_ptr, "device_ptr");
+        *device_ptr = p->impl.MapImage(buffer);
+    });
+}
+
+int vqm_ss2_release_image(vqm_ss2_pipeline* p, uint64_t buffer) {
+    return Guard([&] {
+        Need(p, "pipeline");
+        p->impl.ReleaseImage(buffer);
+    });
+}
+
+int vqm_ss2_submit_buffers(vqm_ss2_pipeline* p, uint64_t a, uint64_t b, float /*background*/,
+                           uint64_t* token) {
+    return Guard([&] {
+        Need(p, "pipeline");
+        Need(token, "token");
+        *token = p->impl.SubmitBuffers(a, b);
+    });
+}
+
+int vqm_ss2_stage_times(const vqm_ss2_pipeline* p, float ms[3]) {
+    return Guard([&] {
+        Need(p, "pipeline");
+        Need(ms, "ms");
+        const auto t = p->impl.StageTimes();
+        for (int i = 0; i < 3; ++i) ms[i] = t[i];
+    });
+}
+
+int vqm_ss2_batch_create(const vqm_ss2_batch_config* config, vqm_ss2_batch** out) {
+    return Guard([&] {
+        Need(config, "config");
+        Need(out, "out");
+        *out = nullptr;
+        *out = new vqm_ss2_batch(*config);
+    });
+}
+
+void vqm_ss2_batch_destroy(vqm_ss2_batch* batch) { delete batch; }
+
+int vqm_ss2_batch_submit(vqm_ss2_batch* batch, const vqm_ss2_device_pair* pairs, uint32_t count,
+                         uint64_t* token) {
+    return Guard([&] {
+        Need(batch, "batch");
+        Need(pairs, "pairs");
+        Need(token, "token");
+        *token = batch->impl.Submit(pairs, count);
+    });
+}
+
+int vqm_ss2_batch_finish(vqm_ss2_batch* batch, uint64_t token, uint32_t* scales, double* norms) {
+    return Guard([&] {
+        Need(batch, "batch");
+        batch->impl.Finish(token, scales, norms);
+    });
+}
+
+int vqm_ss2_batch_stage_times(const vqm_ss2_batch* batch, float ms[3]) {
+    return Guard([&] {
+        Need(batch, "batch");
+        Need(ms, "ms");
+        const auto t = batch->impl.StageTimes();
+        for (int i = 0; i < 3; ++i) ms[i] = t[i];
+    });
+}
+
+int vqm_ss2_batch_pool_bytes(const vqm_ss2_batch* batch, uint64_t* bytes) {
+    return Guard([&] {
+        Need(batch, "batch");
+        Need(bytes, "bytes");
+        *bytes = batch->impl.PoolBytes();
+    });
+}
+
+int vqm_device_count(int* count) {
+    return Guard([&] {
+        Need(count, "count");
+        *count = 0;
+        if (hipGetDeviceCount(count) != hipSuccess) {
+            (void)hipGetLastError();
+            *count = 0;
+        }
+    });
+}
+
+int vqm_device_info(char name[256], char arch[64], int* cus, int* lds_per_block,
+                    uint64_t* hbm_bytes) {
+    return Guard([&] {
+        int dev = 0;
+        vqm::ss2::Check(hipGetDevice(&dev), "hipGetDevice");
+        hipDeviceProp_t prop{};
+        vqm::ss2::Check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+        if (name) std::snprintf(name, 256, "%s", prop.name);
+        if (arch) std::snprintf(arch, 64, "%s", prop.gcnArchName);
+        if (cus) *cus = prop.multiProcessorCount;
+        if (lds_per_block) *lds_per_block = static_cast<int>(prop.sharedMemPerBlock);
+        if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
+    });
+}
+
+int vqm_device_malloc(uint64_t bytes, void** ptr) {
+    return Guard([&] {
+        Need(ptr, "ptr");
+        *ptr = nullptr;
+        vqm::ss2::Check(hipMalloc(ptr, bytes), "hipMalloc");
+    });
+}
+
+int vqm_device_free(void* ptr) {
+    return Guard([&] { vqm::ss2::Check(hipFree(ptr), "hipFree"); });
+}
+
+int vqm_memcpy_htod(void* dst, const void* src, uint64_t bytes) {
+    return Guard([&] { vqm::ss2::Check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy"); });
+}
+
+int vqm_memcpy_dtoh(void* dst, const void* src, uint64_t bytes) {
+    return Guard([&] { vqm::ss2::Check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy"); });
+}
+
+int vqm_device_synchronize(void) {
+    return Guard([&] { vqm::ss2::Check(hipDeviceSynchronize(), "hipDeviceSynchronize"); });
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// Shared host/device definitions of the SSIMULACRA 2.1 HIP path: the
+// per-geometry plan, the per-pair descriptor the kernels read, and the
+// constants of the metric.
+//
+// HBM layout of one pair's workspace (one "slot"):
+//   planes   2 images x (sum over stored (level, channel) planes of
+//            pitch_l * rows_l elements), element type FP32, FP16 or BF16.
+//            Every plane carries a zero border: element (x, y) of a level
+//            lives at (y + kPadTop) * pitch + (x + kPadLeft), and the border
+//            (never written after the workspace is cleared) is the zero
+//            padding of the 13-tap blur (scale.comp:107-124), so the scale
+//            kernel loads whole rows without bounds checks.
+//   partials items x 6 doubles: one FP64 partial per scale work item.
+//   norms    108 doubles: the finished norms of the pair.
+#pragma once
+
+#include <cstdint>
+
+namespace vqm::ss2 {
+
+inline constexpr int kMaxScales = 6;        // ssimulacra2_internal.h:185
+inline constexpr int kChannels = 3;         // X, Y, B
+inline constexpr int kNorms = 108;          // 18 per level
+inline constexpr int kRadius = 6;           // ceil(3.5 * 1.5), pipeline.cc:276
+inline constexpr int kTaps = 13;
+
+// Scale kernel geometry: one 64-lane wave per work item, two output columns
+// per lane.
+inline constexpr int kStripCols = 128;
+inline constexpr int kSrcCols = kStripCols + 2 * kRadius;  // 140
+// Plane padding: 6 zero rows above, 8 zero columns on the left (so rows of
+// four samples stay 16-byte aligned), and enough on the right and below to
+// cover the last strip's halo.
+inline constexpr int kPadTop = kRadius;
+inline constexpr int kPadLeft = 8;
+
+// Prepare kernel: a 256-thread block converts a 64 x 32 base-pixel tile and
+// every level of its sub-pyramid (64 x 32 halves down to 2 x 1 at level 5).
+inline constexpr int kPrepTileW = 64;
+inline constexpr int kPrepTileH = 32;
+
+enum PlaneFormat : uint32_t { kPlaneF32 = 0, kPlaneF16 = 1, kPlaneBF16 = 2 };
+
+struct ScaleItem {
+    uint32_t level;
+    uint32_t channel;
+    uint32_t x0;  // first output column of the strip
+    uint32_t y0;  // first output row of the strip
+};
+
+// Everything about one geometry the kernels need. Built on the host
+// (ss2_plan.cc), copied to the device once per geometry.
+struct Plan {
+    uint32_t width;   // base geometry
+    uint32_t height;
+    uint32_t levels;  // 1..6
+    uint32_t strip_h; // output rows per scale work item
+    uint32_t w[kMaxScales], h[kMaxScales];
+    uint32_t pitch[kMaxScales], rows[kMaxScales];  // padded plane geometry
+    uint32_t mask[kMaxScales];                      // active channels, bit c
+    uint64_t plane_off[kMaxScales][kChannels];      // elements, ~0 = not stored
+    uint64_t image_elems;                           // plane elements per image
+    uint32_t item_begin[kMaxScales][kChannels];
+    uint32_t item_count[kMaxScales][kChannels];
+    uint32_t items;                                 // scale work items per pair
+    uint32_t prep_tiles_x, prep_tiles_y;
+    uint32_t plane_stride;                          // bytes per u8 input plane
+};
+
+// One pair as the kernels see it.
+struct PairDesc {
+    const uint8_t* src[2];     // packed planar u8 sRGB RGB, original then distorted
+    void* planes;              // 2 * image_elems plane elements
+    double* partials;          // items * 6
+    double* norms;             // 108
+    const Plan* plan;          // device copy of the pair's plan
+    const ScaleItem* items;    // device copy of the plan's item table
+};
+
+}  // namespace vqm::ss2

@@ -1,0 +1,80 @@
+// Host-side logic of the SSIMULACRA 2.1 path: scale plan, taps, pruning,
+// score mapping, input validation, decode table and the per-geometry work
+// plan the kernels consume. No GPU calls here.
+#pragma once
+
+#include <array>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "ss2_common.h"
+#include "vqm_ss2.h"
+
+namespace vqm::ss2 {
+
+// Thrown by host logic; the C ABI maps each to its status code.
+struct Error {
+    int status;
+    std::string message;
+};
+
+[[noreturn]] void Fail(int status, std::string message);
+
+struct ScalePlan {
+    uint32_t count = 0;
+    std::array<uint32_t, kMaxScales> w{}, h{};
+};
+
+// pipeline.cc:28-48.
+ScalePlan PlanScales(uint32_t width, uint32_t height);
+
+// pipeline.cc:274-291: 13 FP32 taps of exp(-0.5 (k/1.5)^2) normalised in FP64.
+std::array<float, kTaps> GaussianTaps();
+
+// score.cc:103-114.
+bool ChannelPruned(uint32_t channel, uint32_t scale, uint32_t scales);
+
+// score.cc:118-163. Throws Error{VQM_INVALID_ARGUMENT} on more than six
+// scales, nonfinite norms or an overflowing mapping.
+double Score(uint32_t scales, const double* norms);
+
+// The u8 sRGB decode table the prepare kernel reads: FP32 decode scale
+// (input.cc:714-757) and the IEC 61966-2-1 EOTF (color.glsl:135-137).
+std::array<float, 256> SrgbTable();
+
+// vqm_image.h:122-133 and input.cc:114-145,244-255: packed planes, each
+// starting on a 4-byte boundary.
+uint64_t RequiredBytes(const vqm_image_format& format, uint32_t width, uint32_t height);
+
+// input.cc:152-240 restricted to what this path decodes: opaque full-range
+// 8-bit sRGB RGB with BT.709 primaries. Throws Error{VQM_INVALID_ARGUMENT}.
+void CheckImage(const vqm_image_desc& image, uint32_t width, uint32_t height,
+                const char* name);
+
+// Per-geometry plan: level sizes, padded plane geometry, plane offsets per
+// image, the scale work items and their per-(level, channel) ranges.
+struct GeometryPlan {
+    Plan plan{};
+    std::vector<ScaleItem> items;
+    uint64_t plane_bytes_per_image = 0;  // padded plane storage
+    uint32_t element_bytes = 4;
+    uint32_t prep_tiles = 0;             // prepare blocks per image
+};
+
+GeometryPlan BuildPlan(uint32_t width, uint32_t height, PlaneFormat format,
+                       uint32_t strip_rows);
+
+// Bytes of one pair's workspace: planes of both images, partials, norms,
+// each rounded to 256 bytes.
+struct WorkspaceLayout {
+    uint64_t planes = 0, partials = 0, norms = 0, total = 0;
+};
+WorkspaceLayout LayoutWorkspace(const GeometryPlan& plan);
+
+// Algorithmic HBM bytes of one pair: both u8 inputs read once, every stored
+// plane element written once and read once per image.
+uint64_t AlgorithmicBytes(const GeometryPlan& plan);
+
+}  // namespace vqm::ss2

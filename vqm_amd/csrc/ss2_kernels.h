@@ -1,0 +1,38 @@
+// Host-side launch API of the SSIMULACRA 2.1 kernels (ss2_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ss2_common.h"
+
+namespace vqm::ss2 {
+
+// A flat work list entry: pair index in the batch and the local work index
+// (prepare: tile | image << 31; scale: item). Used when a batch mixes
+// geometries; a batch of one geometry indexes blocks arithmetically.
+struct WorkRef {
+    uint32_t pair;
+    uint32_t index;
+};
+
+// sRGB decode table and taps live in device memory owned by the caller.
+// All launches are asynchronous on `stream`.
+
+// Builds every stored plane of both images of each pair.
+// Uniform geometry: blocks = pairs * 2 * tiles_per_image, work = nullptr.
+// src0/src1, when not null, replace every pair's inputs (single-pair use).
+hipError_t LaunchPrepare(PlaneFormat format, const PairDesc* pairs, const WorkRef* work,
+                         uint32_t blocks, uint32_t tiles_per_image, const float* lut,
+                         const uint8_t* src0, const uint8_t* src1, hipStream_t stream);
+
+// One 64-lane wave per scale work item (level, channel, 128-column strip).
+// Uniform geometry: blocks = pairs * items_per_pair, work = nullptr.
+hipError_t LaunchScale(PlaneFormat format, const PairDesc* pairs, const WorkRef* work,
+                       uint32_t blocks, uint32_t items_per_pair, hipStream_t stream);
+
+// Sums each (level, channel) norm's partials in FP64 and writes the 108 norms.
+hipError_t LaunchReduce(const PairDesc* pairs, uint32_t pairs_count, hipStream_t stream);
+
+}  // namespace vqm::ss2

@@ -1,0 +1,586 @@
+// SSIMULACRA 2.1 kernels for CDNA4 (gfx950), wave64.
+//
+//   prepare  u8 sRGB -> linear (256-entry table) -> XYB, plus the 2x2 box
+//            pyramid of the linear planes, writing only the (level, channel)
+//            planes whose maps survive pruning (prepare.comp:170-232).
+//   scale    one wave per 128-column strip of one (level, channel): the five
+//            13-tap blurs of a, b, a*a, b*b, a*b, the SSIM' and edge maps, and
+//            six FP64 map sums (scale.comp:335-437). Rows stream through the
+//            wave: each input row is staged in LDS, blurred horizontally
+//            (each lane owns two output columns), and the horizontal results
+//            enter a 13-row register ring from which the vertical blur of the
+//            row six above is taken. No map ever reaches HBM.
+//   reduce   FP64 sum of each norm's per-strip partials, 1-norm means and
+//            4-norm fourth roots (reduce.comp:21-63, vulkan.cc:545-583).
+//
+// Arithmetic contract. Every FP32 operation below is an IEEE add, multiply,
+// fused multiply-add (written explicitly), correctly rounded divide, min/max
+// or abs, with contraction switched off, so the CPU oracle's FP32 "dev" form
+// reproduces each pixel's values bit for bit. The blur runs horizontal pass
+// first with taps in ascending order, as scale.comp:149-301 does.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+#include <utility>
+
+#include "ss2_common.h"
+#include "ss2_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace vqm::ss2 {
+namespace {
+
+// The 13 taps of MakeGaussian(1.5) (pipeline.cc:274-291), exact FP32 values.
+struct Taps {
+    static constexpr float w[kTaps] = {
+        0x1.763864p-14f, 0x1.0d891ap-10f, 0x1.f1e742p-8f, 0x1.26dd9ep-5f,
+        0x1.bfdc88p-4f,  0x1.b42852p-3f,  0x1.1058fp-2f,  0x1.b42852p-3f,
+        0x1.bfdc88p-4f,  0x1.26dd9ep-5f,  0x1.f1e742p-8f, 0x1.0d891ap-10f,
+        0x1.763864p-14f};
+};
+
+// XYB constants of prepare.comp:40-54 rounded to FP32.
+constexpr float kM00 = 0x1.333334p-2f, kM01 = 0x1.3e76c8p-1f, kM02 = 0x1.3f7ceep-4f;
+constexpr float kM10 = 0x1.d70a3ep-3f, kM11 = 0x1.624dd2p-1f, kM12 = 0x1.3f7ceep-4f;
+constexpr float kM20 = 0x1.f28798p-3f, kM21 = 0x1.a35d1ep-3f, kM22 = 0x1.1a86d2p-1f;
+constexpr float kOpsinBias = 0x1.f12a6cp-9f;
+constexpr float kCbrtBias = 0x1.3f64eap-3f;
+constexpr float kThird = 0x1.555556p-2f;  // 1/3 rounded to FP32
+constexpr float kC2 = 0.0009f;            // scale.comp:87
+
+__device__ __forceinline__ float fma_(float a, float b, float c) {
+    return __builtin_fmaf(a, b, c);
+}
+
+// Cube root: bit-trick seed of x^(-1/3), two division-free Newton steps, then
+// y = x r^2 with one Newton correction. <= 0.75 ulp on the XYB domain.
+__device__ __forceinline__ float cbrt_dev(float x) {
+    const uint32_t i = __float_as_uint(x);
+    float r = __uint_as_float(0x54a21d2au - i / 3u);
+    float r3 = (r * r) * r;
+    float e = fma_(-x, r3, 1.0f);
+    r = fma_(r * e, kThird, r);
+    r3 = (r * r) * r;
+    e = fma_(-x, r3, 1.0f);
+    r = fma_(r * e, kThird, r);
+    float y = (x * r) * r;
+    const float y2 = y * y;
+    e = fma_(-y2, y, x);
+    y = fma_(e * (r * r), kThird, y);
+    return x > 0.0f ? y : 0.0f;
+}
+
+// XYB of one linear sample, only the channels in `mask` (bit 0 X, 1 Y, 2 B).
+// prepare.comp:81-98 with its multiply-adds fused.
+__device__ __forceinline__ void xyb(float r, float g, float b, uint32_t mask, float out[3]) {
+    float m0 = fma_(kM00, r, fma_(kM01, g, fma_(kM02, b, kOpsinBias)));
+    float m1 = fma_(kM10, r, fma_(kM11, g, fma_(kM12, b, kOpsinBias)));
+    m0 = fmaxf(m0, 0.0f);
+    m1 = fmaxf(m1, 0.0f);
+    const float c0 = cbrt_dev(m0) - kCbrtBias;
+    const float c1 = cbrt_dev(m1) - kCbrtBias;
+    const float half_sum = 0.5f * (c0 + c1);
+    out[0] = fma_(0.5f * (c0 - c1), 14.0f, 0.42f);
+    out[1] = half_sum + 0.01f;
+    out[2] = 0.0f;
+    if (mask & 4u) {
+        float m2 = fma_(kM20, r, fma_(kM21, g, fma_(kM22, b, kOpsinBias)));
+        m2 = fmaxf(m2, 0.0f);
+        const float c2 = cbrt_dev(m2) - kCbrtBias;
+        out[2] = (c2 - half_sum) + 0.55f;
+    }
+}
+
+// ---- plane element formats ------------------------------------------------
+
+struct F32 {
+    using E = float;
+    static __device__ __forceinline__ float2 load2(const E* p) {
+        return *reinterpret_cast<const float2*>(p);
+    }
+    static __device__ __forceinline__ void store1(E* p, float v) { *p = v; }
+    static __device__ __forceinline__ void store2(E* p, float a, float b) {
+        *reinterpret_cast<float2*>(p) = make_float2(a, b);
+    }
+    static __device__ __forceinline__ void store4(E* p, const float v[4]) {
+        *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    }
+};
+
+__device__ __forceinline__ uint16_t to_f16(float v) {
+    return __builtin_bit_cast(uint16_t, static_cast<_Float16>(v));
+}
+__device__ __forceinline__ float from_f16(uint16_t v) {
+    return static_cast<float>(__builtin_bit_cast(_Float16, v));
+}
+__device__ __forceinline__ uint16_t to_bf16(float v) {
+    uint32_t bits = __float_as_uint(v);
+    bits += 0x7FFFu + ((bits >> 16) & 1u);
+    return static_cast<uint16_t>(bits >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t v) {
+    return __uint_as_float(static_cast<uint32_t>(v) << 16);
+}
+
+template <uint16_t (*To)(float), float (*From)(uint16_t)>
+struct Narrow {
+    using E = uint16_t;
+    static __device__ __forceinline__ float2 load2(const E* p) {
+        const uint32_t v = *reinterpret_cast<const uint32_t*>(p);
+        return make_float2(From(static_cast<uint16_t>(v & 0xFFFFu)),
+                           From(static_cast<uint16_t>(v >> 16)));
+    }
+    static __device__ __forceinline__ void store1(E* p, float v) { *p = To(v); }
+    static __device__ __forceinline__ void store2(E* p, float a, float b) {
+        *reinterpret_cast<uint32_t*>(p) =
+            static_cast<uint32_t>(To(a)) | (static_cast<uint32_t>(To(b)) << 16);
+    }
+    static __device__ __forceinline__ void store4(E* p, const float v[4]) {
+        uint2 w;
+        w.x = static_cast<uint32_t>(To(v[0])) | (static_cast<uint32_t>(To(v[1])) << 16);
+        w.y = static_cast<uint32_t>(To(v[2])) | (static_cast<uint32_t>(To(v[3])) << 16);
+        *reinterpret_cast<uint2*>(p) = w;
+    }
+};
+using F16 = Narrow<to_f16, from_f16>;
+using BF16 = Narrow<to_bf16, from_bf16>;
+
+// Workgroup b runs on XCD b % 8. Renumber so each XCD receives a contiguous
+// run of logical work items: neighbouring strips and tiles, which share halo
+// rows and input cache lines, then share one L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
+    constexpr uint32_t kXcds = 8;
+    const uint32_t xcd = b % kXcds, idx = b / kXcds;
+    const uint32_t per = n / kXcds, rem = n % kXcds;
+    return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+
+// ---- prepare ----------------------------------------------------------------
+
+template <typename Fmt>
+__device__ __forceinline__ void store_level(const Plan& plan, typename Fmt::E* planes,
+                                            uint32_t level, uint32_t x, uint32_t y,
+                                            const float v[3]) {
+    const uint32_t mask = plan.mask[level];
+    const uint64_t at = static_cast<uint64_t>(y + kPadTop) * plan.pitch[level] + x + kPadLeft;
+#pragma unroll
+    for (uint32_t c = 0; c < 3; ++c)
+        if (mask & (1u << c)) Fmt::store1(planes + plan.plane_off[level][c] + at, v[c]);
+}
+
+template <typename Fmt>
+__global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict__ pairs,
+                                                      const WorkRef* __restrict__ work,
+                                                      uint32_t blocks, uint32_t tiles_per_image,
+                                                      const float* __restrict__ lut_g,
+                                                      const uint8_t* src0, const uint8_t* src1) {
+    __shared__ float lut[256];
+    // Level l-1 linear samples of this tile, three channels interleaved; two
+    // buffers alternate between levels. Level 1's tile is 32 x 16 samples.
+    __shared__ float ladder[2][(kPrepTileW / 2) * (kPrepTileH / 2) * 3];
+
+    const uint32_t tid = threadIdx.x;
+    lut[tid] = lut_g[tid];
+
+    uint32_t pair_idx, image, tile;
+    if (work != nullptr) {
+        const WorkRef w = work[blockIdx.x];
+        pair_idx = w.pair;
+        image = w.index >> 31;
+        tile = w.index & 0x7FFFFFFFu;
+    } else {
+        const uint32_t l = xcd_remap(blockIdx.x, blocks);
+        tile = l % tiles_per_image;
+        image = (l / tiles_per_image) & 1u;
+        pair_idx = l / (2 * tiles_per_image);
+    }
+    const PairDesc pd = pairs[pair_idx];
+    const Plan& plan = *pd.plan;
+    const uint32_t W = plan.width, H = plan.height;
+    const uint32_t tile_x = tile % plan.prep_tiles_x, tile_y = tile / plan.prep_tiles_x;
+    const uint32_t X0 = tile_x * kPrepTileW, Y0 = tile_y * kPrepTileH;
+    const uint32_t tx = tid % 16u, ty = tid / 16u;
+    using E = typename Fmt::E;
+    E* planes = static_cast<E*>(pd.planes) + (image ? plan.image_elems : 0);
+    // Single-pair launches name their inputs directly; batches use pd.src.
+    const uint8_t* src = image ? (src1 != nullptr ? src1 : pd.src[1])
+                               : (src0 != nullptr ? src0 : pd.src[0]);
+    __syncthreads();
+
+    // Level 0: this thread's 4 x 2 base pixels, coordinates clamped to the
+    // image so the level-1 box below sees the clamped samples
+    // (color.glsl:453-462).
+    float lin[2][4][3];
+    const uint32_t bx = X0 + 4u * tx;
+    const bool vec = (W % 4u == 0u) && (bx + 3u < W);
+    #pragma unroll
+    for (uint32_t r = 0; r < 2; ++r) {
+        const uint32_t y = min(Y0 + 2u * ty + r, H - 1u);
+        #pragma unroll
+        for (uint32_t c = 0; c < 3; ++c) {
+            const uint8_t* row = src + static_cast<uint64_t>(c) * plan.plane_stride +
+                                 static_cast<uint64_t>(y) * W;
+            if (vec) {
+                const uint32_t v = *reinterpret_cast<const uint32_t*>(row + bx);
+                #pragma unroll
+                for (uint32_t j = 0; j < 4; ++j) lin[r][j][c] = lut[(v >> (8u * j)) & 255u];
+            } else {
+                #pragma unroll
+                for (uint32_t j = 0; j < 4; ++j) lin[r][j][c] = lut[row[min(bx + j, W - 1u)]];
+            }
+        }
+    }
+    const uint32_t mask0 = plan.mask[0];
+    if (mask0 != 0u) {
+        #pragma unroll
+        for (uint32_t r = 0; r < 2; ++r) {
+            const uint32_t y = Y0 + 2u * ty + r;
+            if (y >= H) continue;
+            float v[4][3];
+            #pragma unroll
+            for (uint32_t j = 0; j < 4; ++j)
+                xyb(lin[r][j][0], lin[r][j][1], lin[r][j][2], mask0, v[j]);
+            const uint64_t at = static_cast<uint64_t>(y + kPadTop) * plan.pitch[0] + bx + kPadLeft;
+            #pragma unroll
+            for (uint32_t c = 0; c < 3; ++c) {
+                if (!(mask0 & (1u << c))) continue;
+                E* p = planes + plan.plane_off[0][c] + at;
+                if (bx + 3u < W) {
+                    const float q[4] = {v[0][c], v[1][c], v[2][c], v[3][c]};
+                    Fmt::store4(p, q);
+                } else {
+                    #pragma unroll
+                    for (uint32_t j = 0; j < 4; ++j)
+                        if (bx + j < W) Fmt::store1(p + j, v[j][c]);
+                }
+            }
+        }
+    }
+    if (plan.levels < 2) return;
+
+    // Level 1: two samples per thread, each the box of a 2x2 block in
+    // (0,0), (1,0), (0,1), (1,1) order (prepare.comp:100-111).
+    {
+        const uint32_t W1 = plan.w[1], H1 = plan.h[1];
+        const uint32_t gy = Y0 / 2u + ty;
+        float v1[2][3];
+        #pragma unroll
+        for (uint32_t jj = 0; jj < 2; ++jj) {
+            #pragma unroll
+            for (uint32_t c = 0; c < 3; ++c) {
+                float s = lin[0][2 * jj][c];
+                s = s + lin[0][2 * jj + 1][c];
+                s = s + lin[1][2 * jj][c];
+                s = s + lin[1][2 * jj + 1][c];
+                v1[jj][c] = s * 0.25f;
+                // Level l publishes into buffer l & 1; level 2 reads buffer 1.
+                ladder[1][(ty * 32u + 2u * tx + jj) * 3u + c] = v1[jj][c];
+            }
+        }
+        const uint32_t mask1 = plan.mask[1];
+        if (mask1 != 0u && gy < H1) {
+            const uint32_t gx = X0 / 2u + 2u * tx;
+            float x0[3], x1[3];
+            xyb(v1[0][0], v1[0][1], v1[0][2], mask1, x0);
+            xyb(v1[1][0], v1[1][1], v1[1][2], mask1, x1);
+            const uint64_t at = static_cast<uint64_t>(gy + kPadTop) * plan.pitch[1] + gx + kPadLeft;
+            #pragma unroll
+            for (uint32_t c = 0; c < 3; ++c) {
+                if (!(mask1 & (1u << c))) continue;
+                E* p = planes + plan.plane_off[1][c] + at;
+                if (gx + 1u < W1) {
+                    Fmt::store2(p, x0[c], x1[c]);
+                } else if (gx < W1) {
+                    Fmt::store1(p, x0[c]);
+                }
+            }
+        }
+    }
+
+    // Levels 2 and up from the ladder, with the clamp of the previous level's
+    // last row and column applied in this tile's coordinates
+    // (prepare.comp:146-168).
+    for (uint32_t l = 2; l < plan.levels; ++l) {
+        __syncthreads();
+        const uint32_t side_x = kPrepTileW >> l, side_y = kPrepTileH >> l;
+        const uint32_t prev_x = side_x * 2u;
+        const float* prev = ladder[(l - 1u) & 1u];
+        float* next = ladder[l & 1u];
+        if (tid < side_x * side_y) {
+            const uint32_t jx = tid % side_x, jy = tid / side_x;
+            const uint32_t gx = (X0 >> l) + jx, gy = (Y0 >> l) + jy;
+            float v[3] = {0.0f, 0.0f, 0.0f};
+            if (gx < plan.w[l] && gy < plan.h[l]) {
+                const uint32_t ox = X0 >> (l - 1u), oy = Y0 >> (l - 1u);
+                const uint32_t cx0 = 2u * jx, cy0 = 2u * jy;
+                const uint32_t cx1 = min(2u * gx + 1u, plan.w[l - 1] - 1u) - ox;
+                const uint32_t cy1 = min(2u * gy + 1u, plan.h[l - 1] - 1u) - oy;
+                #pragma unroll
+                for (uint32_t c = 0; c < 3; ++c) {
+                    float s = prev[(cy0 * prev_x + cx0) * 3u + c];
+                    s = s + prev[(cy0 * prev_x + cx1) * 3u + c];
+                    s = s + prev[(cy1 * prev_x + cx0) * 3u + c];
+                    s = s + prev[(cy1 * prev_x + cx1) * 3u + c];
+                    v[c] = s * 0.25f;
+                }
+                if (plan.mask[l] != 0u) {
+                    float o[3];
+                    xyb(v[0], v[1], v[2], plan.mask[l], o);
+                    store_level<Fmt>(plan, planes, l, gx, gy, o);
+                }
+            }
+            #pragma unroll
+            for (uint32_t c = 0; c < 3; ++c) next[(jy * side_x + jx) * 3u + c] = v[c];
+        }
+    }
+}
+
+// ---- scale ------------------------------------------------------------------
+
+constexpr int kRing = 8;  // raw rows kept in LDS: the map needs the row six back
+
+template <int... P, typename F>
+__device__ __forceinline__ void unroll(std::integer_sequence<int, P...>, F&& f) {
+    (f(std::integral_constant<int, P>{}), ...);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_xor(v, offset, 64);
+    return v;
+}
+
+template <typename Fmt>
+__global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ pairs,
+                                                   const WorkRef* __restrict__ work,
+                                                   uint32_t blocks, uint32_t items_per_pair) {
+    __shared__ float ring_a[kRing][kSrcCols];
+    __shared__ float ring_b[kRing][kSrcCols];
+    __shared__ float prod[3][kSrcCols];
+
+    uint32_t pair_idx, item_idx;
+    if (work != nullptr) {
+        const WorkRef w = work[blockIdx.x];
+        pair_idx = w.pair;
+        item_idx = w.index;
+    } else {
+        const uint32_t l = xcd_remap(blockIdx.x, blocks);
+        pair_idx = l / items_per_pair;
+        item_idx = l % items_per_pair;
+    }
+    const PairDesc pd = pairs[pair_idx];
+    const Plan& plan = *pd.plan;
+    const ScaleItem item = pd.items[item_idx];
+    const uint32_t level = item.level, channel = item.channel;
+    const uint32_t W = plan.w[level], H = plan.h[level], pitch = plan.pitch[level];
+    const uint32_t x0 = item.x0, y0 = item.y0;
+    const int total = static_cast<int>(plan.strip_h) + 2 * kRadius;
+
+    using E = typename Fmt::E;
+    const E* pa = static_cast<const E*>(pd.planes) + plan.plane_off[level][channel];
+    const E* pb = pa + plan.image_elems;
+    // Source row r (0 .. total-1) is image row y0 - 6 + r, padded row y0 + r;
+    // source column j (0 .. 139) is image column x0 - 6 + j, padded x0 + 2 + j.
+    const uint64_t origin = static_cast<uint64_t>(y0) * pitch + x0 + 2u;
+    pa += origin;
+    pb += origin;
+
+    const int lane = static_cast<int>(threadIdx.x);
+    const bool halo = lane < (kSrcCols - kStripCols) / 2;  // lanes 0..5
+    const int jm = 2 * lane, jh = kStripCols + 2 * lane;
+
+    float2 cur_a = Fmt::load2(pa + jm), cur_b = Fmt::load2(pb + jm);
+    float2 cur_ha = make_float2(0.0f, 0.0f), cur_hb = make_float2(0.0f, 0.0f);
+    if (halo) {
+        cur_ha = Fmt::load2(pa + jh);
+        cur_hb = Fmt::load2(pb + jh);
+    }
+
+    float win[kTaps][5][2];
+    double sums[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const uint32_t col = x0 + 2u * static_cast<uint32_t>(lane);
+
+    for (int base = 0; base < total; base += kTaps) {
+        unroll(std::make_integer_sequence<int, kTaps>{}, [&](auto pc) {
+            constexpr int P = decltype(pc)::value;
+            const int i = base + P;
+            if (i >= total) return;
+            // Stage input row i: raw samples into the ring, products alongside.
+            float* ra = ring_a[i & (kRing - 1)];
+            float* rb = ring_b[i & (kRing - 1)];
+            *reinterpret_cast<float2*>(ra + jm) = cur_a;
+            *reinterpret_cast<float2*>(rb + jm) = cur_b;
+            *reinterpret_cast<float2*>(&prod[0][jm]) = make_float2(cur_a.x * cur_a.x, cur_a.y * cur_a.y);
+            *reinterpret_cast<float2*>(&prod[1][jm]) = make_float2(cur_b.x * cur_b.x, cur_b.y * cur_b.y);
+            *reinterpret_cast<float2*>(&prod[2][jm]) = make_float2(cur_a.x * cur_b.x, cur_a.y * cur_b.y);
+            if (halo) {
+                *reinterpret_cast<float2*>(ra + jh) = cur_ha;
+                *reinterpret_cast<float2*>(rb + jh) = cur_hb;
+                *reinterpret_cast<float2*>(&prod[0][jh]) = make_float2(cur_ha.x * cur_ha.x, cur_ha.y * cur_ha.y);
+                *reinterpret_cast<float2*>(&prod[1][jh]) = make_float2(cur_hb.x * cur_hb.x, cur_hb.y * cur_hb.y);
+                *reinterpret_cast<float2*>(&prod[2][jh]) = make_float2(cur_ha.x * cur_hb.x, cur_ha.y * cur_hb.y);
+            }
+            // Prefetch input row i + 1.
+            if (i + 1 < total) {
+                const uint64_t off = static_cast<uint64_t>(i + 1) * pitch;
+                cur_a = Fmt::load2(pa + off + jm);
+                cur_b = Fmt::load2(pb + off + jm);
+                if (halo) {
+                    cur_ha = Fmt::load2(pa + off + jh);
+                    cur_hb = Fmt::load2(pb + off + jh);
+                }
+            }
+            __syncthreads();
+
+            // Horizontal 13-tap pass of the five quantities at this lane's two
+            // output columns (source columns jm + 6 and jm + 7).
+            for (int q = 0; q < 5; ++q) {
+                const float* s = q == 0 ? ra : q == 1 ? rb : prod[q - 2];
+                float v[14];
+                for (int k = 0; k < 7; ++k) {
+                    const float2 t = *reinterpret_cast<const float2*>(s + jm + 2 * k);
+                    v[2 * k] = t.x;
+                    v[2 * k + 1] = t.y;
+                }
+                float h0 = Taps::w[0] * v[0];
+                float h1 = Taps::w[0] * v[1];
+                for (int k = 1; k < kTaps; ++k) {
+                    h0 = fma_(Taps::w[k], v[k], h0);
+                    h1 = fma_(Taps::w[k], v[k + 1], h1);
+                }
+                win[P][q][0] = h0;
+                win[P][q][1] = h1;
+            }
+
+            // Vertical pass for output row i - 12 (image row y0 + i - 12),
+            // oldest ring entry first: scale.comp:243-298's ascending taps.
+            if (i >= 2 * kRadius) {
+                float bl[5][2];
+                for (int q = 0; q < 5; ++q) {
+                    for (int c = 0; c < 2; ++c) {
+                        float acc = Taps::w[0] * win[(P + 1) % kTaps][q][c];
+                        for (int k = 1; k < kTaps; ++k)
+                            acc = fma_(Taps::w[k], win[(P + 1 + k) % kTaps][q][c], acc);
+                        bl[q][c] = acc;
+                    }
+                }
+                const uint32_t y = y0 + static_cast<uint32_t>(i - 2 * kRadius);
+                const int raw_slot = (i - kRadius) & (kRing - 1);
+                const float2 a2 = *reinterpret_cast<const float2*>(&ring_a[raw_slot][jm + kRadius]);
+                const float2 b2 = *reinterpret_cast<const float2*>(&ring_b[raw_slot][jm + kRadius]);
+                const float av[2] = {a2.x, a2.y}, bv[2] = {b2.x, b2.y};
+                if (y < H) {
+                    for (int c = 0; c < 2; ++c) {
+                        if (col + static_cast<uint32_t>(c) >= W) continue;
+                        // scale.comp:305-333.
+                        const float mu1 = bl[0][c], mu2 = bl[1][c];
+                        const float t = mu1 - mu2;
+                        const float num_m = fma_(-t, t, 1.0f);
+                        const float num_s = fma_(2.0f, fma_(-mu1, mu2, bl[4][c]), kC2);
+                        const float den = (fma_(-mu1, mu1, bl[2][c]) + fma_(-mu2, mu2, bl[3][c])) + kC2;
+                        const float d = fmaxf(fma_(-num_m, num_s, den) / den, 0.0f);
+                        const float dd = d * d;
+                        const float det_dist = 1.0f + fabsf(bv[c] - mu2);
+                        const float det_orig = 1.0f + fabsf(av[c] - mu1);
+                        const float d1 = (det_dist - det_orig) / det_orig;
+                        const float ring = fmaxf(d1, 0.0f);
+                        const float blur = fmaxf(-d1, 0.0f);
+                        const float rr = ring * ring, bb = blur * blur;
+                        sums[0] += static_cast<double>(d);
+                        sums[1] += static_cast<double>(dd * dd);
+                        sums[2] += static_cast<double>(ring);
+                        sums[3] += static_cast<double>(rr * rr);
+                        sums[4] += static_cast<double>(blur);
+                        sums[5] += static_cast<double>(bb * bb);
+                    }
+                }
+            }
+        });
+    }
+
+    double* out = pd.partials + static_cast<uint64_t>(item_idx) * 6u;
+    for (int k = 0; k < 6; ++k) {
+        const double s = wave_sum(sums[k]);
+        if (lane == 0) out[k] = s;
+    }
+}
+
+// ---- reduce -----------------------------------------------------------------
+
+__global__ __launch_bounds__(128) void reduce_kernel(const PairDesc* __restrict__ pairs) {
+    const PairDesc pd = pairs[blockIdx.x];
+    const Plan& plan = *pd.plan;
+    const uint32_t j = threadIdx.x;
+    if (j >= static_cast<uint32_t>(kNorms)) return;
+    const uint32_t level = j / 18u, k = j % 18u;
+    double value = 0.0;
+    if (level < plan.levels) {
+        const uint32_t c = k < 6u ? k / 2u : (k - 6u) / 4u;
+        const uint32_t slot = k < 6u ? k % 2u : 2u + (k - 6u) % 4u;
+        if (plan.mask[level] & (1u << c)) {
+            const uint32_t begin = plan.item_begin[level][c], count = plan.item_count[level][c];
+            double sum = 0.0;
+            for (uint32_t it = 0; it < count; ++it) sum += pd.partials[(begin + it) * 6u + slot];
+            const double one_per_pixel =
+                1.0 / (static_cast<double>(plan.w[level]) * static_cast<double>(plan.h[level]));
+            value = one_per_pixel * sum;
+            if (slot & 1u) value = sqrt(sqrt(value));
+        }
+    }
+    pd.norms[j] = value;
+}
+
+template <typename Fmt>
+hipError_t launch_prepare_t(const PairDesc* pairs, const WorkRef* work, uint32_t blocks,
+                            uint32_t tiles, const float* lut, const uint8_t* src0,
+                            const uint8_t* src1, hipStream_t stream) {
+    hipLaunchKernelGGL(prepare_kernel<Fmt>, dim3(blocks), dim3(256), 0, stream, pairs, work,
+                       blocks, tiles, lut, src0, src1);
+    return hipGetLastError();
+}
+
+template <typename Fmt>
+hipError_t launch_scale_t(const PairDesc* pairs, const WorkRef* work, uint32_t blocks,
+                          uint32_t items, hipStream_t stream) {
+    hipLaunchKernelGGL(scale_kernel<Fmt>, dim3(blocks), dim3(64), 0, stream, pairs, work,
+                       blocks, items);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t LaunchPrepare(PlaneFormat format, const PairDesc* pairs, const WorkRef* work,
+                         uint32_t blocks, uint32_t tiles_per_image, const float* lut,
+                         const uint8_t* src0, const uint8_t* src1, hipStream_t stream) {
+    if (blocks == 0) return hipSuccess;
+    switch (format) {
+        case kPlaneF32:
+            return launch_prepare_t<F32>(pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
+        case kPlaneF16:
+            return launch_prepare_t<F16>(pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
+        case kPlaneBF16:
+            return launch_prepare_t<BF16>(pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t LaunchScale(PlaneFormat format, const PairDesc* pairs, const WorkRef* work,
+                       uint32_t blocks, uint32_t items_per_pair, hipStream_t stream) {
+    if (blocks == 0) return hipSuccess;
+    switch (format) {
+        case kPlaneF32: return launch_scale_t<F32>(pairs, work, blocks, items_per_pair, stream);
+        case kPlaneF16: return launch_scale_t<F16>(pairs, work, blocks, items_per_pair, stream);
+        case kPlaneBF16: return launch_scale_t<BF16>(pairs, work, blocks, items_per_pair, stream);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t LaunchReduce(const PairDesc* pairs, uint32_t pairs_count, hipStream_t stream) {
+    if (pairs_count == 0) return hipSuccess;
+    hipLaunchKernelGGL(reduce_kernel, dim3(pairs_count), dim3(128), 0, stream, pairs);
+    return hipGetLastError();
+}
+
+}  // namespace vqm::ss2

@@ -1,0 +1,546 @@
+// HIP runtime of the SSIMULACRA 2.1 path (see ss2_runtime.h).
+
+#include "ss2_runtime.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+namespace vqm::ss2 {
+
+void Check(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        Fail(VQM_RUNTIME_ERROR, std::string("hip: ") + what + ": " + hipGetErrorString(e));
+}
+
+uint64_t NextId() {
+    static std::atomic<uint64_t> counter{0};
+    return ++counter;
+}
+
+// ---- buffers --------------------------------------------------------------
+
+DeviceBuffer::DeviceBuffer(uint64_t bytes) : bytes_(bytes) {
+    if (bytes == 0) return;
+    if (hipMalloc(&ptr_, bytes) != hipSuccess) {
+        ptr_ = nullptr;
+        (void)hipGetLastError();
+        Fail(VQM_RUNTIME_ERROR, "hip: device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+}
+
+DeviceBuffer::~DeviceBuffer() {
+    if (ptr_) (void)hipFree(ptr_);
+}
+
+DeviceBuffer& DeviceBuffer::operator=(DeviceBuffer&& o) noexcept {
+    if (this != &o) {
+        if (ptr_) (void)hipFree(ptr_);
+        ptr_ = std::exchange(o.ptr_, nullptr);
+        bytes_ = std::exchange(o.bytes_, 0);
+    }
+    return *this;
+}
+
+PinnedBuffer::PinnedBuffer(uint64_t bytes) : bytes_(bytes) {
+    if (bytes == 0) return;
+    if (hipHostMalloc(&ptr_, bytes, hipHostMallocDefault) != hipSuccess) {
+        ptr_ = nullptr;
+        (void)hipGetLastError();
+        Fail(VQM_RUNTIME_ERROR, "hip: pinned host allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+}
+
+PinnedBuffer::~PinnedBuffer() {
+    if (ptr_) (void)hipHostFree(ptr_);
+}
+
+PinnedBuffer& PinnedBuffer::operator=(PinnedBuffer&& o) noexcept {
+    if (this != &o) {
+        if (ptr_) (void)hipHostFree(ptr_);
+        ptr_ = std::exchange(o.ptr_, nullptr);
+        bytes_ = std::exchange(o.bytes_, 0);
+    }
+    return *this;
+}
+
+Stream::Stream() { Check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate"); }
+Stream::~Stream() {
+    if (stream_) {
+        (void)hipStreamSynchronize(stream_);
+        (void)hipStreamDestroy(stream_);
+    }
+}
+
+Event::Event() { Check(hipEventCreate(&event_), "hipEventCreate"); }
+Event::~Event() {
+    if (event_) (void)hipEventDestroy(event_);
+}
+
+// ---- device context ---------------------------------------------------------
+
+DeviceContext& DeviceContext::Get() {
+    static std::mutex mutex;
+    static std::unique_ptr<DeviceContext> ctx;
+    static DeviceBuffer lut;
+    std::lock_guard<std::mutex> lock(mutex);
+    if (ctx) return *ctx;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        Fail(VQM_RUNTIME_ERROR, "hip: no usable compute device");
+    }
+    auto c = std::make_unique<DeviceContext>();
+    Check(hipGetDevice(&c->device), "hipGetDevice");
+    hipDeviceProp_t prop{};
+    Check(hipGetDeviceProperties(&prop, c->device), "hipGetDeviceProperties");
+    c->arch = prop.gcnArchName;
+    c->cus = prop.multiProcessorCount;
+    if (c->arch.rfind("gfx950", 0) != 0)
+        Fail(VQM_RUNTIME_ERROR, "hip: this library is built for gfx950 only, device is " + c->arch);
+    const std::array<float, 256> table = SrgbTable();
+    lut = DeviceBuffer(sizeof(table));
+    Check(hipMemcpy(lut.get(), table.data(), sizeof(table), hipMemcpyHostToDevice), "upload decode table");
+    c->lut = lut.as<float>();
+    ctx = std::move(c);
+    return *ctx;
+}
+
+DevicePlan::DevicePlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows)
+    : host(BuildPlan(width, height, format, strip_rows)),
+      layout(LayoutWorkspace(host)),
+      plan(sizeof(Plan)),
+      items(std::max<uint64_t>(host.items.size(), 1) * sizeof(ScaleItem)) {
+    Check(hipMemcpy(plan.get(), &host.plan, sizeof(Plan), hipMemcpyHostToDevice), "upload plan");
+    if (!host.items.empty())
+        Check(hipMemcpy(items.get(), host.items.data(), host.items.size() * sizeof(ScaleItem),
+                        hipMemcpyHostToDevice),
+              "upload work items");
+}
+
+Workspace::Workspace(const WorkspaceLayout& layout) : memory(layout.total) {
+    // The plane borders must read as zero for every later pair: they are the
+    // blur's zero padding and nothing writes them.
+    Check(hipMemset(memory.get(), 0, layout.total), "clear workspace");
+    // The clear runs on the null stream; the pipelines' streams do not wait
+    // for it, so finish it before any kernel can touch the planes.
+    Check(hipStreamSynchronize(nullptr), "clear workspace");
+    auto* base = memory.as<uint8_t>();
+    planes = base;
+    partials = reinterpret_cast<double*>(base + layout.planes);
+    norms = reinterpret_cast<double*>(base + layout.planes + layout.partials);
+}
+
+// ---- pipeline -----------------------------------------------------------------
+
+namespace {
+
+vqm_image_format Srgb8() {
+    vqm_image_format f{};
+    f.type = 0;
+    f.family = 1;
+    f.alpha = 0;
+    f.range = 1;
+    f.matrix = 0;
+    f.transfer = 13;
+    f.primaries = 1;
+    return f;
+}
+
+PlaneFormat ToFormat(uint32_t f) {
+    if (f > 2) Fail(VQM_INVALID_ARGUMENT, "plane_format must be VQM_PLANES_F32, _F16 or _BF16");
+    return static_cast<PlaneFormat>(f);
+}
+
+constexpr uint32_t kMaxResident = 4096;
+
+float Elapsed(const Event& a, const Event& b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a.get(), b.get()) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.0f;
+    }
+    return ms;
+}
+
+}  // namespace
+
+// Geometry and option checks run before any device work, so an impossible
+// size names the minimum rather than failing on the device
+// (pipeline_test.cc:318-328).
+static const vqm_ss2_config& Validated(const vqm_ss2_config& config) {
+    PlanScales(config.width, config.height);
+    ToFormat(config.plane_format);
+    if (config.resident_images > kMaxResident)
+        Fail(VQM_INVALID_ARGUMENT, "gpu_resident_images is limited to 4096 slots");
+    return config;
+}
+
+Pipeline::Pipeline(const vqm_ss2_config& config)
+    : config_(Validated(config)),
+      ctx_(DeviceContext::Get()),
+      plan_(config.width, config.height, static_cast<PlaneFormat>(config.plane_format),
+            config.strip_rows),
+      ws_(plan_.layout),
+      pair_(sizeof(PairDesc)) {
+    const vqm_image_format srgb = Srgb8();
+    image_bytes_ = RequiredBytes(srgb, config.width, config.height);
+    PairDesc pd{};
+    pd.planes = ws_.planes;
+    pd.partials = ws_.partials;
+    pd.norms = ws_.norms;
+    pd.plan = plan_.plan.as<Plan>();
+    pd.items = plan_.items.as<ScaleItem>();
+    Check(hipMemcpy(pair_.get(), &pd, sizeof(pd), hipMemcpyHostToDevice), "upload pair descriptor");
+    slots_.resize(config.resident_images);
+    for (Slot& s : slots_) s.data = DeviceBuffer(image_bytes_);
+    if (config.streaming_inputs) {
+        stream_inputs_ = DeviceBuffer(2 * image_bytes_);
+        staging_ = PinnedBuffer(2 * image_bytes_);
+    }
+    host_norms_ = PinnedBuffer(kNorms * sizeof(double));
+    if (config.input_bytes > 0) input_pool_ = DeviceBuffer(config.input_bytes);
+}
+
+Pipeline::~Pipeline() {
+    if (in_flight_ != 0) (void)hipEventSynchronize(ev_[4].get());
+}
+
+void Pipeline::RequireIdle() const {
+    if (in_flight_ != 0)
+        Fail(VQM_LOGIC_ERROR, "a pair is in flight on this pipeline: call Finish first");
+}
+
+const Pipeline::Slot& Pipeline::RequireSlot(uint64_t handle, const char* name) const {
+    if (handle != 0)
+        for (const Slot& s : slots_)
+            if (s.handle == handle) return s;
+    Fail(VQM_INVALID_ARGUMENT, std::string(name) + ": stale or unknown resident image handle");
+}
+
+const Pipeline::InputBuffer& Pipeline::RequireBuffer(uint64_t handle, const char* name) const {
+    if (handle != 0)
+        for (const InputBuffer& b : inputs_)
+            if (b.handle == handle) return b;
+    Fail(VQM_INVALID_ARGUMENT, std::string(name) + ": stale or unknown image buffer");
+}
+
+uint64_t Pipeline::Upload(const vqm_image_desc& image, const void* data) {
+    RequireIdle();
+    CheckImage(image, config_.width, config_.height, "image");
+    if (data == nullptr) Fail(VQM_INVALID_ARGUMENT, "image: data is null");
+    for (Slot& s : slots_) {
+        if (s.handle != 0) continue;
+        Check(hipMemcpy(s.data.get(), data, image_bytes_, hipMemcpyHostToDevice), "upload image");
+        s.handle = NextId();
+        return s.handle;
+    }
+    Fail(VQM_INVALID_ARGUMENT, "every resident image slot is in use");
+}
+
+void Pipeline::Release(uint64_t handle) {
+    RequireIdle();
+    for (Slot& s : slots_)
+        if (handle != 0 && s.handle == handle) s.handle = 0;
+}
+
+uint64_t Pipeline::Launch(const uint8_t* src0, const uint8_t* src1) {
+    const hipStream_t s = stream_.get();
+    const PairDesc* pd = pair_.as<PairDesc>();
+    const PlaneFormat fmt = static_cast<PlaneFormat>(config_.plane_format);
+    const GeometryPlan& g = plan_.host;
+    Check(hipEventRecord(ev_[0].get(), s), "hipEventRecord");
+    Check(LaunchPrepare(fmt, pd, nullptr, 2 * g.prep_tiles, g.prep_tiles, ctx_.lut, src0, src1, s),
+          "launch prepare");
+    Check(hipEventRecord(ev_[1].get(), s), "hipEventRecord");
+    Check(LaunchScale(fmt, pd, nullptr, g.plan.items, g.plan.items, s), "launch scale");
+    Check(hipEventRecord(ev_[2].get(), s), "hipEventRecord");
+    Check(LaunchReduce(pd, 1, s), "launch reduce");
+    Check(hipEventRecord(ev_[3].get(), s), "hipEventRecord");
+    Check(hipMemcpyAsync(host_norms_.get(), ws_.norms, kNorms * sizeof(double),
+                         hipMemcpyDeviceToHost, s),
+          "read back norms");
+    Check(hipEventRecord(ev_[4].get(), s), "hipEventRecord");
+    in_flight_ = NextId();
+    return in_flight_;
+}
+
+uint64_t Pipeline::SubmitHost(const vqm_image_desc& a, const void* da, const vqm_image_desc& b,
+                              const void* db) {
+    RequireIdle();
+    if (!config_.streaming_inputs)
+        Fail(VQM_INVALID_ARGUMENT,
+             "streaming_inputs is false, so the pixel overloads are unavailable; use Upload() "
+             "and Compute() with resident handles");
+    CheckImage(a, config_.width, config_.height, "original");
+    CheckImage(b, config_.width, config_.height, "distorted");
+    if (da == nullptr || db == nullptr) Fail(VQM_INVALID_ARGUMENT, "image data is null");
+    auto* staging = staging_.as<uint8_t>();
+    std::memcpy(staging, da, image_bytes_);
+    std::memcpy(staging + image_bytes_, db, image_bytes_);
+    Check(hipMemcpyAsync(stream_inputs_.get(), staging, 2 * image_bytes_, hipMemcpyHostToDevice,
+                         stream_.get()),
+          "upload pair");
+    const auto* base = stream_inputs_.as<uint8_t>();
+    return Launch(base, base + image_bytes_);
+}
+
+uint64_t Pipeline::SubmitResident(uint64_t a, uint64_t b) {
+    RequireIdle();
+    const Slot& sa = RequireSlot(a, "original");
+    const Slot& sb = RequireSlot(b, "distorted");
+    return Launch(sa.data.as<uint8_t>(), sb.data.as<uint8_t>());
+}
+
+uint64_t Pipeline::SubmitResidentHost(uint64_t a, const vqm_image_desc& b, const void* db) {
+    RequireIdle();
+    if (!config_.streaming_inputs)
+        Fail(VQM_INVALID_ARGUMENT, "streaming_inputs is false, so the pixel overloads are unavailable");
+    const Slot& sa = RequireSlot(a, "original");
+    CheckImage(b, config_.width, config_.height, "distorted");
+    if (db == nullptr) Fail(VQM_INVALID_ARGUMENT, "image data is null");
+    auto* staging = staging_.as<uint8_t>() + image_bytes_;
+    std::memcpy(staging, db, image_bytes_);
+    auto* dst = stream_inputs_.as<uint8_t>() + image_bytes_;
+    Check(hipMemcpyAsync(dst, staging, image_bytes_, hipMemcpyHostToDevice, stream_.get()),
+          "upload distorted");
+    return Launch(sa.data.as<uint8_t>(), dst);
+}
+
+uint64_t Pipeline::SubmitDevice(const vqm_device_image& a, const vqm_device_image& b,
+                                hipStream_t wait_stream) {
+    RequireIdle();
+    const vqm_image_desc da{a.width, a.height, a.format, a.bytes};
+    const vqm_image_desc db{b.width, b.height, b.format, b.bytes};
+    CheckImage(da, config_.width, config_.height, "original");
+    CheckImage(db, config_.width, config_.height, "distorted");
+    if (a.data == nullptr || b.data == nullptr) Fail(VQM_INVALID_ARGUMENT, "device image data is null");
+    if (wait_stream != nullptr) {
+        Event ready;
+        Check(hipEventRecord(ready.get(), wait_stream), "record caller stream");
+        Check(hipStreamWaitEvent(stream_.get(), ready.get(), 0), "wait caller stream");
+    }
+    return Launch(static_cast<const uint8_t*>(a.data), static_cast<const uint8_t*>(b.data));
+}
+
+uint64_t Pipeline::AllocateImage(const vqm_image_format& format, uint64_t* bytes) {
+    RequireIdle();
+    if (input_pool_.bytes() == 0)
+        Fail(VQM_INVALID_ARGUMENT, "no caller-owned image buffers: Ssimulacra2Config::input_bytes is 0");
+    const vqm_image_desc desc{config_.width, config_.height, format,
+                              RequiredBytes(format, config_.width, config_.height)};
+    CheckImage(desc, config_.width, config_.height, "image buffer");
+    const uint64_t need = desc.bytes;
+    // First fit over the pool, 256-byte aligned, in address order.
+    std::vector<InputBuffer> sorted = inputs_;
+    std::sort(sorted.begin(), sorted.end(),
+              [](const InputBuffer& x, const InputBuffer& y) { return x.offset < y.offset; });
+    uint64_t cursor = 0;
+    for (const InputBuffer& b : sorted) {
+        if (cursor + need <= b.offset) break;
+        cursor = (b.offset + b.bytes + 255) / 256 * 256;
+    }
+    if (cursor + need > input_pool_.bytes())
+        Fail(VQM_INVALID_ARGUMENT, "the caller-owned image pool is full");
+    InputBuffer ib{NextId(), cursor, need};
+    inputs_.push_back(ib);
+    if (bytes) *bytes = need;
+    return ib.handle;
+}
+
+void* Pipeline::MapImage(uint64_t buffer) {
+    RequireIdle();
+    return input_pool_.as<uint8_t>() + RequireBuffer(buffer, "image buffer").offset;
+}
+
+void Pipeline::ReleaseImage(uint64_t buffer) {
+    RequireIdle();
+    inputs_.erase(std::remove_if(inputs_.begin(), inputs_.end(),
+                                 [&](const InputBuffer& b) { return buffer != 0 && b.handle == buffer; }),
+                  inputs_.end());
+}
+
+uint64_t Pipeline::SubmitBuffers(uint64_t a, uint64_t b) {
+    RequireIdle();
+    const InputBuffer& ba = RequireBuffer(a, "original");
+    const InputBuffer& bb = RequireBuffer(b, "distorted");
+    const auto* base = input_pool_.as<uint8_t>();
+    return Launch(base + ba.offset, base + bb.offset);
+}
+
+uint32_t Pipeline::Finish(uint64_t token, double* norms) {
+    if (token == 0 || token != in_flight_)
+        Fail(VQM_INVALID_ARGUMENT, "unknown or already finished submission token");
+    in_flight_ = 0;
+    Check(hipEventSynchronize(ev_[4].get()), "wait for pair");
+    stage_ms_ = {Elapsed(ev_[0], ev_[1]), Elapsed(ev_[1], ev_[2]), Elapsed(ev_[2], ev_[3])};
+    std::memcpy(norms, host_norms_.get(), kNorms * sizeof(double));
+    return plan_.host.plan.levels;
+}
+
+vqm_ss2_memory Pipeline::Memory() const {
+    vqm_ss2_memory m{};
+    m.workspace_bytes = ws_.memory.bytes() + pair_.bytes() + plan_.plan.bytes() + plan_.items.bytes();
+    for (const Slot& s : slots_) m.resident_bytes += s.data.bytes();
+    m.streaming_bytes = stream_inputs_.bytes();
+    m.input_pool_bytes = input_pool_.bytes();
+    m.pinned_host_bytes = staging_.bytes() + host_norms_.bytes();
+    m.total_device_bytes = m.workspace_bytes + m.resident_bytes + m.streaming_bytes + m.input_pool_bytes;
+    return m;
+}
+
+// ---- batch --------------------------------------------------------------------
+
+Batch::Batch(const vqm_ss2_batch_config& config) : config_(config), ctx_(DeviceContext::Get()) {
+    ToFormat(config.plane_format);
+    if (config_.max_pairs == 0) Fail(VQM_INVALID_ARGUMENT, "max_pairs must be at least 1");
+    pairs_ = DeviceBuffer(config_.max_pairs * sizeof(PairDesc));
+    h_pairs_ = PinnedBuffer(config_.max_pairs * sizeof(PairDesc));
+    norms_ = DeviceBuffer(static_cast<uint64_t>(config_.max_pairs) * kNorms * sizeof(double));
+    h_norms_ = PinnedBuffer(static_cast<uint64_t>(config_.max_pairs) * kNorms * sizeof(double));
+}
+
+Batch::~Batch() {
+    if (in_flight_ != 0) (void)hipEventSynchronize(ev_[4].get());
+}
+
+Batch::Pool& Batch::PoolFor(uint32_t width, uint32_t height) {
+    auto key = std::make_pair(width, height);
+    auto it = pools_.find(key);
+    if (it != pools_.end()) return it->second;
+    Pool pool;
+    pool.plan = std::make_unique<DevicePlan>(width, height,
+                                             static_cast<PlaneFormat>(config_.plane_format),
+                                             config_.strip_rows);
+    return pools_.emplace(key, std::move(pool)).first->second;
+}
+
+uint64_t Batch::Submit(const vqm_ss2_device_pair* pairs, uint32_t count) {
+    if (in_flight_ != 0) Fail(VQM_LOGIC_ERROR, "a batch is in flight: call Finish first");
+    if (count == 0 || count > config_.max_pairs)
+        Fail(VQM_INVALID_ARGUMENT, "pair count must be between 1 and max_pairs");
+    // Plans first (validates every geometry before anything is acquired).
+    std::vector<Pool*> pool_of(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        if (pairs[i].original == nullptr || pairs[i].distorted == nullptr)
+            Fail(VQM_INVALID_ARGUMENT, "pair " + std::to_string(i) + ": image data is null");
+        pool_of[i] = &PoolFor(pairs[i].width, pairs[i].height);
+    }
+    // Workspaces from the per-geometry pools, growing within the budget.
+    std::vector<std::pair<Pool*, std::unique_ptr<Workspace>>> taken;
+    try {
+        for (uint32_t i = 0; i < count; ++i) {
+            Pool& p = *pool_of[i];
+            std::unique_ptr<Workspace> ws;
+            if (!p.free.empty()) {
+                ws = std::move(p.free.back());
+                p.free.pop_back();
+            } else {
+                const uint64_t need = p.plan->layout.total;
+                if (config_.hbm_budget != 0 && pool_bytes_ + need > config_.hbm_budget)
+                    Fail(VQM_INVALID_ARGUMENT, "the submission does not fit the HBM budget");
+                ws = std::make_unique<Workspace>(p.plan->layout);
+                pool_bytes_ += need;
+            }
+            taken.emplace_back(&p, std::move(ws));
+        }
+    } catch (...) {
+        for (auto& [p, ws] : taken) p->free.push_back(std::move(ws));
+        throw;
+    }
+
+    auto* hp = h_pairs_.as<PairDesc>();
+    bool uniform = true;
+    uint64_t prep_work = 0, scale_work = 0;
+    levels_.assign(count, 0);
+    for (uint32_t i = 0; i < count; ++i) {
+        const DevicePlan& dp = *taken[i].first->plan;
+        const Workspace& ws = *taken[i].second;
+        PairDesc& pd = hp[i];
+        pd.src[0] = static_cast<const uint8_t*>(pairs[i].original);
+        pd.src[1] = static_cast<const uint8_t*>(pairs[i].distorted);
+        pd.planes = ws.planes;
+        pd.partials = ws.partials;
+        pd.norms = norms_.as<double>() + static_cast<uint64_t>(i) * kNorms;
+        pd.plan = dp.plan.as<Plan>();
+        pd.items = dp.items.as<ScaleItem>();
+        uniform = uniform && taken[i].first == taken[0].first;
+        prep_work += 2ull * dp.host.prep_tiles;
+        scale_work += dp.host.plan.items;
+        levels_[i] = dp.host.plan.levels;
+    }
+    if (prep_work > 0x7FFFFFFFull || scale_work > 0x7FFFFFFFull)
+        Fail(VQM_INVALID_ARGUMENT, "the submission has too many work items");
+
+    const hipStream_t s = stream_.get();
+    Check(hipMemcpyAsync(pairs_.get(), hp, count * sizeof(PairDesc), hipMemcpyHostToDevice, s),
+          "upload pair descriptors");
+    const WorkRef* wprep = nullptr;
+    const WorkRef* wscale = nullptr;
+    if (!uniform) {
+        if (h_work_prep_.bytes() < prep_work * sizeof(WorkRef)) {
+            h_work_prep_ = PinnedBuffer(prep_work * sizeof(WorkRef));
+            work_prep_ = DeviceBuffer(prep_work * sizeof(WorkRef));
+        }
+        if (h_work_scale_.bytes() < scale_work * sizeof(WorkRef)) {
+            h_work_scale_ = PinnedBuffer(scale_work * sizeof(WorkRef));
+            work_scale_ = DeviceBuffer(scale_work * sizeof(WorkRef));
+        }
+        auto* wp = h_work_prep_.as<WorkRef>();
+        auto* ws = h_work_scale_.as<WorkRef>();
+        uint64_t np = 0, ns = 0;
+        for (uint32_t i = 0; i < count; ++i) {
+            const GeometryPlan& g = taken[i].first->plan->host;
+            for (uint32_t image = 0; image < 2; ++image)
+                for (uint32_t t = 0; t < g.prep_tiles; ++t) wp[np++] = WorkRef{i, t | (image << 31)};
+            for (uint32_t t = 0; t < g.plan.items; ++t) ws[ns++] = WorkRef{i, t};
+        }
+        Check(hipMemcpyAsync(work_prep_.get(), wp, np * sizeof(WorkRef), hipMemcpyHostToDevice, s),
+              "upload prepare work");
+        Check(hipMemcpyAsync(work_scale_.get(), ws, ns * sizeof(WorkRef), hipMemcpyHostToDevice, s),
+              "upload scale work");
+        wprep = work_prep_.as<WorkRef>();
+        wscale = work_scale_.as<WorkRef>();
+    }
+    const PlaneFormat fmt = static_cast<PlaneFormat>(config_.plane_format);
+    const GeometryPlan& g0 = taken[0].first->plan->host;
+    const PairDesc* dpairs = pairs_.as<PairDesc>();
+    Check(hipEventRecord(ev_[0].get(), s), "hipEventRecord");
+    Check(LaunchPrepare(fmt, dpairs, wprep, static_cast<uint32_t>(prep_work), g0.prep_tiles,
+                        ctx_.lut, nullptr, nullptr, s),
+          "launch prepare");
+    Check(hipEventRecord(ev_[1].get(), s), "hipEventRecord");
+    Check(LaunchScale(fmt, dpairs, wscale, static_cast<uint32_t>(scale_work), g0.plan.items, s),
+          "launch scale");
+    Check(hipEventRecord(ev_[2].get(), s), "hipEventRecord");
+    Check(LaunchReduce(dpairs, count, s), "launch reduce");
+    Check(hipEventRecord(ev_[3].get(), s), "hipEventRecord");
+    Check(hipMemcpyAsync(h_norms_.get(), norms_.get(), static_cast<uint64_t>(count) * kNorms * sizeof(double),
+                         hipMemcpyDeviceToHost, s),
+          "read back norms");
+    Check(hipEventRecord(ev_[4].get(), s), "hipEventRecord");
+    for (auto& [p, ws] : taken) p->busy.push_back(std::move(ws));
+    in_flight_count_ = count;
+    in_flight_ = NextId();
+    return in_flight_;
+}
+
+void Batch::Finish(uint64_t token, uint32_t* scales, double* norms) {
+    if (token == 0 || token != in_flight_)
+        Fail(VQM_INVALID_ARGUMENT, "unknown or already finished submission token");
+    in_flight_ = 0;
+    for (auto& [key, pool] : pools_) {
+        for (auto& ws : pool.busy) pool.free.push_back(std::move(ws));
+        pool.busy.clear();
+    }
+    Check(hipEventSynchronize(ev_[4].get()), "wait for batch");
+    stage_ms_ = {Elapsed(ev_[0], ev_[1]), Elapsed(ev_[1], ev_[2]), Elapsed(ev_[2], ev_[3])};
+    if (norms)
+        std::memcpy(norms, h_norms_.get(), static_cast<uint64_t>(in_flight_count_) * kNorms * sizeof(double));
+    if (scales)
+        for (uint32_t i = 0; i < in_flight_count_; ++i) scales[i] = levels_[i];
+}
+
+}  // namespace vqm::ss2

@@ -1,0 +1,202 @@
+// HIP runtime of the SSIMULACRA 2.1 path: device context, device/pinned
+// buffers, the single-geometry pipeline (the backend behind
+// internal::Ssimulacra2GpuPipeline, ssimulacra2_internal.h:267-317) and the
+// multi-geometry batch engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <utility>
+#include <vector>
+
+#include "ss2_common.h"
+#include "ss2_host.h"
+#include "ss2_kernels.h"
+#include "vqm_ss2.h"
+
+namespace vqm::ss2 {
+
+void Check(hipError_t e, const char* what);
+
+// Process-unique 64-bit ids for tokens and handles (handles.h:21-32): never
+// zero, never reused.
+uint64_t NextId();
+
+class DeviceBuffer {
+public:
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(uint64_t bytes);
+    ~DeviceBuffer();
+    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(std::exchange(o.ptr_, nullptr)), bytes_(std::exchange(o.bytes_, 0)) {}
+    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    void* get() const { return ptr_; }
+    template <typename T> T* as() const { return static_cast<T*>(ptr_); }
+    uint64_t bytes() const { return bytes_; }
+
+private:
+    void* ptr_ = nullptr;
+    uint64_t bytes_ = 0;
+};
+
+class PinnedBuffer {
+public:
+    PinnedBuffer() = default;
+    explicit PinnedBuffer(uint64_t bytes);
+    ~PinnedBuffer();
+    PinnedBuffer(PinnedBuffer&& o) noexcept : ptr_(std::exchange(o.ptr_, nullptr)), bytes_(std::exchange(o.bytes_, 0)) {}
+    PinnedBuffer& operator=(PinnedBuffer&& o) noexcept;
+    PinnedBuffer(const PinnedBuffer&) = delete;
+    PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+    void* get() const { return ptr_; }
+    template <typename T> T* as() const { return static_cast<T*>(ptr_); }
+    uint64_t bytes() const { return bytes_; }
+
+private:
+    void* ptr_ = nullptr;
+    uint64_t bytes_ = 0;
+};
+
+// One HIP stream at default priority and its events.
+class Stream {
+public:
+    Stream();
+    ~Stream();
+    Stream(const Stream&) = delete;
+    Stream& operator=(const Stream&) = delete;
+    hipStream_t get() const { return stream_; }
+
+private:
+    hipStream_t stream_ = nullptr;
+};
+
+class Event {
+public:
+    Event();
+    ~Event();
+    Event(const Event&) = delete;
+    Event& operator=(const Event&) = delete;
+    hipEvent_t get() const { return event_; }
+
+private:
+    hipEvent_t event_ = nullptr;
+};
+
+// Per-process device state: the current device must be a gfx950; the sRGB
+// decode table lives in its memory.
+struct DeviceContext {
+    int device = 0;
+    std::string arch;
+    int cus = 0;
+    const float* lut = nullptr;
+    static DeviceContext& Get();
+};
+
+// A geometry's plan on the host and its device copies.
+struct DevicePlan {
+    GeometryPlan host;
+    WorkspaceLayout layout;
+    DeviceBuffer plan;   // one Plan
+    DeviceBuffer items;  // ScaleItem[items]
+    DevicePlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows);
+};
+
+// One pair's device workspace: planes (zero borders), partials, norms.
+struct Workspace {
+    DeviceBuffer memory;
+    void* planes = nullptr;
+    double* partials = nullptr;
+    double* norms = nullptr;
+    Workspace(const WorkspaceLayout& layout);
+};
+
+class Pipeline {
+public:
+    explicit Pipeline(const vqm_ss2_config& config);
+    ~Pipeline();
+
+    uint64_t Upload(const vqm_image_desc& image, const void* data);
+    void Release(uint64_t handle);
+    uint64_t SubmitHost(const vqm_image_desc& a, const void* da, const vqm_image_desc& b,
+                        const void* db);
+    uint64_t SubmitResident(uint64_t a, uint64_t b);
+    uint64_t SubmitResidentHost(uint64_t a, const vqm_image_desc& b, const void* db);
+    uint64_t SubmitDevice(const vqm_device_image& a, const vqm_device_image& b,
+                          hipStream_t wait_stream);
+    uint64_t AllocateImage(const vqm_image_format& format, uint64_t* bytes);
+    void* MapImage(uint64_t buffer);
+    void ReleaseImage(uint64_t buffer);
+    uint64_t SubmitBuffers(uint64_t a, uint64_t b);
+    uint32_t Finish(uint64_t token, double* norms);
+    vqm_ss2_memory Memory() const;
+    std::array<float, 3> StageTimes() const { return stage_ms_; }
+
+private:
+    struct Slot {
+        uint64_t handle = 0;  // 0 = free
+        DeviceBuffer data;
+    };
+    struct InputBuffer {
+        uint64_t handle = 0;
+        uint64_t offset = 0;
+        uint64_t bytes = 0;
+    };
+    void RequireIdle() const;
+    const Slot& RequireSlot(uint64_t handle, const char* name) const;
+    const InputBuffer& RequireBuffer(uint64_t handle, const char* name) const;
+    uint64_t Launch(const uint8_t* src0, const uint8_t* src1);
+
+    vqm_ss2_config config_;
+    DeviceContext& ctx_;
+    DevicePlan plan_;
+    Workspace ws_;
+    DeviceBuffer pair_;  // one PairDesc
+    std::vector<Slot> slots_;
+    DeviceBuffer stream_inputs_;   // two images (streaming_inputs)
+    PinnedBuffer staging_;         // two images, pinned
+    PinnedBuffer host_norms_;
+    DeviceBuffer input_pool_;      // caller-owned images (input_bytes)
+    std::vector<InputBuffer> inputs_;
+    uint64_t image_bytes_ = 0;
+    Stream stream_;
+    Event ev_[5];
+    uint64_t in_flight_ = 0;
+    std::array<float, 3> stage_ms_{};
+};
+
+class Batch {
+public:
+    explicit Batch(const vqm_ss2_batch_config& config);
+    ~Batch();
+    uint64_t Submit(const vqm_ss2_device_pair* pairs, uint32_t count);
+    void Finish(uint64_t token, uint32_t* scales, double* norms);
+    std::array<float, 3> StageTimes() const { return stage_ms_; }
+    uint64_t PoolBytes() const { return pool_bytes_; }
+
+private:
+    struct Pool {
+        std::unique_ptr<DevicePlan> plan;
+        std::vector<std::unique_ptr<Workspace>> free, busy;
+    };
+    Pool& PoolFor(uint32_t width, uint32_t height);
+
+    vqm_ss2_batch_config config_;
+    DeviceContext& ctx_;
+    std::map<std::pair<uint32_t, uint32_t>, Pool> pools_;
+    uint64_t pool_bytes_ = 0;
+    DeviceBuffer pairs_, work_prep_, work_scale_, norms_;
+    PinnedBuffer h_pairs_, h_work_prep_, h_work_scale_, h_norms_;
+    std::vector<uint32_t> levels_;
+    Stream stream_;
+    Event ev_[5];
+    uint64_t in_flight_ = 0;
+    uint32_t in_flight_count_ = 0;
+    std::array<float, 3> stage_ms_{};
+};
+
+}  // namespace vqm::ss2

@@ -1,0 +1,650 @@
+"""Python mirror of vqm's SSIMULACRA 2 API over the HIP C ABI.
+
+Names, argument meaning and error behaviour follow the reference's C++ public
+API (include/ssimulacra2.h:17-215, include/vqm_image.h:17-166,
+include/vqm_pool.h:51-86, include/vqm_alpha.h:16-27) so code and tests read
+like the reference's own:
+
+    std::invalid_argument  -> InvalidArgument (a ValueError)
+    std::logic_error       -> LogicError
+    std::runtime_error     -> DeviceError
+    std::overflow_error    -> OverflowError
+
+Every scoring call runs on the GPU through libvqm_hip.so; there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import threading
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Union
+
+import numpy as np
+
+from . import _lib as L
+
+
+class InvalidArgument(ValueError):
+    """std::invalid_argument of the reference API."""
+
+
+class LogicError(RuntimeError):
+    """std::logic_error: a call made while a pair is in flight."""
+
+
+class DeviceError(RuntimeError):
+    """std::runtime_error: no usable device, HIP failure or allocation failure."""
+
+
+def _check(status: int) -> None:
+    if status == L.VQM_OK:
+        return
+    msg = (L.lib().vqm_last_error() or b"").decode("utf-8", "replace")
+    if status == L.VQM_INVALID_ARGUMENT:
+        raise InvalidArgument(msg)
+    if status == L.VQM_LOGIC_ERROR:
+        raise LogicError(msg)
+    if status == L.VQM_OVERFLOW:
+        raise OverflowError(msg)
+    raise DeviceError(msg)
+
+
+# ---- image model (vqm_image.h) ---------------------------------------------
+
+class PixelType(enum.IntEnum):
+    Byte = 0
+    Word = 1
+    Half = 2
+    Float = 3
+
+
+class PixelRange(enum.IntEnum):
+    Limited = 0
+    Full = 1
+
+
+class ColorFamily(enum.IntEnum):
+    Grey = 0
+    Rgb = 1
+    Yuv = 2
+
+
+class AlphaType(enum.IntEnum):
+    NoAlpha = 0
+    Straight = 1
+    Premultiplied = 2
+
+
+class MatrixCoefficients(enum.IntEnum):
+    Rgb = 0
+    Bt709 = 1
+    Unspecified = 2
+    Fcc = 4
+    Bt470Bg = 5
+    St170M = 6
+    St240M = 7
+    YCgCo = 8
+    Bt2020Ncl = 9
+    Bt2020Cl = 10
+
+
+class TransferCharacteristics(enum.IntEnum):
+    Bt709 = 1
+    Unspecified = 2
+    Bt470M = 4
+    Bt470Bg = 5
+    Bt601 = 6
+    St240M = 7
+    Linear = 8
+    Log100 = 9
+    Log316 = 10
+    Iec61966_2_4 = 11
+    Bt1361 = 12
+    Srgb = 13
+    Bt2020_10 = 14
+    Bt2020_12 = 15
+    St2084 = 16
+    St428 = 17
+    AribB67 = 18
+
+
+class ColorPrimaries(enum.IntEnum):
+    Bt709 = 1
+    Unspecified = 2
+    Bt470M = 4
+    Bt470Bg = 5
+    St170M = 6
+    St240M = 7
+    Film = 8
+    Bt2020 = 9
+
+
+@dataclass(frozen=True)
+class Size:
+    width: int = 0
+    height: int = 0
+
+
+@dataclass
+class ImageFormat:
+    """vqm_image.h:100-120. The default is the reference's (Float samples);
+    this path decodes Byte sRGB RGB, see ImageFormat.srgb8()."""
+    type: int = PixelType.Float
+    family: int = ColorFamily.Rgb
+    alpha: int = AlphaType.NoAlpha
+    range: int = PixelRange.Full
+    matrix: int = MatrixCoefficients.Rgb
+    transfer: int = TransferCharacteristics.Srgb
+    primaries: int = ColorPrimaries.Bt709
+    chroma_location: int = 0
+    subsample_w: int = 0
+    subsample_h: int = 0
+    depth: int = 0
+
+    @staticmethod
+    def srgb8() -> "ImageFormat":
+        return ImageFormat(type=PixelType.Byte)
+
+    def to_c(self) -> L.ImageFormatC:
+        f = L.ImageFormatC()
+        for name in ("type", "family", "alpha", "range", "matrix", "transfer", "primaries",
+                     "chroma_location", "subsample_w", "subsample_h", "depth"):
+            value = int(getattr(self, name))
+            if not 0 <= value <= 255:
+                raise InvalidArgument(f"ImageFormat.{name} out of range")
+            setattr(f, name, value)
+        return f
+
+
+def required_bytes(fmt: ImageFormat, size: Size) -> int:
+    """RequiredBytes, vqm_image.h:143-146."""
+    out = L.u64()
+    _check(L.lib().vqm_required_bytes(ctypes.byref(fmt.to_c()), size.width, size.height,
+                                      ctypes.byref(out)))
+    return out.value
+
+
+@dataclass
+class Image:
+    """vqm_image.h:134-141: packed planes, each on a 4-byte boundary."""
+    size: Size
+    format: ImageFormat = field(default_factory=ImageFormat.srgb8)
+    data: Union[bytes, bytearray, np.ndarray] = b""
+
+    @staticmethod
+    def from_planes(r: np.ndarray, g: np.ndarray, b: np.ndarray) -> "Image":
+        """An 8-bit sRGB image from three HxW uint8 planes."""
+        h, w = r.shape
+        stride = (w * h + 3) // 4 * 4
+        buf = np.zeros(3 * stride, np.uint8)
+        for i, p in enumerate((r, g, b)):
+            buf[i * stride:i * stride + w * h] = np.ascontiguousarray(p, np.uint8).reshape(-1)
+        return Image(Size(w, h), ImageFormat.srgb8(), buf)
+
+    def _buffer(self):
+        data = self.data
+        if isinstance(data, np.ndarray):
+            arr = np.ascontiguousarray(data, dtype=np.uint8).reshape(-1)
+            return arr, arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes
+        if isinstance(data, bytearray):
+            arr = (ctypes.c_uint8 * len(data)).from_buffer(data)
+            return arr, ctypes.cast(arr, ctypes.c_void_p), len(data)
+        arr = ctypes.create_string_buffer(bytes(data), len(data)) if data else None
+        return arr, (ctypes.cast(arr, ctypes.c_void_p) if arr is not None else None), len(data)
+
+    def desc(self) -> L.ImageDescC:
+        d = L.ImageDescC()
+        d.width, d.height = self.size.width, self.size.height
+        d.format = self.format.to_c()
+        d.bytes = len(self.data) if not isinstance(self.data, np.ndarray) else self.data.nbytes
+        return d
+
+
+@dataclass(frozen=True)
+class UploadedImage:
+    """vqm_image.h:151-154: a resident slot handle (0 = invalid)."""
+    slot: int = 0
+    size: Size = Size()
+
+
+@dataclass(frozen=True)
+class ImageBuffer:
+    """vqm_image.h:162-166: caller-owned device image memory (0 = invalid)."""
+    slot: int = 0
+    size: Size = Size()
+    bytes: int = 0
+
+
+@dataclass(frozen=True)
+class DeviceImage:
+    """A caller-owned device image (replaces VulkanBufferView): a device
+    pointer to packed planes, kept alive until Finish."""
+    ptr: int
+    size: Size
+    format: ImageFormat = field(default_factory=ImageFormat.srgb8)
+    bytes: int = 0
+
+    def to_c(self) -> L.DeviceImageC:
+        d = L.DeviceImageC()
+        d.data = self.ptr
+        d.width, d.height = self.size.width, self.size.height
+        d.format = self.format.to_c()
+        d.bytes = self.bytes or required_bytes(self.format, self.size)
+        return d
+
+
+# ---- results (ssimulacra2.h:17-33) -----------------------------------------
+
+@dataclass
+class Ssimulacra2Scale:
+    avg_ssim: List[float] = field(default_factory=lambda: [0.0] * 6)
+    avg_edgediff: List[float] = field(default_factory=lambda: [0.0] * 12)
+
+
+@dataclass
+class Ssimulacra2Result:
+    scales: List[Ssimulacra2Scale] = field(default_factory=list)
+
+    @staticmethod
+    def from_norms(count: int, norms: Sequence[float]) -> "Ssimulacra2Result":
+        return Ssimulacra2Result([
+            Ssimulacra2Scale(list(norms[s * 18:s * 18 + 6]), list(norms[s * 18 + 6:s * 18 + 18]))
+            for s in range(count)])
+
+    def norms(self) -> np.ndarray:
+        """The 108-entry flat layout of include/vqm_ss2.h."""
+        out = np.zeros(108)
+        for s, sc in enumerate(self.scales[:6]):
+            out[s * 18:s * 18 + 6] = sc.avg_ssim
+            out[s * 18 + 6:s * 18 + 18] = sc.avg_edgediff
+        return out
+
+    def Score(self) -> float:
+        """score.cc:118-163; InvalidArgument for >6 scales or nonfinite norms."""
+        if len(self.scales) > 6:
+            raise InvalidArgument("Ssimulacra2Result has too many scales")
+        flat = self.norms()
+        score = L.f64()
+        _check(L.lib().vqm_ss2_score(len(self.scales), flat.ctypes.data_as(ctypes.POINTER(L.f64)),
+                                     ctypes.byref(score)))
+        return score.value
+
+
+def plan_scales(size: Size) -> List[Size]:
+    """PlanSsimulacra2Scales (pipeline.cc:28-48)."""
+    sizes = (L.u32 * 12)()
+    count = L.u32()
+    _check(L.lib().vqm_ss2_plan_scales(size.width, size.height, sizes, ctypes.byref(count)))
+    return [Size(sizes[2 * i], sizes[2 * i + 1]) for i in range(count.value)]
+
+
+def channel_pruned(channel: int, scale: int, scales: int) -> bool:
+    """Ssimulacra2ChannelPruned (score.cc:103-114)."""
+    return bool(L.lib().vqm_ss2_channel_pruned(channel, scale, scales))
+
+
+def gaussian_taps() -> np.ndarray:
+    t = (L.f32 * 13)()
+    L.lib().vqm_ss2_gaussian_taps(t)
+    return np.array(t, np.float32)
+
+
+def srgb_table() -> np.ndarray:
+    t = (L.f32 * 256)()
+    L.lib().vqm_ss2_srgb_table(t)
+    return np.array(t, np.float32)
+
+
+# ---- config and memory --------------------------------------------------------
+
+@dataclass
+class Ssimulacra2Config:
+    """ssimulacra2.h:35-64. half_planes selects FP16 planes; plane_format
+    (0 FP32, 1 FP16, 2 BF16) overrides it."""
+    size: Size = Size()
+    gpu_resident_images: int = 2
+    half_planes: bool = False
+    input_bytes: int = 0
+    streaming_inputs: bool = True
+    plane_format: Optional[int] = None
+    strip_rows: int = 0
+
+    def to_c(self) -> L.ConfigC:
+        c = L.ConfigC()
+        c.width, c.height = self.size.width, self.size.height
+        c.resident_images = self.gpu_resident_images
+        fmt = self.plane_format
+        if fmt is None:
+            fmt = L.PLANES_F16 if self.half_planes else L.PLANES_F32
+        c.plane_format = fmt
+        c.streaming_inputs = 1 if self.streaming_inputs else 0
+        c.strip_rows = self.strip_rows
+        c.input_bytes = self.input_bytes
+        return c
+
+
+@dataclass
+class PipelineMemoryUsage:
+    workspace_bytes: int
+    resident_bytes: int
+    streaming_bytes: int
+    input_pool_bytes: int
+    pinned_host_bytes: int
+    total_device_bytes: int
+
+    @staticmethod
+    def from_c(m: L.MemoryC) -> "PipelineMemoryUsage":
+        return PipelineMemoryUsage(m.workspace_bytes, m.resident_bytes, m.streaming_bytes,
+                                   m.input_pool_bytes, m.pinned_host_bytes, m.total_device_bytes)
+
+
+def memory_plan(config: Ssimulacra2Config) -> PipelineMemoryUsage:
+    """Device bytes a pipeline with `config` allocates (layout.cc:106-325)."""
+    m = L.MemoryC()
+    _check(L.lib().vqm_ss2_memory_plan(ctypes.byref(config.to_c()), ctypes.byref(m)))
+    return PipelineMemoryUsage.from_c(m)
+
+
+def algorithmic_bytes(size: Size, plane_format: int = L.PLANES_F32) -> int:
+    out = L.u64()
+    _check(L.lib().vqm_ss2_algorithmic_bytes(size.width, size.height, plane_format,
+                                             ctypes.byref(out)))
+    return out.value
+
+
+# ---- the pipeline (ssimulacra2.h:83-215) ----------------------------------------
+
+class Ssimulacra2Pipeline:
+    """One geometry, one submission in flight, not thread-safe (use one per
+    thread). All scoring runs on the GPU."""
+
+    def __init__(self, config: Ssimulacra2Config):
+        if isinstance(config, Size):
+            config = Ssimulacra2Config(size=config)
+        self.config = config
+        handle = ctypes.c_void_p()
+        _check(L.lib().vqm_ss2_create(ctypes.byref(config.to_c()), ctypes.byref(handle)))
+        self._h = handle
+        self._keep = None
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            L.lib().vqm_ss2_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def size(self) -> Size:
+        return self.config.size
+
+    # -- resident images
+    def Upload(self, image: Image) -> UploadedImage:
+        buf, ptr, _ = image._buffer()
+        handle = L.u64()
+        _check(L.lib().vqm_ss2_upload(self._h, ctypes.byref(image.desc()), ptr, ctypes.byref(handle)))
+        return UploadedImage(handle.value, image.size)
+
+    def Release(self, handle: Union[UploadedImage, ImageBuffer]) -> None:
+        if isinstance(handle, ImageBuffer):
+            _check(L.lib().vqm_ss2_release_image(self._h, handle.slot))
+        else:
+            _check(L.lib().vqm_ss2_release(self._h, handle.slot))
+
+    # -- caller-owned device buffers
+    def AllocateImage(self, fmt: ImageFormat) -> ImageBuffer:
+        handle, nbytes = L.u64(), L.u64()
+        _check(L.lib().vqm_ss2_allocate_image(self._h, ctypes.byref(fmt.to_c()),
+                                              ctypes.byref(handle), ctypes.byref(nbytes)))
+        return ImageBuffer(handle.value, self.size, nbytes.value)
+
+    def MapImage(self, buffer: ImageBuffer) -> int:
+        """Device pointer of the buffer (fill it with memcpy_htod)."""
+        ptr = ctypes.c_void_p()
+        _check(L.lib().vqm_ss2_map_image(self._h, buffer.slot, ctypes.byref(ptr)))
+        return ptr.value
+
+    # -- split submission
+    def Submit(self, original, distorted, background: float = 0.5, wait_stream: int = 0) -> int:
+        token = L.u64()
+        lib = L.lib()
+        if isinstance(original, Image) and isinstance(distorted, Image):
+            ba, pa, _ = original._buffer()
+            bb, pb, _ = distorted._buffer()
+            self._keep = (ba, bb)
+            _check(lib.vqm_ss2_submit_host(self._h, ctypes.byref(original.desc()), pa,
+                                           ctypes.byref(distorted.desc()), pb, background,
+                                           ctypes.byref(token)))
+        elif isinstance(original, UploadedImage) and isinstance(distorted, UploadedImage):
+            _check(lib.vqm_ss2_submit_resident(self._h, original.slot, distorted.slot, background,
+                                               ctypes.byref(token)))
+        elif isinstance(original, UploadedImage) and isinstance(distorted, Image):
+            bb, pb, _ = distorted._buffer()
+            self._keep = (bb,)
+            _check(lib.vqm_ss2_submit_resident_host(self._h, original.slot,
+                                                    ctypes.byref(distorted.desc()), pb, background,
+                                                    ctypes.byref(token)))
+        elif isinstance(original, ImageBuffer) and isinstance(distorted, ImageBuffer):
+            _check(lib.vqm_ss2_submit_buffers(self._h, original.slot, distorted.slot, background,
+                                              ctypes.byref(token)))
+        elif isinstance(original, DeviceImage) and isinstance(distorted, DeviceImage):
+            _check(lib.vqm_ss2_submit_device(self._h, ctypes.byref(original.to_c()),
+                                             ctypes.byref(distorted.to_c()),
+                                             ctypes.c_void_p(wait_stream or None), background,
+                                             ctypes.byref(token)))
+        else:
+            raise InvalidArgument("unsupported pair of input kinds")
+        return token.value
+
+    def Finish(self, token: int) -> Ssimulacra2Result:
+        norms = (L.f64 * 108)()
+        scales = L.u32()
+        _check(L.lib().vqm_ss2_finish(self._h, token, ctypes.byref(scales), norms))
+        self._keep = None
+        return Ssimulacra2Result.from_norms(scales.value, list(norms))
+
+    def Compute(self, original, distorted, background: float = 0.5) -> Ssimulacra2Result:
+        return self.Finish(self.Submit(original, distorted, background))
+
+    def GetMemoryUsage(self) -> PipelineMemoryUsage:
+        m = L.MemoryC()
+        _check(L.lib().vqm_ss2_memory_usage(self._h, ctypes.byref(m)))
+        return PipelineMemoryUsage.from_c(m)
+
+    def StageTimes(self) -> List[float]:
+        """Prepare, scale and reduce times of the last finished pair, in ms."""
+        ms = (L.f32 * 3)()
+        _check(L.lib().vqm_ss2_stage_times(self._h, ms))
+        return list(ms)
+
+
+def ComputeWorstBackground(pipeline: Ssimulacra2Pipeline, original, distorted) -> Ssimulacra2Result:
+    """vqm_alpha.h:16-27: opaque inputs score once; with alpha, the worse of
+    the dark (0.1) and bright (0.9) backgrounds."""
+    fa = getattr(original, "format", None)
+    fb = getattr(distorted, "format", None)
+    if (fa is None or fa.alpha == AlphaType.NoAlpha) and (fb is None or fb.alpha == AlphaType.NoAlpha):
+        return pipeline.Compute(original, distorted)
+    dark = pipeline.Compute(original, distorted, 0.1)
+    bright = pipeline.Compute(original, distorted, 0.9)
+    return dark if dark.Score() <= bright.Score() else bright
+
+
+class Ssimulacra2Pool:
+    """vqm_pool.h:51-86 / pool.cc:24-116: N pipelines round-robin, results in
+    input order; after a failure the pending pairs are drained and the error
+    re-raised."""
+
+    def __init__(self, config: Ssimulacra2Config, workers: int):
+        if workers < 1:
+            raise InvalidArgument("a pool needs at least one worker")
+        self._pipes = [Ssimulacra2Pipeline(config) for _ in range(workers)]
+
+    def Workers(self) -> int:
+        return len(self._pipes)
+
+    def ComputeMany(self, pairs, background: float = 0.5) -> List[Ssimulacra2Result]:
+        n = len(self._pipes)
+        tokens: List[Optional[int]] = [None] * n
+        owner: List[int] = [-1] * n
+        out: List[Optional[Ssimulacra2Result]] = [None] * len(pairs)
+        try:
+            for i, (a, b) in enumerate(pairs):
+                slot = i % n
+                if tokens[slot] is not None:
+                    out[owner[slot]] = self._pipes[slot].Finish(tokens[slot])
+                    tokens[slot] = None
+                tokens[slot] = self._pipes[slot].Submit(a, b, background)
+                owner[slot] = i
+            for slot in range(n):
+                if tokens[slot] is not None:
+                    out[owner[slot]] = self._pipes[slot].Finish(tokens[slot])
+                    tokens[slot] = None
+        except Exception:
+            for slot in range(n):
+                if tokens[slot] is not None:
+                    try:
+                        self._pipes[slot].Finish(tokens[slot])
+                    except Exception:
+                        pass
+                    tokens[slot] = None
+            raise
+        return out  # type: ignore[return-value]
+
+    def ScoreMany(self, pairs, background: float = 0.5) -> List[float]:
+        return [r.Score() for r in self.ComputeMany(pairs, background)]
+
+
+# ---- device memory plumbing ----------------------------------------------------
+
+class DeviceBuffer:
+    """Device memory through the C ABI (hipMalloc/hipMemcpy)."""
+
+    def __init__(self, nbytes: int):
+        ptr = ctypes.c_void_p()
+        _check(L.lib().vqm_device_malloc(nbytes, ctypes.byref(ptr)))
+        self.ptr = ptr.value
+        self.nbytes = nbytes
+
+    def upload(self, data, offset: int = 0) -> None:
+        arr = np.ascontiguousarray(data, dtype=np.uint8).reshape(-1)
+        if offset + arr.nbytes > self.nbytes:
+            raise InvalidArgument("upload beyond the buffer")
+        _check(L.lib().vqm_memcpy_htod(ctypes.c_void_p(self.ptr + offset),
+                                       arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes))
+
+    def download(self, nbytes: Optional[int] = None, offset: int = 0) -> np.ndarray:
+        n = self.nbytes - offset if nbytes is None else nbytes
+        out = np.empty(n, np.uint8)
+        _check(L.lib().vqm_memcpy_dtoh(out.ctypes.data_as(ctypes.c_void_p),
+                                       ctypes.c_void_p(self.ptr + offset), n))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            L.lib().vqm_device_free(ctypes.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def memcpy_htod(dst: int, data) -> None:
+    arr = np.ascontiguousarray(data, dtype=np.uint8).reshape(-1)
+    _check(L.lib().vqm_memcpy_htod(ctypes.c_void_p(dst), arr.ctypes.data_as(ctypes.c_void_p),
+                                   arr.nbytes))
+
+
+def device_count() -> int:
+    n = ctypes.c_int()
+    _check(L.lib().vqm_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def device_info() -> dict:
+    name = ctypes.create_string_buffer(256)
+    arch = ctypes.create_string_buffer(64)
+    cus, lds = ctypes.c_int(), ctypes.c_int()
+    hbm = L.u64()
+    _check(L.lib().vqm_device_info(name, arch, ctypes.byref(cus), ctypes.byref(lds),
+                                   ctypes.byref(hbm)))
+    return {"name": name.value.decode(), "arch": arch.value.decode(), "cus": cus.value,
+            "lds_per_block": lds.value, "hbm_bytes": hbm.value}
+
+
+# ---- batched scoring ------------------------------------------------------------
+
+class Ssimulacra2Batch:
+    """Many pairs of any geometries per launch (no reference counterpart).
+    Pairs are device-resident (DeviceImage pointers)."""
+
+    def __init__(self, max_pairs: int, plane_format: int = L.PLANES_F32, hbm_budget: int = 0,
+                 strip_rows: int = 0):
+        cfg = L.BatchConfigC()
+        cfg.plane_format = plane_format
+        cfg.strip_rows = strip_rows
+        cfg.max_pairs = max_pairs
+        cfg.hbm_budget = hbm_budget
+        h = ctypes.c_void_p()
+        _check(L.lib().vqm_ss2_batch_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self.max_pairs = max_pairs
+        self._count = 0
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            L.lib().vqm_ss2_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def Submit(self, pairs: Sequence[tuple]) -> int:
+        """pairs: (original_ptr, distorted_ptr, width, height) tuples."""
+        arr = (L.DevicePairC * len(pairs))()
+        for i, (a, b, w, h) in enumerate(pairs):
+            arr[i].original, arr[i].distorted, arr[i].width, arr[i].height = a, b, w, h
+        token = L.u64()
+        _check(L.lib().vqm_ss2_batch_submit(self._h, arr, len(pairs), ctypes.byref(token)))
+        self._count = len(pairs)
+        return token.value
+
+    def Finish(self, token: int, raw: bool = False):
+        n = self._count
+        norms = np.zeros(n * 108)
+        scales = np.zeros(n, np.uint32)
+        _check(L.lib().vqm_ss2_batch_finish(self._h, token,
+                                            scales.ctypes.data_as(ctypes.POINTER(L.u32)),
+                                            norms.ctypes.data_as(ctypes.POINTER(L.f64))))
+        if raw:
+            return scales, norms.reshape(n, 108)
+        return [Ssimulacra2Result.from_norms(int(scales[i]), norms[i * 108:(i + 1) * 108])
+                for i in range(n)]
+
+    def StageTimes(self) -> List[float]:
+        ms = (L.f32 * 3)()
+        _check(L.lib().vqm_ss2_batch_stage_times(self._h, ms))
+        return list(ms)
+
+    def PoolBytes(self) -> int:
+        b = L.u64()
+        _check(L.lib().vqm_ss2_batch_pool_bytes(self._h, ctypes.byref(b)))
+        return b.value
