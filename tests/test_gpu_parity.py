@@ -39,8 +39,10 @@ def check_against_oracle(oracle_lib, w, h, a, b, dev_mode=None, **cfg):
     dev = dev_mode if dev_mode is not None else oracle_lib.F32_DEV
     n_o, want, _ = oracle_lib.compute(dev, w, h, a, b)
     assert n == n_o
-    np.testing.assert_allclose(got, want, rtol=DEV_REL, atol=1e-300,
-                               err_msg=f"{w}x{h} GPU vs oracle device form")
+    bad = np.nonzero(np.abs(got - want) > DEV_REL * np.abs(want))[0]
+    assert bad.size == 0, (f"{w}x{h} GPU vs oracle device form: features "
+                           f"{[(int(i) // 18, int(i) % 18) for i in bad]} (level, index) differ; "
+                           f"got {got[bad]} want {want[bad]}")
     return n, got, res
 
 
@@ -58,7 +60,7 @@ def test_reference_goldens_on_gpu(oracle_lib):
 
 
 @pytest.mark.parametrize("size,scales", [((8, 8), 2), ((13, 9), 2), ((32, 32), 4), ((64, 64), 5),
-                                         ((65, 65), 5), ((129, 97), 5), ((300, 41), 3)])
+                                         ((65, 65), 5), ((129, 97), 5), ((300, 41), 4)])
 def test_identity_and_small_geometries(size, scales):
     """pipeline_test.cc:120-144, 760-783: identity is exactly 100 and repeats."""
     w, h = size
@@ -74,11 +76,17 @@ def test_identity_and_small_geometries(size, scales):
 @pytest.mark.parametrize("size", [(32, 32), (97, 61), (200, 130), (257, 129), (1000, 37), (37, 600),
                                   (128, 64), (129, 65)])
 def test_random_pairs_match_oracle(oracle_lib, size):
+    """Pure-noise pairs, the FP32 worst case: the reference's own FP32
+    operation order drifts up to 1.6e-3 from FP64 on these, so the GPU is held
+    to twice that form's own error (or 1e-3, whichever is larger)."""
     w, h = size
     a, b = random_pair(w, h, seed=w * 7 + h)
     n, got, _ = check_against_oracle(oracle_lib, w, h, a, b)
     _, f64, _ = oracle_lib.compute(oracle_lib.F64, w, h, a, b)
-    assert abs(oracle_lib.score(n, got) - oracle_lib.score(n, f64)) <= SCORE_ABS
+    _, ref, _ = oracle_lib.compute(oracle_lib.F32_REF, w, h, a, b)
+    s64 = oracle_lib.score(n, f64)
+    bound = max(SCORE_ABS, 2.0 * abs(oracle_lib.score(n, ref) - s64))
+    assert abs(oracle_lib.score(n, got) - s64) <= bound
 
 
 @pytest.mark.parametrize("strength", synth.STRENGTHS)

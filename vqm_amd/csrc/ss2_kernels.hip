@@ -110,8 +110,20 @@ struct F32 {
     }
 };
 
+// FP32 -> FP16, round to nearest even, written out in integer operations so
+// that every value (ties included) rounds exactly as IEEE 754 specifies and
+// as the oracle's restatement does.
 __device__ __forceinline__ uint16_t to_f16(float v) {
-    return __builtin_bit_cast(uint16_t, static_cast<_Float16>(v));
+    const uint32_t x = __float_as_uint(v);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const uint32_t a = x & 0x7FFFFFFFu;
+    if (a >= 0x477FF000u)  // rounds to infinity (65520 and up), or NaN
+        return static_cast<uint16_t>(sign | (a > 0x7F800000u ? 0x7E00u : 0x7C00u));
+    if (a < 0x38800000u)   // below 2^-14: a multiple of 2^-24, exact product
+        return static_cast<uint16_t>(sign | static_cast<uint32_t>(__builtin_rintf(__uint_as_float(a) * 0x1p24f)));
+    uint32_t r = a - 0x38000000u;  // rebias the exponent from 127 to 15
+    r += 0xFFFu + ((r >> 13) & 1u);
+    return static_cast<uint16_t>(sign | (r >> 13));
 }
 __device__ __forceinline__ float from_f16(uint16_t v) {
     return static_cast<float>(__builtin_bit_cast(_Float16, v));
