@@ -82,6 +82,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-modes", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run the plain copy kernel over MIB MiB (traffic counter reference)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -110,6 +112,20 @@ def main() -> int:
         dev.upload(b, (2 * i + 1) * slot)
         refs.append((dev.ptr + 2 * i * slot, dev.ptr + (2 * i + 1) * slot, w, h))
     log(f"uploaded {2 * len(pairs)} images ({2 * slot * len(pairs) / 1e9:.2f} GB)")
+
+    if args.copy_calibration:
+        import ctypes
+        from vqm_amd import _lib
+        nbytes = args.copy_calibration << 20
+        src_buf, dst_buf = DeviceBuffer(nbytes), DeviceBuffer(nbytes)
+        for _ in range(3):
+            status = _lib.lib().vqm_device_copy(ctypes.c_void_p(dst_buf.ptr), ctypes.c_void_p(src_buf.ptr), nbytes)
+            if status != 0:
+                log("copy calibration failed")
+                return 1
+        log(f"copy calibration: 3 copies of {args.copy_calibration} MiB")
+        src_buf.free()
+        dst_buf.free()
 
     batch = Ssimulacra2Batch(max_pairs=len(refs), plane_format=fmt)
     for _ in range(args.warmup):

@@ -1,0 +1,405 @@
+// ssimulacra2.h -- C++20 facade of the MI355X-native SSIMULACRA 2.1 path.
+//
+// Header-only: rebuilds the reference's public API (GreatValueCreamSoda/vqm
+// include/ssimulacra2.h:17-215, include/vqm_image.h:17-166,
+// include/vqm_pool.h:51-86, include/vqm_alpha.h:16-27) on the C ABI of
+// libvqm_hip.so (vqm_ss2.h). Names, argument meaning and exceptions follow
+// the reference:
+//   std::invalid_argument  bad geometry, size or format mismatch, stale
+//                          handle or token, full pool
+//   std::logic_error       Submit/Upload/Release/AllocateImage/MapImage while
+//                          a pair is in flight
+//   std::runtime_error     no usable device, HIP failure, allocation failure
+// The Vulkan-specific overloads are replaced by DeviceImageView (a device
+// pointer to packed planes) and an optional HIP stream to wait on.
+#pragma once
+
+#include <array>
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "vqm_ss2.h"
+
+namespace vqm {
+
+struct Size {
+    std::size_t width = 0;
+    std::size_t height = 0;
+    [[nodiscard]] constexpr bool operator==(const Size&) const noexcept = default;
+};
+
+// zimg numeric values, as in the reference (vqm_image.h:25-96).
+enum class PixelType : std::uint8_t { Byte = 0, Word = 1, Half = 2, Float = 3 };
+enum class PixelRange : std::uint8_t { Limited = 0, Full = 1 };
+enum class ColorFamily : std::uint8_t { Grey = 0, Rgb = 1, Yuv = 2 };
+enum class AlphaType : std::uint8_t { None = 0, Straight = 1, Premultiplied = 2 };
+enum class MatrixCoefficients : std::uint8_t { Rgb = 0, Bt709 = 1, Unspecified = 2 };
+enum class TransferCharacteristics : std::uint8_t { Bt709 = 1, Unspecified = 2, Linear = 8, Srgb = 13 };
+enum class ColorPrimaries : std::uint8_t { Bt709 = 1, Unspecified = 2, Bt2020 = 9 };
+enum class PlaneFormat : std::uint32_t { F32 = VQM_PLANES_F32, F16 = VQM_PLANES_F16, BF16 = VQM_PLANES_BF16 };
+
+// vqm_image.h:100-120. The reference's default sample type is Float; this
+// path decodes 8-bit sRGB RGB (ImageFormat::Srgb8()).
+struct ImageFormat {
+    PixelType type = PixelType::Float;
+    ColorFamily family = ColorFamily::Rgb;
+    AlphaType alpha = AlphaType::None;
+    PixelRange range = PixelRange::Full;
+    MatrixCoefficients matrix = MatrixCoefficients::Rgb;
+    TransferCharacteristics transfer = TransferCharacteristics::Srgb;
+    ColorPrimaries primaries = ColorPrimaries::Bt709;
+    std::uint8_t chroma_location = 0;
+    std::uint8_t subsample_w = 0;
+    std::uint8_t subsample_h = 0;
+    std::uint8_t depth = 0;
+
+    static ImageFormat Srgb8() {
+        ImageFormat f;
+        f.type = PixelType::Byte;
+        return f;
+    }
+    [[nodiscard]] vqm_image_format ToC() const {
+        vqm_image_format f{};
+        f.type = static_cast<std::uint8_t>(type);
+        f.family = static_cast<std::uint8_t>(family);
+        f.alpha = static_cast<std::uint8_t>(alpha);
+        f.range = static_cast<std::uint8_t>(range);
+        f.matrix = static_cast<std::uint8_t>(matrix);
+        f.transfer = static_cast<std::uint8_t>(transfer);
+        f.primaries = static_cast<std::uint8_t>(primaries);
+        f.chroma_location = chroma_location;
+        f.subsample_w = subsample_w;
+        f.subsample_h = subsample_h;
+        f.depth = depth;
+        return f;
+    }
+};
+
+// vqm_image.h:134-141: packed planes, each starting on a 4-byte boundary.
+struct Image {
+    Size size;
+    ImageFormat format = ImageFormat::Srgb8();
+    std::vector<std::uint8_t> data;
+    [[nodiscard]] bool HasAlpha() const noexcept { return format.alpha != AlphaType::None; }
+};
+
+struct UploadedImage {  // vqm_image.h:151-154
+    std::uint64_t slot = 0;
+    Size size;
+};
+
+struct ImageBuffer {  // vqm_image.h:162-166
+    std::uint64_t slot = 0;
+    Size size;
+    std::size_t bytes = 0;
+};
+
+// Caller-owned device image (replaces VulkanBufferView, vqm_vulkan.h:74-101):
+// must stay alive and unmodified until Finish returns.
+struct DeviceImageView {
+    const void* data = nullptr;
+    Size size;
+    ImageFormat format = ImageFormat::Srgb8();
+};
+
+namespace detail {
+
+[[noreturn]] inline void Throw(int status) {
+    const std::string msg = vqm_last_error();
+    switch (status) {
+        case VQM_INVALID_ARGUMENT: throw std::invalid_argument(msg);
+        case VQM_LOGIC_ERROR: throw std::logic_error(msg);
+        case VQM_OVERFLOW: throw std::overflow_error(msg);
+        default: throw std::runtime_error(msg);
+    }
+}
+
+inline void Check(int status) {
+    if (status != VQM_OK) Throw(status);
+}
+
+inline std::uint32_t Dim(std::size_t v) {
+    if (v > 0xFFFFFFFFull) throw std::invalid_argument("image dimension exceeds 32 bits");
+    return static_cast<std::uint32_t>(v);
+}
+
+inline vqm_image_desc Desc(const Image& image) {
+    return vqm_image_desc{Dim(image.size.width), Dim(image.size.height), image.format.ToC(),
+                          image.data.size()};
+}
+
+}  // namespace detail
+
+// RequiredBytes, vqm_image.h:143-146.
+[[nodiscard]] inline std::size_t RequiredBytes(const ImageFormat& format, Size size) {
+    std::uint64_t bytes = 0;
+    const auto f = format.ToC();
+    detail::Check(vqm_required_bytes(&f, detail::Dim(size.width), detail::Dim(size.height), &bytes));
+    return static_cast<std::size_t>(bytes);
+}
+
+// ssimulacra2.h:17-25.
+struct Ssimulacra2Scale {
+    static constexpr std::size_t kSsimNormCount = 6;
+    static constexpr std::size_t kEdgeNormCount = 12;
+    std::array<double, kSsimNormCount> avg_ssim{};
+    std::array<double, kEdgeNormCount> avg_edgediff{};
+};
+
+// ssimulacra2.h:27-33.
+struct Ssimulacra2Result {
+    std::vector<Ssimulacra2Scale> scales;
+
+    static Ssimulacra2Result FromNorms(std::uint32_t count, const double* norms) {
+        Ssimulacra2Result r;
+        r.scales.resize(count);
+        for (std::uint32_t s = 0; s < count; ++s) {
+            for (std::size_t k = 0; k < 6; ++k) r.scales[s].avg_ssim[k] = norms[s * 18 + k];
+            for (std::size_t k = 0; k < 12; ++k) r.scales[s].avg_edgediff[k] = norms[s * 18 + 6 + k];
+        }
+        return r;
+    }
+    // Weighted sum of up to 108 norms mapped to a score of at most 100;
+    // std::invalid_argument for more than six scales or nonfinite norms.
+    [[nodiscard]] double Score() const {
+        if (scales.size() > 6) throw std::invalid_argument("Ssimulacra2Result has too many scales");
+        std::array<double, 108> flat{};
+        for (std::size_t s = 0; s < scales.size(); ++s) {
+            for (std::size_t k = 0; k < 6; ++k) flat[s * 18 + k] = scales[s].avg_ssim[k];
+            for (std::size_t k = 0; k < 12; ++k) flat[s * 18 + 6 + k] = scales[s].avg_edgediff[k];
+        }
+        double score = 0.0;
+        detail::Check(vqm_ss2_score(static_cast<std::uint32_t>(scales.size()), flat.data(), &score));
+        return score;
+    }
+};
+
+// ssimulacra2.h:35-64 (chroma_filter and memory_policy have no meaning here).
+struct Ssimulacra2Config {
+    Size size;
+    std::size_t gpu_resident_images = 2;
+    bool half_planes = false;             // FP16 planes
+    std::size_t input_bytes = 0;          // caller-owned image pool
+    bool streaming_inputs = true;
+    PlaneFormat plane_format = PlaneFormat::F32;  // overridden to F16 by half_planes
+    std::uint32_t strip_rows = 0;
+
+    [[nodiscard]] vqm_ss2_config ToC() const {
+        vqm_ss2_config c{};
+        vqm_ss2_config_init(&c, detail::Dim(size.width), detail::Dim(size.height));
+        if (gpu_resident_images > 0xFFFFFFFFull)
+            throw std::invalid_argument("gpu_resident_images is limited to 4096 slots");
+        c.resident_images = static_cast<std::uint32_t>(gpu_resident_images);
+        c.plane_format = half_planes ? VQM_PLANES_F16 : static_cast<std::uint32_t>(plane_format);
+        c.streaming_inputs = streaming_inputs ? 1u : 0u;
+        c.strip_rows = strip_rows;
+        c.input_bytes = input_bytes;
+        return c;
+    }
+};
+
+// vqm_memory_types.h:29-38, by region.
+struct PipelineMemoryUsage {
+    std::uint64_t workspace_bytes = 0;
+    std::uint64_t resident_bytes = 0;
+    std::uint64_t streaming_bytes = 0;
+    std::uint64_t input_pool_bytes = 0;
+    std::uint64_t pinned_host_bytes = 0;
+    std::uint64_t total_device_bytes = 0;
+};
+
+// ssimulacra2.h:66-215. One geometry, one submission in flight, not
+// thread-safe; use one pipeline per thread.
+class Ssimulacra2Pipeline {
+public:
+    explicit Ssimulacra2Pipeline(Ssimulacra2Config config) : size_(config.size) {
+        const vqm_ss2_config c = config.ToC();
+        detail::Check(vqm_ss2_create(&c, &p_));
+    }
+    ~Ssimulacra2Pipeline() { vqm_ss2_destroy(p_); }
+    Ssimulacra2Pipeline(Ssimulacra2Pipeline&& o) noexcept
+        : p_(std::exchange(o.p_, nullptr)), size_(o.size_) {}
+    Ssimulacra2Pipeline& operator=(Ssimulacra2Pipeline&& o) noexcept {
+        if (this != &o) {
+            vqm_ss2_destroy(p_);
+            p_ = std::exchange(o.p_, nullptr);
+            size_ = o.size_;
+        }
+        return *this;
+    }
+    Ssimulacra2Pipeline(const Ssimulacra2Pipeline&) = delete;
+    Ssimulacra2Pipeline& operator=(const Ssimulacra2Pipeline&) = delete;
+
+    [[nodiscard]] Ssimulacra2Result Compute(const Image& a, const Image& b, float background = 0.5f) {
+        return Finish(Submit(a, b, background));
+    }
+    [[nodiscard]] Ssimulacra2Result Compute(const UploadedImage& a, const UploadedImage& b,
+                                            float background = 0.5f) {
+        return Finish(Submit(a, b, background));
+    }
+    [[nodiscard]] Ssimulacra2Result Compute(const UploadedImage& a, const Image& b,
+                                            float background = 0.5f) {
+        return Finish(Submit(a, b, background));
+    }
+    [[nodiscard]] Ssimulacra2Result Compute(const ImageBuffer& a, const ImageBuffer& b,
+                                            float background = 0.5f) {
+        return Finish(Submit(a, b, background));
+    }
+    [[nodiscard]] Ssimulacra2Result Compute(const DeviceImageView& a, const DeviceImageView& b,
+                                            float background = 0.5f) {
+        return Finish(Submit(a, b, background));
+    }
+
+    [[nodiscard]] UploadedImage Upload(const Image& image) {
+        const vqm_image_desc d = detail::Desc(image);
+        std::uint64_t handle = 0;
+        detail::Check(vqm_ss2_upload(p_, &d, image.data.data(), &handle));
+        return UploadedImage{handle, image.size};
+    }
+    void Release(UploadedImage image) { detail::Check(vqm_ss2_release(p_, image.slot)); }
+
+    [[nodiscard]] std::uint64_t Submit(const Image& a, const Image& b, float background = 0.5f) {
+        const vqm_image_desc da = detail::Desc(a), db = detail::Desc(b);
+        std::uint64_t token = 0;
+        detail::Check(vqm_ss2_submit_host(p_, &da, a.data.data(), &db, b.data.data(), background, &token));
+        return token;
+    }
+    [[nodiscard]] std::uint64_t Submit(const UploadedImage& a, const UploadedImage& b,
+                                       float background = 0.5f) {
+        std::uint64_t token = 0;
+        detail::Check(vqm_ss2_submit_resident(p_, a.slot, b.slot, background, &token));
+        return token;
+    }
+    [[nodiscard]] std::uint64_t Submit(const UploadedImage& a, const Image& b, float background = 0.5f) {
+        const vqm_image_desc db = detail::Desc(b);
+        std::uint64_t token = 0;
+        detail::Check(vqm_ss2_submit_resident_host(p_, a.slot, &db, b.data.data(), background, &token));
+        return token;
+    }
+    [[nodiscard]] std::uint64_t Submit(const ImageBuffer& a, const ImageBuffer& b,
+                                       float background = 0.5f) {
+        std::uint64_t token = 0;
+        detail::Check(vqm_ss2_submit_buffers(p_, a.slot, b.slot, background, &token));
+        return token;
+    }
+    // Optional `wait_stream` (a hipStream_t): the pair starts after the work
+    // already queued there (the VulkanSubmitSync wait, ssimulacra2.h:196-201).
+    [[nodiscard]] std::uint64_t Submit(const DeviceImageView& a, const DeviceImageView& b,
+                                       float background = 0.5f, void* wait_stream = nullptr) {
+        const vqm_device_image da = View(a), db = View(b);
+        std::uint64_t token = 0;
+        detail::Check(vqm_ss2_submit_device(p_, &da, &db, wait_stream, background, &token));
+        return token;
+    }
+
+    [[nodiscard]] Ssimulacra2Result Finish(std::uint64_t token) {
+        std::array<double, 108> norms{};
+        std::uint32_t scales = 0;
+        detail::Check(vqm_ss2_finish(p_, token, &scales, norms.data()));
+        return Ssimulacra2Result::FromNorms(scales, norms.data());
+    }
+
+    // Caller-owned device memory carved from input_bytes; MapImage returns a
+    // device pointer (fill it with hipMemcpy or a kernel before Submit).
+    [[nodiscard]] ImageBuffer AllocateImage(const ImageFormat& format) {
+        const vqm_image_format f = format.ToC();
+        std::uint64_t handle = 0, bytes = 0;
+        detail::Check(vqm_ss2_allocate_image(p_, &f, &handle, &bytes));
+        return ImageBuffer{handle, size_, static_cast<std::size_t>(bytes)};
+    }
+    [[nodiscard]] void* MapImage(ImageBuffer buffer) {
+        void* ptr = nullptr;
+        detail::Check(vqm_ss2_map_image(p_, buffer.slot, &ptr));
+        return ptr;
+    }
+    void Release(ImageBuffer buffer) { detail::Check(vqm_ss2_release_image(p_, buffer.slot)); }
+
+    [[nodiscard]] PipelineMemoryUsage GetMemoryUsage() const {
+        vqm_ss2_memory m{};
+        detail::Check(vqm_ss2_memory_usage(p_, &m));
+        return PipelineMemoryUsage{m.workspace_bytes, m.resident_bytes, m.streaming_bytes,
+                                   m.input_pool_bytes, m.pinned_host_bytes, m.total_device_bytes};
+    }
+
+private:
+    vqm_device_image View(const DeviceImageView& v) const {
+        return vqm_device_image{v.data, detail::Dim(v.size.width), detail::Dim(v.size.height),
+                                v.format.ToC(), RequiredBytes(v.format, v.size)};
+    }
+    vqm_ss2_pipeline* p_ = nullptr;
+    Size size_;
+};
+
+// vqm_alpha.h:16-27: opaque inputs score once; with alpha, the worse of the
+// dark (0.1) and bright (0.9) background.
+template <typename Frame>
+[[nodiscard]] Ssimulacra2Result ComputeWorstBackground(Ssimulacra2Pipeline& pipeline,
+                                                       const Frame& original, const Frame& distorted) {
+    if (original.format.alpha == AlphaType::None && distorted.format.alpha == AlphaType::None)
+        return pipeline.Compute(original, distorted);
+    auto dark = pipeline.Compute(original, distorted, 0.1f);
+    auto bright = pipeline.Compute(original, distorted, 0.9f);
+    return dark.Score() <= bright.Score() ? dark : bright;
+}
+
+struct ImagePair {  // vqm_pool.h:15-18
+    const Image& original;
+    const Image& distorted;
+};
+
+// vqm_pool.h:51-86 / pool.cc:24-116: N pipelines round-robin, one pair in
+// flight each; results in input order. After a failure the pending pairs
+// are drained and the error rethrown.
+class Ssimulacra2Pool {
+public:
+    Ssimulacra2Pool(Ssimulacra2Config config, std::size_t workers) {
+        if (workers == 0) throw std::invalid_argument("a pool needs at least one worker");
+        for (std::size_t i = 0; i < workers; ++i) pipes_.emplace_back(config);
+    }
+    [[nodiscard]] std::size_t Workers() const noexcept { return pipes_.size(); }
+
+    [[nodiscard]] std::vector<Ssimulacra2Result> ComputeMany(const std::vector<ImagePair>& pairs,
+                                                             float background = 0.5f) {
+        const std::size_t n = pipes_.size();
+        std::vector<std::uint64_t> tokens(n, 0);
+        std::vector<std::size_t> owner(n, 0);
+        std::vector<Ssimulacra2Result> out(pairs.size());
+        try {
+            for (std::size_t i = 0; i < pairs.size(); ++i) {
+                const std::size_t slot = i % n;
+                if (tokens[slot] != 0) {
+                    out[owner[slot]] = pipes_[slot].Finish(std::exchange(tokens[slot], 0));
+                }
+                tokens[slot] = pipes_[slot].Submit(pairs[i].original, pairs[i].distorted, background);
+                owner[slot] = i;
+            }
+            for (std::size_t slot = 0; slot < n; ++slot)
+                if (tokens[slot] != 0) out[owner[slot]] = pipes_[slot].Finish(std::exchange(tokens[slot], 0));
+        } catch (...) {
+            for (std::size_t slot = 0; slot < n; ++slot) {
+                if (tokens[slot] == 0) continue;
+                try {
+                    (void)pipes_[slot].Finish(std::exchange(tokens[slot], 0));
+                } catch (...) {
+                }
+            }
+            throw;
+        }
+        return out;
+    }
+    [[nodiscard]] std::vector<double> ScoreMany(const std::vector<ImagePair>& pairs,
+                                                float background = 0.5f) {
+        std::vector<double> scores;
+        for (const auto& r : ComputeMany(pairs, background)) scores.push_back(r.Score());
+        return scores;
+    }
+
+private:
+    std::vector<Ssimulacra2Pipeline> pipes_;
+};
+
+}  // namespace vqm

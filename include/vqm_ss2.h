@@ -281,6 +281,9 @@ int vqm_device_free(void* ptr);
 int vqm_memcpy_htod(void* dst, const void* src, uint64_t bytes);
 int vqm_memcpy_dtoh(void* dst, const void* src, uint64_t bytes);
 int vqm_device_synchronize(void);
+/* Copies `bytes` (a multiple of 16) between device buffers with a plain
+ * streaming kernel and waits: the reference point for HBM traffic counters. */
+int vqm_device_copy(void* dst, const void* src, uint64_t bytes);
 
 #ifdef __cplusplus
 }

@@ -117,6 +117,7 @@ SIGNATURES = {
     "vqm_memcpy_htod": (ctypes.c_int, [vp, vp, u64]),
     "vqm_memcpy_dtoh": (ctypes.c_int, [vp, vp, u64]),
     "vqm_device_synchronize": (ctypes.c_int, []),
+    "vqm_device_copy": (ctypes.c_int, [vp, vp, u64]),
 }
 
 _lock = threading.Lock()

@@ -381,6 +381,14 @@ int vqm_memcpy_dtoh(void* dst, const void* src, uint64_t bytes) {
     return Guard([&] { vqm::ss2::Check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy"); });
 }
 
+int vqm_device_copy(void* dst, const void* src, uint64_t bytes) {
+    return Guard([&] {
+        if (bytes % 16 != 0) vqm::ss2::Fail(VQM_INVALID_ARGUMENT, "copy size must be a multiple of 16");
+        vqm::ss2::Check(vqm::ss2::LaunchCopy(dst, src, bytes, nullptr), "copy kernel");
+        vqm::ss2::Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    });
+}
+
 int vqm_device_synchronize(void) {
     return Guard([&] { vqm::ss2::Check(hipDeviceSynchronize(), "hipDeviceSynchronize"); });
 }

@@ -35,4 +35,7 @@ hipError_t LaunchScale(PlaneFormat format, const PairDesc* pairs, const WorkRef*
 // Sums each (level, channel) norm's partials in FP64 and writes the 108 norms.
 hipError_t LaunchReduce(const PairDesc* pairs, uint32_t pairs_count, hipStream_t stream);
 
+// Device-to-device copy of `bytes` (a multiple of 16) by a plain kernel.
+hipError_t LaunchCopy(void* dst, const void* src, uint64_t bytes, hipStream_t stream);
+
 }  // namespace vqm::ss2

@@ -21,6 +21,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <type_traits>
 #include <utility>
@@ -561,7 +562,25 @@ hipError_t launch_scale_t(const PairDesc* pairs, const WorkRef* work, uint32_t b
     return hipGetLastError();
 }
 
+// A plain streaming copy (16-byte loads and stores): the reference point that
+// scales the FETCH_SIZE / WRITE_SIZE counters to bytes.
+__global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src,
+                                                   uint4* __restrict__ dst, uint64_t n) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        dst[i] = src[i];
+}
+
 }  // namespace
+
+hipError_t LaunchCopy(void* dst, const void* src, uint64_t bytes, hipStream_t stream) {
+    const uint64_t n = bytes / 16;
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(copy_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, stream,
+                       static_cast<const uint4*>(src), static_cast<uint4*>(dst), n);
+    return hipGetLastError();
+}
 
 hipError_t LaunchPrepare(PlaneFormat format, const PairDesc* pairs, const WorkRef* work,
                          uint32_t blocks, uint32_t tiles_per_image, const float* lut,
