@@ -18,6 +18,10 @@
 // or abs, with contraction switched off, so the CPU oracle's FP32 "dev" form
 // reproduces each pixel's values bit for bit. The blur runs horizontal pass
 // first with taps in ascending order, as scale.comp:149-301 does.
+//
+// Memory access. Descriptors (pairs, plans, work items) are read through
+// constant-address-space pointers so that their uniform fields become scalar
+// loads; images and planes through global-address-space pointers.
 
 #include <hip/hip_runtime.h>
 
@@ -33,6 +37,23 @@
 
 namespace vqm::ss2 {
 namespace {
+
+template <typename T> using ConstAS = __attribute__((address_space(4))) const T;
+template <typename T> using GlobalAS = __attribute__((address_space(1))) T;
+
+// Builtin vector types for wide loads and stores through qualified pointers.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+template <typename T>
+__device__ __forceinline__ ConstAS<T>* as_const(const T* p) {
+    return (ConstAS<T>*)p;
+}
+template <typename T>
+__device__ __forceinline__ GlobalAS<T>* as_global(T* p) {
+    return (GlobalAS<T>*)p;
+}
 
 // The 13 taps of MakeGaussian(1.5) (pipeline.cc:274-291), exact FP32 values.
 struct Taps {
@@ -99,15 +120,16 @@ __device__ __forceinline__ void xyb(float r, float g, float b, uint32_t mask, fl
 
 struct F32 {
     using E = float;
-    static __device__ __forceinline__ float2 load2(const E* p) {
-        return *reinterpret_cast<const float2*>(p);
+    static __device__ __forceinline__ float2 load2(GlobalAS<const E>* p) {
+        const f32x2 v = *reinterpret_cast<GlobalAS<const f32x2>*>(p);
+        return make_float2(v.x, v.y);
     }
-    static __device__ __forceinline__ void store1(E* p, float v) { *p = v; }
-    static __device__ __forceinline__ void store2(E* p, float a, float b) {
-        *reinterpret_cast<float2*>(p) = make_float2(a, b);
+    static __device__ __forceinline__ void store1(GlobalAS<E>* p, float v) { *p = v; }
+    static __device__ __forceinline__ void store2(GlobalAS<E>* p, float a, float b) {
+        *reinterpret_cast<GlobalAS<f32x2>*>(p) = f32x2{a, b};
     }
-    static __device__ __forceinline__ void store4(E* p, const float v[4]) {
-        *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    static __device__ __forceinline__ void store4(GlobalAS<E>* p, const float v[4]) {
+        *reinterpret_cast<GlobalAS<f32x4>*>(p) = f32x4{v[0], v[1], v[2], v[3]};
     }
 };
 
@@ -141,21 +163,20 @@ __device__ __forceinline__ float from_bf16(uint16_t v) {
 template <uint16_t (*To)(float), float (*From)(uint16_t)>
 struct Narrow {
     using E = uint16_t;
-    static __device__ __forceinline__ float2 load2(const E* p) {
-        const uint32_t v = *reinterpret_cast<const uint32_t*>(p);
+    static __device__ __forceinline__ float2 load2(GlobalAS<const E>* p) {
+        const uint32_t v = *reinterpret_cast<GlobalAS<const uint32_t>*>(p);
         return make_float2(From(static_cast<uint16_t>(v & 0xFFFFu)),
                            From(static_cast<uint16_t>(v >> 16)));
     }
-    static __device__ __forceinline__ void store1(E* p, float v) { *p = To(v); }
-    static __device__ __forceinline__ void store2(E* p, float a, float b) {
-        *reinterpret_cast<uint32_t*>(p) =
+    static __device__ __forceinline__ void store1(GlobalAS<E>* p, float v) { *p = To(v); }
+    static __device__ __forceinline__ void store2(GlobalAS<E>* p, float a, float b) {
+        *reinterpret_cast<GlobalAS<uint32_t>*>(p) =
             static_cast<uint32_t>(To(a)) | (static_cast<uint32_t>(To(b)) << 16);
     }
-    static __device__ __forceinline__ void store4(E* p, const float v[4]) {
-        uint2 w;
-        w.x = static_cast<uint32_t>(To(v[0])) | (static_cast<uint32_t>(To(v[1])) << 16);
-        w.y = static_cast<uint32_t>(To(v[2])) | (static_cast<uint32_t>(To(v[3])) << 16);
-        *reinterpret_cast<uint2*>(p) = w;
+    static __device__ __forceinline__ void store4(GlobalAS<E>* p, const float v[4]) {
+        *reinterpret_cast<GlobalAS<u32x2>*>(p) =
+            u32x2{static_cast<uint32_t>(To(v[0])) | (static_cast<uint32_t>(To(v[1])) << 16),
+                  static_cast<uint32_t>(To(v[2])) | (static_cast<uint32_t>(To(v[3])) << 16)};
     }
 };
 using F16 = Narrow<to_f16, from_f16>;
@@ -173,15 +194,50 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
 
 // ---- prepare ----------------------------------------------------------------
 
-template <typename Fmt>
-__device__ __forceinline__ void store_level(const Plan& plan, typename Fmt::E* planes,
-                                            uint32_t level, uint32_t x, uint32_t y,
-                                            const float v[3]) {
-    const uint32_t mask = plan.mask[level];
-    const uint64_t at = static_cast<uint64_t>(y + kPadTop) * plan.pitch[level] + x + kPadLeft;
+// Level 1's tile is 32 x 16 samples; every later level is a quarter of the
+// previous one. The ladder keeps a level's linear samples channel-planar.
+constexpr uint32_t kLadder = (kPrepTileW / 2) * (kPrepTileH / 2);
+
+template <typename Fmt, uint32_t L>
+__device__ __forceinline__ void prepare_level(ConstAS<Plan>* plan, GlobalAS<typename Fmt::E>* planes,
+                                              uint32_t X0, uint32_t Y0, uint32_t tid,
+                                              float (*ladder)[3][kLadder]) {
+    // Level L from the ladder level L-1 published, with the clamp of level
+    // L-1's last row and column applied in this tile's coordinates
+    // (prepare.comp:146-168).
+    constexpr uint32_t side_x = kPrepTileW >> L, side_y = kPrepTileH >> L;
+    constexpr uint32_t prev_x = side_x * 2u;
+    const float (*prev)[kLadder] = ladder[(L - 1u) & 1u];
+    float (*next)[kLadder] = ladder[L & 1u];
+    if (tid >= side_x * side_y) return;
+    const uint32_t jx = tid % side_x, jy = tid / side_x;
+    const uint32_t gx = (X0 >> L) + jx, gy = (Y0 >> L) + jy;
+    float v[3] = {0.0f, 0.0f, 0.0f};
+    if (gx < plan->w[L] && gy < plan->h[L]) {
+        const uint32_t ox = X0 >> (L - 1u), oy = Y0 >> (L - 1u);
+        const uint32_t cx0 = 2u * jx, cy0 = 2u * jy;
+        const uint32_t cx1 = min(2u * gx + 1u, plan->w[L - 1] - 1u) - ox;
+        const uint32_t cy1 = min(2u * gy + 1u, plan->h[L - 1] - 1u) - oy;
 #pragma unroll
-    for (uint32_t c = 0; c < 3; ++c)
-        if (mask & (1u << c)) Fmt::store1(planes + plan.plane_off[level][c] + at, v[c]);
+        for (uint32_t c = 0; c < 3; ++c) {
+            float s = prev[c][cy0 * prev_x + cx0];
+            s = s + prev[c][cy0 * prev_x + cx1];
+            s = s + prev[c][cy1 * prev_x + cx0];
+            s = s + prev[c][cy1 * prev_x + cx1];
+            v[c] = s * 0.25f;
+        }
+        const uint32_t mask = plan->mask[L];
+        if (mask != 0u) {
+            float o[3];
+            xyb(v[0], v[1], v[2], mask, o);
+            const uint64_t at = static_cast<uint64_t>(gy + kPadTop) * plan->pitch[L] + gx + kPadLeft;
+#pragma unroll
+            for (uint32_t c = 0; c < 3; ++c)
+                if (mask & (1u << c)) Fmt::store1(planes + plan->plane_off[L][c] + at, o[c]);
+        }
+    }
+#pragma unroll
+    for (uint32_t c = 0; c < 3; ++c) next[c][jy * side_x + jx] = v[c];
 }
 
 template <typename Fmt>
@@ -191,36 +247,38 @@ __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict
                                                       const float* __restrict__ lut_g,
                                                       const uint8_t* src0, const uint8_t* src1) {
     __shared__ float lut[256];
-    // Level l-1 linear samples of this tile, three channels interleaved; two
-    // buffers alternate between levels. Level 1's tile is 32 x 16 samples.
-    __shared__ float ladder[2][(kPrepTileW / 2) * (kPrepTileH / 2) * 3];
+    // Two ladders alternate between levels: level l publishes into l & 1.
+    __shared__ __align__(16) float ladder[2][3][kLadder];
 
     const uint32_t tid = threadIdx.x;
-    lut[tid] = lut_g[tid];
+    lut[tid] = as_global(lut_g)[tid];
 
     uint32_t pair_idx, image, tile;
     if (work != nullptr) {
-        const WorkRef w = work[blockIdx.x];
-        pair_idx = w.pair;
-        image = w.index >> 31;
-        tile = w.index & 0x7FFFFFFFu;
+        ConstAS<WorkRef>* w = as_const(work) + blockIdx.x;
+        pair_idx = w->pair;
+        image = w->index >> 31;
+        tile = w->index & 0x7FFFFFFFu;
     } else {
         const uint32_t l = xcd_remap(blockIdx.x, blocks);
         tile = l % tiles_per_image;
         image = (l / tiles_per_image) & 1u;
         pair_idx = l / (2 * tiles_per_image);
     }
-    const PairDesc pd = pairs[pair_idx];
-    const Plan& plan = *pd.plan;
-    const uint32_t W = plan.width, H = plan.height;
-    const uint32_t tile_x = tile % plan.prep_tiles_x, tile_y = tile / plan.prep_tiles_x;
+    ConstAS<PairDesc>* pdp = as_const(pairs) + pair_idx;
+    ConstAS<Plan>* plan = as_const(pdp->plan);
+    const uint32_t W = plan->width, H = plan->height, levels = plan->levels;
+    const uint32_t tiles_x = plan->prep_tiles_x;
+    const uint32_t tile_x = tile % tiles_x, tile_y = tile / tiles_x;
     const uint32_t X0 = tile_x * kPrepTileW, Y0 = tile_y * kPrepTileH;
     const uint32_t tx = tid % 16u, ty = tid / 16u;
     using E = typename Fmt::E;
-    E* planes = static_cast<E*>(pd.planes) + (image ? plan.image_elems : 0);
+    GlobalAS<E>* planes = as_global(static_cast<E*>(pdp->planes)) + (image ? plan->image_elems : 0);
     // Single-pair launches name their inputs directly; batches use pd.src.
-    const uint8_t* src = image ? (src1 != nullptr ? src1 : pd.src[1])
-                               : (src0 != nullptr ? src0 : pd.src[0]);
+    const uint8_t* src_ptr = image ? (src1 != nullptr ? src1 : pdp->src[1])
+                                   : (src0 != nullptr ? src0 : pdp->src[0]);
+    GlobalAS<const uint8_t>* src = as_global(src_ptr);
+    const uint64_t plane_stride = plan->plane_stride;
     __syncthreads();
 
     // Level 0: this thread's 4 x 2 base pixels, coordinates clamped to the
@@ -229,81 +287,80 @@ __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict
     float lin[2][4][3];
     const uint32_t bx = X0 + 4u * tx;
     const bool vec = (W % 4u == 0u) && (bx + 3u < W);
-    #pragma unroll
+#pragma unroll
     for (uint32_t r = 0; r < 2; ++r) {
         const uint32_t y = min(Y0 + 2u * ty + r, H - 1u);
-        #pragma unroll
+#pragma unroll
         for (uint32_t c = 0; c < 3; ++c) {
-            const uint8_t* row = src + static_cast<uint64_t>(c) * plan.plane_stride +
-                                 static_cast<uint64_t>(y) * W;
+            GlobalAS<const uint8_t>* row = src + c * plane_stride + static_cast<uint64_t>(y) * W;
             if (vec) {
-                const uint32_t v = *reinterpret_cast<const uint32_t*>(row + bx);
-                #pragma unroll
+                const uint32_t v = *reinterpret_cast<GlobalAS<const uint32_t>*>(row + bx);
+#pragma unroll
                 for (uint32_t j = 0; j < 4; ++j) lin[r][j][c] = lut[(v >> (8u * j)) & 255u];
             } else {
-                #pragma unroll
+#pragma unroll
                 for (uint32_t j = 0; j < 4; ++j) lin[r][j][c] = lut[row[min(bx + j, W - 1u)]];
             }
         }
     }
-    const uint32_t mask0 = plan.mask[0];
+    const uint32_t mask0 = plan->mask[0];
     if (mask0 != 0u) {
-        #pragma unroll
+        const uint32_t pitch0 = plan->pitch[0];
+#pragma unroll
         for (uint32_t r = 0; r < 2; ++r) {
             const uint32_t y = Y0 + 2u * ty + r;
             if (y >= H) continue;
             float v[4][3];
-            #pragma unroll
+#pragma unroll
             for (uint32_t j = 0; j < 4; ++j)
                 xyb(lin[r][j][0], lin[r][j][1], lin[r][j][2], mask0, v[j]);
-            const uint64_t at = static_cast<uint64_t>(y + kPadTop) * plan.pitch[0] + bx + kPadLeft;
-            #pragma unroll
+            const uint64_t at = static_cast<uint64_t>(y + kPadTop) * pitch0 + bx + kPadLeft;
+#pragma unroll
             for (uint32_t c = 0; c < 3; ++c) {
                 if (!(mask0 & (1u << c))) continue;
-                E* p = planes + plan.plane_off[0][c] + at;
+                GlobalAS<E>* p = planes + plan->plane_off[0][c] + at;
                 if (bx + 3u < W) {
                     const float q[4] = {v[0][c], v[1][c], v[2][c], v[3][c]};
                     Fmt::store4(p, q);
                 } else {
-                    #pragma unroll
+#pragma unroll
                     for (uint32_t j = 0; j < 4; ++j)
                         if (bx + j < W) Fmt::store1(p + j, v[j][c]);
                 }
             }
         }
     }
-    if (plan.levels < 2) return;
+    if (levels < 2) return;
 
     // Level 1: two samples per thread, each the box of a 2x2 block in
     // (0,0), (1,0), (0,1), (1,1) order (prepare.comp:100-111).
     {
-        const uint32_t W1 = plan.w[1], H1 = plan.h[1];
+        const uint32_t W1 = plan->w[1], H1 = plan->h[1];
         const uint32_t gy = Y0 / 2u + ty;
         float v1[2][3];
-        #pragma unroll
-        for (uint32_t jj = 0; jj < 2; ++jj) {
-            #pragma unroll
-            for (uint32_t c = 0; c < 3; ++c) {
+#pragma unroll
+        for (uint32_t c = 0; c < 3; ++c) {
+#pragma unroll
+            for (uint32_t jj = 0; jj < 2; ++jj) {
                 float s = lin[0][2 * jj][c];
                 s = s + lin[0][2 * jj + 1][c];
                 s = s + lin[1][2 * jj][c];
                 s = s + lin[1][2 * jj + 1][c];
                 v1[jj][c] = s * 0.25f;
-                // Level l publishes into buffer l & 1; level 2 reads buffer 1.
-                ladder[1][(ty * 32u + 2u * tx + jj) * 3u + c] = v1[jj][c];
             }
+            *reinterpret_cast<float2*>(&ladder[1][c][ty * 32u + 2u * tx]) = make_float2(v1[0][c], v1[1][c]);
         }
-        const uint32_t mask1 = plan.mask[1];
+        const uint32_t mask1 = plan->mask[1];
         if (mask1 != 0u && gy < H1) {
             const uint32_t gx = X0 / 2u + 2u * tx;
             float x0[3], x1[3];
             xyb(v1[0][0], v1[0][1], v1[0][2], mask1, x0);
             xyb(v1[1][0], v1[1][1], v1[1][2], mask1, x1);
-            const uint64_t at = static_cast<uint64_t>(gy + kPadTop) * plan.pitch[1] + gx + kPadLeft;
-            #pragma unroll
+            const uint64_t at = static_cast<uint64_t>(gy + kPadTop) * plan->pitch[1] + gx + kPadLeft;
+#pragma unroll
             for (uint32_t c = 0; c < 3; ++c) {
                 if (!(mask1 & (1u << c))) continue;
-                E* p = planes + plan.plane_off[1][c] + at;
+                GlobalAS<E>* p = planes + plan->plane_off[1][c] + at;
                 if (gx + 1u < W1) {
                     Fmt::store2(p, x0[c], x1[c]);
                 } else if (gx < W1) {
@@ -313,42 +370,11 @@ __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict
         }
     }
 
-    // Levels 2 and up from the ladder, with the clamp of the previous level's
-    // last row and column applied in this tile's coordinates
-    // (prepare.comp:146-168).
-    for (uint32_t l = 2; l < plan.levels; ++l) {
-        __syncthreads();
-        const uint32_t side_x = kPrepTileW >> l, side_y = kPrepTileH >> l;
-        const uint32_t prev_x = side_x * 2u;
-        const float* prev = ladder[(l - 1u) & 1u];
-        float* next = ladder[l & 1u];
-        if (tid < side_x * side_y) {
-            const uint32_t jx = tid % side_x, jy = tid / side_x;
-            const uint32_t gx = (X0 >> l) + jx, gy = (Y0 >> l) + jy;
-            float v[3] = {0.0f, 0.0f, 0.0f};
-            if (gx < plan.w[l] && gy < plan.h[l]) {
-                const uint32_t ox = X0 >> (l - 1u), oy = Y0 >> (l - 1u);
-                const uint32_t cx0 = 2u * jx, cy0 = 2u * jy;
-                const uint32_t cx1 = min(2u * gx + 1u, plan.w[l - 1] - 1u) - ox;
-                const uint32_t cy1 = min(2u * gy + 1u, plan.h[l - 1] - 1u) - oy;
-                #pragma unroll
-                for (uint32_t c = 0; c < 3; ++c) {
-                    float s = prev[(cy0 * prev_x + cx0) * 3u + c];
-                    s = s + prev[(cy0 * prev_x + cx1) * 3u + c];
-                    s = s + prev[(cy1 * prev_x + cx0) * 3u + c];
-                    s = s + prev[(cy1 * prev_x + cx1) * 3u + c];
-                    v[c] = s * 0.25f;
-                }
-                if (plan.mask[l] != 0u) {
-                    float o[3];
-                    xyb(v[0], v[1], v[2], plan.mask[l], o);
-                    store_level<Fmt>(plan, planes, l, gx, gy, o);
-                }
-            }
-            #pragma unroll
-            for (uint32_t c = 0; c < 3; ++c) next[(jy * side_x + jx) * 3u + c] = v[c];
-        }
-    }
+    // Levels 2 .. 5: each waits for the previous level's ladder.
+    if (levels > 2) { __syncthreads(); prepare_level<Fmt, 2>(plan, planes, X0, Y0, tid, ladder); }
+    if (levels > 3) { __syncthreads(); prepare_level<Fmt, 3>(plan, planes, X0, Y0, tid, ladder); }
+    if (levels > 4) { __syncthreads(); prepare_level<Fmt, 4>(plan, planes, X0, Y0, tid, ladder); }
+    if (levels > 5) { __syncthreads(); prepare_level<Fmt, 5>(plan, planes, X0, Y0, tid, ladder); }
 }
 
 // ---- scale ------------------------------------------------------------------
@@ -369,36 +395,34 @@ template <typename Fmt>
 __global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ pairs,
                                                    const WorkRef* __restrict__ work,
                                                    uint32_t blocks, uint32_t items_per_pair) {
-    __shared__ float ring_a[kRing][kSrcCols];
-    __shared__ float ring_b[kRing][kSrcCols];
-    __shared__ float prod[3][kSrcCols];
+    __shared__ __align__(16) float ring_a[kRing][kSrcCols];
+    __shared__ __align__(16) float ring_b[kRing][kSrcCols];
+    __shared__ __align__(16) float prod[3][kSrcCols];
 
     uint32_t pair_idx, item_idx;
     if (work != nullptr) {
-        const WorkRef w = work[blockIdx.x];
-        pair_idx = w.pair;
-        item_idx = w.index;
+        ConstAS<WorkRef>* w = as_const(work) + blockIdx.x;
+        pair_idx = w->pair;
+        item_idx = w->index;
     } else {
         const uint32_t l = xcd_remap(blockIdx.x, blocks);
         pair_idx = l / items_per_pair;
         item_idx = l % items_per_pair;
     }
-    const PairDesc pd = pairs[pair_idx];
-    const Plan& plan = *pd.plan;
-    const ScaleItem item = pd.items[item_idx];
-    const uint32_t level = item.level, channel = item.channel;
-    const uint32_t W = plan.w[level], H = plan.h[level], pitch = plan.pitch[level];
-    const uint32_t x0 = item.x0, y0 = item.y0;
-    const int total = static_cast<int>(plan.strip_h) + 2 * kRadius;
+    ConstAS<PairDesc>* pdp = as_const(pairs) + pair_idx;
+    ConstAS<Plan>* plan = as_const(pdp->plan);
+    ConstAS<ScaleItem>* item = as_const(pdp->items) + item_idx;
+    const uint32_t level = item->level, channel = item->channel;
+    const uint32_t x0 = item->x0, y0 = item->y0;
+    const uint32_t W = plan->w[level], H = plan->h[level], pitch = plan->pitch[level];
+    const int total = static_cast<int>(plan->strip_h) + 2 * kRadius;
 
     using E = typename Fmt::E;
-    const E* pa = static_cast<const E*>(pd.planes) + plan.plane_off[level][channel];
-    const E* pb = pa + plan.image_elems;
     // Source row r (0 .. total-1) is image row y0 - 6 + r, padded row y0 + r;
     // source column j (0 .. 139) is image column x0 - 6 + j, padded x0 + 2 + j.
-    const uint64_t origin = static_cast<uint64_t>(y0) * pitch + x0 + 2u;
-    pa += origin;
-    pb += origin;
+    const uint64_t origin = plan->plane_off[level][channel] + static_cast<uint64_t>(y0) * pitch + x0 + 2u;
+    GlobalAS<const E>* pa = as_global(static_cast<const E*>(pdp->planes)) + origin;
+    GlobalAS<const E>* pb = pa + plan->image_elems;
 
     const int lane = static_cast<int>(threadIdx.x);
     const bool halo = lane < (kSrcCols - kStripCols) / 2;  // lanes 0..5
@@ -513,7 +537,7 @@ __global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ 
         });
     }
 
-    double* out = pd.partials + static_cast<uint64_t>(item_idx) * 6u;
+    GlobalAS<double>* out = as_global(pdp->partials) + static_cast<uint64_t>(item_idx) * 6u;
     for (int k = 0; k < 6; ++k) {
         const double s = wave_sum(sums[k]);
         if (lane == 0) out[k] = s;
@@ -523,26 +547,36 @@ __global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ 
 // ---- reduce -----------------------------------------------------------------
 
 __global__ __launch_bounds__(128) void reduce_kernel(const PairDesc* __restrict__ pairs) {
-    const PairDesc pd = pairs[blockIdx.x];
-    const Plan& plan = *pd.plan;
+    ConstAS<PairDesc>* pdp = as_const(pairs) + blockIdx.x;
+    ConstAS<Plan>* plan = as_const(pdp->plan);
     const uint32_t j = threadIdx.x;
     if (j >= static_cast<uint32_t>(kNorms)) return;
     const uint32_t level = j / 18u, k = j % 18u;
     double value = 0.0;
-    if (level < plan.levels) {
+    if (level < plan->levels) {
         const uint32_t c = k < 6u ? k / 2u : (k - 6u) / 4u;
         const uint32_t slot = k < 6u ? k % 2u : 2u + (k - 6u) % 4u;
-        if (plan.mask[level] & (1u << c)) {
-            const uint32_t begin = plan.item_begin[level][c], count = plan.item_count[level][c];
+        if (plan->mask[level] & (1u << c)) {
+            const uint32_t begin = plan->item_begin[level][c], count = plan->item_count[level][c];
+            GlobalAS<const double>* partials = as_global(static_cast<const double*>(pdp->partials));
             double sum = 0.0;
-            for (uint32_t it = 0; it < count; ++it) sum += pd.partials[(begin + it) * 6u + slot];
+            for (uint32_t it = 0; it < count; ++it) sum += partials[(begin + it) * 6u + slot];
             const double one_per_pixel =
-                1.0 / (static_cast<double>(plan.w[level]) * static_cast<double>(plan.h[level]));
+                1.0 / (static_cast<double>(plan->w[level]) * static_cast<double>(plan->h[level]));
             value = one_per_pixel * sum;
             if (slot & 1u) value = sqrt(sqrt(value));
         }
     }
-    pd.norms[j] = value;
+    as_global(pdp->norms)[j] = value;
+}
+
+// A plain streaming copy (16-byte loads and stores): the reference point that
+// scales the FETCH_SIZE / WRITE_SIZE counters to bytes.
+__global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src,
+                                                   uint4* __restrict__ dst, uint64_t n) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        dst[i] = src[i];
 }
 
 template <typename Fmt>
@@ -560,15 +594,6 @@ hipError_t launch_scale_t(const PairDesc* pairs, const WorkRef* work, uint32_t b
     hipLaunchKernelGGL(scale_kernel<Fmt>, dim3(blocks), dim3(64), 0, stream, pairs, work,
                        blocks, items);
     return hipGetLastError();
-}
-
-// A plain streaming copy (16-byte loads and stores): the reference point that
-// scales the FETCH_SIZE / WRITE_SIZE counters to bytes.
-__global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src,
-                                                   uint4* __restrict__ dst, uint64_t n) {
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
-        dst[i] = src[i];
 }
 
 }  // namespace
