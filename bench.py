@@ -79,6 +79,10 @@ def main() -> int:
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="1080p")
     ap.add_argument("--pairs-per-strength", type=int, default=100)
     ap.add_argument("--planes", choices=sorted(PLANES), default="f32")
+    ap.add_argument("--strip-rows", type=int, default=0,
+                    help="output rows per scale work item of the batch (0 = the library default)")
+    ap.add_argument("--single-strip-rows", type=int, default=16,
+                    help="output rows per scale work item of the single-pair modes (latency)")
     ap.add_argument("--cpu-sample", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-modes", action="store_true")
@@ -127,7 +131,7 @@ def main() -> int:
         src_buf.free()
         dst_buf.free()
 
-    batch = Ssimulacra2Batch(max_pairs=len(refs), plane_format=fmt)
+    batch = Ssimulacra2Batch(max_pairs=len(refs), plane_format=fmt, strip_rows=args.strip_rows)
     for _ in range(args.warmup):
         scales, norms = batch.Finish(batch.Submit(refs), raw=True)
     step_ms, stages = [], []
@@ -190,7 +194,8 @@ def main() -> int:
     if not args.no_modes:
         modes = {}
         # Single pair at a time, resident inputs: per-pair latency.
-        pipe = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(w, h), plane_format=fmt))
+        pipe = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(w, h), plane_format=fmt,
+                                                     strip_rows=args.single_strip_rows))
         from vqm_amd import DeviceImage
         lat = []
         for i in range(min(40, len(refs))):

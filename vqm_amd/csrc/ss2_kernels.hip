@@ -118,12 +118,16 @@ __device__ __forceinline__ void xyb(float r, float g, float b, uint32_t mask, fl
 
 // ---- plane element formats ------------------------------------------------
 
+// A format loads two adjacent elements as Raw bits and unpacks them to FP32
+// only where they are consumed, so that the wait for a prefetched row lands
+// one row later instead of right behind the load.
 struct F32 {
     using E = float;
-    static __device__ __forceinline__ float2 load2(GlobalAS<const E>* p) {
-        const f32x2 v = *reinterpret_cast<GlobalAS<const f32x2>*>(p);
-        return make_float2(v.x, v.y);
+    using Raw = f32x2;
+    static __device__ __forceinline__ Raw load2(GlobalAS<const E>* p) {
+        return *reinterpret_cast<GlobalAS<const f32x2>*>(p);
     }
+    static __device__ __forceinline__ float2 unpack(Raw v) { return make_float2(v.x, v.y); }
     static __device__ __forceinline__ void store1(GlobalAS<E>* p, float v) { *p = v; }
     static __device__ __forceinline__ void store2(GlobalAS<E>* p, float a, float b) {
         *reinterpret_cast<GlobalAS<f32x2>*>(p) = f32x2{a, b};
@@ -163,8 +167,11 @@ __device__ __forceinline__ float from_bf16(uint16_t v) {
 template <uint16_t (*To)(float), float (*From)(uint16_t)>
 struct Narrow {
     using E = uint16_t;
-    static __device__ __forceinline__ float2 load2(GlobalAS<const E>* p) {
-        const uint32_t v = *reinterpret_cast<GlobalAS<const uint32_t>*>(p);
+    using Raw = uint32_t;
+    static __device__ __forceinline__ Raw load2(GlobalAS<const E>* p) {
+        return *reinterpret_cast<GlobalAS<const uint32_t>*>(p);
+    }
+    static __device__ __forceinline__ float2 unpack(Raw v) {
         return make_float2(From(static_cast<uint16_t>(v & 0xFFFFu)),
                            From(static_cast<uint16_t>(v >> 16)));
     }
@@ -428,8 +435,9 @@ __global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ 
     const bool halo = lane < (kSrcCols - kStripCols) / 2;  // lanes 0..5
     const int jm = 2 * lane, jh = kStripCols + 2 * lane;
 
-    float2 cur_a = Fmt::load2(pa + jm), cur_b = Fmt::load2(pb + jm);
-    float2 cur_ha = make_float2(0.0f, 0.0f), cur_hb = make_float2(0.0f, 0.0f);
+    using Raw = typename Fmt::Raw;
+    Raw cur_a = Fmt::load2(pa + jm), cur_b = Fmt::load2(pb + jm);
+    Raw cur_ha{}, cur_hb{};
     if (halo) {
         cur_ha = Fmt::load2(pa + jh);
         cur_hb = Fmt::load2(pb + jh);
@@ -447,17 +455,21 @@ __global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ 
             // Stage input row i: raw samples into the ring, products alongside.
             float* ra = ring_a[i & (kRing - 1)];
             float* rb = ring_b[i & (kRing - 1)];
-            *reinterpret_cast<float2*>(ra + jm) = cur_a;
-            *reinterpret_cast<float2*>(rb + jm) = cur_b;
-            *reinterpret_cast<float2*>(&prod[0][jm]) = make_float2(cur_a.x * cur_a.x, cur_a.y * cur_a.y);
-            *reinterpret_cast<float2*>(&prod[1][jm]) = make_float2(cur_b.x * cur_b.x, cur_b.y * cur_b.y);
-            *reinterpret_cast<float2*>(&prod[2][jm]) = make_float2(cur_a.x * cur_b.x, cur_a.y * cur_b.y);
+            {
+                const float2 a = Fmt::unpack(cur_a), b = Fmt::unpack(cur_b);
+                *reinterpret_cast<float2*>(ra + jm) = a;
+                *reinterpret_cast<float2*>(rb + jm) = b;
+                *reinterpret_cast<float2*>(&prod[0][jm]) = make_float2(a.x * a.x, a.y * a.y);
+                *reinterpret_cast<float2*>(&prod[1][jm]) = make_float2(b.x * b.x, b.y * b.y);
+                *reinterpret_cast<float2*>(&prod[2][jm]) = make_float2(a.x * b.x, a.y * b.y);
+            }
             if (halo) {
-                *reinterpret_cast<float2*>(ra + jh) = cur_ha;
-                *reinterpret_cast<float2*>(rb + jh) = cur_hb;
-                *reinterpret_cast<float2*>(&prod[0][jh]) = make_float2(cur_ha.x * cur_ha.x, cur_ha.y * cur_ha.y);
-                *reinterpret_cast<float2*>(&prod[1][jh]) = make_float2(cur_hb.x * cur_hb.x, cur_hb.y * cur_hb.y);
-                *reinterpret_cast<float2*>(&prod[2][jh]) = make_float2(cur_ha.x * cur_hb.x, cur_ha.y * cur_hb.y);
+                const float2 a = Fmt::unpack(cur_ha), b = Fmt::unpack(cur_hb);
+                *reinterpret_cast<float2*>(ra + jh) = a;
+                *reinterpret_cast<float2*>(rb + jh) = b;
+                *reinterpret_cast<float2*>(&prod[0][jh]) = make_float2(a.x * a.x, a.y * a.y);
+                *reinterpret_cast<float2*>(&prod[1][jh]) = make_float2(b.x * b.x, b.y * b.y);
+                *reinterpret_cast<float2*>(&prod[2][jh]) = make_float2(a.x * b.x, a.y * b.y);
             }
             // Prefetch input row i + 1.
             if (i + 1 < total) {
