@@ -58,6 +58,134 @@ def make_pairs(w, h, cfg, per_strength):
     return pairs
 
 
+class LadderPool:
+    """Distinct seeded pairs of every ladder size and strength, resident on
+    the device and, for host-input requests, in pinned host memory."""
+
+    def __init__(self, cfg, variants, pinned):
+        from vqm_amd import DeviceBuffer, PinnedBuffer, synth
+        self.entries = {}
+        self.keep = []
+        t0 = time.perf_counter()
+        for si, (w, h) in enumerate(synth.LADDER):
+            for sk, strength in enumerate(synth.STRENGTHS):
+                for v in range(variants):
+                    idx = (si * len(synth.STRENGTHS) + sk) * variants + v
+                    a, b = synth.pair(cfg, idx, w, h, strength)
+                    da, db = DeviceBuffer(a.nbytes), DeviceBuffer(b.nbytes)
+                    da.upload(a)
+                    db.upload(b)
+                    entry = {"dev": (da.ptr, db.ptr), "host": None}
+                    self.keep += [da, db]
+                    if pinned:
+                        pa, pb = PinnedBuffer(a.nbytes), PinnedBuffer(b.nbytes)
+                        pa.array[:] = a
+                        pb.array[:] = b
+                        entry["host"] = (pa.ptr, pb.ptr)
+                        self.keep += [pa, pb]
+                    self.entries[(w, h, sk, v)] = entry
+        self.variants = variants
+        log(f"ladder pool: {len(self.entries)} pairs ({variants} per size and strength) "
+            f"in {time.perf_counter() - t0:.1f} s")
+
+    def pick(self, cfg, index, weights):
+        from vqm_amd import synth
+        (w, h), strength = synth.ladder_pair_shape(cfg, index, weights)
+        sk = synth.STRENGTHS.index(strength)
+        v = min(int(synth.draw(cfg, index, 3) * self.variants), self.variants - 1)
+        return w, h, self.entries[(w, h, sk, v)]
+
+
+def serving(args, pool):
+    """Poisson arrivals over the five-size ladder (weights 0.1/0.2/0.35/0.2/
+    0.15) replayed open-loop against the continuous-batching server: the
+    capacity of a burst, then latency at fixed offered rates."""
+    import math
+    from vqm_amd import Ssimulacra2Server, synth
+    n = args.serve_requests
+    picks = [pool.pick(4, i, synth.SERVE_WEIGHTS) for i in range(n)]
+    unit = [-math.log(1.0 - synth.draw(4, i, 2)) for i in range(n)]  # Exp(1) gaps
+    budget = args.serve_budget_gib << 30
+
+    def replay(server, host, qps):
+        t, reqs = 0.0, []
+        for (w, h, e), gap in zip(picks, unit):
+            if qps:
+                t += gap / qps
+            a, b = e["host"] if host else e["dev"]
+            reqs.append((a, b, w, h, host, int(t * 1e9)))
+        out = server.Replay(reqs)
+        lat = np.array([r.latency_ms for r in out])
+        start = min(r.submit_ns for r in out)
+        end = max(r.done_ns for r in out)
+        return {"offered_qps": round(qps, 1) if qps else "burst",
+                "pairs_per_s": round(n / ((end - start) * 1e-9), 1),
+                "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "mean_batch": round(float(np.mean([r.batch_size for r in out])), 2)}
+
+    report = {"requests": n, "hbm_budget_bytes": budget, "lanes": args.serve_lanes,
+              "max_batch": args.serve_max_batch}
+    for mode, host, fractions in (("resident", False, (0.25, 0.5, 0.75, 0.9)),
+                                  ("streaming", True, (0.5, 0.9))):
+        base_capacity = None
+        for planes in ("f32", "bf16"):
+            server = Ssimulacra2Server(plane_format=PLANES[planes], lanes=args.serve_lanes,
+                                       max_batch=args.serve_max_batch, hbm_budget=budget)
+            replay(server, host, 0)  # warm the workspace pools
+            burst = replay(server, host, 0)
+            if base_capacity is None:
+                base_capacity = burst["pairs_per_s"]
+            points = [burst] + [replay(server, host, f * base_capacity) for f in fractions]
+            stats = server.Stats()
+            server.close()
+            report[f"{mode}_{planes}"] = {"capacity_pairs_per_s": burst["pairs_per_s"],
+                                          "points": points,
+                                          "peak_pool_bytes": stats["peak_pool_bytes"]}
+            log(f"serving {mode} {planes}: {points}")
+    return report
+
+
+def ladder_batch(args, pool):
+    """256 pairs drawn uniformly from the ladder in one batched launch per
+    kernel, against the same pairs scored one at a time through one
+    single-pair pipeline per size."""
+    from vqm_amd import DeviceImage, Size, Ssimulacra2Batch, Ssimulacra2Config, Ssimulacra2Pipeline
+    n = 256
+    picks = [pool.pick(3, i, None) for i in range(n)]
+    refs = [(e["dev"][0], e["dev"][1], w, h) for w, h, e in picks]
+    out = {"pairs": n}
+    for planes in ("f32", "bf16"):
+        batch = Ssimulacra2Batch(max_pairs=n, plane_format=PLANES[planes])
+        for _ in range(2):
+            batch.Finish(batch.Submit(refs), raw=True)
+        t0 = time.perf_counter()
+        steps = 5
+        for _ in range(steps):
+            batch.Finish(batch.Submit(refs), raw=True)
+        dt = (time.perf_counter() - t0) / steps
+        stages = batch.StageTimes()
+        batch.close()
+        out[f"batched_{planes}"] = {"pairs_per_s": round(n / dt, 1), "ms_per_launch": round(dt * 1e3, 3),
+                                    "stage_ms": [round(s, 3) for s in stages]}
+    pipes = {}
+    for planes in ("f32",):
+        for w, h, _ in picks:
+            if (w, h) not in pipes:
+                pipes[(w, h)] = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(w, h), plane_format=PLANES[planes]))
+        for w, h, e in picks[:10]:
+            pipes[(w, h)].Compute(DeviceImage(e["dev"][0], Size(w, h)), DeviceImage(e["dev"][1], Size(w, h)))
+        t0 = time.perf_counter()
+        for w, h, e in picks:
+            pipes[(w, h)].Compute(DeviceImage(e["dev"][0], Size(w, h)), DeviceImage(e["dev"][1], Size(w, h)))
+        dt = time.perf_counter() - t0
+        out[f"single_pair_loop_{planes}"] = {"pairs_per_s": round(n / dt, 1)}
+        for p in pipes.values():
+            p.close()
+    log(f"ladder batch: {out}")
+    return out
+
+
 def cpu_baseline(w, h, pairs, sample):
     import oracle
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -81,8 +209,16 @@ def main() -> int:
     ap.add_argument("--planes", choices=sorted(PLANES), default="f32")
     ap.add_argument("--strip-rows", type=int, default=0,
                     help="output rows per scale work item of the batch (0 = the library default)")
-    ap.add_argument("--single-strip-rows", type=int, default=16,
-                    help="output rows per scale work item of the single-pair modes (latency)")
+    ap.add_argument("--single-strip-rows", type=int, default=0,
+                    help="output rows per scale work item of the single-pair modes (0 = the library default)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--no-ladder", action="store_true", help="skip the uneven-size batch workload")
+    ap.add_argument("--serve-requests", type=int, default=10000)
+    ap.add_argument("--serve-lanes", type=int, default=3)
+    ap.add_argument("--serve-max-batch", type=int, default=64)
+    ap.add_argument("--serve-budget-gib", type=int, default=8)
+    ap.add_argument("--ladder-variants", type=int, default=2,
+                    help="distinct pairs per ladder size and strength")
     ap.add_argument("--cpu-sample", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-modes", action="store_true")
@@ -222,6 +358,14 @@ def main() -> int:
         pipe.close()
         out["modes"] = modes
         log(f"modes: {modes}")
+
+    if not (args.no_serve and args.no_ladder):
+        pool = LadderPool(3, args.ladder_variants, pinned=not args.no_serve)
+        if not args.no_ladder:
+            out["ladder_batch"] = ladder_batch(args, pool)
+        if not args.no_serve:
+            out["serving"] = serving(args, pool)
+        del pool
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, h, pairs, args.cpu_sample)

@@ -270,6 +270,107 @@ int vqm_ss2_batch_stage_times(const vqm_ss2_batch* batch, float ms[3]);
 /* Device bytes held by the batch's workspace pool. */
 int vqm_ss2_batch_pool_bytes(const vqm_ss2_batch* batch, uint64_t* bytes);
 
+/* ---- serving (no counterpart in the reference: its closest caller is the
+ * worker pool of Ssimulacra2Pool, vqm_pool.h:35-86, pool.cc:24-116, and
+ * the bench driver's workers, bench/driver/ssimulacra2.cc:175-285) --------
+ *
+ * A request server with continuous batching. Requests of any geometry
+ * (>= 8x8) are queued as they arrive. A dispatcher thread forms a batch from
+ * the queue head whenever one of `lanes` HIP streams is free, admitting
+ * requests in arrival order while their workspaces fit `hbm_budget`, and
+ * launches it; batches on different lanes overlap on the device. A
+ * completion thread hands results back as batches finish and returns their
+ * workspaces to the per-geometry pools. Each request's result does not
+ * depend on what it was batched with. All entry points are thread-safe. */
+
+typedef struct vqm_ss2_server vqm_ss2_server;
+
+typedef struct vqm_ss2_server_config {
+    uint32_t plane_format;   /* VQM_PLANES_* */
+    uint32_t lanes;          /* batches in flight at once, 1..16; 0 = 3 */
+    uint32_t max_batch;      /* requests per launch, 1..4096; 0 = 64 */
+    uint32_t strip_rows;     /* 0 = default */
+    uint64_t hbm_budget;     /* device bytes for workspaces and host-input
+                                staging; 0 = unlimited */
+} vqm_ss2_server_config;
+
+/* The outcome of one request. Times are vqm_now_ns() readings. */
+typedef struct vqm_ss2_request_result {
+    int32_t status;          /* VQM_OK or the request's error */
+    uint32_t scales;
+    uint32_t batch_size;     /* requests in the launch that scored it */
+    uint32_t lane;
+    uint64_t submit_ns;      /* accepted by vqm_ss2_server_submit_* */
+    uint64_t launch_ns;      /* its batch was queued on the device */
+    uint64_t done_ns;        /* its batch's completion was observed */
+    double norms[108];
+} vqm_ss2_request_result;
+
+typedef struct vqm_ss2_server_stats {
+    uint64_t submitted;
+    uint64_t completed;      /* including failed */
+    uint64_t failed;
+    uint64_t batches;
+    uint64_t batched_requests; /* sum of batch sizes */
+    uint64_t max_batch;
+    uint64_t pool_bytes;     /* workspace bytes held now */
+    uint64_t peak_pool_bytes;
+    uint32_t max_lanes_busy; /* most batches in flight at once */
+    uint32_t queued;         /* requests waiting now */
+} vqm_ss2_server_stats;
+
+void vqm_ss2_server_config_init(vqm_ss2_server_config* config);
+/* INVALID_ARGUMENT for a bad config; RUNTIME_ERROR without a device. */
+int vqm_ss2_server_create(const vqm_ss2_server_config* config, vqm_ss2_server** out);
+/* Scores every queued request, then stops; results not collected are lost. */
+void vqm_ss2_server_destroy(vqm_ss2_server* server);
+
+/* Queues a pair of caller-owned device images; they must stay alive and
+ * unmodified until the request completes. INVALID_ARGUMENT below 8x8 or for
+ * null data. */
+int vqm_ss2_server_submit_device(vqm_ss2_server* server, const vqm_ss2_device_pair* pair,
+                                 uint64_t* ticket);
+
+/* Queues a pair of host images (packed u8 sRGB RGB); they are copied to the
+ * device when the request's batch launches, so they must stay readable until
+ * the request completes. Pinned memory (vqm_host_malloc) makes the copy
+ * asynchronous. */
+int vqm_ss2_server_submit_host(vqm_ss2_server* server, const vqm_image_desc* original,
+                               const void* original_data, const vqm_image_desc* distorted,
+                               const void* distorted_data, uint64_t* ticket);
+
+/* Blocks until the request completes and consumes its ticket. Returns the
+ * request's own status (INVALID_ARGUMENT when it cannot fit the budget even
+ * alone; RUNTIME_ERROR on a device failure), or INVALID_ARGUMENT for an
+ * unknown ticket. `result` is filled in either case when the ticket is
+ * known. */
+int vqm_ss2_server_wait(vqm_ss2_server* server, uint64_t ticket, vqm_ss2_request_result* result);
+
+int vqm_ss2_server_get_stats(vqm_ss2_server* server, vqm_ss2_server_stats* stats);
+
+/* The clock of the request times (steady, nanoseconds). */
+uint64_t vqm_now_ns(void);
+
+/* One request of an open-loop replay: device or host images. */
+typedef struct vqm_ss2_replay_request {
+    const void* original;
+    const void* distorted;
+    uint32_t width;
+    uint32_t height;
+    uint32_t host;           /* 1: host images (vqm_ss2_server_submit_host) */
+    uint32_t reserved;
+    uint64_t arrival_ns;     /* offset from the start of the replay */
+} vqm_ss2_replay_request;
+
+/* Submits `count` requests at their arrival offsets (open loop: a request is
+ * submitted at its arrival time whether or not earlier ones are done) from
+ * the calling thread, collects them on a second thread, and writes one
+ * result per request, with submit_ns replaced by the scheduled arrival time
+ * so that done_ns - submit_ns is the latency the request saw. Returns the
+ * first failing request's status, else VQM_OK. */
+int vqm_ss2_server_replay(vqm_ss2_server* server, const vqm_ss2_replay_request* requests,
+                          uint32_t count, vqm_ss2_request_result* results);
+
 /* ---- device plumbing for callers without a HIP toolchain ---------------- */
 
 int vqm_device_count(int* count);
@@ -278,6 +379,9 @@ int vqm_device_info(char name[256], char arch[64], int* cus, int* lds_per_block,
                     uint64_t* hbm_bytes);
 int vqm_device_malloc(uint64_t bytes, void** ptr);
 int vqm_device_free(void* ptr);
+/* Pinned (page-locked) host memory, for host inputs copied asynchronously. */
+int vqm_host_malloc(uint64_t bytes, void** ptr);
+int vqm_host_free(void* ptr);
 int vqm_memcpy_htod(void* dst, const void* src, uint64_t bytes);
 int vqm_memcpy_dtoh(void* dst, const void* src, uint64_t bytes);
 int vqm_device_synchronize(void);

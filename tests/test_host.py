@@ -150,3 +150,15 @@ def test_last_error_is_thread_local_and_cleared():
     assert b"8x8" in lib.vqm_last_error()
     assert lib.vqm_ss2_plan_scales(8, 8, sizes, ctypes.byref(count)) == L.VQM_OK
     assert lib.vqm_last_error() == b""
+
+
+def test_server_config_validation_before_device_work():
+    """The server rejects a bad configuration before touching the device."""
+    for kwargs in ({"lanes": 17}, {"max_batch": 5000}, {"plane_format": 3}, {"strip_rows": 6}):
+        with pytest.raises(InvalidArgument):
+            vqm_amd.Ssimulacra2Server(**kwargs)
+    cfg = L.ServerConfigC()
+    L.lib().vqm_ss2_server_config_init(ctypes.byref(cfg))
+    assert (cfg.plane_format, cfg.lanes, cfg.max_batch, cfg.strip_rows, cfg.hbm_budget) == (0, 3, 64, 0, 0)
+    t0 = vqm_amd.now_ns()
+    assert vqm_amd.now_ns() >= t0 > 0

@@ -66,6 +66,27 @@ class DevicePairC(ctypes.Structure):
     _fields_ = [("original", vp), ("distorted", vp), ("width", u32), ("height", u32)]
 
 
+class ServerConfigC(ctypes.Structure):
+    _fields_ = [("plane_format", u32), ("lanes", u32), ("max_batch", u32), ("strip_rows", u32),
+                ("hbm_budget", u64)]
+
+
+class RequestResultC(ctypes.Structure):
+    _fields_ = [("status", i32), ("scales", u32), ("batch_size", u32), ("lane", u32),
+                ("submit_ns", u64), ("launch_ns", u64), ("done_ns", u64), ("norms", f64 * 108)]
+
+
+class ServerStatsC(ctypes.Structure):
+    _fields_ = [("submitted", u64), ("completed", u64), ("failed", u64), ("batches", u64),
+                ("batched_requests", u64), ("max_batch", u64), ("pool_bytes", u64),
+                ("peak_pool_bytes", u64), ("max_lanes_busy", u32), ("queued", u32)]
+
+
+class ReplayRequestC(ctypes.Structure):
+    _fields_ = [("original", vp), ("distorted", vp), ("width", u32), ("height", u32),
+                ("host", u32), ("reserved", u32), ("arrival_ns", u64)]
+
+
 # Every function include/vqm_ss2.h declares: name -> (restype, argtypes).
 SIGNATURES = {
     "vqm_last_error": (ctypes.c_char_p, []),
@@ -108,12 +129,27 @@ SIGNATURES = {
     "vqm_ss2_batch_finish": (ctypes.c_int, [vp, u64, ctypes.POINTER(u32), ctypes.POINTER(f64)]),
     "vqm_ss2_batch_stage_times": (ctypes.c_int, [vp, ctypes.POINTER(f32)]),
     "vqm_ss2_batch_pool_bytes": (ctypes.c_int, [vp, ctypes.POINTER(u64)]),
+    "vqm_ss2_server_config_init": (None, [ctypes.POINTER(ServerConfigC)]),
+    "vqm_ss2_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfigC), ctypes.POINTER(vp)]),
+    "vqm_ss2_server_destroy": (None, [vp]),
+    "vqm_ss2_server_submit_device": (ctypes.c_int, [vp, ctypes.POINTER(DevicePairC),
+                                                    ctypes.POINTER(u64)]),
+    "vqm_ss2_server_submit_host": (ctypes.c_int, [vp, ctypes.POINTER(ImageDescC), vp,
+                                                  ctypes.POINTER(ImageDescC), vp,
+                                                  ctypes.POINTER(u64)]),
+    "vqm_ss2_server_wait": (ctypes.c_int, [vp, u64, ctypes.POINTER(RequestResultC)]),
+    "vqm_ss2_server_get_stats": (ctypes.c_int, [vp, ctypes.POINTER(ServerStatsC)]),
+    "vqm_now_ns": (u64, []),
+    "vqm_ss2_server_replay": (ctypes.c_int, [vp, ctypes.POINTER(ReplayRequestC), u32,
+                                             ctypes.POINTER(RequestResultC)]),
     "vqm_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
     "vqm_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p,
                                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(u64)]),
     "vqm_device_malloc": (ctypes.c_int, [u64, ctypes.POINTER(vp)]),
     "vqm_device_free": (ctypes.c_int, [vp]),
+    "vqm_host_malloc": (ctypes.c_int, [u64, ctypes.POINTER(vp)]),
+    "vqm_host_free": (ctypes.c_int, [vp]),
     "vqm_memcpy_htod": (ctypes.c_int, [vp, vp, u64]),
     "vqm_memcpy_dtoh": (ctypes.c_int, [vp, vp, u64]),
     "vqm_device_synchronize": (ctypes.c_int, []),

@@ -8,6 +8,7 @@
 
 #include "ss2_host.h"
 #include "ss2_runtime.h"
+#include "ss2_server.h"
 #include "vqm_ss2.h"
 
 struct vqm_ss2_pipeline {
@@ -18,6 +19,11 @@ struct vqm_ss2_pipeline {
 struct vqm_ss2_batch {
     vqm::ss2::Batch impl;
     explicit vqm_ss2_batch(const vqm_ss2_batch_config& c) : impl(c) {}
+};
+
+struct vqm_ss2_server {
+    vqm::ss2::Server impl;
+    explicit vqm_ss2_server(const vqm_ss2_server_config& c) : impl(c) {}
 };
 
 namespace {
@@ -335,6 +341,82 @@ int vqm_ss2_batch_pool_bytes(const vqm_ss2_batch* batch, uint64_t* bytes) {
     });
 }
 
+void vqm_ss2_server_config_init(vqm_ss2_server_config* config) {
+    if (config == nullptr) return;
+    std::memset(config, 0, sizeof(*config));
+    config->plane_format = VQM_PLANES_F32;
+    config->lanes = 3;
+    config->max_batch = 64;
+}
+
+int vqm_ss2_server_create(const vqm_ss2_server_config* config, vqm_ss2_server** out) {
+    return Guard([&] {
+        Need(config, "config");
+        Need(out, "out");
+        *out = nullptr;
+        *out = new vqm_ss2_server(*config);
+    });
+}
+
+void vqm_ss2_server_destroy(vqm_ss2_server* server) { delete server; }
+
+int vqm_ss2_server_submit_device(vqm_ss2_server* server, const vqm_ss2_device_pair* pair,
+                                 uint64_t* ticket) {
+    return Guard([&] {
+        Need(server, "server");
+        Need(pair, "pair");
+        Need(ticket, "ticket");
+        *ticket = server->impl.SubmitDevice(*pair);
+    });
+}
+
+int vqm_ss2_server_submit_host(vqm_ss2_server* server, const vqm_image_desc* original,
+                               const void* original_data, const vqm_image_desc* distorted,
+                               const void* distorted_data, uint64_t* ticket) {
+    return Guard([&] {
+        Need(server, "server");
+        Need(original, "original");
+        Need(distorted, "distorted");
+        Need(ticket, "ticket");
+        *ticket = server->impl.SubmitHost(*original, original_data, *distorted, distorted_data);
+    });
+}
+
+int vqm_ss2_server_wait(vqm_ss2_server* server, uint64_t ticket, vqm_ss2_request_result* result) {
+    std::string message;
+    int status = VQM_OK;
+    const int call = Guard([&] {
+        Need(server, "server");
+        status = server->impl.Wait(ticket, result, &message);
+    });
+    if (call != VQM_OK) return call;
+    return status == VQM_OK ? VQM_OK : SetError(status, message);
+}
+
+int vqm_ss2_server_get_stats(vqm_ss2_server* server, vqm_ss2_server_stats* stats) {
+    return Guard([&] {
+        Need(server, "server");
+        Need(stats, "stats");
+        *stats = server->impl.Stats();
+    });
+}
+
+uint64_t vqm_now_ns(void) { return vqm::ss2::NowNs(); }
+
+int vqm_ss2_server_replay(vqm_ss2_server* server, const vqm_ss2_replay_request* requests,
+                          uint32_t count, vqm_ss2_request_result* results) {
+    std::string message;
+    int status = VQM_OK;
+    const int call = Guard([&] {
+        Need(server, "server");
+        Need(requests, "requests");
+        Need(results, "results");
+        status = vqm::ss2::Replay(server->impl, requests, count, results, &message);
+    });
+    if (call != VQM_OK) return call;
+    return status == VQM_OK ? VQM_OK : SetError(status, message);
+}
+
 int vqm_device_count(int* count) {
     return Guard([&] {
         Need(count, "count");
@@ -371,6 +453,18 @@ int vqm_device_malloc(uint64_t bytes, void** ptr) {
 
 int vqm_device_free(void* ptr) {
     return Guard([&] { vqm::ss2::Check(hipFree(ptr), "hipFree"); });
+}
+
+int vqm_host_malloc(uint64_t bytes, void** ptr) {
+    return Guard([&] {
+        Need(ptr, "ptr");
+        *ptr = nullptr;
+        vqm::ss2::Check(hipHostMalloc(ptr, bytes, hipHostMallocDefault), "hipHostMalloc");
+    });
+}
+
+int vqm_host_free(void* ptr) {
+    return Guard([&] { vqm::ss2::Check(hipHostFree(ptr), "hipHostFree"); });
 }
 
 int vqm_memcpy_htod(void* dst, const void* src, uint64_t bytes) {

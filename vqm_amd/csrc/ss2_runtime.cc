@@ -20,6 +20,32 @@ uint64_t NextId() {
     return ++counter;
 }
 
+vqm_image_format Srgb8Format() {
+    vqm_image_format f{};
+    f.type = 0;
+    f.family = 1;
+    f.alpha = 0;
+    f.range = 1;
+    f.matrix = 0;
+    f.transfer = 13;
+    f.primaries = 1;
+    return f;
+}
+
+PlaneFormat ToFormat(uint32_t f) {
+    if (f > 2) Fail(VQM_INVALID_ARGUMENT, "plane_format must be VQM_PLANES_F32, _F16 or _BF16");
+    return static_cast<PlaneFormat>(f);
+}
+
+float Elapsed(const Event& a, const Event& b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a.get(), b.get()) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.0f;
+    }
+    return ms;
+}
+
 // ---- buffers --------------------------------------------------------------
 
 DeviceBuffer::DeviceBuffer(uint64_t bytes) : bytes_(bytes) {
@@ -120,7 +146,12 @@ DevicePlan::DevicePlan(uint32_t width, uint32_t height, PlaneFormat format, uint
               "upload work items");
 }
 
-Workspace::Workspace(const WorkspaceLayout& layout) : memory(layout.total) {
+namespace {
+uint64_t InputArea(uint64_t input_image_bytes) { return 2 * ((input_image_bytes + 255) / 256 * 256); }
+}  // namespace
+
+Workspace::Workspace(const WorkspaceLayout& layout, uint64_t input_image_bytes)
+    : memory(layout.total + InputArea(input_image_bytes)) {
     // The plane borders must read as zero for every later pair: they are the
     // blur's zero padding and nothing writes them.
     Check(hipMemset(memory.get(), 0, layout.total), "clear workspace");
@@ -131,40 +162,183 @@ Workspace::Workspace(const WorkspaceLayout& layout) : memory(layout.total) {
     planes = base;
     partials = reinterpret_cast<double*>(base + layout.planes);
     norms = reinterpret_cast<double*>(base + layout.planes + layout.partials);
+    if (input_image_bytes > 0) inputs = base + layout.total;
+}
+
+// ---- workspace pools ------------------------------------------------------------
+
+WorkspacePools::WorkspacePools(PlaneFormat format, uint32_t strip_rows, uint64_t budget)
+    : format_(format), strip_rows_(strip_rows), budget_(budget) {}
+
+const DevicePlan& WorkspacePools::PlanFor(uint32_t width, uint32_t height) {
+    auto& slot = plans_[std::make_pair(width, height)];
+    if (!slot) {
+        try {
+            slot = std::make_unique<DevicePlan>(width, height, format_, strip_rows_);
+        } catch (...) {
+            plans_.erase(std::make_pair(width, height));
+            throw;
+        }
+    }
+    return *slot;
+}
+
+uint64_t WorkspacePools::Need(uint32_t width, uint32_t height, bool inputs) {
+    const DevicePlan& plan = PlanFor(width, height);
+    uint64_t need = plan.layout.total;
+    if (inputs) need += InputArea(RequiredBytes(Srgb8Format(), width, height));
+    return need;
+}
+
+std::unique_ptr<Workspace> WorkspacePools::Acquire(uint32_t width, uint32_t height, bool inputs) {
+    Pool& pool = pools_[Key{width, height, inputs}];
+    if (!pool.idle.empty()) {
+        std::unique_ptr<Workspace> ws = std::move(pool.idle.back());
+        pool.idle.pop_back();
+        return ws;
+    }
+    const DevicePlan& plan = PlanFor(width, height);
+    const uint64_t need = Need(width, height, inputs);
+    if (budget_ != 0 && bytes_ + need > budget_) {
+        // Free idle workspaces of other geometries, largest first.
+        std::vector<std::pair<uint64_t, Pool*>> idle;
+        for (auto& [key, p] : pools_)
+            if (!p.idle.empty()) idle.emplace_back(p.each, &p);
+        std::sort(idle.begin(), idle.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+        for (auto& [each, p] : idle) {
+            while (!p->idle.empty() && bytes_ + need > budget_) {
+                p->idle.pop_back();
+                bytes_ -= each;
+            }
+        }
+        if (bytes_ + need > budget_) return nullptr;
+    }
+    auto ws = std::make_unique<Workspace>(
+        plan.layout, inputs ? RequiredBytes(Srgb8Format(), width, height) : 0);
+    pool.each = need;
+    bytes_ += need;
+    peak_ = std::max(peak_, bytes_);
+    return ws;
+}
+
+void WorkspacePools::Return(uint32_t width, uint32_t height, bool inputs, std::unique_ptr<Workspace> ws) {
+    if (ws) pools_[Key{width, height, inputs}].idle.push_back(std::move(ws));
+}
+
+// ---- lane -----------------------------------------------------------------------
+
+Lane::Lane(uint32_t max_pairs)
+    : max_pairs_(max_pairs),
+      pairs_(max_pairs * sizeof(PairDesc)),
+      norms_(static_cast<uint64_t>(max_pairs) * kNorms * sizeof(double)),
+      h_pairs_(max_pairs * sizeof(PairDesc)),
+      h_norms_(static_cast<uint64_t>(max_pairs) * kNorms * sizeof(double)) {}
+
+Lane::~Lane() {
+    if (pending_) (void)hipEventSynchronize(ev_[4].get());
+}
+
+void Lane::Launch(PlaneFormat fmt, const float* lut, const LaunchEntry* entries, uint32_t count) {
+    if (pending_) Fail(VQM_LOGIC_ERROR, "lane: the previous launch has not finished");
+    if (count == 0 || count > max_pairs_) Fail(VQM_INVALID_ARGUMENT, "lane: bad pair count");
+    auto* hp = h_pairs_.as<PairDesc>();
+    bool uniform = true;
+    uint64_t prep_work = 0, scale_work = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const LaunchEntry& e = entries[i];
+        PairDesc& pd = hp[i];
+        for (int k = 0; k < 2; ++k)
+            pd.src[k] = e.host[k] ? e.ws->inputs + k * ((e.image_bytes + 255) / 256 * 256) : e.src[k];
+        pd.planes = e.ws->planes;
+        pd.partials = e.ws->partials;
+        pd.norms = norms_.as<double>() + static_cast<uint64_t>(i) * kNorms;
+        pd.plan = e.plan->plan.as<Plan>();
+        pd.items = e.plan->items.as<ScaleItem>();
+        uniform = uniform && e.plan == entries[0].plan;
+        prep_work += 2ull * e.plan->host.prep_tiles;
+        scale_work += e.plan->host.plan.items;
+    }
+    if (prep_work > 0x7FFFFFFFull || scale_work > 0x7FFFFFFFull)
+        Fail(VQM_INVALID_ARGUMENT, "the submission has too many work items");
+
+    const hipStream_t s = stream_.get();
+    for (uint32_t i = 0; i < count; ++i)
+        for (int k = 0; k < 2; ++k)
+            if (entries[i].host[k])
+                Check(hipMemcpyAsync(const_cast<uint8_t*>(hp[i].src[k]), entries[i].host[k],
+                                     entries[i].image_bytes, hipMemcpyHostToDevice, s),
+                      "upload input image");
+    Check(hipMemcpyAsync(pairs_.get(), hp, count * sizeof(PairDesc), hipMemcpyHostToDevice, s),
+          "upload pair descriptors");
+    const WorkRef* wprep = nullptr;
+    const WorkRef* wscale = nullptr;
+    if (!uniform) {
+        if (h_work_prep_.bytes() < prep_work * sizeof(WorkRef)) {
+            h_work_prep_ = PinnedBuffer(prep_work * sizeof(WorkRef));
+            work_prep_ = DeviceBuffer(prep_work * sizeof(WorkRef));
+        }
+        if (h_work_scale_.bytes() < scale_work * sizeof(WorkRef)) {
+            h_work_scale_ = PinnedBuffer(scale_work * sizeof(WorkRef));
+            work_scale_ = DeviceBuffer(scale_work * sizeof(WorkRef));
+        }
+        auto* wp = h_work_prep_.as<WorkRef>();
+        auto* ws = h_work_scale_.as<WorkRef>();
+        uint64_t np = 0, ns = 0;
+        for (uint32_t i = 0; i < count; ++i) {
+            const GeometryPlan& g = entries[i].plan->host;
+            for (uint32_t image = 0; image < 2; ++image)
+                for (uint32_t t = 0; t < g.prep_tiles; ++t) wp[np++] = WorkRef{i, t | (image << 31)};
+            for (uint32_t t = 0; t < g.plan.items; ++t) ws[ns++] = WorkRef{i, t};
+        }
+        Check(hipMemcpyAsync(work_prep_.get(), wp, np * sizeof(WorkRef), hipMemcpyHostToDevice, s),
+              "upload prepare work");
+        Check(hipMemcpyAsync(work_scale_.get(), ws, ns * sizeof(WorkRef), hipMemcpyHostToDevice, s),
+              "upload scale work");
+        wprep = work_prep_.as<WorkRef>();
+        wscale = work_scale_.as<WorkRef>();
+    }
+    const GeometryPlan& g0 = entries[0].plan->host;
+    const PairDesc* dpairs = pairs_.as<PairDesc>();
+    Check(hipEventRecord(ev_[0].get(), s), "hipEventRecord");
+    Check(LaunchPrepare(fmt, dpairs, wprep, static_cast<uint32_t>(prep_work), g0.prep_tiles, lut,
+                        nullptr, nullptr, s),
+          "launch prepare");
+    Check(hipEventRecord(ev_[1].get(), s), "hipEventRecord");
+    Check(LaunchScale(fmt, dpairs, wscale, static_cast<uint32_t>(scale_work), g0.plan.items, s),
+          "launch scale");
+    Check(hipEventRecord(ev_[2].get(), s), "hipEventRecord");
+    Check(LaunchReduce(dpairs, count, s), "launch reduce");
+    Check(hipEventRecord(ev_[3].get(), s), "hipEventRecord");
+    Check(hipMemcpyAsync(h_norms_.get(), norms_.get(), static_cast<uint64_t>(count) * kNorms * sizeof(double),
+                         hipMemcpyDeviceToHost, s),
+          "read back norms");
+    Check(hipEventRecord(ev_[4].get(), s), "hipEventRecord");
+    pending_ = true;
+}
+
+bool Lane::Done() const {
+    if (!pending_) return true;
+    const hipError_t e = hipEventQuery(ev_[4].get());
+    if (e == hipErrorNotReady) return false;
+    Check(e, "query lane");
+    return true;
+}
+
+void Lane::Wait() {
+    if (!pending_) return;
+    pending_ = false;
+    Check(hipEventSynchronize(ev_[4].get()), "wait for launch");
+}
+
+std::array<float, 3> Lane::StageTimes() const {
+    return {Elapsed(ev_[0], ev_[1]), Elapsed(ev_[1], ev_[2]), Elapsed(ev_[2], ev_[3])};
 }
 
 // ---- pipeline -----------------------------------------------------------------
 
 namespace {
-
-vqm_image_format Srgb8() {
-    vqm_image_format f{};
-    f.type = 0;
-    f.family = 1;
-    f.alpha = 0;
-    f.range = 1;
-    f.matrix = 0;
-    f.transfer = 13;
-    f.primaries = 1;
-    return f;
-}
-
-PlaneFormat ToFormat(uint32_t f) {
-    if (f > 2) Fail(VQM_INVALID_ARGUMENT, "plane_format must be VQM_PLANES_F32, _F16 or _BF16");
-    return static_cast<PlaneFormat>(f);
-}
-
 constexpr uint32_t kMaxResident = 4096;
-
-float Elapsed(const Event& a, const Event& b) {
-    float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, a.get(), b.get()) != hipSuccess) {
-        (void)hipGetLastError();
-        return 0.0f;
-    }
-    return ms;
-}
-
+vqm_image_format Srgb8() { return Srgb8Format(); }
 }  // namespace
 
 // Geometry and option checks run before any device work, so an impossible
@@ -393,136 +567,53 @@ vqm_ss2_memory Pipeline::Memory() const {
 
 // ---- batch --------------------------------------------------------------------
 
-Batch::Batch(const vqm_ss2_batch_config& config) : config_(config), ctx_(DeviceContext::Get()) {
+namespace {
+const vqm_ss2_batch_config& ValidatedBatch(const vqm_ss2_batch_config& config) {
     ToFormat(config.plane_format);
-    if (config_.max_pairs == 0) Fail(VQM_INVALID_ARGUMENT, "max_pairs must be at least 1");
-    pairs_ = DeviceBuffer(config_.max_pairs * sizeof(PairDesc));
-    h_pairs_ = PinnedBuffer(config_.max_pairs * sizeof(PairDesc));
-    norms_ = DeviceBuffer(static_cast<uint64_t>(config_.max_pairs) * kNorms * sizeof(double));
-    h_norms_ = PinnedBuffer(static_cast<uint64_t>(config_.max_pairs) * kNorms * sizeof(double));
+    if (config.max_pairs == 0) Fail(VQM_INVALID_ARGUMENT, "max_pairs must be at least 1");
+    return config;
 }
+}  // namespace
 
-Batch::~Batch() {
-    if (in_flight_ != 0) (void)hipEventSynchronize(ev_[4].get());
-}
+Batch::Batch(const vqm_ss2_batch_config& config)
+    : config_(ValidatedBatch(config)),
+      ctx_(DeviceContext::Get()),
+      pools_(static_cast<PlaneFormat>(config.plane_format), config.strip_rows, config.hbm_budget),
+      lane_(config.max_pairs) {}
 
-Batch::Pool& Batch::PoolFor(uint32_t width, uint32_t height) {
-    auto key = std::make_pair(width, height);
-    auto it = pools_.find(key);
-    if (it != pools_.end()) return it->second;
-    Pool pool;
-    pool.plan = std::make_unique<DevicePlan>(width, height,
-                                             static_cast<PlaneFormat>(config_.plane_format),
-                                             config_.strip_rows);
-    return pools_.emplace(key, std::move(pool)).first->second;
-}
+Batch::~Batch() = default;
 
 uint64_t Batch::Submit(const vqm_ss2_device_pair* pairs, uint32_t count) {
     if (in_flight_ != 0) Fail(VQM_LOGIC_ERROR, "a batch is in flight: call Finish first");
     if (count == 0 || count > config_.max_pairs)
         Fail(VQM_INVALID_ARGUMENT, "pair count must be between 1 and max_pairs");
-    // Plans first (validates every geometry before anything is acquired).
-    std::vector<Pool*> pool_of(count);
+    // Plans first: every geometry is validated before anything is acquired.
     for (uint32_t i = 0; i < count; ++i) {
         if (pairs[i].original == nullptr || pairs[i].distorted == nullptr)
             Fail(VQM_INVALID_ARGUMENT, "pair " + std::to_string(i) + ": image data is null");
-        pool_of[i] = &PoolFor(pairs[i].width, pairs[i].height);
+        pools_.PlanFor(pairs[i].width, pairs[i].height);
     }
-    // Workspaces from the per-geometry pools, growing within the budget.
-    std::vector<std::pair<Pool*, std::unique_ptr<Workspace>>> taken;
+    std::vector<Taken> taken;
+    std::vector<LaunchEntry> entries(count);
     try {
         for (uint32_t i = 0; i < count; ++i) {
-            Pool& p = *pool_of[i];
-            std::unique_ptr<Workspace> ws;
-            if (!p.free.empty()) {
-                ws = std::move(p.free.back());
-                p.free.pop_back();
-            } else {
-                const uint64_t need = p.plan->layout.total;
-                if (config_.hbm_budget != 0 && pool_bytes_ + need > config_.hbm_budget)
-                    Fail(VQM_INVALID_ARGUMENT, "the submission does not fit the HBM budget");
-                ws = std::make_unique<Workspace>(p.plan->layout);
-                pool_bytes_ += need;
-            }
-            taken.emplace_back(&p, std::move(ws));
+            auto ws = pools_.Acquire(pairs[i].width, pairs[i].height, false);
+            if (!ws) Fail(VQM_INVALID_ARGUMENT, "the submission does not fit the HBM budget");
+            LaunchEntry& e = entries[i];
+            e.plan = &pools_.PlanFor(pairs[i].width, pairs[i].height);
+            e.ws = ws.get();
+            e.src[0] = static_cast<const uint8_t*>(pairs[i].original);
+            e.src[1] = static_cast<const uint8_t*>(pairs[i].distorted);
+            taken.push_back(Taken{pairs[i].width, pairs[i].height, std::move(ws)});
         }
+        lane_.Launch(static_cast<PlaneFormat>(config_.plane_format), ctx_.lut, entries.data(), count);
     } catch (...) {
-        for (auto& [p, ws] : taken) p->free.push_back(std::move(ws));
+        for (Taken& t : taken) pools_.Return(t.width, t.height, false, std::move(t.ws));
         throw;
     }
-
-    auto* hp = h_pairs_.as<PairDesc>();
-    bool uniform = true;
-    uint64_t prep_work = 0, scale_work = 0;
-    levels_.assign(count, 0);
-    for (uint32_t i = 0; i < count; ++i) {
-        const DevicePlan& dp = *taken[i].first->plan;
-        const Workspace& ws = *taken[i].second;
-        PairDesc& pd = hp[i];
-        pd.src[0] = static_cast<const uint8_t*>(pairs[i].original);
-        pd.src[1] = static_cast<const uint8_t*>(pairs[i].distorted);
-        pd.planes = ws.planes;
-        pd.partials = ws.partials;
-        pd.norms = norms_.as<double>() + static_cast<uint64_t>(i) * kNorms;
-        pd.plan = dp.plan.as<Plan>();
-        pd.items = dp.items.as<ScaleItem>();
-        uniform = uniform && taken[i].first == taken[0].first;
-        prep_work += 2ull * dp.host.prep_tiles;
-        scale_work += dp.host.plan.items;
-        levels_[i] = dp.host.plan.levels;
-    }
-    if (prep_work > 0x7FFFFFFFull || scale_work > 0x7FFFFFFFull)
-        Fail(VQM_INVALID_ARGUMENT, "the submission has too many work items");
-
-    const hipStream_t s = stream_.get();
-    Check(hipMemcpyAsync(pairs_.get(), hp, count * sizeof(PairDesc), hipMemcpyHostToDevice, s),
-          "upload pair descriptors");
-    const WorkRef* wprep = nullptr;
-    const WorkRef* wscale = nullptr;
-    if (!uniform) {
-        if (h_work_prep_.bytes() < prep_work * sizeof(WorkRef)) {
-            h_work_prep_ = PinnedBuffer(prep_work * sizeof(WorkRef));
-            work_prep_ = DeviceBuffer(prep_work * sizeof(WorkRef));
-        }
-        if (h_work_scale_.bytes() < scale_work * sizeof(WorkRef)) {
-            h_work_scale_ = PinnedBuffer(scale_work * sizeof(WorkRef));
-            work_scale_ = DeviceBuffer(scale_work * sizeof(WorkRef));
-        }
-        auto* wp = h_work_prep_.as<WorkRef>();
-        auto* ws = h_work_scale_.as<WorkRef>();
-        uint64_t np = 0, ns = 0;
-        for (uint32_t i = 0; i < count; ++i) {
-            const GeometryPlan& g = taken[i].first->plan->host;
-            for (uint32_t image = 0; image < 2; ++image)
-                for (uint32_t t = 0; t < g.prep_tiles; ++t) wp[np++] = WorkRef{i, t | (image << 31)};
-            for (uint32_t t = 0; t < g.plan.items; ++t) ws[ns++] = WorkRef{i, t};
-        }
-        Check(hipMemcpyAsync(work_prep_.get(), wp, np * sizeof(WorkRef), hipMemcpyHostToDevice, s),
-              "upload prepare work");
-        Check(hipMemcpyAsync(work_scale_.get(), ws, ns * sizeof(WorkRef), hipMemcpyHostToDevice, s),
-              "upload scale work");
-        wprep = work_prep_.as<WorkRef>();
-        wscale = work_scale_.as<WorkRef>();
-    }
-    const PlaneFormat fmt = static_cast<PlaneFormat>(config_.plane_format);
-    const GeometryPlan& g0 = taken[0].first->plan->host;
-    const PairDesc* dpairs = pairs_.as<PairDesc>();
-    Check(hipEventRecord(ev_[0].get(), s), "hipEventRecord");
-    Check(LaunchPrepare(fmt, dpairs, wprep, static_cast<uint32_t>(prep_work), g0.prep_tiles,
-                        ctx_.lut, nullptr, nullptr, s),
-          "launch prepare");
-    Check(hipEventRecord(ev_[1].get(), s), "hipEventRecord");
-    Check(LaunchScale(fmt, dpairs, wscale, static_cast<uint32_t>(scale_work), g0.plan.items, s),
-          "launch scale");
-    Check(hipEventRecord(ev_[2].get(), s), "hipEventRecord");
-    Check(LaunchReduce(dpairs, count, s), "launch reduce");
-    Check(hipEventRecord(ev_[3].get(), s), "hipEventRecord");
-    Check(hipMemcpyAsync(h_norms_.get(), norms_.get(), static_cast<uint64_t>(count) * kNorms * sizeof(double),
-                         hipMemcpyDeviceToHost, s),
-          "read back norms");
-    Check(hipEventRecord(ev_[4].get(), s), "hipEventRecord");
-    for (auto& [p, ws] : taken) p->busy.push_back(std::move(ws));
-    in_flight_count_ = count;
+    levels_.resize(count);
+    for (uint32_t i = 0; i < count; ++i) levels_[i] = entries[i].plan->host.plan.levels;
+    busy_ = std::move(taken);
     in_flight_ = NextId();
     return in_flight_;
 }
@@ -531,16 +622,14 @@ void Batch::Finish(uint64_t token, uint32_t* scales, double* norms) {
     if (token == 0 || token != in_flight_)
         Fail(VQM_INVALID_ARGUMENT, "unknown or already finished submission token");
     in_flight_ = 0;
-    for (auto& [key, pool] : pools_) {
-        for (auto& ws : pool.busy) pool.free.push_back(std::move(ws));
-        pool.busy.clear();
-    }
-    Check(hipEventSynchronize(ev_[4].get()), "wait for batch");
-    stage_ms_ = {Elapsed(ev_[0], ev_[1]), Elapsed(ev_[1], ev_[2]), Elapsed(ev_[2], ev_[3])};
-    if (norms)
-        std::memcpy(norms, h_norms_.get(), static_cast<uint64_t>(in_flight_count_) * kNorms * sizeof(double));
+    for (Taken& t : busy_) pools_.Return(t.width, t.height, false, std::move(t.ws));
+    busy_.clear();
+    lane_.Wait();
+    stage_ms_ = lane_.StageTimes();
+    const auto count = static_cast<uint32_t>(levels_.size());
+    if (norms) std::memcpy(norms, lane_.Norms(0), static_cast<uint64_t>(count) * kNorms * sizeof(double));
     if (scales)
-        for (uint32_t i = 0; i < in_flight_count_; ++i) scales[i] = levels_[i];
+        for (uint32_t i = 0; i < count; ++i) scales[i] = levels_[i];
 }
 
 }  // namespace vqm::ss2

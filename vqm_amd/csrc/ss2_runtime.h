@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <map>
 #include <memory>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -87,6 +88,15 @@ private:
     hipEvent_t event_ = nullptr;
 };
 
+// Milliseconds between two recorded events (0 when unavailable).
+float Elapsed(const Event& a, const Event& b);
+
+// The one decoded input format: 8-bit full-range sRGB RGB, BT.709, opaque.
+vqm_image_format Srgb8Format();
+
+// VQM_PLANES_* to PlaneFormat; INVALID_ARGUMENT otherwise.
+PlaneFormat ToFormat(uint32_t plane_format);
+
 // Per-process device state: the current device must be a gfx950; the sRGB
 // decode table lives in its memory.
 struct DeviceContext {
@@ -106,13 +116,92 @@ struct DevicePlan {
     DevicePlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows);
 };
 
-// One pair's device workspace: planes (zero borders), partials, norms.
+// One pair's device workspace: planes (zero borders), partials, norms and,
+// for pairs whose inputs arrive from the host, room for both input images.
 struct Workspace {
     DeviceBuffer memory;
     void* planes = nullptr;
     double* partials = nullptr;
     double* norms = nullptr;
-    Workspace(const WorkspaceLayout& layout);
+    uint8_t* inputs = nullptr;  // 2 * input_image_bytes, or null
+    Workspace(const WorkspaceLayout& layout, uint64_t input_image_bytes = 0);
+};
+
+// Per-geometry workspace pools under one device-byte budget. A geometry's
+// plan is built on first use; workspaces are reused, new ones are allocated
+// while the budget allows, and idle workspaces of other geometries are freed
+// to make room. Not thread-safe.
+class WorkspacePools {
+public:
+    WorkspacePools(PlaneFormat format, uint32_t strip_rows, uint64_t budget);
+    const DevicePlan& PlanFor(uint32_t width, uint32_t height);
+    // Device bytes one workspace of this geometry holds.
+    uint64_t Need(uint32_t width, uint32_t height, bool inputs);
+    // A workspace, or null when it cannot fit the budget until busy
+    // workspaces come back.
+    std::unique_ptr<Workspace> Acquire(uint32_t width, uint32_t height, bool inputs);
+    void Return(uint32_t width, uint32_t height, bool inputs, std::unique_ptr<Workspace> ws);
+    uint64_t bytes() const { return bytes_; }
+    uint64_t peak() const { return peak_; }
+    uint64_t budget() const { return budget_; }
+
+private:
+    struct Key {
+        uint32_t width, height;
+        bool inputs;
+        bool operator<(const Key& o) const {
+            return std::tie(width, height, inputs) < std::tie(o.width, o.height, o.inputs);
+        }
+    };
+    struct Pool {
+        std::vector<std::unique_ptr<Workspace>> idle;
+        uint64_t each = 0;
+    };
+    PlaneFormat format_;
+    uint32_t strip_rows_;
+    uint64_t budget_;
+    std::map<std::pair<uint32_t, uint32_t>, std::unique_ptr<DevicePlan>> plans_;
+    std::map<Key, Pool> pools_;
+    uint64_t bytes_ = 0, peak_ = 0;
+};
+
+// One pair of a launch: its plan, workspace, device inputs and, when the
+// inputs are on the host, the host images to copy into the workspace first.
+struct LaunchEntry {
+    const DevicePlan* plan = nullptr;
+    Workspace* ws = nullptr;
+    const uint8_t* src[2] = {nullptr, nullptr};
+    const void* host[2] = {nullptr, nullptr};
+    uint64_t image_bytes = 0;
+};
+
+// A stream with the descriptor, work-list and norm buffers of one launch of
+// the three kernels over up to `max_pairs` pairs of any geometries. A lane
+// holds one launch at a time.
+class Lane {
+public:
+    explicit Lane(uint32_t max_pairs);
+    ~Lane();
+    // Queues the uploads, the prepare, scale and reduce launches and the norm
+    // read-back on the lane's stream, then returns.
+    void Launch(PlaneFormat format, const float* lut, const LaunchEntry* entries, uint32_t count);
+    bool Done() const;
+    void Wait();
+    // Forgets a launch that failed part-way (its stream already synchronized).
+    void Abandon() { pending_ = false; }
+    // Norms of entry i of the finished launch (pinned host memory).
+    const double* Norms(uint32_t i) const { return h_norms_.as<double>() + static_cast<uint64_t>(i) * kNorms; }
+    std::array<float, 3> StageTimes() const;
+    hipStream_t stream() const { return stream_.get(); }
+    uint32_t max_pairs() const { return max_pairs_; }
+
+private:
+    uint32_t max_pairs_;
+    DeviceBuffer pairs_, work_prep_, work_scale_, norms_;
+    PinnedBuffer h_pairs_, h_work_prep_, h_work_scale_, h_norms_;
+    Stream stream_;
+    Event ev_[5];
+    bool pending_ = false;
 };
 
 class Pipeline {
@@ -176,26 +265,20 @@ public:
     uint64_t Submit(const vqm_ss2_device_pair* pairs, uint32_t count);
     void Finish(uint64_t token, uint32_t* scales, double* norms);
     std::array<float, 3> StageTimes() const { return stage_ms_; }
-    uint64_t PoolBytes() const { return pool_bytes_; }
+    uint64_t PoolBytes() const { return pools_.bytes(); }
 
 private:
-    struct Pool {
-        std::unique_ptr<DevicePlan> plan;
-        std::vector<std::unique_ptr<Workspace>> free, busy;
+    struct Taken {
+        uint32_t width, height;
+        std::unique_ptr<Workspace> ws;
     };
-    Pool& PoolFor(uint32_t width, uint32_t height);
-
     vqm_ss2_batch_config config_;
     DeviceContext& ctx_;
-    std::map<std::pair<uint32_t, uint32_t>, Pool> pools_;
-    uint64_t pool_bytes_ = 0;
-    DeviceBuffer pairs_, work_prep_, work_scale_, norms_;
-    PinnedBuffer h_pairs_, h_work_prep_, h_work_scale_, h_norms_;
+    WorkspacePools pools_;
+    Lane lane_;
+    std::vector<Taken> busy_;
     std::vector<uint32_t> levels_;
-    Stream stream_;
-    Event ev_[5];
     uint64_t in_flight_ = 0;
-    uint32_t in_flight_count_ = 0;
     std::array<float, 3> stage_ms_{};
 };
 

@@ -648,3 +648,148 @@ class Ssimulacra2Batch:
         b = L.u64()
         _check(L.lib().vqm_ss2_batch_pool_bytes(self._h, ctypes.byref(b)))
         return b.value
+
+
+class PinnedBuffer:
+    """Page-locked host memory through the C ABI (hipHostMalloc), viewed as a
+    numpy uint8 array: host inputs in it are copied to the device
+    asynchronously."""
+
+    def __init__(self, nbytes: int):
+        ptr = ctypes.c_void_p()
+        _check(L.lib().vqm_host_malloc(nbytes, ctypes.byref(ptr)))
+        self.ptr = ptr.value
+        self.nbytes = nbytes
+        self.array = np.ctypeslib.as_array((ctypes.c_uint8 * nbytes).from_address(self.ptr))
+
+    def free(self) -> None:
+        if self.ptr:
+            self.array = None
+            L.lib().vqm_host_free(ctypes.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---- serving ----------------------------------------------------------------------
+
+@dataclass
+class RequestResult:
+    """Outcome of one served request. Times are vqm_now_ns() readings; for a
+    replay, submit_ns is the scheduled arrival time."""
+    status: int
+    scales: int
+    batch_size: int
+    lane: int
+    submit_ns: int
+    launch_ns: int
+    done_ns: int
+    norms: np.ndarray
+
+    @staticmethod
+    def from_c(r: "L.RequestResultC") -> "RequestResult":
+        return RequestResult(r.status, r.scales, r.batch_size, r.lane, r.submit_ns, r.launch_ns,
+                             r.done_ns, np.ctypeslib.as_array(r.norms).copy())
+
+    @property
+    def latency_ms(self) -> float:
+        return (self.done_ns - self.submit_ns) * 1e-6
+
+    def result(self) -> Ssimulacra2Result:
+        return Ssimulacra2Result.from_norms(self.scales, self.norms)
+
+
+def now_ns() -> int:
+    return int(L.lib().vqm_now_ns())
+
+
+class Ssimulacra2Server:
+    """Request server with continuous batching (no reference counterpart; the
+    reference overlaps pairs through Ssimulacra2Pool workers, vqm_pool.h:35-86).
+
+    Requests of any geometry are queued; batches form from the queue head
+    whenever one of `lanes` HIP streams is free, admitted in arrival order
+    against `hbm_budget`, and overlap on the device. Thread-safe."""
+
+    def __init__(self, plane_format: int = L.PLANES_F32, lanes: int = 3, max_batch: int = 64,
+                 hbm_budget: int = 0, strip_rows: int = 0):
+        cfg = L.ServerConfigC()
+        L.lib().vqm_ss2_server_config_init(ctypes.byref(cfg))
+        cfg.plane_format = plane_format
+        cfg.lanes = lanes
+        cfg.max_batch = max_batch
+        cfg.hbm_budget = hbm_budget
+        cfg.strip_rows = strip_rows
+        h = ctypes.c_void_p()
+        self._keep = {}
+        _check(L.lib().vqm_ss2_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            L.lib().vqm_ss2_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def SubmitDevice(self, original: int, distorted: int, width: int, height: int) -> int:
+        pair = L.DevicePairC(original, distorted, width, height)
+        ticket = L.u64()
+        _check(L.lib().vqm_ss2_server_submit_device(self._h, ctypes.byref(pair), ctypes.byref(ticket)))
+        return ticket.value
+
+    def SubmitHost(self, original: Image, distorted: Image) -> int:
+        """The server copies the host images when the request's batch
+        launches; this object keeps them alive until Wait."""
+        keep, ptrs = [], []
+        for img in (original, distorted):
+            arr, ptr, _ = img._buffer()
+            keep.append(arr)
+            ptrs.append(ptr)
+        da, db = original.desc(), distorted.desc()
+        ticket = L.u64()
+        _check(L.lib().vqm_ss2_server_submit_host(self._h, ctypes.byref(da), ptrs[0],
+                                                  ctypes.byref(db), ptrs[1], ctypes.byref(ticket)))
+        self._keep[ticket.value] = (original, distorted, keep)
+        return ticket.value
+
+    def Wait(self, ticket: int) -> RequestResult:
+        out = L.RequestResultC()
+        try:
+            _check(L.lib().vqm_ss2_server_wait(self._h, ticket, ctypes.byref(out)))
+        finally:
+            self._keep.pop(ticket, None)
+        return RequestResult.from_c(out)
+
+    def Stats(self) -> dict:
+        s = L.ServerStatsC()
+        _check(L.lib().vqm_ss2_server_get_stats(self._h, ctypes.byref(s)))
+        return {name: getattr(s, name) for name, _ in L.ServerStatsC._fields_}
+
+    def Replay(self, requests: Sequence[tuple], check: bool = True) -> List[RequestResult]:
+        """Open-loop replay. requests: (original_ptr, distorted_ptr, width,
+        height, host, arrival_ns) tuples; host=True means host pointers."""
+        n = len(requests)
+        arr = (L.ReplayRequestC * n)()
+        for i, (a, b, w, h, host, t) in enumerate(requests):
+            arr[i].original, arr[i].distorted = a, b
+            arr[i].width, arr[i].height, arr[i].host, arr[i].arrival_ns = w, h, int(bool(host)), int(t)
+        out = (L.RequestResultC * n)()
+        status = L.lib().vqm_ss2_server_replay(self._h, arr, n, out)
+        if check:
+            _check(status)
+        return [RequestResult.from_c(out[i]) for i in range(n)]
