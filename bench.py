@@ -126,12 +126,17 @@ def serving(args, pool):
 
     report = {"requests": n, "hbm_budget_bytes": budget, "lanes": args.serve_lanes,
               "max_batch": args.serve_max_batch}
+    # The plain FP32 single-pair form (one pair on the device at a time, no
+    # batching) faces the same arrivals at the same offered rates.
+    forms = (("f32", args.serve_lanes, args.serve_max_batch), ("bf16", args.serve_lanes, args.serve_max_batch),
+             ("single_pair_f32", 1, 1))
     for mode, host, fractions in (("resident", False, (0.25, 0.5, 0.75, 0.9)),
                                   ("streaming", True, (0.5, 0.9))):
         base_capacity = None
-        for planes in ("f32", "bf16"):
-            server = Ssimulacra2Server(plane_format=PLANES[planes], lanes=args.serve_lanes,
-                                       max_batch=args.serve_max_batch, hbm_budget=budget)
+        for name, lanes, max_batch in forms:
+            planes = "f32" if name.endswith("f32") else name
+            server = Ssimulacra2Server(plane_format=PLANES[planes], lanes=lanes,
+                                       max_batch=max_batch, hbm_budget=budget)
             replay(server, host, 0)  # warm the workspace pools
             burst = replay(server, host, 0)
             if base_capacity is None:
@@ -139,10 +144,10 @@ def serving(args, pool):
             points = [burst] + [replay(server, host, f * base_capacity) for f in fractions]
             stats = server.Stats()
             server.close()
-            report[f"{mode}_{planes}"] = {"capacity_pairs_per_s": burst["pairs_per_s"],
-                                          "points": points,
-                                          "peak_pool_bytes": stats["peak_pool_bytes"]}
-            log(f"serving {mode} {planes}: {points}")
+            report[f"{mode}_{name}"] = {"capacity_pairs_per_s": burst["pairs_per_s"],
+                                        "points": points,
+                                        "peak_pool_bytes": stats["peak_pool_bytes"]}
+            log(f"serving {mode} {name}: {points}")
     return report
 
 
@@ -329,20 +334,24 @@ def main() -> int:
 
     if not args.no_modes:
         modes = {}
-        # Single pair at a time, resident inputs: per-pair latency.
-        pipe = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(w, h), plane_format=fmt,
-                                                     strip_rows=args.single_strip_rows))
         from vqm_amd import DeviceImage
-        lat = []
-        for i in range(min(40, len(refs))):
-            a, b = refs[i][0], refs[i][1]
-            t = time.perf_counter()
-            pipe.Compute(DeviceImage(a, Size(w, h)), DeviceImage(b, Size(w, h)))
-            lat.append((time.perf_counter() - t) * 1e3)
-        lat = lat[5:]
-        modes["resident_single_pair"] = {"pairs_per_s": round(1e3 / statistics.mean(lat), 1),
-                                         "p50_ms": round(float(np.percentile(lat, 50)), 4),
-                                         "p99_ms": round(float(np.percentile(lat, 99)), 4)}
+        # Single pair at a time, resident inputs: per-pair latency, with the
+        # pair's launches replayed from one HIP graph and launched eagerly.
+        for key, graph in (("resident_single_pair_graph", True), ("resident_single_pair", False)):
+            pipe = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(w, h), plane_format=fmt,
+                                                         strip_rows=args.single_strip_rows, graph=graph))
+            lat = []
+            for i in range(min(60, len(refs))):
+                a, b = refs[i][0], refs[i][1]
+                t = time.perf_counter()
+                pipe.Compute(DeviceImage(a, Size(w, h)), DeviceImage(b, Size(w, h)))
+                lat.append((time.perf_counter() - t) * 1e3)
+            lat = lat[5:]
+            modes[key] = {"pairs_per_s": round(1e3 / statistics.mean(lat), 1),
+                          "p50_ms": round(float(np.percentile(lat, 50)), 4),
+                          "p99_ms": round(float(np.percentile(lat, 99)), 4)}
+            if graph:
+                pipe.close()
         # Streaming: host images uploaded per call (PCIe-inclusive).
         imgs = [(Image(Size(w, h), ImageFormat.srgb8(), a), Image(Size(w, h), ImageFormat.srgb8(), b))
                 for a, b in pairs[:30]]

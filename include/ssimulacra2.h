@@ -187,6 +187,7 @@ struct Ssimulacra2Config {
     bool streaming_inputs = true;
     PlaneFormat plane_format = PlaneFormat::F32;  // overridden to F16 by half_planes
     std::uint32_t strip_rows = 0;
+    bool graph = false;  // replay one captured HIP graph per pair (no reference field)
 
     [[nodiscard]] vqm_ss2_config ToC() const {
         vqm_ss2_config c{};
@@ -198,6 +199,7 @@ struct Ssimulacra2Config {
         c.streaming_inputs = streaming_inputs ? 1u : 0u;
         c.strip_rows = strip_rows;
         c.input_bytes = input_bytes;
+        c.graph = graph ? 1u : 0u;
         return c;
     }
 };

@@ -100,6 +100,12 @@ typedef struct vqm_ss2_config {
     uint32_t streaming_inputs;  /* default 1 */
     uint32_t strip_rows;        /* output rows per scale work item, 0 = default */
     uint64_t input_bytes;       /* caller-owned image pool (AllocateImage), default 0 */
+    uint32_t graph;             /* 1: each pair replays one HIP graph of the
+                                   descriptor upload, the three kernels and the
+                                   norm read-back, captured on first use; 0
+                                   (default): eager launches, which measured
+                                   slightly faster. No reference counterpart. */
+    uint32_t reserved;
 } vqm_ss2_config;
 
 /* PipelineMemoryUsage, include/vqm_memory_types.h:29-38: device bytes the

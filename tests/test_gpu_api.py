@@ -191,8 +191,13 @@ def test_pool_drains_on_invalid_pair():
 
 
 def test_stage_times_are_measured():
+    """Eager launches time each stage; a graph replay times the whole pair."""
     a, b = ramp_image(256, 256), ramp_image(256, 256, seed=5)
-    p = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(256, 256)))
+    p = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(256, 256), graph=False))
     p.Compute(Image(Size(256, 256), ImageFormat.srgb8(), a), Image(Size(256, 256), ImageFormat.srgb8(), b))
     t = p.StageTimes()
     assert len(t) == 3 and all(v > 0.0 for v in t)
+    g = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(256, 256), graph=True))
+    g.Compute(Image(Size(256, 256), ImageFormat.srgb8(), a), Image(Size(256, 256), ImageFormat.srgb8(), b))
+    t = g.StageTimes()
+    assert t[0] > 0.0 and t[1] == 0.0 and t[2] == 0.0

@@ -20,3 +20,17 @@ def test_plan_bounds():
                         "-o", exe], check=True)
         r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
         assert r.returncode == 0, r.stderr
+
+
+def test_range_allocator():
+    """The workspace arena's first-fit bookkeeping against a byte-map model."""
+    with tempfile.TemporaryDirectory() as tmp:
+        exe = os.path.join(tmp, "range_allocator_test")
+        subprocess.run(["g++", "-std=c++20", "-O2", "-Wall",
+                        "-I", os.path.join(ROOT, "include"),
+                        "-I", os.path.join(ROOT, "vqm_amd", "csrc"),
+                        os.path.join(ROOT, "tests", "cpp", "range_allocator_test.cc"),
+                        os.path.join(ROOT, "vqm_amd", "csrc", "ss2_host.cc"),
+                        "-o", exe], check=True)
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr

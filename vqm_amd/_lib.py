@@ -48,7 +48,7 @@ class DeviceImageC(ctypes.Structure):
 class ConfigC(ctypes.Structure):
     _fields_ = [("width", u32), ("height", u32), ("resident_images", u32),
                 ("plane_format", u32), ("streaming_inputs", u32), ("strip_rows", u32),
-                ("input_bytes", u64)]
+                ("input_bytes", u64), ("graph", u32), ("reserved", u32)]
 
 
 class MemoryC(ctypes.Structure):

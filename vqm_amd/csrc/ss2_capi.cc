@@ -70,6 +70,7 @@ void vqm_ss2_config_init(vqm_ss2_config* config, uint32_t width, uint32_t height
     config->resident_images = 2;
     config->plane_format = VQM_PLANES_F32;
     config->streaming_inputs = 1;
+    config->graph = 0;
 }
 
 void vqm_image_format_srgb8(vqm_image_format* f) {

@@ -267,4 +267,43 @@ uint64_t AlgorithmicBytes(const GeometryPlan& plan) {
     return 2 * input + 2 * (2 * elements * plan.element_bytes);
 }
 
+RangeAllocator::RangeAllocator(uint64_t capacity) : capacity_(capacity / kAlign * kAlign) {
+    if (capacity_ > 0) free_[0] = capacity_;
+}
+
+uint64_t RangeAllocator::Alloc(uint64_t bytes) {
+    bytes = Align(std::max<uint64_t>(bytes, 1), kAlign);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second < bytes) continue;
+        const uint64_t offset = it->first, length = it->second;
+        free_.erase(it);
+        if (length > bytes) free_[offset + bytes] = length - bytes;
+        return offset;
+    }
+    return kNone;
+}
+
+void RangeAllocator::Free(uint64_t offset, uint64_t bytes) {
+    bytes = Align(std::max<uint64_t>(bytes, 1), kAlign);
+    auto next = free_.lower_bound(offset);
+    if (next != free_.end() && offset + bytes == next->first) {  // merge with the range after
+        bytes += next->second;
+        next = free_.erase(next);
+    }
+    if (next != free_.begin()) {  // merge with the range before
+        auto prev = std::prev(next);
+        if (prev->first + prev->second == offset) {
+            prev->second += bytes;
+            return;
+        }
+    }
+    free_[offset] = bytes;
+}
+
+uint64_t RangeAllocator::free_bytes() const {
+    uint64_t total = 0;
+    for (const auto& [offset, length] : free_) total += length;
+    return total;
+}
+
 }  // namespace vqm::ss2

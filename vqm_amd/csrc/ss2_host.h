@@ -6,6 +6,7 @@
 #include <array>
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -72,6 +73,26 @@ struct WorkspaceLayout {
     uint64_t planes = 0, partials = 0, norms = 0, total = 0;
 };
 WorkspaceLayout LayoutWorkspace(const GeometryPlan& plan);
+
+// Offset bookkeeping of a device arena: ranges of a fixed capacity handed
+// out in 256-byte multiples by first fit over an address-ordered free list,
+// neighbours merged on release.
+class RangeAllocator {
+public:
+    static constexpr uint64_t kAlign = 256;
+    explicit RangeAllocator(uint64_t capacity);
+    // Offset of a new range of `bytes`, or ~0 when no free range fits.
+    uint64_t Alloc(uint64_t bytes);
+    void Free(uint64_t offset, uint64_t bytes);
+    uint64_t capacity() const { return capacity_; }
+    uint64_t free_bytes() const;
+    size_t free_ranges() const { return free_.size(); }
+    static constexpr uint64_t kNone = ~0ull;
+
+private:
+    uint64_t capacity_;
+    std::map<uint64_t, uint64_t> free_;  // offset -> length
+};
 
 // Algorithmic HBM bytes of one pair: both u8 inputs read once, every stored
 // plane element written once and read once per image.

@@ -147,18 +147,28 @@ DevicePlan::DevicePlan(uint32_t width, uint32_t height, PlaneFormat format, uint
 }
 
 namespace {
-uint64_t InputArea(uint64_t input_image_bytes) { return 2 * ((input_image_bytes + 255) / 256 * 256); }
+constexpr uint64_t kAlign = 256;
+uint64_t AlignUp(uint64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+uint64_t InputArea(uint64_t input_image_bytes) { return 2 * AlignUp(input_image_bytes); }
 }  // namespace
 
-Workspace::Workspace(const WorkspaceLayout& layout, uint64_t input_image_bytes)
-    : memory(layout.total + InputArea(input_image_bytes)) {
+uint64_t Workspace::Bytes(const WorkspaceLayout& layout, uint64_t input_image_bytes) {
+    return layout.total + InputArea(input_image_bytes);
+}
+
+Workspace::Workspace(const WorkspaceLayout& layout, uint64_t input_image_bytes, void* memory)
+    : bytes(Bytes(layout, input_image_bytes)) {
+    if (memory == nullptr) {
+        owned = DeviceBuffer(bytes);
+        memory = owned.get();
+    }
+    base = static_cast<uint8_t*>(memory);
     // The plane borders must read as zero for every later pair: they are the
     // blur's zero padding and nothing writes them.
-    Check(hipMemset(memory.get(), 0, layout.total), "clear workspace");
+    Check(hipMemset(base, 0, layout.total), "clear workspace");
     // The clear runs on the null stream; the pipelines' streams do not wait
     // for it, so finish it before any kernel can touch the planes.
     Check(hipStreamSynchronize(nullptr), "clear workspace");
-    auto* base = memory.as<uint8_t>();
     planes = base;
     partials = reinterpret_cast<double*>(base + layout.planes);
     norms = reinterpret_cast<double*>(base + layout.planes + layout.partials);
@@ -169,6 +179,11 @@ Workspace::Workspace(const WorkspaceLayout& layout, uint64_t input_image_bytes)
 
 WorkspacePools::WorkspacePools(PlaneFormat format, uint32_t strip_rows, uint64_t budget)
     : format_(format), strip_rows_(strip_rows), budget_(budget) {}
+
+void WorkspacePools::Drop(std::unique_ptr<Workspace> ws) {
+    bytes_ -= ws->bytes;
+    if (ws->owned.bytes() == 0) arena_->Free(ws->base, ws->bytes);
+}
 
 const DevicePlan& WorkspacePools::PlanFor(uint32_t width, uint32_t height) {
     auto& slot = plans_[std::make_pair(width, height)];
@@ -198,23 +213,32 @@ std::unique_ptr<Workspace> WorkspacePools::Acquire(uint32_t width, uint32_t heig
         return ws;
     }
     const DevicePlan& plan = PlanFor(width, height);
-    const uint64_t need = Need(width, height, inputs);
-    if (budget_ != 0 && bytes_ + need > budget_) {
-        // Free idle workspaces of other geometries, largest first.
-        std::vector<std::pair<uint64_t, Pool*>> idle;
-        for (auto& [key, p] : pools_)
-            if (!p.idle.empty()) idle.emplace_back(p.each, &p);
-        std::sort(idle.begin(), idle.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
-        for (auto& [each, p] : idle) {
-            while (!p->idle.empty() && bytes_ + need > budget_) {
-                p->idle.pop_back();
-                bytes_ -= each;
-            }
+    const uint64_t input_bytes = inputs ? RequiredBytes(Srgb8Format(), width, height) : 0;
+    const uint64_t need = Workspace::Bytes(plan.layout, input_bytes);
+    void* memory = nullptr;
+    if (budget_ != 0) {
+        if (need > budget_) return nullptr;
+        if (!arena_) arena_ = std::make_unique<Arena>(budget_);
+        memory = arena_->Alloc(need);
+        // Hand idle workspaces of other geometries back, largest first, until
+        // a range fits.
+        while (memory == nullptr) {
+            Pool* victim = nullptr;
+            for (auto& [key, p] : pools_)
+                if (!p.idle.empty() && (victim == nullptr || p.each > victim->each)) victim = &p;
+            if (victim == nullptr) return nullptr;
+            Drop(std::move(victim->idle.back()));
+            victim->idle.pop_back();
+            memory = arena_->Alloc(need);
         }
-        if (bytes_ + need > budget_) return nullptr;
     }
-    auto ws = std::make_unique<Workspace>(
-        plan.layout, inputs ? RequiredBytes(Srgb8Format(), width, height) : 0);
+    std::unique_ptr<Workspace> ws;
+    try {
+        ws = std::make_unique<Workspace>(plan.layout, input_bytes, memory);
+    } catch (...) {
+        if (memory) arena_->Free(memory, need);
+        throw;
+    }
     pool.each = need;
     bytes_ += need;
     peak_ = std::max(peak_, bytes_);
@@ -368,6 +392,10 @@ Pipeline::Pipeline(const vqm_ss2_config& config)
     pd.plan = plan_.plan.as<Plan>();
     pd.items = plan_.items.as<ScaleItem>();
     Check(hipMemcpy(pair_.get(), &pd, sizeof(pd), hipMemcpyHostToDevice), "upload pair descriptor");
+    if (config.graph) {
+        h_pair_ = PinnedBuffer(sizeof(PairDesc));
+        std::memcpy(h_pair_.get(), &pd, sizeof(pd));
+    }
     slots_.resize(config.resident_images);
     for (Slot& s : slots_) s.data = DeviceBuffer(image_bytes_);
     if (config.streaming_inputs) {
@@ -380,6 +408,34 @@ Pipeline::Pipeline(const vqm_ss2_config& config)
 
 Pipeline::~Pipeline() {
     if (in_flight_ != 0) (void)hipEventSynchronize(ev_[4].get());
+    if (graph_) (void)hipGraphExecDestroy(graph_);
+}
+
+// Captures, on the pipeline's stream, the upload of the pinned pair
+// descriptor (its input pointers change per pair), the three kernels (the
+// prepare kernel reads its inputs through the descriptor) and the norm
+// read-back. The graph holds no memset node: the plane borders are cleared
+// once, when the workspace is created.
+void Pipeline::BuildGraph() {
+    const hipStream_t s = stream_.get();
+    const PairDesc* pd = pair_.as<PairDesc>();
+    const PlaneFormat fmt = static_cast<PlaneFormat>(config_.plane_format);
+    const GeometryPlan& g = plan_.host;
+    Check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin graph capture");
+    hipError_t e = hipMemcpyAsync(pair_.get(), h_pair_.get(), sizeof(PairDesc), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = LaunchPrepare(fmt, pd, nullptr, 2 * g.prep_tiles, g.prep_tiles, ctx_.lut, nullptr, nullptr, s);
+    if (e == hipSuccess) e = LaunchScale(fmt, pd, nullptr, g.plan.items, g.plan.items, s);
+    if (e == hipSuccess) e = LaunchReduce(pd, 1, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(host_norms_.get(), ws_.norms, kNorms * sizeof(double), hipMemcpyDeviceToHost, s);
+    hipGraph_t graph = nullptr;
+    const hipError_t end = hipStreamEndCapture(s, &graph);
+    Check(e, "capture the pair's launches");
+    Check(end, "end graph capture");
+    e = hipGraphInstantiate(&graph_, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) graph_ = nullptr;
+    Check(e, "instantiate the pair's graph");
 }
 
 void Pipeline::RequireIdle() const {
@@ -425,6 +481,19 @@ uint64_t Pipeline::Launch(const uint8_t* src0, const uint8_t* src1) {
     const PairDesc* pd = pair_.as<PairDesc>();
     const PlaneFormat fmt = static_cast<PlaneFormat>(config_.plane_format);
     const GeometryPlan& g = plan_.host;
+    if (config_.graph) {
+        if (graph_ == nullptr) BuildGraph();
+        auto* hp = h_pair_.as<PairDesc>();
+        hp->src[0] = src0;
+        hp->src[1] = src1;
+        Check(hipEventRecord(ev_[0].get(), s), "hipEventRecord");
+        Check(hipGraphLaunch(graph_, s), "launch the pair's graph");
+        Check(hipEventRecord(ev_[4].get(), s), "hipEventRecord");
+        graph_launched_ = true;
+        in_flight_ = NextId();
+        return in_flight_;
+    }
+    graph_launched_ = false;
     Check(hipEventRecord(ev_[0].get(), s), "hipEventRecord");
     Check(LaunchPrepare(fmt, pd, nullptr, 2 * g.prep_tiles, g.prep_tiles, ctx_.lut, src0, src1, s),
           "launch prepare");
@@ -549,14 +618,17 @@ uint32_t Pipeline::Finish(uint64_t token, double* norms) {
         Fail(VQM_INVALID_ARGUMENT, "unknown or already finished submission token");
     in_flight_ = 0;
     Check(hipEventSynchronize(ev_[4].get()), "wait for pair");
-    stage_ms_ = {Elapsed(ev_[0], ev_[1]), Elapsed(ev_[1], ev_[2]), Elapsed(ev_[2], ev_[3])};
+    if (graph_launched_)
+        stage_ms_ = {Elapsed(ev_[0], ev_[4]), 0.0f, 0.0f};
+    else
+        stage_ms_ = {Elapsed(ev_[0], ev_[1]), Elapsed(ev_[1], ev_[2]), Elapsed(ev_[2], ev_[3])};
     std::memcpy(norms, host_norms_.get(), kNorms * sizeof(double));
     return plan_.host.plan.levels;
 }
 
 vqm_ss2_memory Pipeline::Memory() const {
     vqm_ss2_memory m{};
-    m.workspace_bytes = ws_.memory.bytes() + pair_.bytes() + plan_.plan.bytes() + plan_.items.bytes();
+    m.workspace_bytes = ws_.bytes + pair_.bytes() + plan_.plan.bytes() + plan_.items.bytes();
     for (const Slot& s : slots_) m.resident_bytes += s.data.bytes();
     m.streaming_bytes = stream_inputs_.bytes();
     m.input_pool_bytes = input_pool_.bytes();

@@ -118,19 +118,46 @@ struct DevicePlan {
 
 // One pair's device workspace: planes (zero borders), partials, norms and,
 // for pairs whose inputs arrive from the host, room for both input images.
+// It owns its memory, or uses `bytes` at `memory` carved from an arena.
 struct Workspace {
-    DeviceBuffer memory;
+    DeviceBuffer owned;
+    uint8_t* base = nullptr;
+    uint64_t bytes = 0;
     void* planes = nullptr;
     double* partials = nullptr;
     double* norms = nullptr;
     uint8_t* inputs = nullptr;  // 2 * input_image_bytes, or null
-    Workspace(const WorkspaceLayout& layout, uint64_t input_image_bytes = 0);
+    Workspace(const WorkspaceLayout& layout, uint64_t input_image_bytes = 0, void* memory = nullptr);
+    // Bytes a workspace of this layout needs.
+    static uint64_t Bytes(const WorkspaceLayout& layout, uint64_t input_image_bytes);
+};
+
+// One device allocation carved into ranges by a RangeAllocator. Not
+// thread-safe.
+class Arena {
+public:
+    explicit Arena(uint64_t bytes) : memory_(bytes), ranges_(bytes) {}
+    // A range of `bytes`, or null when no free range is large enough.
+    void* Alloc(uint64_t bytes) {
+        const uint64_t offset = ranges_.Alloc(bytes);
+        return offset == RangeAllocator::kNone ? nullptr : memory_.as<uint8_t>() + offset;
+    }
+    void Free(void* p, uint64_t bytes) {
+        ranges_.Free(static_cast<uint64_t>(static_cast<uint8_t*>(p) - memory_.as<uint8_t>()), bytes);
+    }
+
+private:
+    DeviceBuffer memory_;
+    RangeAllocator ranges_;
 };
 
 // Per-geometry workspace pools under one device-byte budget. A geometry's
-// plan is built on first use; workspaces are reused, new ones are allocated
-// while the budget allows, and idle workspaces of other geometries are freed
-// to make room. Not thread-safe.
+// plan is built on first use and workspaces are reused. With a budget, the
+// workspaces are carved from one arena of that size (allocated on first
+// use), and idle workspaces of other geometries are handed back to it to
+// make room, so admission under pressure never frees or allocates device
+// memory. Without a budget each workspace is its own allocation and nothing
+// is freed. Not thread-safe.
 class WorkspacePools {
 public:
     WorkspacePools(PlaneFormat format, uint32_t strip_rows, uint64_t budget);
@@ -157,9 +184,11 @@ private:
         std::vector<std::unique_ptr<Workspace>> idle;
         uint64_t each = 0;
     };
+    void Drop(std::unique_ptr<Workspace> ws);
     PlaneFormat format_;
     uint32_t strip_rows_;
     uint64_t budget_;
+    std::unique_ptr<Arena> arena_;
     std::map<std::pair<uint32_t, uint32_t>, std::unique_ptr<DevicePlan>> plans_;
     std::map<Key, Pool> pools_;
     uint64_t bytes_ = 0, peak_ = 0;
@@ -239,12 +268,16 @@ private:
     const Slot& RequireSlot(uint64_t handle, const char* name) const;
     const InputBuffer& RequireBuffer(uint64_t handle, const char* name) const;
     uint64_t Launch(const uint8_t* src0, const uint8_t* src1);
+    void BuildGraph();
 
     vqm_ss2_config config_;
     DeviceContext& ctx_;
     DevicePlan plan_;
     Workspace ws_;
-    DeviceBuffer pair_;  // one PairDesc
+    DeviceBuffer pair_;    // one PairDesc
+    PinnedBuffer h_pair_;  // its host copy; the graph uploads it per pair
+    hipGraphExec_t graph_ = nullptr;
+    bool graph_launched_ = false;
     std::vector<Slot> slots_;
     DeviceBuffer stream_inputs_;   // two images (streaming_inputs)
     PinnedBuffer staging_;         // two images, pinned

@@ -310,9 +310,12 @@ class Ssimulacra2Config:
     streaming_inputs: bool = True
     plane_format: Optional[int] = None
     strip_rows: int = 0
+    graph: bool = False  # replay one captured HIP graph per pair (no reference field)
 
     def to_c(self) -> L.ConfigC:
         c = L.ConfigC()
+        L.lib().vqm_ss2_config_init(ctypes.byref(c), self.size.width, self.size.height)
+        c.graph = 1 if self.graph else 0
         c.width, c.height = self.size.width, self.size.height
         c.resident_images = self.gpu_resident_images
         fmt = self.plane_format
