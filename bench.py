@@ -2,7 +2,7 @@
 """SSIMULACRA 2.1 throughput on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1080p|4k|512]
-                    [--planes f32|f16|bf16] [--cpu-sample P | --no-cpu]
+                    [--planes f32|u16|f16|bf16] [--cpu-sample P | --no-cpu]
 
 A step scores one batch of synthetic pairs already resident in HBM: the 1080p
 workload is 3 distortion strengths x 100 pairs (block step 4/12/24, noise
@@ -33,20 +33,18 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 RTX5070_1080P_RESIDENT = 3836.7  # 1080p resident, 3 workers, RTX 5070
 WORKLOADS = {"1080p": (1920, 1080, 1), "4k": (3840, 2160, 2), "512": (512, 512, 0)}
-PLANES = {"f32": 0, "f16": 1, "bf16": 2}
+PLANES = {"f32": 0, "f16": 1, "bf16": 2, "u16": 3}  # u16: the mixed form
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def plane_elements(w: int, h: int) -> int:
-    """Stored plane elements of one image (pruned planes excluded)."""
-    from vqm_amd import Size, channel_pruned, plan_scales
-    sizes = plan_scales(Size(w, h))
-    n = len(sizes)
-    return sum(s.width * s.height for l, s in enumerate(sizes) for c in range(3)
-               if not channel_pruned(c, l, n))
+def plane_bytes(w: int, h: int, fmt: int) -> int:
+    """Stored plane bytes of one image (pruned planes excluded, no padding):
+    the library's algorithmic pair bytes are 6N input + 4 x these."""
+    from vqm_amd import Size, algorithmic_bytes
+    return (algorithmic_bytes(Size(w, h), fmt) - 6 * w * h) // 4
 
 
 def make_pairs(w, h, cfg, per_strength):
@@ -128,7 +126,7 @@ def serving(args, pool):
               "max_batch": args.serve_max_batch}
     # The plain FP32 single-pair form (one pair on the device at a time, no
     # batching) faces the same arrivals at the same offered rates.
-    forms = (("f32", args.serve_lanes, args.serve_max_batch), ("bf16", args.serve_lanes, args.serve_max_batch),
+    forms = (("f32", args.serve_lanes, args.serve_max_batch), ("u16", args.serve_lanes, args.serve_max_batch),
              ("single_pair_f32", 1, 1))
     for mode, host, fractions in (("resident", False, (0.25, 0.5, 0.75, 0.9)),
                                   ("streaming", True, (0.5, 0.9))):
@@ -160,7 +158,7 @@ def ladder_batch(args, pool):
     picks = [pool.pick(3, i, None) for i in range(n)]
     refs = [(e["dev"][0], e["dev"][1], w, h) for w, h, e in picks]
     out = {"pairs": n}
-    for planes in ("f32", "bf16"):
+    for planes in ("f32", "u16"):
         batch = Ssimulacra2Batch(max_pairs=n, plane_format=PLANES[planes])
         for _ in range(2):
             batch.Finish(batch.Submit(refs), raw=True)
@@ -214,6 +212,8 @@ def main() -> int:
     ap.add_argument("--planes", choices=sorted(PLANES), default="f32")
     ap.add_argument("--strip-rows", type=int, default=0,
                     help="output rows per scale work item of the batch (0 = the library default)")
+    ap.add_argument("--chunks", type=int, default=1,
+                    help="runs of pairs per batch whose scale overlaps the next run's prepare")
     ap.add_argument("--single-strip-rows", type=int, default=0,
                     help="output rows per scale work item of the single-pair modes (0 = the library default)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
@@ -241,7 +241,6 @@ def main() -> int:
 
     w, h, cfg = WORKLOADS[args.workload]
     fmt = PLANES[args.planes]
-    elem = 4 if fmt == 0 else 2
     info = device_info()
     log(f"device {info['name']} {info['arch']} cus={info['cus']}")
 
@@ -272,7 +271,8 @@ def main() -> int:
         src_buf.free()
         dst_buf.free()
 
-    batch = Ssimulacra2Batch(max_pairs=len(refs), plane_format=fmt, strip_rows=args.strip_rows)
+    batch = Ssimulacra2Batch(max_pairs=len(refs), plane_format=fmt, strip_rows=args.strip_rows,
+                             chunks=args.chunks)
     for _ in range(args.warmup):
         scales, norms = batch.Finish(batch.Submit(refs), raw=True)
     step_ms, stages = [], []
@@ -292,7 +292,7 @@ def main() -> int:
     # Roofline of the dominant kernel: algorithmic bytes per launch over the
     # average HIP-event duration of that launch on the batch stream.
     n_px = w * h
-    planes_bytes = plane_elements(w, h) * elem
+    planes_bytes = plane_bytes(w, h, fmt)
     launch_bytes = {
         "prepare": len(refs) * (2 * 3 * n_px + 2 * planes_bytes),
         "scale": len(refs) * (2 * planes_bytes),
@@ -322,7 +322,8 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": round(value / RTX5070_1080P_RESIDENT, 3) if args.workload == "1080p" else None,
-        "dtype": {0: "f32", 1: "f32 (f16 planes)", 2: "f32 (bf16 planes)"}[fmt],
+        "dtype": {0: "f32", 1: "f32 (f16 planes)", 2: "f32 (bf16 planes)",
+                  3: "f32 (u16 fixed-point planes at levels 0-1)"}[fmt],
         "data": "synthetic (vqm_amd/synth.py: sinusoid field + block quantisation + Gaussian noise)",
         "config": {"workload": f"{w}x{h} 8-bit sRGB RGB, 6 scales, {len(refs)} pairs per step "
                                f"(3 strengths x {args.pairs_per_strength}), resident in HBM, "

@@ -30,8 +30,18 @@ F32_DEV = Mode(1, 0, 1, 0, 0, 0)
 
 
 def f32_dev_planes(plane_format: int, levels_mask: int = 0x3F) -> Mode:
-    """The device form with stored planes rounded to FP16 (1) or BF16 (2)."""
+    """The device form with the stored planes of the levels in `levels_mask`
+    rounded to FP16 (1), BF16 (2) or unsigned 16-bit fixed point (3)."""
     return Mode(1, 0, 1, plane_format, levels_mask, 0)
+
+
+def f32_dev_for(plane_format: int) -> Mode:
+    """The device form of a library plane format: FP32 (0), FP16 (1) and BF16
+    (2) planes at every level; the mixed form (3) rounds levels 0 and 1 to
+    unsigned 16-bit fixed point and keeps FP32 above."""
+    if plane_format == 0:
+        return F32_DEV
+    return f32_dev_planes(plane_format, 0x03 if plane_format == 3 else 0x3F)
 
 
 def build() -> None:

@@ -103,6 +103,19 @@ float round_bf16(float v) {
     return bits_to_float(bits);
 }
 
+// Round-to-nearest-even float -> unsigned 16-bit fixed point q / 65536
+// (clamped to 0..65535) -> float. XYB values of linear RGB in [0, 1] lie in
+// [0, 0.94], so the step is a uniform 2^-16 and zero stays exactly zero.
+float round_unorm16(float v) {
+    if (!(v > 0.0f)) return 0.0f;
+    const double scaled = static_cast<double>(v) * 65536.0;  // exact
+    if (scaled >= 65535.0) return 65535.0f / 65536.0f;
+    double fl = std::floor(scaled);
+    const double rem = scaled - fl;
+    if (rem > 0.5 || (rem == 0.5 && std::fmod(fl, 2.0) != 0.0)) fl += 1.0;
+    return static_cast<float>(fl / 65536.0);
+}
+
 template <typename T> T fmaT(T a, T b, T c) { return std::fma(a, b, c); }
 
 // Device cube root: a bit-trick seed for x^(-1/3), two division-free Newton
@@ -420,7 +433,9 @@ int32_t compute(const ss2o_mode& mode, uint32_t width, uint32_t height,
                             reinterpret_cast<float*>(v));
                     if (round_plane) {
                         for (int c = 0; c < 3; ++c)
-                            v[c] = mode.plane_format == 1 ? round_half(v[c]) : round_bf16(v[c]);
+                            v[c] = mode.plane_format == 1   ? round_half(v[c])
+                                   : mode.plane_format == 2 ? round_bf16(v[c])
+                                                            : round_unorm16(v[c]);
                     }
                 } else {
                     xyb_f64(lin[0][i], lin[1][i], lin[2][i], reinterpret_cast<double*>(v));

@@ -44,7 +44,7 @@ typedef struct ss2o_mode {
     int32_t arith;        /* 0 = FP64, 1 = FP32 */
     int32_t blur_order;   /* 0 = horizontal then vertical, 1 = vertical then horizontal */
     int32_t cbrt_kind;    /* 0 = libm cbrt (correctly rounded to FP32), 1 = device Newton */
-    int32_t plane_format; /* FP32 arith only: 0 = FP32, 1 = FP16, 2 = BF16 stored planes */
+    int32_t plane_format; /* FP32 arith only: 0 = FP32, 1 = FP16, 2 = BF16, 3 = unorm16 stored planes */
     uint32_t plane_round_levels; /* levels whose stored planes use plane_format */
     int32_t sum_quads;    /* 1 = FP32 sums of 4 adjacent pixels, then FP64; 0 = FP64 per pixel */
 } ss2o_mode;

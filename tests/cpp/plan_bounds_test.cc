@@ -32,9 +32,12 @@ static void check(uint32_t width, uint32_t height, PlaneFormat fmt, uint32_t str
     const GeometryPlan g = BuildPlan(width, height, fmt, strip_rows);
     const Plan& p = g.plan;
     const uint32_t sh = p.strip_h;
-    // Planes: aligned, disjoint, inside image_elems.
+    // Planes: aligned, disjoint, inside image_bytes (offsets in bytes, the
+    // element size per level).
     uint64_t prev_end = 0;
     for (uint32_t l = 0; l < p.levels; ++l) {
+        const uint32_t e = ElementBytes(fmt, l);
+        EXPECT(e == (fmt == kPlaneF32 || (fmt == kPlaneU16 && l >= kFineLevels) ? 4u : 2u), "element size");
         EXPECT(p.pitch[l] % 4 == 0, "pitch %u not a multiple of 4", p.pitch[l]);
         const uint32_t strips = (p.w[l] + kStripCols - 1) / kStripCols;
         EXPECT(p.pitch[l] >= strips * kStripCols + 14, "pitch too small at level %u", l);
@@ -46,12 +49,14 @@ static void check(uint32_t width, uint32_t height, PlaneFormat fmt, uint32_t str
                 EXPECT(p.plane_off[l][c] == ~0ull, "pruned plane has storage");
                 continue;
             }
-            EXPECT(p.plane_off[l][c] % 64 == 0, "plane offset not 64-aligned");
+            EXPECT(p.plane_off[l][c] % 256 == 0, "plane offset not 256-byte aligned");
             EXPECT(p.plane_off[l][c] >= prev_end, "planes overlap");
-            prev_end = p.plane_off[l][c] + static_cast<uint64_t>(p.pitch[l]) * p.rows[l];
+            prev_end = p.plane_off[l][c] + static_cast<uint64_t>(p.pitch[l]) * p.rows[l] * e;
         }
     }
-    EXPECT(prev_end <= p.image_elems, "planes beyond image_elems");
+    EXPECT(prev_end <= p.image_bytes, "planes beyond image_bytes");
+    EXPECT(p.image_bytes % 256 == 0, "image planes not 256-byte aligned");
+    EXPECT(g.plane_bytes_per_image == p.image_bytes, "plane bytes");
 
     // Scale items: reads of rows y0 .. y0+sh+11 and columns x0+2 .. x0+141 of
     // the padded plane; the map covers each output pixel once.
@@ -93,7 +98,7 @@ static void check(uint32_t width, uint32_t height, PlaneFormat fmt, uint32_t str
 }
 
 int main() {
-    const PlaneFormat fmts[] = {kPlaneF32, kPlaneF16, kPlaneBF16};
+    const PlaneFormat fmts[] = {kPlaneF32, kPlaneF16, kPlaneBF16, kPlaneU16};
     std::vector<std::pair<uint32_t, uint32_t>> sizes = {
         {8, 8}, {13, 9}, {32, 32}, {64, 64}, {65, 65}, {129, 97}, {512, 512}, {1920, 1080},
         {3840, 2160}, {640, 360}, {1280, 720}, {2560, 1440}, {127, 8}, {8, 300}, {255, 257}};

@@ -79,3 +79,19 @@ def test_batch_budget_and_misuse():
         big.Finish(token)
     with pytest.raises(InvalidArgument):
         big.Submit([(pairs[0][0], pairs[0][1], 4, 4)])  # below 8x8
+
+
+@pytest.mark.parametrize("chunks", [2, 3, 16])
+def test_chunked_batch_equals_each_pair_alone(chunks):
+    """Runs of pairs whose scale overlaps the next run's prepare on a second
+    stream: uniform and mixed batches, more chunks than pairs included."""
+    for shapes in ([(160, 90)] * 7, [(64, 64), (200, 130), (33, 17), (640, 360), (129, 97)]):
+        pairs, keep, host = upload_pairs(shapes, seed=51)
+        batch = Ssimulacra2Batch(max_pairs=8, chunks=chunks)
+        _, norms = batch.Finish(batch.Submit(pairs), raw=True)
+        for i, ((w, h), (a, b)) in enumerate(zip(shapes, host)):
+            assert np.array_equal(norms[i], alone(w, h, a, b)), f"pair {i} {w}x{h}"
+        assert all(v > 0 for v in batch.StageTimes())
+        batch.close()
+    with pytest.raises(InvalidArgument):
+        Ssimulacra2Batch(max_pairs=2, chunks=17)

@@ -119,24 +119,37 @@ def test_4k_matches_oracle(oracle_lib):
     assert_features_close(got, f64, what="4K GPU vs FP64")
 
 
-@pytest.mark.parametrize("fmt", [vqm_amd.PLANES_F16, vqm_amd.PLANES_BF16])
+@pytest.mark.parametrize("fmt", [vqm_amd.PLANES_F16, vqm_amd.PLANES_BF16, vqm_amd.PLANES_U16])
 def test_narrow_planes_match_oracle(oracle_lib, fmt):
-    """FP16/BF16 stored planes: bit-level parity with the oracle's device form
-    rounding the same planes; FP16 stays within the reference's half-plane
-    bounds of the FP32 result (pipeline_test.cc:785-834: score 0.1, norm
-    2e-4)."""
+    """FP16/BF16/mixed stored planes: bit-level parity with the oracle's
+    device form rounding the same planes; FP16 stays within the reference's
+    half-plane bounds of the FP32 result (pipeline_test.cc:785-834: score 0.1,
+    norm 2e-4), and the mixed form within them by far."""
     orig = gradient_image(0)
     for variant in (1, 2):
         dist = gradient_image(variant)
         n, got, res = check_against_oracle(oracle_lib, 32, 32, orig, dist,
-                                           dev_mode=oracle_lib.f32_dev_planes(fmt), plane_format=fmt)
+                                           dev_mode=oracle_lib.f32_dev_for(fmt), plane_format=fmt)
         _, full, _ = oracle_lib.compute(oracle_lib.F32_DEV, 32, 32, orig, dist)
-        if fmt == vqm_amd.PLANES_F16:
+        if fmt != vqm_amd.PLANES_BF16:
             assert abs(res.Score() - oracle_lib.score(n, full)) <= 0.1
             assert np.max(np.abs(got - full)) <= 2e-4
     a, b = synth.pair(0, 0, 512, 512, synth.STRENGTHS[1])
-    check_against_oracle(oracle_lib, 512, 512, a, b, dev_mode=oracle_lib.f32_dev_planes(fmt),
+    check_against_oracle(oracle_lib, 512, 512, a, b, dev_mode=oracle_lib.f32_dev_for(fmt),
                          plane_format=fmt)
+
+
+@pytest.mark.parametrize("strength", synth.STRENGTHS)
+def test_mixed_form_keeps_the_fp32_contract_at_1080p(oracle_lib, strength):
+    """The mixed form (unsigned 16-bit fixed-point planes at levels 0-1, FP32
+    above) is bit-exact with its oracle form and meets the same score and
+    feature bounds against FP64 as the FP32 form."""
+    a, b = synth.pair(1, 7, 1920, 1080, strength)
+    n, got, res = check_against_oracle(oracle_lib, 1920, 1080, a, b, dev_mode=oracle_lib.f32_dev_for(3),
+                                       plane_format=vqm_amd.PLANES_U16)
+    _, f64, _ = oracle_lib.compute(oracle_lib.F64, 1920, 1080, a, b)
+    assert abs(res.Score() - oracle_lib.score(n, f64)) <= SCORE_ABS
+    assert_features_close(got, f64, what="1080p mixed form vs FP64")
 
 
 @pytest.mark.parametrize("strip_rows", [4, 16, 128])

@@ -132,6 +132,12 @@ def test_memory_plan_and_algorithmic_bytes():
     assert b32 / n == pytest.approx(32.94, abs=0.01)
     assert b16 / n == pytest.approx(19.47, abs=0.01)
     assert b32 == pytest.approx(68.30e6, rel=1e-3)
+    # The mixed form: 2-byte elements at levels 0-1, 4-byte above.
+    sizes = vqm_amd.plan_scales(Size(1920, 1080))
+    stored = [sum(not vqm_amd.channel_pruned(c, l, len(sizes)) for c in range(3)) for l in range(len(sizes))]
+    mixed_planes = sum(k * s.width * s.height * (2 if l < 2 else 4) for l, (k, s) in enumerate(zip(stored, sizes)))
+    assert vqm_amd.algorithmic_bytes(Size(1920, 1080), L.PLANES_U16) == 6 * n + 4 * mixed_planes
+    assert b16 < 6 * n + 4 * mixed_planes < 0.65 * b32
     m = vqm_amd.memory_plan(Ssimulacra2Config(size=Size(1920, 1080)))
     assert m.resident_bytes == 2 * 3 * n
     assert m.streaming_bytes == 2 * 3 * n
@@ -154,7 +160,7 @@ def test_last_error_is_thread_local_and_cleared():
 
 def test_server_config_validation_before_device_work():
     """The server rejects a bad configuration before touching the device."""
-    for kwargs in ({"lanes": 17}, {"max_batch": 5000}, {"plane_format": 3}, {"strip_rows": 6}):
+    for kwargs in ({"lanes": 17}, {"max_batch": 5000}, {"plane_format": 4}, {"strip_rows": 6}):
         with pytest.raises(InvalidArgument):
             vqm_amd.Ssimulacra2Server(**kwargs)
     cfg = L.ServerConfigC()

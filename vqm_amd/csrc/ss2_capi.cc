@@ -135,7 +135,7 @@ int vqm_ss2_memory_plan(const vqm_ss2_config* config, vqm_ss2_memory* plan) {
     return Guard([&] {
         Need(config, "config");
         Need(plan, "plan");
-        if (config->plane_format > 2) vqm::ss2::Fail(VQM_INVALID_ARGUMENT, "invalid plane_format");
+        if (config->plane_format > 3) vqm::ss2::Fail(VQM_INVALID_ARGUMENT, "invalid plane_format");
         const auto g = vqm::ss2::BuildPlan(config->width, config->height,
                                            static_cast<vqm::ss2::PlaneFormat>(config->plane_format),
                                            config->strip_rows);
@@ -159,7 +159,7 @@ int vqm_ss2_algorithmic_bytes(uint32_t width, uint32_t height, uint32_t plane_fo
                               uint64_t* bytes) {
     return Guard([&] {
         Need(bytes, "bytes");
-        if (plane_format > 2) vqm::ss2::Fail(VQM_INVALID_ARGUMENT, "invalid plane_format");
+        if (plane_format > 3) vqm::ss2::Fail(VQM_INVALID_ARGUMENT, "invalid plane_format");
         const auto g = vqm::ss2::BuildPlan(width, height,
                                            static_cast<vqm::ss2::PlaneFormat>(plane_format), 0);
         *bytes = vqm::ss2::AlgorithmicBytes(g);

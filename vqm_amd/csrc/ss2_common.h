@@ -4,7 +4,9 @@
 //
 // HBM layout of one pair's workspace (one "slot"):
 //   planes   2 images x (sum over stored (level, channel) planes of
-//            pitch_l * rows_l elements), element type FP32, FP16 or BF16.
+//            pitch_l * rows_l elements). The element type is per level:
+//            FP32, FP16 or BF16 at every level, or unsigned 16-bit fixed
+//            point (q / 65536) at levels 0 and 1 with FP32 at levels 2..5.
 //            Every plane carries a zero border: element (x, y) of a level
 //            lives at (y + kPadTop) * pitch + (x + kPadLeft), and the border
 //            (never written after the workspace is cleared) is the zero
@@ -39,7 +41,16 @@ inline constexpr int kPadLeft = 8;
 inline constexpr int kPrepTileW = 64;
 inline constexpr int kPrepTileH = 32;
 
-enum PlaneFormat : uint32_t { kPlaneF32 = 0, kPlaneF16 = 1, kPlaneBF16 = 2 };
+enum PlaneFormat : uint32_t { kPlaneF32 = 0, kPlaneF16 = 1, kPlaneBF16 = 2, kPlaneU16 = 3 };
+
+// Levels below this one store the format's fine element type; the U16 format
+// stores FP32 from here on.
+inline constexpr uint32_t kFineLevels = 2;
+
+// Bytes of one stored element of `level` in `format`.
+constexpr uint32_t ElementBytes(PlaneFormat format, uint32_t level) {
+    return format == kPlaneF32 ? 4u : format == kPlaneU16 ? (level < kFineLevels ? 2u : 4u) : 2u;
+}
 
 struct ScaleItem {
     uint32_t level;
@@ -58,8 +69,8 @@ struct Plan {
     uint32_t w[kMaxScales], h[kMaxScales];
     uint32_t pitch[kMaxScales], rows[kMaxScales];  // padded plane geometry
     uint32_t mask[kMaxScales];                      // active channels, bit c
-    uint64_t plane_off[kMaxScales][kChannels];      // elements, ~0 = not stored
-    uint64_t image_elems;                           // plane elements per image
+    uint64_t plane_off[kMaxScales][kChannels];      // bytes, ~0 = not stored
+    uint64_t image_bytes;                           // plane bytes per image
     uint32_t item_begin[kMaxScales][kChannels];
     uint32_t item_count[kMaxScales][kChannels];
     uint32_t items;                                 // scale work items per pair
@@ -70,7 +81,7 @@ struct Plan {
 // One pair as the kernels see it.
 struct PairDesc {
     const uint8_t* src[2];     // packed planar u8 sRGB RGB, original then distorted
-    void* planes;              // 2 * image_elems plane elements
+    void* planes;              // 2 * image_bytes of planes
     double* partials;          // items * 6
     double* norms;             // 108
     const Plan* plan;          // device copy of the pair's plan

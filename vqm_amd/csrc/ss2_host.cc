@@ -208,8 +208,8 @@ GeometryPlan BuildPlan(uint32_t width, uint32_t height, PlaneFormat format, uint
     p.levels = scales.count;
     p.strip_h = strip_rows;
     p.plane_stride = static_cast<uint32_t>(Align(static_cast<uint64_t>(width) * height, 4));
-    g.element_bytes = format == kPlaneF32 ? 4 : 2;
-    uint64_t cursor = 0;
+    g.format = format;
+    uint64_t cursor = 0;  // bytes
     for (uint32_t l = 0; l < static_cast<uint32_t>(kMaxScales); ++l) {
         for (int c = 0; c < kChannels; ++c) p.plane_off[l][c] = ~0ull;
         if (l >= scales.count) continue;
@@ -223,12 +223,12 @@ GeometryPlan BuildPlan(uint32_t width, uint32_t height, PlaneFormat format, uint
             if (ChannelPruned(c, l, scales.count)) continue;
             p.mask[l] |= 1u << c;
             p.plane_off[l][c] = cursor;
-            // 64-element alignment keeps every row start and vector store aligned.
-            cursor = Align(cursor + CheckedMul(p.pitch[l], p.rows[l]), 64);
+            // 256-byte alignment keeps every row start and vector store aligned.
+            cursor = Align(cursor + CheckedMul(CheckedMul(p.pitch[l], p.rows[l]), ElementBytes(format, l)), 256);
         }
     }
-    p.image_elems = cursor;
-    g.plane_bytes_per_image = CheckedMul(cursor, g.element_bytes);
+    p.image_bytes = cursor;
+    g.plane_bytes_per_image = cursor;
     // Scale work: per level and active channel, bands of strip_rows rows by
     // 128-column strips, row-major so that consecutive items are neighbours.
     for (uint32_t l = 0; l < scales.count; ++l) {
@@ -259,12 +259,13 @@ WorkspaceLayout LayoutWorkspace(const GeometryPlan& plan) {
 
 uint64_t AlgorithmicBytes(const GeometryPlan& plan) {
     const Plan& p = plan.plan;
-    uint64_t elements = 0;  // stored plane elements of one image, unpadded
+    uint64_t bytes = 0;  // stored plane bytes of one image, unpadded
     for (uint32_t l = 0; l < p.levels; ++l)
         for (int c = 0; c < kChannels; ++c)
-            if (p.mask[l] & (1u << c)) elements += static_cast<uint64_t>(p.w[l]) * p.h[l];
+            if (p.mask[l] & (1u << c))
+                bytes += static_cast<uint64_t>(p.w[l]) * p.h[l] * ElementBytes(plan.format, l);
     const uint64_t input = 3ull * p.width * p.height;
-    return 2 * input + 2 * (2 * elements * plan.element_bytes);
+    return 2 * input + 2 * (2 * bytes);
 }
 
 RangeAllocator::RangeAllocator(uint64_t capacity) : capacity_(capacity / kAlign * kAlign) {

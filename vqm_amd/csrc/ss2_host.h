@@ -60,7 +60,7 @@ struct GeometryPlan {
     Plan plan{};
     std::vector<ScaleItem> items;
     uint64_t plane_bytes_per_image = 0;  // padded plane storage
-    uint32_t element_bytes = 4;
+    PlaneFormat format = kPlaneF32;
     uint32_t prep_tiles = 0;             // prepare blocks per image
 };
 

@@ -186,8 +186,42 @@ struct Narrow {
                   static_cast<uint32_t>(To(v[2])) | (static_cast<uint32_t>(To(v[3])) << 16)};
     }
 };
+// FP32 -> unsigned 16-bit fixed point q / 65536, round to nearest even,
+// clamped to 0 .. 65535 (NaN and negatives to 0). The XYB values of linear
+// RGB in [0, 1] lie in [0, 0.94]: the step is a uniform 2^-16 and zero, the
+// plane borders' value, is exact. Scaling by 2^16 and back is exact.
+__device__ __forceinline__ uint16_t to_unorm16(float v) {
+    const float s = fminf(fmaxf(v * 65536.0f, 0.0f), 65535.0f);
+    return static_cast<uint16_t>(__builtin_rintf(s));
+}
+__device__ __forceinline__ float from_unorm16(uint16_t v) {
+    return static_cast<float>(v) * (1.0f / 65536.0f);
+}
+
 using F16 = Narrow<to_f16, from_f16>;
 using BF16 = Narrow<to_bf16, from_bf16>;
+using U16 = Narrow<to_unorm16, from_unorm16>;
+
+// Element formats per pyramid level: Fine below kFineLevels, Coarse above.
+template <typename FineT, typename CoarseT>
+struct Layout {
+    using Fine = FineT;
+    using Coarse = CoarseT;
+    template <uint32_t L>
+    using At = std::conditional_t<(L < kFineLevels), Fine, Coarse>;
+    static constexpr bool kUniform = std::is_same_v<Fine, Coarse>;
+};
+using LayoutF32 = Layout<F32, F32>;
+using LayoutF16 = Layout<F16, F16>;
+using LayoutBF16 = Layout<BF16, BF16>;
+using LayoutU16 = Layout<U16, F32>;
+
+// Element pointer of a plane: planes of one image start at `base` and
+// `offset` is in bytes.
+template <typename Fmt>
+__device__ __forceinline__ GlobalAS<typename Fmt::E>* plane_at(GlobalAS<uint8_t>* base, uint64_t offset) {
+    return reinterpret_cast<GlobalAS<typename Fmt::E>*>(base + offset);
+}
 
 // Workgroup b runs on XCD b % 8. Renumber so each XCD receives a contiguous
 // run of logical work items: neighbouring strips and tiles, which share halo
@@ -205,10 +239,11 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
 // previous one. The ladder keeps a level's linear samples channel-planar.
 constexpr uint32_t kLadder = (kPrepTileW / 2) * (kPrepTileH / 2);
 
-template <typename Fmt, uint32_t L>
-__device__ __forceinline__ void prepare_level(ConstAS<Plan>* plan, GlobalAS<typename Fmt::E>* planes,
+template <typename Lay, uint32_t L>
+__device__ __forceinline__ void prepare_level(ConstAS<Plan>* plan, GlobalAS<uint8_t>* planes,
                                               uint32_t X0, uint32_t Y0, uint32_t tid,
                                               float (*ladder)[3][kLadder]) {
+    using Fmt = typename Lay::template At<L>;
     // Level L from the ladder level L-1 published, with the clamp of level
     // L-1's last row and column applied in this tile's coordinates
     // (prepare.comp:146-168).
@@ -240,14 +275,14 @@ __device__ __forceinline__ void prepare_level(ConstAS<Plan>* plan, GlobalAS<type
             const uint64_t at = static_cast<uint64_t>(gy + kPadTop) * plan->pitch[L] + gx + kPadLeft;
 #pragma unroll
             for (uint32_t c = 0; c < 3; ++c)
-                if (mask & (1u << c)) Fmt::store1(planes + plan->plane_off[L][c] + at, o[c]);
+                if (mask & (1u << c)) Fmt::store1(plane_at<Fmt>(planes, plan->plane_off[L][c]) + at, o[c]);
         }
     }
 #pragma unroll
     for (uint32_t c = 0; c < 3; ++c) next[c][jy * side_x + jx] = v[c];
 }
 
-template <typename Fmt>
+template <typename Lay>
 __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict__ pairs,
                                                       const WorkRef* __restrict__ work,
                                                       uint32_t blocks, uint32_t tiles_per_image,
@@ -279,8 +314,7 @@ __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict
     const uint32_t tile_x = tile % tiles_x, tile_y = tile / tiles_x;
     const uint32_t X0 = tile_x * kPrepTileW, Y0 = tile_y * kPrepTileH;
     const uint32_t tx = tid % 16u, ty = tid / 16u;
-    using E = typename Fmt::E;
-    GlobalAS<E>* planes = as_global(static_cast<E*>(pdp->planes)) + (image ? plan->image_elems : 0);
+    GlobalAS<uint8_t>* planes = as_global(static_cast<uint8_t*>(pdp->planes)) + (image ? plan->image_bytes : 0);
     // Single-pair launches name their inputs directly; batches use pd.src.
     const uint8_t* src_ptr = image ? (src1 != nullptr ? src1 : pdp->src[1])
                                    : (src0 != nullptr ? src0 : pdp->src[0]);
@@ -325,7 +359,8 @@ __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict
 #pragma unroll
             for (uint32_t c = 0; c < 3; ++c) {
                 if (!(mask0 & (1u << c))) continue;
-                GlobalAS<E>* p = planes + plan->plane_off[0][c] + at;
+                using Fmt = typename Lay::template At<0>;
+                auto* p = plane_at<Fmt>(planes, plan->plane_off[0][c]) + at;
                 if (bx + 3u < W) {
                     const float q[4] = {v[0][c], v[1][c], v[2][c], v[3][c]};
                     Fmt::store4(p, q);
@@ -367,7 +402,8 @@ __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict
 #pragma unroll
             for (uint32_t c = 0; c < 3; ++c) {
                 if (!(mask1 & (1u << c))) continue;
-                GlobalAS<E>* p = planes + plan->plane_off[1][c] + at;
+                using Fmt = typename Lay::template At<1>;
+                auto* p = plane_at<Fmt>(planes, plan->plane_off[1][c]) + at;
                 if (gx + 1u < W1) {
                     Fmt::store2(p, x0[c], x1[c]);
                 } else if (gx < W1) {
@@ -378,10 +414,10 @@ __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict
     }
 
     // Levels 2 .. 5: each waits for the previous level's ladder.
-    if (levels > 2) { __syncthreads(); prepare_level<Fmt, 2>(plan, planes, X0, Y0, tid, ladder); }
-    if (levels > 3) { __syncthreads(); prepare_level<Fmt, 3>(plan, planes, X0, Y0, tid, ladder); }
-    if (levels > 4) { __syncthreads(); prepare_level<Fmt, 4>(plan, planes, X0, Y0, tid, ladder); }
-    if (levels > 5) { __syncthreads(); prepare_level<Fmt, 5>(plan, planes, X0, Y0, tid, ladder); }
+    if (levels > 2) { __syncthreads(); prepare_level<Lay, 2>(plan, planes, X0, Y0, tid, ladder); }
+    if (levels > 3) { __syncthreads(); prepare_level<Lay, 3>(plan, planes, X0, Y0, tid, ladder); }
+    if (levels > 4) { __syncthreads(); prepare_level<Lay, 4>(plan, planes, X0, Y0, tid, ladder); }
+    if (levels > 5) { __syncthreads(); prepare_level<Lay, 5>(plan, planes, X0, Y0, tid, ladder); }
 }
 
 // ---- scale ------------------------------------------------------------------
@@ -398,38 +434,24 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// One scale work item: the strip of `level`, `channel` starting at (x0, y0),
+// with the plane elements in format Fmt.
 template <typename Fmt>
-__global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ pairs,
-                                                   const WorkRef* __restrict__ work,
-                                                   uint32_t blocks, uint32_t items_per_pair) {
-    __shared__ __align__(16) float ring_a[kRing][kSrcCols];
-    __shared__ __align__(16) float ring_b[kRing][kSrcCols];
-    __shared__ __align__(16) float prod[3][kSrcCols];
-
-    uint32_t pair_idx, item_idx;
-    if (work != nullptr) {
-        ConstAS<WorkRef>* w = as_const(work) + blockIdx.x;
-        pair_idx = w->pair;
-        item_idx = w->index;
-    } else {
-        const uint32_t l = xcd_remap(blockIdx.x, blocks);
-        pair_idx = l / items_per_pair;
-        item_idx = l % items_per_pair;
-    }
-    ConstAS<PairDesc>* pdp = as_const(pairs) + pair_idx;
-    ConstAS<Plan>* plan = as_const(pdp->plan);
-    ConstAS<ScaleItem>* item = as_const(pdp->items) + item_idx;
-    const uint32_t level = item->level, channel = item->channel;
-    const uint32_t x0 = item->x0, y0 = item->y0;
+__device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>* plan, uint32_t item_idx,
+                                           uint32_t level, uint32_t channel, uint32_t x0, uint32_t y0,
+                                           float (*ring_a)[kSrcCols], float (*ring_b)[kSrcCols],
+                                           float (*prod)[kSrcCols]) {
     const uint32_t W = plan->w[level], H = plan->h[level], pitch = plan->pitch[level];
     const int total = static_cast<int>(plan->strip_h) + 2 * kRadius;
 
     using E = typename Fmt::E;
     // Source row r (0 .. total-1) is image row y0 - 6 + r, padded row y0 + r;
     // source column j (0 .. 139) is image column x0 - 6 + j, padded x0 + 2 + j.
-    const uint64_t origin = plan->plane_off[level][channel] + static_cast<uint64_t>(y0) * pitch + x0 + 2u;
-    GlobalAS<const E>* pa = as_global(static_cast<const E*>(pdp->planes)) + origin;
-    GlobalAS<const E>* pb = pa + plan->image_elems;
+    GlobalAS<const uint8_t>* base = as_global(static_cast<const uint8_t*>(pdp->planes));
+    GlobalAS<const E>* pa = reinterpret_cast<GlobalAS<const E>*>(base + plan->plane_off[level][channel]) +
+                            (static_cast<uint64_t>(y0) * pitch + x0 + 2u);
+    GlobalAS<const E>* pb = reinterpret_cast<GlobalAS<const E>*>(
+        reinterpret_cast<GlobalAS<const uint8_t>*>(pa) + plan->image_bytes);
 
     const int lane = static_cast<int>(threadIdx.x);
     const bool halo = lane < (kSrcCols - kStripCols) / 2;  // lanes 0..5
@@ -556,6 +578,36 @@ __global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ 
     }
 }
 
+template <typename Lay>
+__global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ pairs,
+                                                   const WorkRef* __restrict__ work,
+                                                   uint32_t blocks, uint32_t items_per_pair) {
+    __shared__ __align__(16) float ring_a[kRing][kSrcCols];
+    __shared__ __align__(16) float ring_b[kRing][kSrcCols];
+    __shared__ __align__(16) float prod[3][kSrcCols];
+
+    uint32_t pair_idx, item_idx;
+    if (work != nullptr) {
+        ConstAS<WorkRef>* w = as_const(work) + blockIdx.x;
+        pair_idx = w->pair;
+        item_idx = w->index;
+    } else {
+        const uint32_t l = xcd_remap(blockIdx.x, blocks);
+        pair_idx = l / items_per_pair;
+        item_idx = l % items_per_pair;
+    }
+    ConstAS<PairDesc>* pdp = as_const(pairs) + pair_idx;
+    ConstAS<Plan>* plan = as_const(pdp->plan);
+    ConstAS<ScaleItem>* item = as_const(pdp->items) + item_idx;
+    const uint32_t level = item->level, channel = item->channel;
+    const uint32_t x0 = item->x0, y0 = item->y0;
+    // The item's level picks the element format (wave-uniform).
+    if (Lay::kUniform || level < kFineLevels)
+        scale_item<typename Lay::Fine>(pdp, plan, item_idx, level, channel, x0, y0, ring_a, ring_b, prod);
+    else
+        scale_item<typename Lay::Coarse>(pdp, plan, item_idx, level, channel, x0, y0, ring_a, ring_b, prod);
+}
+
 // ---- reduce -----------------------------------------------------------------
 
 __global__ __launch_bounds__(128) void reduce_kernel(const PairDesc* __restrict__ pairs) {
@@ -591,19 +643,19 @@ __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src
         dst[i] = src[i];
 }
 
-template <typename Fmt>
+template <typename Lay>
 hipError_t launch_prepare_t(const PairDesc* pairs, const WorkRef* work, uint32_t blocks,
                             uint32_t tiles, const float* lut, const uint8_t* src0,
                             const uint8_t* src1, hipStream_t stream) {
-    hipLaunchKernelGGL(prepare_kernel<Fmt>, dim3(blocks), dim3(256), 0, stream, pairs, work,
+    hipLaunchKernelGGL(prepare_kernel<Lay>, dim3(blocks), dim3(256), 0, stream, pairs, work,
                        blocks, tiles, lut, src0, src1);
     return hipGetLastError();
 }
 
-template <typename Fmt>
+template <typename Lay>
 hipError_t launch_scale_t(const PairDesc* pairs, const WorkRef* work, uint32_t blocks,
                           uint32_t items, hipStream_t stream) {
-    hipLaunchKernelGGL(scale_kernel<Fmt>, dim3(blocks), dim3(64), 0, stream, pairs, work,
+    hipLaunchKernelGGL(scale_kernel<Lay>, dim3(blocks), dim3(64), 0, stream, pairs, work,
                        blocks, items);
     return hipGetLastError();
 }
@@ -625,11 +677,13 @@ hipError_t LaunchPrepare(PlaneFormat format, const PairDesc* pairs, const WorkRe
     if (blocks == 0) return hipSuccess;
     switch (format) {
         case kPlaneF32:
-            return launch_prepare_t<F32>(pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
+            return launch_prepare_t<LayoutF32>(pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
         case kPlaneF16:
-            return launch_prepare_t<F16>(pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
+            return launch_prepare_t<LayoutF16>(pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
         case kPlaneBF16:
-            return launch_prepare_t<BF16>(pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
+            return launch_prepare_t<LayoutBF16>(pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
+        case kPlaneU16:
+            return launch_prepare_t<LayoutU16>(pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
     }
     return hipErrorInvalidValue;
 }
@@ -638,9 +692,10 @@ hipError_t LaunchScale(PlaneFormat format, const PairDesc* pairs, const WorkRef*
                        uint32_t blocks, uint32_t items_per_pair, hipStream_t stream) {
     if (blocks == 0) return hipSuccess;
     switch (format) {
-        case kPlaneF32: return launch_scale_t<F32>(pairs, work, blocks, items_per_pair, stream);
-        case kPlaneF16: return launch_scale_t<F16>(pairs, work, blocks, items_per_pair, stream);
-        case kPlaneBF16: return launch_scale_t<BF16>(pairs, work, blocks, items_per_pair, stream);
+        case kPlaneF32: return launch_scale_t<LayoutF32>(pairs, work, blocks, items_per_pair, stream);
+        case kPlaneF16: return launch_scale_t<LayoutF16>(pairs, work, blocks, items_per_pair, stream);
+        case kPlaneBF16: return launch_scale_t<LayoutBF16>(pairs, work, blocks, items_per_pair, stream);
+        case kPlaneU16: return launch_scale_t<LayoutU16>(pairs, work, blocks, items_per_pair, stream);
     }
     return hipErrorInvalidValue;
 }

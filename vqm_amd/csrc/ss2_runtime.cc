@@ -33,7 +33,7 @@ vqm_image_format Srgb8Format() {
 }
 
 PlaneFormat ToFormat(uint32_t f) {
-    if (f > 2) Fail(VQM_INVALID_ARGUMENT, "plane_format must be VQM_PLANES_F32, _F16 or _BF16");
+    if (f > 3) Fail(VQM_INVALID_ARGUMENT, "plane_format must be VQM_PLANES_F32, _F16, _BF16 or _U16");
     return static_cast<PlaneFormat>(f);
 }
 
@@ -256,18 +256,23 @@ Lane::Lane(uint32_t max_pairs)
       pairs_(max_pairs * sizeof(PairDesc)),
       norms_(static_cast<uint64_t>(max_pairs) * kNorms * sizeof(double)),
       h_pairs_(max_pairs * sizeof(PairDesc)),
-      h_norms_(static_cast<uint64_t>(max_pairs) * kNorms * sizeof(double)) {}
-
-Lane::~Lane() {
-    if (pending_) (void)hipEventSynchronize(ev_[4].get());
+      h_norms_(static_cast<uint64_t>(max_pairs) * kNorms * sizeof(double)) {
+    for (uint32_t i = 0; i < 4 * kMaxChunks + 3; ++i) ev_.push_back(std::make_unique<Event>());
 }
 
-void Lane::Launch(PlaneFormat fmt, const float* lut, const LaunchEntry* entries, uint32_t count) {
+Lane::~Lane() {
+    if (pending_) (void)hipEventSynchronize(Ev(DoneEvent()).get());
+}
+
+void Lane::Launch(PlaneFormat fmt, const float* lut, const LaunchEntry* entries, uint32_t count,
+                  uint32_t chunks) {
     if (pending_) Fail(VQM_LOGIC_ERROR, "lane: the previous launch has not finished");
     if (count == 0 || count > max_pairs_) Fail(VQM_INVALID_ARGUMENT, "lane: bad pair count");
+    chunks = std::max(1u, std::min({chunks, count, kMaxChunks}));
     auto* hp = h_pairs_.as<PairDesc>();
     bool uniform = true;
     uint64_t prep_work = 0, scale_work = 0;
+    std::vector<uint64_t> prep_at(count + 1, 0), scale_at(count + 1, 0);  // work-list offsets
     for (uint32_t i = 0; i < count; ++i) {
         const LaunchEntry& e = entries[i];
         PairDesc& pd = hp[i];
@@ -281,11 +286,14 @@ void Lane::Launch(PlaneFormat fmt, const float* lut, const LaunchEntry* entries,
         uniform = uniform && e.plan == entries[0].plan;
         prep_work += 2ull * e.plan->host.prep_tiles;
         scale_work += e.plan->host.plan.items;
+        prep_at[i + 1] = prep_work;
+        scale_at[i + 1] = scale_work;
     }
     if (prep_work > 0x7FFFFFFFull || scale_work > 0x7FFFFFFFull)
         Fail(VQM_INVALID_ARGUMENT, "the submission has too many work items");
 
     const hipStream_t s = stream_.get();
+    const hipStream_t t = chunks > 1 ? scale_stream_.get() : s;
     for (uint32_t i = 0; i < count; ++i)
         for (int k = 0; k < 2; ++k)
             if (entries[i].host[k])
@@ -311,8 +319,8 @@ void Lane::Launch(PlaneFormat fmt, const float* lut, const LaunchEntry* entries,
         for (uint32_t i = 0; i < count; ++i) {
             const GeometryPlan& g = entries[i].plan->host;
             for (uint32_t image = 0; image < 2; ++image)
-                for (uint32_t t = 0; t < g.prep_tiles; ++t) wp[np++] = WorkRef{i, t | (image << 31)};
-            for (uint32_t t = 0; t < g.plan.items; ++t) ws[ns++] = WorkRef{i, t};
+                for (uint32_t k = 0; k < g.prep_tiles; ++k) wp[np++] = WorkRef{i, k | (image << 31)};
+            for (uint32_t k = 0; k < g.plan.items; ++k) ws[ns++] = WorkRef{i, k};
         }
         Check(hipMemcpyAsync(work_prep_.get(), wp, np * sizeof(WorkRef), hipMemcpyHostToDevice, s),
               "upload prepare work");
@@ -323,26 +331,44 @@ void Lane::Launch(PlaneFormat fmt, const float* lut, const LaunchEntry* entries,
     }
     const GeometryPlan& g0 = entries[0].plan->host;
     const PairDesc* dpairs = pairs_.as<PairDesc>();
-    Check(hipEventRecord(ev_[0].get(), s), "hipEventRecord");
-    Check(LaunchPrepare(fmt, dpairs, wprep, static_cast<uint32_t>(prep_work), g0.prep_tiles, lut,
-                        nullptr, nullptr, s),
-          "launch prepare");
-    Check(hipEventRecord(ev_[1].get(), s), "hipEventRecord");
-    Check(LaunchScale(fmt, dpairs, wscale, static_cast<uint32_t>(scale_work), g0.plan.items, s),
-          "launch scale");
-    Check(hipEventRecord(ev_[2].get(), s), "hipEventRecord");
-    Check(LaunchReduce(dpairs, count, s), "launch reduce");
-    Check(hipEventRecord(ev_[3].get(), s), "hipEventRecord");
+    chunks_ = chunks;
+    for (uint32_t c = 0; c < chunks; ++c) {
+        const uint32_t i0 = static_cast<uint32_t>(uint64_t(c) * count / chunks);
+        const uint32_t i1 = static_cast<uint32_t>(uint64_t(c + 1) * count / chunks);
+        // Uniform runs index pairs from their first descriptor; work lists
+        // hold absolute pair indices and are cut at the run's offsets.
+        const PairDesc* base = uniform ? dpairs + i0 : dpairs;
+        const WorkRef* wp = uniform ? nullptr : wprep + prep_at[i0];
+        const auto np = static_cast<uint32_t>(prep_at[i1] - prep_at[i0]);
+        Check(hipEventRecord(Ev(4 * c).get(), s), "hipEventRecord");
+        Check(LaunchPrepare(fmt, base, wp, np, g0.prep_tiles, lut, nullptr, nullptr, s), "launch prepare");
+        Check(hipEventRecord(Ev(4 * c + 1).get(), s), "hipEventRecord");
+    }
+    for (uint32_t c = 0; c < chunks; ++c) {
+        const uint32_t i0 = static_cast<uint32_t>(uint64_t(c) * count / chunks);
+        const uint32_t i1 = static_cast<uint32_t>(uint64_t(c + 1) * count / chunks);
+        const PairDesc* base = uniform ? dpairs + i0 : dpairs;
+        const WorkRef* ws = uniform ? nullptr : wscale + scale_at[i0];
+        const auto ns = static_cast<uint32_t>(scale_at[i1] - scale_at[i0]);
+        if (t != s) Check(hipStreamWaitEvent(t, Ev(4 * c + 1).get(), 0), "order scale after prepare");
+        Check(hipEventRecord(Ev(4 * c + 2).get(), t), "hipEventRecord");
+        Check(LaunchScale(fmt, base, ws, ns, g0.plan.items, t), "launch scale");
+        Check(hipEventRecord(Ev(4 * c + 3).get(), t), "hipEventRecord");
+    }
+    const uint32_t r = 4 * chunks;
+    Check(hipEventRecord(Ev(r).get(), t), "hipEventRecord");
+    Check(LaunchReduce(dpairs, count, t), "launch reduce");
+    Check(hipEventRecord(Ev(r + 1).get(), t), "hipEventRecord");
     Check(hipMemcpyAsync(h_norms_.get(), norms_.get(), static_cast<uint64_t>(count) * kNorms * sizeof(double),
-                         hipMemcpyDeviceToHost, s),
+                         hipMemcpyDeviceToHost, t),
           "read back norms");
-    Check(hipEventRecord(ev_[4].get(), s), "hipEventRecord");
+    Check(hipEventRecord(Ev(r + 2).get(), t), "hipEventRecord");
     pending_ = true;
 }
 
 bool Lane::Done() const {
     if (!pending_) return true;
-    const hipError_t e = hipEventQuery(ev_[4].get());
+    const hipError_t e = hipEventQuery(Ev(DoneEvent()).get());
     if (e == hipErrorNotReady) return false;
     Check(e, "query lane");
     return true;
@@ -351,11 +377,22 @@ bool Lane::Done() const {
 void Lane::Wait() {
     if (!pending_) return;
     pending_ = false;
-    Check(hipEventSynchronize(ev_[4].get()), "wait for launch");
+    Check(hipEventSynchronize(Ev(DoneEvent()).get()), "wait for launch");
+}
+
+void Lane::Synchronize() {
+    (void)hipStreamSynchronize(stream_.get());
+    (void)hipStreamSynchronize(scale_stream_.get());
 }
 
 std::array<float, 3> Lane::StageTimes() const {
-    return {Elapsed(ev_[0], ev_[1]), Elapsed(ev_[1], ev_[2]), Elapsed(ev_[2], ev_[3])};
+    std::array<float, 3> t{};
+    for (uint32_t c = 0; c < chunks_; ++c) {
+        t[0] += Elapsed(Ev(4 * c), Ev(4 * c + 1));
+        t[1] += Elapsed(Ev(4 * c + 2), Ev(4 * c + 3));
+    }
+    t[2] = Elapsed(Ev(4 * chunks_), Ev(4 * chunks_ + 1));
+    return t;
 }
 
 // ---- pipeline -----------------------------------------------------------------
@@ -643,6 +680,7 @@ namespace {
 const vqm_ss2_batch_config& ValidatedBatch(const vqm_ss2_batch_config& config) {
     ToFormat(config.plane_format);
     if (config.max_pairs == 0) Fail(VQM_INVALID_ARGUMENT, "max_pairs must be at least 1");
+    if (config.chunks > Lane::kMaxChunks) Fail(VQM_INVALID_ARGUMENT, "chunks must be at most 16");
     return config;
 }
 }  // namespace
@@ -678,7 +716,8 @@ uint64_t Batch::Submit(const vqm_ss2_device_pair* pairs, uint32_t count) {
             e.src[1] = static_cast<const uint8_t*>(pairs[i].distorted);
             taken.push_back(Taken{pairs[i].width, pairs[i].height, std::move(ws)});
         }
-        lane_.Launch(static_cast<PlaneFormat>(config_.plane_format), ctx_.lut, entries.data(), count);
+        lane_.Launch(static_cast<PlaneFormat>(config_.plane_format), ctx_.lut, entries.data(), count,
+                     config_.chunks);
     } catch (...) {
         for (Taken& t : taken) pools_.Return(t.width, t.height, false, std::move(t.ws));
         throw;

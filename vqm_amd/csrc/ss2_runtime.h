@@ -209,17 +209,26 @@ struct LaunchEntry {
 // holds one launch at a time.
 class Lane {
 public:
+    static constexpr uint32_t kMaxChunks = 16;
     explicit Lane(uint32_t max_pairs);
     ~Lane();
     // Queues the uploads, the prepare, scale and reduce launches and the norm
-    // read-back on the lane's stream, then returns.
-    void Launch(PlaneFormat format, const float* lut, const LaunchEntry* entries, uint32_t count);
+    // read-back, then returns. With chunks > 1 the pairs are split into that
+    // many runs of consecutive pairs: the prepare launches of the runs follow
+    // one another on the lane's first stream, and each run's scale launch
+    // follows on a second stream as soon as its prepare is done, so the
+    // scale of one run overlaps the prepare of the next.
+    void Launch(PlaneFormat format, const float* lut, const LaunchEntry* entries, uint32_t count,
+                uint32_t chunks = 1);
     bool Done() const;
     void Wait();
-    // Forgets a launch that failed part-way (its stream already synchronized).
+    // Forgets a launch that failed part-way (its streams already synchronized).
     void Abandon() { pending_ = false; }
+    void Synchronize();
     // Norms of entry i of the finished launch (pinned host memory).
     const double* Norms(uint32_t i) const { return h_norms_.as<double>() + static_cast<uint64_t>(i) * kNorms; }
+    // Milliseconds the prepare, scale and reduce launches of the finished
+    // launch ran, each summed over its chunks (chunks may overlap).
     std::array<float, 3> StageTimes() const;
     hipStream_t stream() const { return stream_.get(); }
     uint32_t max_pairs() const { return max_pairs_; }
@@ -228,9 +237,15 @@ private:
     uint32_t max_pairs_;
     DeviceBuffer pairs_, work_prep_, work_scale_, norms_;
     PinnedBuffer h_pairs_, h_work_prep_, h_work_scale_, h_norms_;
-    Stream stream_;
-    Event ev_[5];
+    Stream stream_, scale_stream_;
+    // Per chunk: prepare start/end, scale start/end; then reduce start/end
+    // and the end of the read-back.
+    std::vector<std::unique_ptr<Event>> ev_;
+    uint32_t chunks_ = 1;
     bool pending_ = false;
+    Event& Ev(uint32_t i) { return *ev_[i]; }
+    const Event& Ev(uint32_t i) const { return *ev_[i]; }
+    uint32_t DoneEvent() const { return 4 * chunks_ + 2; }
 };
 
 class Pipeline {

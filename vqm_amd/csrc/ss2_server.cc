@@ -199,7 +199,7 @@ void Server::DispatchLoop() {
         if (status != VQM_OK) {
             // Whatever was queued before the failure must end before the
             // workspaces are reused.
-            (void)hipStreamSynchronize(lanes_[lane]->stream());
+            lanes_[lane]->Synchronize();
             lanes_[lane]->Abandon();
             {
                 std::lock_guard<std::mutex> pl(pool_mu_);

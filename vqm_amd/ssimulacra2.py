@@ -597,10 +597,11 @@ class Ssimulacra2Batch:
     Pairs are device-resident (DeviceImage pointers)."""
 
     def __init__(self, max_pairs: int, plane_format: int = L.PLANES_F32, hbm_budget: int = 0,
-                 strip_rows: int = 0):
+                 strip_rows: int = 0, chunks: int = 1):
         cfg = L.BatchConfigC()
         cfg.plane_format = plane_format
         cfg.strip_rows = strip_rows
+        cfg.chunks = chunks
         cfg.max_pairs = max_pairs
         cfg.hbm_budget = hbm_budget
         h = ctypes.c_void_p()
