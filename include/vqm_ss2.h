@@ -256,6 +256,9 @@ typedef struct vqm_ss2_batch_config {
                                 runs of consecutive pairs whose scale launches
                                 (second stream) overlap the next run's prepare */
     uint64_t hbm_budget;     /* device bytes the workspace pool may hold, 0 = unlimited */
+    uint32_t autotune;       /* 1: strip_rows and chunks come from vqm_ss2_autotune
+                                for each geometry the first time it is seen */
+    uint32_t reserved;
 } vqm_ss2_batch_config;
 
 /* One pair of caller-owned device images (packed u8 sRGB RGB). */
@@ -285,6 +288,25 @@ int vqm_ss2_batch_stage_times(const vqm_ss2_batch* batch, float ms[3]);
 
 /* Device bytes held by the batch's workspace pool. */
 int vqm_ss2_batch_pool_bytes(const vqm_ss2_batch* batch, uint64_t* bytes);
+
+/* ---- launch-shape autotuner (no reference counterpart) ------------------ */
+
+typedef struct vqm_ss2_tune_result {
+    uint32_t strip_rows;     /* scale work-item height chosen */
+    uint32_t chunks;         /* batch chunk count chosen */
+    uint32_t candidates;     /* shapes timed */
+    uint32_t cached;         /* 1 when the choice came from the process cache */
+    float ms_per_launch;     /* measured time of the chosen shape */
+    float default_ms;        /* measured time of 64 rows, 1 chunk */
+} vqm_ss2_tune_result;
+
+/* Times every (strip height 16/32/64/128) x (chunks 1/2/4/8) shape for a
+ * launch of `pairs` pairs of this geometry and format (rounded up to a power
+ * of two, at most 512) on the device and returns the fastest; the choice is
+ * cached for the process. Shapes do not change results, only time. Holds
+ * up to 16 GiB of temporary device memory while it runs. */
+int vqm_ss2_autotune(uint32_t width, uint32_t height, uint32_t plane_format, uint32_t pairs,
+                     vqm_ss2_tune_result* result);
 
 /* ---- serving (no counterpart in the reference: its closest caller is the
  * worker pool of Ssimulacra2Pool, vqm_pool.h:35-86, pool.cc:24-116, and

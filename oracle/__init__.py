@@ -60,6 +60,7 @@ def lib() -> ctypes.CDLL:
         P = ctypes.POINTER
         u8p, f32p, f64p, u32p = P(ctypes.c_uint8), P(ctypes.c_float), P(ctypes.c_double), P(ctypes.c_uint32)
         sig = {
+            "ss2o_round_plane": (ctypes.c_float, [ctypes.c_int32, ctypes.c_float]),
             "ss2o_plan_scales": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_uint32, u32p]),
             "ss2o_gaussian_taps": (None, [f32p]),
             "ss2o_channel_pruned": (ctypes.c_int32, [ctypes.c_int32] * 3),
@@ -217,3 +218,8 @@ def pyramid(width: int, height: int, image: np.ndarray, f64: bool = True):
 
 def image_bytes(width: int, height: int) -> int:
     return int(lib().ss2o_image_bytes(width, height))
+
+
+def round_plane(plane_format: int, v: float) -> float:
+    """A stored plane value as `plane_format` keeps it (see ss2_oracle.h)."""
+    return float(lib().ss2o_round_plane(plane_format, v))

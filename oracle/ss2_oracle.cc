@@ -477,6 +477,15 @@ int32_t compute(const ss2o_mode& mode, uint32_t width, uint32_t height,
 
 extern "C" {
 
+float ss2o_round_plane(int32_t plane_format, float v) {
+    switch (plane_format) {
+        case 1: return round_half(v);
+        case 2: return round_bf16(v);
+        case 3: return round_unorm16(v);
+        default: return v;
+    }
+}
+
 int32_t ss2o_plan_scales(uint32_t width, uint32_t height, uint32_t sizes[12]) {
     if (width < kMinDim || height < kMinDim) return -1;
     int32_t count = 0;

@@ -49,6 +49,11 @@ typedef struct ss2o_mode {
     int32_t sum_quads;    /* 1 = FP32 sums of 4 adjacent pixels, then FP64; 0 = FP64 per pixel */
 } ss2o_mode;
 
+/* One stored plane value as plane_format keeps it: 1 FP16 and 2 BF16 (round
+ * to nearest even), 3 unsigned 16-bit fixed point q / 65536 (round to
+ * nearest even, clamped to 0..65535, NaN to 0); 0 returns v. */
+float ss2o_round_plane(int32_t plane_format, float v);
+
 /* Scale plan: returns the level count (1..6) and writes width,height pairs,
  * or -1 below 8x8. pipeline.cc:28-48. */
 int32_t ss2o_plan_scales(uint32_t width, uint32_t height, uint32_t sizes[12]);

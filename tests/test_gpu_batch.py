@@ -95,3 +95,23 @@ def test_chunked_batch_equals_each_pair_alone(chunks):
         batch.close()
     with pytest.raises(InvalidArgument):
         Ssimulacra2Batch(max_pairs=2, chunks=17)
+
+
+def test_autotune_picks_a_shape_and_keeps_results():
+    """The tuner times every (strip height, chunks) shape and caches the
+    fastest; launch shapes never change a result bit."""
+    first = vqm_amd.autotune(Size(320, 180), vqm_amd.PLANES_F32, pairs=6)
+    assert first["strip_rows"] in (16, 32, 64, 128) and first["chunks"] in (1, 2, 4, 8)
+    assert first["candidates"] == 4 * 4 and not first["cached"]
+    assert 0.0 < first["ms_per_launch"] <= first["default_ms"] * 1.0001
+    again = vqm_amd.autotune(Size(320, 180), vqm_amd.PLANES_F32, pairs=5)  # same size class
+    assert again["cached"] and again["strip_rows"] == first["strip_rows"]
+    shapes = [(320, 180)] * 6 + [(64, 48), (200, 120)]
+    pairs, keep, host = upload_pairs(shapes, seed=61)
+    plain = Ssimulacra2Batch(max_pairs=8)
+    tuned = Ssimulacra2Batch(max_pairs=8, autotune=True)
+    _, a = plain.Finish(plain.Submit(pairs), raw=True)
+    _, b = tuned.Finish(tuned.Submit(pairs), raw=True)
+    assert np.array_equal(a, b)
+    with pytest.raises(InvalidArgument):
+        vqm_amd.autotune(Size(4, 4))

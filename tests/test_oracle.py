@@ -274,3 +274,35 @@ def test_fp32_forms_within_contract_of_fp64(oracle_lib, strength):
         _, f32, _ = oracle_lib.compute(form, 512, 512, a, b)
         assert abs(oracle_lib.score(n, f32) - oracle_lib.score(n, f64)) <= SCORE_ABS
         assert_features_close(f32, f64, what="fp32 vs fp64")
+
+
+def _bf16_reference(v: np.ndarray) -> np.ndarray:
+    """Round float32 values to the nearest bfloat16 (ties to even) through
+    float64: keep 8 significant bits of the float32 significand."""
+    m, e = np.frexp(v.astype(np.float64))        # v = m * 2^e, 0.5 <= |m| < 1
+    scaled = np.ldexp(m, 8)                      # 8 significant bits above the point
+    return np.ldexp(np.rint(scaled), e - 8).astype(np.float32)
+
+
+def test_plane_roundings(oracle_lib):
+    """The narrow plane formats against independent references: numpy's
+    IEEE binary16 conversion (round to nearest even), a float64 restatement
+    of bfloat16 rounding, and numpy's rint for unsigned 16-bit fixed point
+    q / 65536 with clamping; ties and edges included."""
+    rng = np.random.default_rng(5)
+    vals = np.concatenate([
+        rng.uniform(0.0, 1.0, 4000), rng.uniform(-2.0, 2.0, 1000),
+        (np.arange(0, 2000) + 0.5) / 65536.0,             # unorm16 ties
+        np.array([0.0, -0.0, 1e-9, 0.93, 0.9999, 65535.5 / 65536, 1.0, 3.0, -1e-3]),
+        (np.arange(1, 400) + 0.5) * 2.0 ** -11,           # binary16 ties near 0.1-0.2
+        np.float32(1.0) + (np.arange(0, 200) * 2 + 1) * np.float32(2.0 ** -8) / 2,  # bf16 ties
+    ]).astype(np.float32)
+    f16 = vals.astype(np.float16).astype(np.float32)
+    bf16 = _bf16_reference(vals)
+    u16 = (np.clip(np.rint(np.maximum(vals.astype(np.float64), 0.0) * 65536.0), 0, 65535) / 65536.0).astype(np.float32)
+    for v, a, b, c in zip(vals, f16, bf16, u16):
+        assert oracle_lib.round_plane(1, float(v)) == a, v
+        assert oracle_lib.round_plane(2, float(v)) == b, v
+        assert oracle_lib.round_plane(3, float(v)) == c, v
+    assert oracle_lib.round_plane(3, float("nan")) == 0.0
+    assert oracle_lib.round_plane(0, 0.123) == np.float32(0.123)

@@ -60,7 +60,12 @@ class MemoryC(ctypes.Structure):
 
 class BatchConfigC(ctypes.Structure):
     _fields_ = [("plane_format", u32), ("strip_rows", u32), ("max_pairs", u32),
-                ("chunks", u32), ("hbm_budget", u64)]
+                ("chunks", u32), ("hbm_budget", u64), ("autotune", u32), ("reserved", u32)]
+
+
+class TuneResultC(ctypes.Structure):
+    _fields_ = [("strip_rows", u32), ("chunks", u32), ("candidates", u32), ("cached", u32),
+                ("ms_per_launch", f32), ("default_ms", f32)]
 
 
 class DevicePairC(ctypes.Structure):
@@ -130,6 +135,7 @@ SIGNATURES = {
     "vqm_ss2_batch_finish": (ctypes.c_int, [vp, u64, ctypes.POINTER(u32), ctypes.POINTER(f64)]),
     "vqm_ss2_batch_stage_times": (ctypes.c_int, [vp, ctypes.POINTER(f32)]),
     "vqm_ss2_batch_pool_bytes": (ctypes.c_int, [vp, ctypes.POINTER(u64)]),
+    "vqm_ss2_autotune": (ctypes.c_int, [u32, u32, u32, u32, ctypes.POINTER(TuneResultC)]),
     "vqm_ss2_server_config_init": (None, [ctypes.POINTER(ServerConfigC)]),
     "vqm_ss2_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfigC), ctypes.POINTER(vp)]),
     "vqm_ss2_server_destroy": (None, [vp]),

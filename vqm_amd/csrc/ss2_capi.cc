@@ -9,6 +9,7 @@
 #include "ss2_host.h"
 #include "ss2_runtime.h"
 #include "ss2_server.h"
+#include "ss2_tune.h"
 #include "vqm_ss2.h"
 
 struct vqm_ss2_pipeline {
@@ -339,6 +340,21 @@ int vqm_ss2_batch_pool_bytes(const vqm_ss2_batch* batch, uint64_t* bytes) {
         Need(batch, "batch");
         Need(bytes, "bytes");
         *bytes = batch->impl.PoolBytes();
+    });
+}
+
+int vqm_ss2_autotune(uint32_t width, uint32_t height, uint32_t plane_format, uint32_t pairs,
+                     vqm_ss2_tune_result* result) {
+    return Guard([&] {
+        Need(result, "result");
+        const vqm::ss2::TuneChoice c = vqm::ss2::Tuner::Get().Choose(
+            width, height, vqm::ss2::ToFormat(plane_format), pairs);
+        result->strip_rows = c.strip_rows;
+        result->chunks = c.chunks;
+        result->candidates = c.candidates;
+        result->cached = c.cached ? 1u : 0u;
+        result->ms_per_launch = c.ms_per_launch;
+        result->default_ms = c.default_ms;
     });
 }
 

@@ -2,6 +2,8 @@
 
 #include "ss2_runtime.h"
 
+#include "ss2_tune.h"
+
 #include <algorithm>
 #include <atomic>
 #include <cstring>
@@ -188,8 +190,10 @@ void WorkspacePools::Drop(std::unique_ptr<Workspace> ws) {
 const DevicePlan& WorkspacePools::PlanFor(uint32_t width, uint32_t height) {
     auto& slot = plans_[std::make_pair(width, height)];
     if (!slot) {
+        const auto rows = strip_rows_for_.find(std::make_pair(width, height));
         try {
-            slot = std::make_unique<DevicePlan>(width, height, format_, strip_rows_);
+            slot = std::make_unique<DevicePlan>(width, height, format_,
+                                                rows != strip_rows_for_.end() ? rows->second : strip_rows_);
         } catch (...) {
             plans_.erase(std::make_pair(width, height));
             throw;
@@ -698,11 +702,30 @@ uint64_t Batch::Submit(const vqm_ss2_device_pair* pairs, uint32_t count) {
     if (count == 0 || count > config_.max_pairs)
         Fail(VQM_INVALID_ARGUMENT, "pair count must be between 1 and max_pairs");
     // Plans first: every geometry is validated before anything is acquired.
+    std::map<std::pair<uint32_t, uint32_t>, uint32_t> per_geometry;
     for (uint32_t i = 0; i < count; ++i) {
         if (pairs[i].original == nullptr || pairs[i].distorted == nullptr)
             Fail(VQM_INVALID_ARGUMENT, "pair " + std::to_string(i) + ": image data is null");
-        pools_.PlanFor(pairs[i].width, pairs[i].height);
+        PlanScales(pairs[i].width, pairs[i].height);
+        ++per_geometry[std::make_pair(pairs[i].width, pairs[i].height)];
     }
+    uint32_t chunks = config_.chunks;
+    if (config_.autotune) {
+        // Each new geometry's strip height comes from the tuner, timed for
+        // this batch's count of it; the chunk count follows the geometry
+        // with the most pairs.
+        uint32_t most = 0;
+        for (const auto& [geometry, n] : per_geometry) {
+            const auto [w, h] = geometry;
+            const TuneChoice c = Tuner::Get().Choose(w, h, static_cast<PlaneFormat>(config_.plane_format), n);
+            if (!pools_.HasPlan(w, h)) pools_.SetStripRows(w, h, c.strip_rows);
+            if (n > most) {
+                most = n;
+                chunks = c.chunks;
+            }
+        }
+    }
+    for (const auto& [geometry, n] : per_geometry) pools_.PlanFor(geometry.first, geometry.second);
     std::vector<Taken> taken;
     std::vector<LaunchEntry> entries(count);
     try {
@@ -717,7 +740,7 @@ uint64_t Batch::Submit(const vqm_ss2_device_pair* pairs, uint32_t count) {
             taken.push_back(Taken{pairs[i].width, pairs[i].height, std::move(ws)});
         }
         lane_.Launch(static_cast<PlaneFormat>(config_.plane_format), ctx_.lut, entries.data(), count,
-                     config_.chunks);
+                     chunks);
     } catch (...) {
         for (Taken& t : taken) pools_.Return(t.width, t.height, false, std::move(t.ws));
         throw;

@@ -162,6 +162,13 @@ class WorkspacePools {
 public:
     WorkspacePools(PlaneFormat format, uint32_t strip_rows, uint64_t budget);
     const DevicePlan& PlanFor(uint32_t width, uint32_t height);
+    bool HasPlan(uint32_t width, uint32_t height) const {
+        return plans_.count(std::make_pair(width, height)) != 0;
+    }
+    // The strip height of a geometry whose plan does not exist yet.
+    void SetStripRows(uint32_t width, uint32_t height, uint32_t strip_rows) {
+        strip_rows_for_[std::make_pair(width, height)] = strip_rows;
+    }
     // Device bytes one workspace of this geometry holds.
     uint64_t Need(uint32_t width, uint32_t height, bool inputs);
     // A workspace, or null when it cannot fit the budget until busy
@@ -190,6 +197,7 @@ private:
     uint64_t budget_;
     std::unique_ptr<Arena> arena_;
     std::map<std::pair<uint32_t, uint32_t>, std::unique_ptr<DevicePlan>> plans_;
+    std::map<std::pair<uint32_t, uint32_t>, uint32_t> strip_rows_for_;
     std::map<Key, Pool> pools_;
     uint64_t bytes_ = 0, peak_ = 0;
 };

@@ -597,11 +597,12 @@ class Ssimulacra2Batch:
     Pairs are device-resident (DeviceImage pointers)."""
 
     def __init__(self, max_pairs: int, plane_format: int = L.PLANES_F32, hbm_budget: int = 0,
-                 strip_rows: int = 0, chunks: int = 1):
+                 strip_rows: int = 0, chunks: int = 1, autotune: bool = False):
         cfg = L.BatchConfigC()
         cfg.plane_format = plane_format
         cfg.strip_rows = strip_rows
         cfg.chunks = chunks
+        cfg.autotune = 1 if autotune else 0
         cfg.max_pairs = max_pairs
         cfg.hbm_budget = hbm_budget
         h = ctypes.c_void_p()
@@ -677,6 +678,14 @@ class PinnedBuffer:
             self.free()
         except Exception:
             pass
+
+
+def autotune(size: Size, plane_format: int = L.PLANES_F32, pairs: int = 1) -> dict:
+    """The launch shape (strip height, chunk count) measured fastest for a
+    batch of `pairs` pairs of this size and plane format (vqm_ss2_autotune)."""
+    r = L.TuneResultC()
+    _check(L.lib().vqm_ss2_autotune(size.width, size.height, plane_format, pairs, ctypes.byref(r)))
+    return {name: getattr(r, name) for name, _ in L.TuneResultC._fields_}
 
 
 # ---- serving ----------------------------------------------------------------------
