@@ -243,6 +243,14 @@ int vqm_ss2_submit_buffers(vqm_ss2_pipeline* pipeline, uint64_t original, uint64
  * VQM_STAGE_TIMING stages (src/timing.h:18-21). */
 int vqm_ss2_stage_times(const vqm_ss2_pipeline* pipeline, float ms[3]);
 
+/* The resident reference cache (src/metrics/ssimulacra2/vulkan.cc:5-11,
+ * 238-241, 716-735): a submission whose original is the same resident
+ * handle as the previous submission's original reuses that image's planes
+ * and prepares the distorted image only. Uploads get new handles and
+ * Release, or any submission with a non-resident original, ends the reuse.
+ * Writes how many images the last submission prepared (1 or 2). */
+int vqm_ss2_prepared_images(const vqm_ss2_pipeline* pipeline, uint32_t* images);
+
 /* ---- batched scoring (no counterpart in the reference: it overlaps pairs
  * only by holding several pipelines, vqm_pool.h:35-50) -------------------- */
 

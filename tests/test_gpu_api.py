@@ -201,3 +201,40 @@ def test_stage_times_are_measured():
     g.Compute(Image(Size(256, 256), ImageFormat.srgb8(), a), Image(Size(256, 256), ImageFormat.srgb8(), b))
     t = g.StageTimes()
     assert t[0] > 0.0 and t[1] == 0.0 and t[2] == 0.0
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_resident_reference_is_prepared_once(graph):
+    """vulkan.cc:5-11,716-735: a resident original scored against several
+    distorted images is prepared once; results equal fresh pipelines bit for
+    bit, and a release, a new upload or a non-resident original ends the
+    reuse."""
+    S = Size(320, 200)
+    a, b1 = ramp_image(320, 200, seed=1), ramp_image(320, 200, seed=2)
+    b2, c = ramp_image(320, 200, seed=3), ramp_image(320, 200, seed=4)
+    img = lambda d: Image(S, ImageFormat.srgb8(), d)  # noqa: E731
+
+    def fresh(x, y):
+        q = Ssimulacra2Pipeline(Ssimulacra2Config(size=S))
+        r = q.Compute(img(x), img(y)).norms()
+        q.close()
+        return r
+
+    p = Ssimulacra2Pipeline(Ssimulacra2Config(size=S, gpu_resident_images=4, graph=graph))
+    ra, rb1, rb2 = p.Upload(img(a)), p.Upload(img(b1)), p.Upload(img(b2))
+    assert np.array_equal(p.Compute(ra, rb1).norms(), fresh(a, b1))
+    assert p.PreparedImages() == 2
+    assert np.array_equal(p.Compute(ra, rb2).norms(), fresh(a, b2))
+    assert p.PreparedImages() == 1
+    assert np.array_equal(p.Compute(ra, img(b1)).norms(), fresh(a, b1))   # host distorted
+    assert p.PreparedImages() == 1
+    assert np.array_equal(p.Compute(img(c), img(b2)).norms(), fresh(c, b2))  # host original
+    assert p.PreparedImages() == 2
+    assert np.array_equal(p.Compute(ra, rb2).norms(), fresh(a, b2))
+    assert p.PreparedImages() == 2
+    p.Release(ra)
+    rc = p.Upload(img(c))  # may land in the released slot: a new handle
+    assert np.array_equal(p.Compute(rc, rb1).norms(), fresh(c, b1))
+    assert p.PreparedImages() == 2
+    assert np.array_equal(p.Compute(rc, rb2).norms(), fresh(c, b2))
+    assert p.PreparedImages() == 1

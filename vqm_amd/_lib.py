@@ -128,6 +128,7 @@ SIGNATURES = {
     "vqm_ss2_release_image": (ctypes.c_int, [vp, u64]),
     "vqm_ss2_submit_buffers": (ctypes.c_int, [vp, u64, u64, f32, ctypes.POINTER(u64)]),
     "vqm_ss2_stage_times": (ctypes.c_int, [vp, ctypes.POINTER(f32)]),
+    "vqm_ss2_prepared_images": (ctypes.c_int, [vp, ctypes.POINTER(u32)]),
     "vqm_ss2_batch_create": (ctypes.c_int, [ctypes.POINTER(BatchConfigC), ctypes.POINTER(vp)]),
     "vqm_ss2_batch_destroy": (None, [vp]),
     "vqm_ss2_batch_submit": (ctypes.c_int, [vp, ctypes.POINTER(DevicePairC), u32,

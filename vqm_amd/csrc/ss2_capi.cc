@@ -298,6 +298,14 @@ int vqm_ss2_stage_times(const vqm_ss2_pipeline* p, float ms[3]) {
     });
 }
 
+int vqm_ss2_prepared_images(const vqm_ss2_pipeline* p, uint32_t* images) {
+    return Guard([&] {
+        Need(p, "pipeline");
+        Need(images, "images");
+        *images = p->impl.PreparedImages();
+    });
+}
+
 int vqm_ss2_batch_create(const vqm_ss2_batch_config* config, vqm_ss2_batch** out) {
     return Guard([&] {
         Need(config, "config");

@@ -445,6 +445,16 @@ Pipeline::Pipeline(const vqm_ss2_config& config)
     }
     host_norms_ = PinnedBuffer(kNorms * sizeof(double));
     if (config.input_bytes > 0) input_pool_ = DeviceBuffer(config.input_bytes);
+    if (config.resident_images > 0) {
+        // The prepare work of the distorted image alone, for pairs whose
+        // resident original is still in the workspace.
+        std::vector<WorkRef> work(plan_.host.prep_tiles);
+        for (uint32_t t = 0; t < plan_.host.prep_tiles; ++t) work[t] = WorkRef{0, t | (1u << 31)};
+        distorted_work_ = DeviceBuffer(work.size() * sizeof(WorkRef));
+        Check(hipMemcpy(distorted_work_.get(), work.data(), work.size() * sizeof(WorkRef),
+                        hipMemcpyHostToDevice),
+              "upload distorted work list");
+    }
 }
 
 Pipeline::~Pipeline() {
@@ -513,16 +523,18 @@ uint64_t Pipeline::Upload(const vqm_image_desc& image, const void* data) {
 
 void Pipeline::Release(uint64_t handle) {
     RequireIdle();
+    if (handle != 0 && handle == cached_original_) cached_original_ = 0;
     for (Slot& s : slots_)
         if (handle != 0 && s.handle == handle) s.handle = 0;
 }
 
-uint64_t Pipeline::Launch(const uint8_t* src0, const uint8_t* src1) {
+uint64_t Pipeline::Launch(const uint8_t* src0, const uint8_t* src1, bool reuse_original) {
     const hipStream_t s = stream_.get();
     const PairDesc* pd = pair_.as<PairDesc>();
     const PlaneFormat fmt = static_cast<PlaneFormat>(config_.plane_format);
     const GeometryPlan& g = plan_.host;
-    if (config_.graph) {
+    prepared_images_ = reuse_original ? 1 : 2;
+    if (config_.graph && !reuse_original) {
         if (graph_ == nullptr) BuildGraph();
         auto* hp = h_pair_.as<PairDesc>();
         hp->src[0] = src0;
@@ -536,8 +548,13 @@ uint64_t Pipeline::Launch(const uint8_t* src0, const uint8_t* src1) {
     }
     graph_launched_ = false;
     Check(hipEventRecord(ev_[0].get(), s), "hipEventRecord");
-    Check(LaunchPrepare(fmt, pd, nullptr, 2 * g.prep_tiles, g.prep_tiles, ctx_.lut, src0, src1, s),
-          "launch prepare");
+    if (reuse_original)
+        Check(LaunchPrepare(fmt, pd, distorted_work_.as<WorkRef>(), g.prep_tiles, g.prep_tiles, ctx_.lut,
+                            src0, src1, s),
+              "launch prepare");
+    else
+        Check(LaunchPrepare(fmt, pd, nullptr, 2 * g.prep_tiles, g.prep_tiles, ctx_.lut, src0, src1, s),
+              "launch prepare");
     Check(hipEventRecord(ev_[1].get(), s), "hipEventRecord");
     Check(LaunchScale(fmt, pd, nullptr, g.plan.items, g.plan.items, s), "launch scale");
     Check(hipEventRecord(ev_[2].get(), s), "hipEventRecord");
@@ -568,6 +585,7 @@ uint64_t Pipeline::SubmitHost(const vqm_image_desc& a, const void* da, const vqm
                          stream_.get()),
           "upload pair");
     const auto* base = stream_inputs_.as<uint8_t>();
+    cached_original_ = 0;  // the workspace now holds a host image's planes
     return Launch(base, base + image_bytes_);
 }
 
@@ -575,7 +593,11 @@ uint64_t Pipeline::SubmitResident(uint64_t a, uint64_t b) {
     RequireIdle();
     const Slot& sa = RequireSlot(a, "original");
     const Slot& sb = RequireSlot(b, "distorted");
-    return Launch(sa.data.as<uint8_t>(), sb.data.as<uint8_t>());
+    const bool reuse = a == cached_original_;
+    cached_original_ = 0;
+    const uint64_t token = Launch(sa.data.as<uint8_t>(), sb.data.as<uint8_t>(), reuse);
+    cached_original_ = a;
+    return token;
 }
 
 uint64_t Pipeline::SubmitResidentHost(uint64_t a, const vqm_image_desc& b, const void* db) {
@@ -590,7 +612,11 @@ uint64_t Pipeline::SubmitResidentHost(uint64_t a, const vqm_image_desc& b, const
     auto* dst = stream_inputs_.as<uint8_t>() + image_bytes_;
     Check(hipMemcpyAsync(dst, staging, image_bytes_, hipMemcpyHostToDevice, stream_.get()),
           "upload distorted");
-    return Launch(sa.data.as<uint8_t>(), dst);
+    const bool reuse = a == cached_original_;
+    cached_original_ = 0;
+    const uint64_t token = Launch(sa.data.as<uint8_t>(), dst, reuse);
+    cached_original_ = a;
+    return token;
 }
 
 uint64_t Pipeline::SubmitDevice(const vqm_device_image& a, const vqm_device_image& b,
@@ -606,6 +632,7 @@ uint64_t Pipeline::SubmitDevice(const vqm_device_image& a, const vqm_device_imag
         Check(hipEventRecord(ready.get(), wait_stream), "record caller stream");
         Check(hipStreamWaitEvent(stream_.get(), ready.get(), 0), "wait caller stream");
     }
+    cached_original_ = 0;  // caller-owned memory may change between submissions
     return Launch(static_cast<const uint8_t*>(a.data), static_cast<const uint8_t*>(b.data));
 }
 
@@ -651,6 +678,7 @@ uint64_t Pipeline::SubmitBuffers(uint64_t a, uint64_t b) {
     const InputBuffer& ba = RequireBuffer(a, "original");
     const InputBuffer& bb = RequireBuffer(b, "distorted");
     const auto* base = input_pool_.as<uint8_t>();
+    cached_original_ = 0;
     return Launch(base + ba.offset, base + bb.offset);
 }
 

@@ -290,7 +290,9 @@ private:
     void RequireIdle() const;
     const Slot& RequireSlot(uint64_t handle, const char* name) const;
     const InputBuffer& RequireBuffer(uint64_t handle, const char* name) const;
-    uint64_t Launch(const uint8_t* src0, const uint8_t* src1);
+    // reuse_original: the original's planes are already in the workspace
+    // (the resident reference cache), so only the distorted image is prepared.
+    uint64_t Launch(const uint8_t* src0, const uint8_t* src1, bool reuse_original = false);
     void BuildGraph();
 
     vqm_ss2_config config_;
@@ -312,6 +314,16 @@ private:
     Event ev_[5];
     uint64_t in_flight_ = 0;
     std::array<float, 3> stage_ms_{};
+    // Resident reference cache (vulkan.cc:238-241,716-735): the handle of
+    // the resident original whose planes the workspace holds, or 0.
+    uint64_t cached_original_ = 0;
+    DeviceBuffer distorted_work_;  // prepare work list of the distorted image only
+    uint32_t prepared_images_ = 0; // images the last submission prepared
+
+public:
+    // Images the last submission prepared: 1 when the resident original was
+    // reused from the previous submission, else 2.
+    uint32_t PreparedImages() const { return prepared_images_; }
 };
 
 class Batch {

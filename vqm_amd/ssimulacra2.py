@@ -466,6 +466,13 @@ class Ssimulacra2Pipeline:
         _check(L.lib().vqm_ss2_memory_usage(self._h, ctypes.byref(m)))
         return PipelineMemoryUsage.from_c(m)
 
+    def PreparedImages(self) -> int:
+        """Images the last submission prepared: 1 when its resident original
+        was the previous submission's and its planes were reused."""
+        n = L.u32()
+        _check(L.lib().vqm_ss2_prepared_images(self._h, ctypes.byref(n)))
+        return n.value
+
     def StageTimes(self) -> List[float]:
         """Prepare, scale and reduce times of the last finished pair, in ms."""
         ms = (L.f32 * 3)()
