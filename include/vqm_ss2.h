@@ -16,10 +16,15 @@
  *   VQM_RUNTIME_ERROR    -> std::runtime_error (no device, HIP failure, allocation)
  *   VQM_OVERFLOW         -> std::overflow_error (layout does not fit)
  *
- * Inputs are planar 8-bit sRGB RGB images in the reference's Image layout
- * (include/vqm_image.h:122-133): planes R, G, B of width*height bytes, each
- * plane starting on a 4-byte boundary. Norm arrays hold 108 doubles, index
- * level * 18 + k with k = 0..5 the Ssimulacra2Scale::avg_ssim entries
+ * Inputs are planar images in the reference's Image layout
+ * (include/vqm_image.h:122-133): planes Y/R, U/G, V/B and alpha, each starting
+ * on a 4-byte boundary, in any format the reference accepts (byte, word,
+ * half or float samples; grey, RGB or YUV with 4:4:4 to 4:1:0 chroma; limited
+ * or full range; its matrix coefficients, transfer characteristics and
+ * primaries; straight or premultiplied alpha). Pipelines decode every format;
+ * the batch engine and the server take 8-bit full-range sRGB RGB. Norm
+ * arrays hold 108 doubles, index level * 18 + k with k = 0..5 the
+ * Ssimulacra2Scale::avg_ssim entries
  * [channel * 2 + norm] and k = 6..17 the avg_edgediff entries
  * [channel * 4 + n] (include/ssimulacra2.h:17-25); pruned (channel, level)
  * pairs and levels beyond `scales` are 0.0.
@@ -60,10 +65,11 @@ enum {
 enum { VQM_PLANES_F32 = 0, VQM_PLANES_F16 = 1, VQM_PLANES_BF16 = 2, VQM_PLANES_U16 = 3 };
 
 /* ImageFormat, include/vqm_image.h:100-120; numeric enum values are zimg's
- * as in the reference (vqm_image.h:25-96). This path decodes type Byte (0),
- * family Rgb (1), alpha None (0), range Full (1), matrix Rgb (0) or
- * Unspecified (2), transfer Srgb (13), primaries Bt709 (1), no subsampling,
- * depth 0 or 8; anything else is VQM_INVALID_ARGUMENT. */
+ * as in the reference (vqm_image.h:25-96). Validation follows the
+ * reference's ValidateImage (input.cc:152-222): an unknown or unsupported
+ * combination is VQM_INVALID_ARGUMENT. 8-bit full-range opaque sRGB RGB with
+ * BT.709 primaries decodes through a table; every other format through a
+ * per-image descriptor. */
 typedef struct vqm_image_format {
     uint8_t type;
     uint8_t family;
@@ -113,7 +119,9 @@ typedef struct vqm_ss2_config {
                                    norm read-back, captured on first use; 0
                                    (default): eager launches, which measured
                                    slightly faster. No reference counterpart. */
-    uint32_t reserved;
+    uint32_t chroma_filter;     /* subsampled chroma upsampling: 0 bilinear
+                                   (default), 1 bicubic, 2 lanczos
+                                   (Ssimulacra2Config::chroma_filter) */
 } vqm_ss2_config;
 
 /* PipelineMemoryUsage, include/vqm_memory_types.h:29-38: device bytes the
@@ -143,6 +151,15 @@ void vqm_image_format_srgb8(vqm_image_format* format);
 /* RequiredBytes, include/vqm_image.h:143-146 (input.cc:244-255). */
 int vqm_required_bytes(const vqm_image_format* format, uint32_t width, uint32_t height,
                        uint64_t* bytes);
+
+/* ValidateImage (input.cc:152-222): VQM_OK when the reference accepts
+ * `format` at this geometry, else INVALID_ARGUMENT naming the reason. */
+int vqm_validate_image_format(const vqm_image_format* format, uint32_t width, uint32_t height);
+
+/* PlaneScaleOffset (input.cc:714-757): the decode scale and offset of a
+ * sample class (luma/RGB, chroma when `chroma`, alpha when `alpha`). */
+int vqm_plane_scale_offset(const vqm_image_format* format, int chroma, int alpha, float* scale,
+                           float* offset);
 
 /* PlanSsimulacra2Scales, pipeline.cc:28-48: writes `count` and width,height
  * pairs of each level. INVALID_ARGUMENT below 8x8. */

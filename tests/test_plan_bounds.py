@@ -17,6 +17,7 @@ def test_plan_bounds():
                         "-I", os.path.join(ROOT, "vqm_amd", "csrc"),
                         os.path.join(ROOT, "tests", "cpp", "plan_bounds_test.cc"),
                         os.path.join(ROOT, "vqm_amd", "csrc", "ss2_host.cc"),
+                        os.path.join(ROOT, "vqm_amd", "csrc", "ss2_source.cc"),
                         "-o", exe], check=True)
         r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
         assert r.returncode == 0, r.stderr
@@ -31,6 +32,7 @@ def test_range_allocator():
                         "-I", os.path.join(ROOT, "vqm_amd", "csrc"),
                         os.path.join(ROOT, "tests", "cpp", "range_allocator_test.cc"),
                         os.path.join(ROOT, "vqm_amd", "csrc", "ss2_host.cc"),
+                        os.path.join(ROOT, "vqm_amd", "csrc", "ss2_source.cc"),
                         "-o", exe], check=True)
         r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
         assert r.returncode == 0, r.stderr

@@ -49,7 +49,7 @@ class DeviceImageC(ctypes.Structure):
 class ConfigC(ctypes.Structure):
     _fields_ = [("width", u32), ("height", u32), ("resident_images", u32),
                 ("plane_format", u32), ("streaming_inputs", u32), ("strip_rows", u32),
-                ("input_bytes", u64), ("graph", u32), ("reserved", u32)]
+                ("input_bytes", u64), ("graph", u32), ("chroma_filter", u32)]
 
 
 class MemoryC(ctypes.Structure):
@@ -100,6 +100,9 @@ SIGNATURES = {
     "vqm_image_format_srgb8": (None, [ctypes.POINTER(ImageFormatC)]),
     "vqm_required_bytes": (ctypes.c_int, [ctypes.POINTER(ImageFormatC), u32, u32,
                                           ctypes.POINTER(u64)]),
+    "vqm_validate_image_format": (ctypes.c_int, [ctypes.POINTER(ImageFormatC), u32, u32]),
+    "vqm_plane_scale_offset": (ctypes.c_int, [ctypes.POINTER(ImageFormatC), ctypes.c_int, ctypes.c_int,
+                                              ctypes.POINTER(f32), ctypes.POINTER(f32)]),
     "vqm_ss2_plan_scales": (ctypes.c_int, [u32, u32, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
     "vqm_ss2_gaussian_taps": (None, [ctypes.POINTER(f32)]),
     "vqm_ss2_channel_pruned": (ctypes.c_int, [u32, u32, u32]),

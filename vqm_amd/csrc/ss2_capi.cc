@@ -9,6 +9,7 @@
 #include "ss2_host.h"
 #include "ss2_runtime.h"
 #include "ss2_server.h"
+#include "ss2_source.h"
 #include "ss2_tune.h"
 #include "vqm_ss2.h"
 
@@ -95,6 +96,25 @@ int vqm_required_bytes(const vqm_image_format* format, uint32_t width, uint32_t 
     });
 }
 
+int vqm_validate_image_format(const vqm_image_format* format, uint32_t width, uint32_t height) {
+    return Guard([&] {
+        Need(format, "format");
+        vqm::ss2::ValidateFormat(*format, width, height, "image");
+    });
+}
+
+int vqm_plane_scale_offset(const vqm_image_format* format, int chroma, int alpha, float* scale,
+                           float* offset) {
+    return Guard([&] {
+        Need(format, "format");
+        Need(scale, "scale");
+        Need(offset, "offset");
+        const vqm::ss2::ScaleOffset so = vqm::ss2::PlaneScaleOffset(*format, chroma != 0, alpha != 0);
+        *scale = so.scale;
+        *offset = so.offset;
+    });
+}
+
 int vqm_ss2_plan_scales(uint32_t width, uint32_t height, uint32_t sizes[12], uint32_t* count) {
     return Guard([&] {
         Need(sizes, "sizes");
@@ -145,12 +165,15 @@ int vqm_ss2_memory_plan(const vqm_ss2_config* config, vqm_ss2_memory* plan) {
         vqm_image_format_srgb8(&f);
         const uint64_t image = vqm::ss2::RequiredBytes(f, config->width, config->height);
         std::memset(plan, 0, sizeof(*plan));
-        plan->workspace_bytes = layout.total + sizeof(vqm::ss2::PairDesc) + sizeof(vqm::ss2::Plan) +
-                                g.items.size() * sizeof(vqm::ss2::ScaleItem);
+        plan->workspace_bytes = layout.total + sizeof(vqm::ss2::PairDesc) + 2 * sizeof(vqm::ss2::SourceDesc) +
+                                sizeof(vqm::ss2::Plan) + g.items.size() * sizeof(vqm::ss2::ScaleItem);
+        // Slots and streaming buffers as created, for 8-bit RGB images; they
+        // grow when a larger format arrives.
+        const uint64_t staged = (image + 255) / 256 * 256;
         plan->resident_bytes = static_cast<uint64_t>(config->resident_images) * image;
-        plan->streaming_bytes = config->streaming_inputs ? 2 * image : 0;
+        plan->streaming_bytes = config->streaming_inputs ? 2 * staged : 0;
         plan->input_pool_bytes = config->input_bytes;
-        plan->pinned_host_bytes = (config->streaming_inputs ? 2 * image : 0) + 108 * sizeof(double);
+        plan->pinned_host_bytes = (config->streaming_inputs ? 2 * staged : 0) + 108 * sizeof(double);
         plan->total_device_bytes = plan->workspace_bytes + plan->resident_bytes +
                                    plan->streaming_bytes + plan->input_pool_bytes;
     });
@@ -196,45 +219,45 @@ int vqm_ss2_release(vqm_ss2_pipeline* p, uint64_t handle) {
 }
 
 int vqm_ss2_submit_host(vqm_ss2_pipeline* p, const vqm_image_desc* a, const void* da,
-                        const vqm_image_desc* b, const void* db, float /*background*/,
+                        const vqm_image_desc* b, const void* db, float background,
                         uint64_t* token) {
     return Guard([&] {
         Need(p, "pipeline");
         Need(a, "original");
         Need(b, "distorted");
         Need(token, "token");
-        *token = p->impl.SubmitHost(*a, da, *b, db);
+        *token = p->impl.SubmitHost(*a, da, *b, db, background);
     });
 }
 
-int vqm_ss2_submit_resident(vqm_ss2_pipeline* p, uint64_t a, uint64_t b, float /*background*/,
+int vqm_ss2_submit_resident(vqm_ss2_pipeline* p, uint64_t a, uint64_t b, float background,
                             uint64_t* token) {
     return Guard([&] {
         Need(p, "pipeline");
         Need(token, "token");
-        *token = p->impl.SubmitResident(a, b);
+        *token = p->impl.SubmitResident(a, b, background);
     });
 }
 
 int vqm_ss2_submit_resident_host(vqm_ss2_pipeline* p, uint64_t a, const vqm_image_desc* b,
-                                 const void* db, float /*background*/, uint64_t* token) {
+                                 const void* db, float background, uint64_t* token) {
     return Guard([&] {
         Need(p, "pipeline");
         Need(b, "distorted");
         Need(token, "token");
-        *token = p->impl.SubmitResidentHost(a, *b, db);
+        *token = p->impl.SubmitResidentHost(a, *b, db, background);
     });
 }
 
 int vqm_ss2_submit_device(vqm_ss2_pipeline* p, const vqm_device_image* a,
-                          const vqm_device_image* b, void* wait_stream, float /*background*/,
+                          const vqm_device_image* b, void* wait_stream, float background,
                           uint64_t* token) {
     return Guard([&] {
         Need(p, "pipeline");
         Need(a, "original");
         Need(b, "distorted");
         Need(token, "token");
-        *token = p->impl.SubmitDevice(*a, *b, static_cast<hipStream_t>(wait_stream));
+        *token = p->impl.SubmitDevice(*a, *b, static_cast<hipStream_t>(wait_stream), background);
     });
 }
 
@@ -280,12 +303,12 @@ int vqm_ss2_release_image(vqm_ss2_pipeline* p, uint64_t buffer) {
     });
 }
 
-int vqm_ss2_submit_buffers(vqm_ss2_pipeline* p, uint64_t a, uint64_t b, float /*background*/,
+int vqm_ss2_submit_buffers(vqm_ss2_pipeline* p, uint64_t a, uint64_t b, float background,
                            uint64_t* token) {
     return Guard([&] {
         Need(p, "pipeline");
         Need(token, "token");
-        *token = p->impl.SubmitBuffers(a, b);
+        *token = p->impl.SubmitBuffers(a, b, background);
     });
 }
 

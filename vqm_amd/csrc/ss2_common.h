@@ -78,14 +78,53 @@ struct Plan {
     uint32_t plane_stride;                          // bytes per u8 input plane
 };
 
+// Decode curve ids of SourceDesc::transfer: linear; the BT.1886 2.4 power
+// (BT.709, BT.601, BT.2020, SMPTE 240M); 2.2 and 2.8 powers (BT.470M/BG);
+// IEC 61966-2-1 sRGB; the two log curves; xvYCC; BT.1361; SMPTE ST 2084 (PQ);
+// SMPTE ST 428; ARIB STD-B67 (HLG).
+enum TransferId : uint32_t {
+    kCurveLinear = 0, kCurvePow24 = 1, kCurvePow22 = 2, kCurvePow28 = 3, kCurveSrgb = 4,
+    kCurveLog100 = 5, kCurveLog316 = 6, kCurveXvycc = 7, kCurveBt1361 = 8, kCurvePq = 9,
+    kCurveSt428 = 10, kCurveHlg = 11
+};
+
+// How the prepare kernel decodes one input image that is not 8-bit sRGB RGB
+// (the general input surface of color.glsl:94-541 and input.cc:114-842):
+// raw sample -> range/depth scale and offset -> chroma upsampling -> matrix
+// (or constant-luminance / ICtCp op) -> alpha -> transfer -> primaries,
+// ending in linear BT.709 RGB.
+struct SourceDesc {
+    uint32_t kind;            // 0: 8-bit sRGB RGB (table decode), 1: this descriptor
+    float background;         // alpha blend target, in encoded RGB
+    uint64_t plane_off[4];    // bytes from the image base: Y/R, U/G, V/B, alpha
+    uint32_t plane_w[4], plane_h[4];
+    uint32_t type;            // 0 byte, 1 word, 2 half, 3 float
+    uint32_t family;          // 0 grey, 1 RGB, 2 YUV
+    uint32_t alpha;           // 0 none, 1 straight, 2 premultiplied
+    uint32_t sub_w, sub_h;    // log2 chroma subsampling
+    uint32_t transfer;        // decode curve id, see TransferId
+    uint32_t mode;            // 0 matrix, 1 constant luminance, 2 ICtCp
+    uint32_t filter;          // chroma filter: 0 bilinear, 1 bicubic, 2 lanczos
+    uint32_t identity1;       // matrix 1 is the identity
+    uint32_t identity2;       // matrix 2 is the identity
+    float scale[3], offset[3];  // luma/RGB, chroma, alpha sample classes
+    float m1[9];              // YUV -> RGB, or ICtCp -> LMS
+    float m2[9];              // gamut to BT.709 (after LMS -> RGB for ICtCp)
+    float kr, kg, kb;         // luma weights of the CL op and the HLG OOTF
+    float to_linear_scale;    // PQ and HLG peak over 100 cd/m^2
+    float shift_x, shift_y;   // chroma siting offsets
+    float cl[4];              // CL op constants nb, pb, nr, pr
+};
+
 // One pair as the kernels see it.
 struct PairDesc {
-    const uint8_t* src[2];     // packed planar u8 sRGB RGB, original then distorted
+    const uint8_t* src[2];     // packed planar images, original then distorted
     void* planes;              // 2 * image_bytes of planes
     double* partials;          // items * 6
     double* norms;             // 108
     const Plan* plan;          // device copy of the pair's plan
     const ScaleItem* items;    // device copy of the plan's item table
+    const SourceDesc* source[2];  // null or kind 0: 8-bit sRGB RGB (table decode)
 };
 
 }  // namespace vqm::ss2

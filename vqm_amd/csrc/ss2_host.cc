@@ -2,6 +2,8 @@
 
 #include "ss2_host.h"
 
+#include "ss2_source.h"
+
 #include <algorithm>
 #include <cmath>
 #include <limits>
@@ -170,25 +172,8 @@ void CheckImage(const vqm_image_desc& image, uint32_t width, uint32_t height, co
     const std::string who(name);
     if (image.width != width || image.height != height)
         Fail(VQM_INVALID_ARGUMENT, who + ": image size does not match the pipeline geometry");
-    const vqm_image_format& f = image.format;
-    if (f.type > 3) Fail(VQM_INVALID_ARGUMENT, who + ": invalid pixel type");
-    if (f.range > 1) Fail(VQM_INVALID_ARGUMENT, who + ": invalid pixel range");
-    if (f.alpha > 2) Fail(VQM_INVALID_ARGUMENT, who + ": invalid alpha type");
-    if (f.family > 2) Fail(VQM_INVALID_ARGUMENT, who + ": invalid color family");
-    if (f.subsample_w != 0 || f.subsample_h != 0) {
-        if (f.family == 1) Fail(VQM_INVALID_ARGUMENT, who + ": subsampled RGB images are not supported");
-        Fail(VQM_INVALID_ARGUMENT, who + ": subsampled input is not supported by this path");
-    }
-    if (f.family == 1 && f.matrix != 0 && f.matrix != 2)
-        Fail(VQM_INVALID_ARGUMENT, who + ": RGB images cannot declare YUV matrix coefficients");
-    if (f.family != 1) Fail(VQM_INVALID_ARGUMENT, who + ": only RGB input is supported by this path");
-    if (f.type != 0 || (f.depth != 0 && f.depth != 8))
-        Fail(VQM_INVALID_ARGUMENT, who + ": only 8-bit samples are supported by this path");
-    if (f.range != 1) Fail(VQM_INVALID_ARGUMENT, who + ": only full-range input is supported by this path");
-    if (f.alpha != 0) Fail(VQM_INVALID_ARGUMENT, who + ": alpha is not supported by this path");
-    if (f.transfer != 13) Fail(VQM_INVALID_ARGUMENT, who + ": only the sRGB transfer is supported by this path");
-    if (f.primaries != 1) Fail(VQM_INVALID_ARGUMENT, who + ": only BT.709 primaries are supported by this path");
-    const uint64_t expected = RequiredBytes(f, width, height);
+    ValidateFormat(image.format, width, height, who);
+    const uint64_t expected = RequiredBytes(image.format, width, height);
     if (image.bytes != expected)
         Fail(VQM_INVALID_ARGUMENT, who + ": data size does not match the image format (expected " +
                                        std::to_string(expected) + " bytes)");

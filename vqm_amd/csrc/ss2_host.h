@@ -49,8 +49,9 @@ std::array<float, 256> SrgbTable();
 // starting on a 4-byte boundary.
 uint64_t RequiredBytes(const vqm_image_format& format, uint32_t width, uint32_t height);
 
-// input.cc:152-240 restricted to what this path decodes: opaque full-range
-// 8-bit sRGB RGB with BT.709 primaries. Throws Error{VQM_INVALID_ARGUMENT}.
+// input.cc:152-240: the image matches the geometry, its format is one the
+// reference accepts, and its byte length is RequiredBytes. Throws
+// Error{VQM_INVALID_ARGUMENT}.
 void CheckImage(const vqm_image_desc& image, uint32_t width, uint32_t height,
                 const char* name);
 

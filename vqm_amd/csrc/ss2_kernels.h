@@ -23,7 +23,9 @@ struct WorkRef {
 // Builds every stored plane of both images of each pair.
 // Uniform geometry: blocks = pairs * 2 * tiles_per_image, work = nullptr.
 // src0/src1, when not null, replace every pair's inputs (single-pair use).
-hipError_t LaunchPrepare(PlaneFormat format, const PairDesc* pairs, const WorkRef* work,
+// general: some image of the launch has a SourceDesc of kind 1 (formats
+// other than 8-bit sRGB RGB); the table-only kernel ignores descriptors.
+hipError_t LaunchPrepare(PlaneFormat format, bool general, const PairDesc* pairs, const WorkRef* work,
                          uint32_t blocks, uint32_t tiles_per_image, const float* lut,
                          const uint8_t* src0, const uint8_t* src1, hipStream_t stream);
 

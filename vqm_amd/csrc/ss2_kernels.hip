@@ -233,6 +233,205 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
     return xcd * per + (xcd < rem ? xcd : rem) + idx;
 }
 
+// ---- general input decode -----------------------------------------------------
+//
+// Images that are not 8-bit sRGB RGB are decoded per pixel from their
+// SourceDesc, in the stage order of color.glsl:447-541: raw sample with the
+// range/depth scale and offset, chroma upsampling, the matrix (or the
+// constant-luminance or ICtCp op), alpha, the transfer curve, and the
+// primaries matrix to linear BT.709. These stages use the device's powf/expf
+// and are held to the A/B tolerances of the input tests, not bit parity.
+
+namespace decode {
+
+constexpr float kRec709Alpha = 1.09929682680944f;
+constexpr float kRec709Beta = 0.018053968510807f;
+
+__device__ float raw_sample(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* src, uint32_t plane,
+                            uint32_t x, uint32_t y) {
+    const uint64_t index = static_cast<uint64_t>(y) * d->plane_w[plane] + x;
+    GlobalAS<const uint8_t>* base = src + d->plane_off[plane];
+    float v;
+    switch (d->type) {
+        case 0: v = static_cast<float>(base[index]); break;
+        case 1: v = static_cast<float>(reinterpret_cast<GlobalAS<const uint16_t>*>(base)[index]); break;
+        case 2: v = static_cast<float>(reinterpret_cast<GlobalAS<const _Float16>*>(base)[index]); break;
+        default: v = reinterpret_cast<GlobalAS<const float>*>(base)[index]; break;
+    }
+    // Sample class: 2 alpha, 1 the chroma planes of YUV, 0 luma or RGB.
+    const uint32_t cls = plane == 3 ? 2u : (plane != 0 && d->family == 2 ? 1u : 0u);
+    return v * d->scale[cls] + d->offset[cls];
+}
+
+// BT.709 inverse OETF (the CL op and the xvYCC / BT.1361 extensions).
+__device__ float rec709_inverse_oetf(float x) {
+    x = fmaxf(x, 0.0f);
+    return x < 4.5f * kRec709Beta ? x / 4.5f
+                                  : powf((x + (kRec709Alpha - 1.0f)) / kRec709Alpha, 1.0f / 0.45f);
+}
+
+// IEC 61966-2-1.
+__device__ float srgb_eotf(float v) {
+    return v <= 0.04045f ? v / 12.92f : powf((v + 0.055f) / 1.055f, 2.4f);
+}
+
+// SMPTE ST 2084, normalised to 10000 cd/m^2.
+__device__ float pq_eotf(float x) {
+    if (x <= 0.0f) return 0.0f;
+    const float p = powf(x, 1.0f / 78.84375f);
+    const float num = fmaxf(p - 0.8359375f, 0.0f);
+    const float den = fmaxf(18.8515625f - 18.6875f * p, 1.17549435e-38f);
+    return powf(num / den, 1.0f / 0.1593017578125f);
+}
+
+// ARIB STD-B67 inverse OETF (scene light).
+__device__ float hlg_inverse_oetf(float x) {
+    x = fmaxf(x, 0.0f);
+    return x <= 0.5f ? (x * x) * (1.0f / 3.0f) : (expf((x - 0.55991073f) / 0.17883277f) + 0.28466892f) / 12.0f;
+}
+
+__device__ float curve(uint32_t id, float v, float scale) {
+    switch (id) {
+        case kCurveLinear: return v;
+        case kCurvePow24: return v < 0.0f ? 0.0f : powf(v, 2.4f);
+        case kCurvePow22: return v < 0.0f ? 0.0f : powf(v, 2.2f);
+        case kCurvePow28: return v < 0.0f ? 0.0f : powf(v, 2.8f);
+        case kCurveSrgb: return srgb_eotf(v);
+        case kCurveLog100: return v <= 0.0f ? 0.01f : powf(10.0f, 2.0f * (v - 1.0f));
+        case kCurveLog316: return v <= 0.0f ? 0.00316227766f : powf(10.0f, 2.5f * (v - 1.0f));
+        case kCurveXvycc:
+            if (v >= 0.0f && v <= 1.0f) return powf(v, 2.4f);
+            return v < 0.0f ? -rec709_inverse_oetf(-v) : rec709_inverse_oetf(v);
+        case kCurveBt1361:
+            if (v >= 0.0f && v <= 1.0f) return powf(v, 2.4f);
+            return v >= 0.0f ? rec709_inverse_oetf(v) : -rec709_inverse_oetf(-4.0f * v) / 4.0f;
+        case kCurvePq: return scale * pq_eotf(v);
+        case kCurveSt428: return v < 0.0f ? 0.0f : (52.37f / 48.0f) * powf(v, 2.6f);
+        default: return v;
+    }
+}
+
+__device__ float3 apply(ConstAS<float>* m, float3 v) {
+    return make_float3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+                       m[6] * v.x + m[7] * v.y + m[8] * v.z);
+}
+
+// Constant-luminance Y'CbCr to linear RGB through the BT.709 curve.
+__device__ float3 cl_to_rgb(ConstAS<SourceDesc>* d, float3 e) {
+    const float u = e.y, v = e.z;
+    const float b_minus_y = u * 2.0f * (u < 0.0f ? d->cl[0] : d->cl[1]);
+    const float r_minus_y = v * 2.0f * (v < 0.0f ? d->cl[2] : d->cl[3]);
+    const float b = rec709_inverse_oetf(b_minus_y + e.x);
+    const float r = rec709_inverse_oetf(r_minus_y + e.x);
+    const float luma = rec709_inverse_oetf(e.x);
+    return make_float3(r, (luma - d->kr * r - d->kb * b) / d->kg, b);
+}
+
+// Resampling kernels: triangle, Catmull-Rom (B = 0, C = 0.5), Lanczos-3.
+__device__ float sinc(float x) {
+    const float pix = 3.14159265358979323846f * x;
+    return x == 0.0f ? 1.0f : sinf(pix) / pix;
+}
+__device__ float filter_weight(uint32_t filter, float x) {
+    x = fabsf(x);
+    if (filter == 0) return fmaxf(1.0f - x, 0.0f);
+    if (filter == 1) {
+        if (x < 1.0f) return 1.0f + x * x * (-2.5f + x * 1.5f);
+        if (x < 2.0f) return 2.0f + x * (-4.0f + x * (2.5f - 0.5f * x));
+        return 0.0f;
+    }
+    return x < 3.0f ? sinc(x) * sinc(x / 3.0f) : 0.0f;
+}
+
+// Rounds half up such that round(x - 1) == round(x) - 1.
+__device__ float round_halfup(float x) { return x < 0.0f ? floorf(x + 0.5f) : floorf(x + 0.49999997f); }
+
+// A tap position mirrored once at the plane edges, then clamped.
+__device__ uint32_t mirror(float pos, uint32_t dim) {
+    const float extent = static_cast<float>(dim);
+    const float m = pos < 0.0f ? -pos : (pos >= extent ? 2.0f * extent - pos : pos);
+    return static_cast<uint32_t>(fminf(fmaxf(m, 0.0f), extent - 1.0f));
+}
+
+constexpr int kMaxTaps = 6;
+
+// The taps of one axis for output sample `out`: chroma position
+// (out + 0.5) / ratio + shift, the first tap half a sample past
+// round(pos - taps / 2). Bilinear weights are 1 - t and t (their sum is
+// exactly one); the others are normalised by their sum.
+__device__ int axis_taps(uint32_t filter, uint32_t out, uint32_t sub, float shift, uint32_t dim,
+                         float w[kMaxTaps], uint32_t idx[kMaxTaps]) {
+    const int taps = filter == 0 ? 2 : filter == 1 ? 4 : 6;
+    const float pos = (static_cast<float>(out) + 0.5f) / static_cast<float>(1u << sub) + shift;
+    const float base = round_halfup(pos - 0.5f * static_cast<float>(taps)) + 0.5f;
+    if (filter == 0) {
+        const float t = pos - base;
+        w[0] = 1.0f - t;
+        w[1] = t;
+    } else {
+        float total = 0.0f;
+        for (int j = 0; j < taps; ++j) total += filter_weight(filter, base + static_cast<float>(j) - pos);
+        for (int j = 0; j < taps; ++j) w[j] = filter_weight(filter, base + static_cast<float>(j) - pos) / total;
+    }
+    for (int j = 0; j < taps; ++j) idx[j] = mirror(base + static_cast<float>(j), dim);
+    return taps;
+}
+
+__device__ float chroma_sample(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* src, uint32_t plane,
+                               uint32_t x, uint32_t y) {
+    float wx[kMaxTaps], wy[kMaxTaps];
+    uint32_t ix[kMaxTaps], iy[kMaxTaps];
+    const int tx = axis_taps(d->filter, x, d->sub_w, d->shift_x, d->plane_w[plane], wx, ix);
+    const int ty = axis_taps(d->filter, y, d->sub_h, d->shift_y, d->plane_h[plane], wy, iy);
+    float sum = 0.0f;
+    for (int j = 0; j < ty; ++j) {
+        float row = 0.0f;
+        for (int k = 0; k < tx; ++k) row += wx[k] * raw_sample(d, src, plane, ix[k], iy[j]);
+        sum += wy[j] * row;
+    }
+    return sum;
+}
+
+// Linear BT.709 RGB of pixel (x, y).
+__device__ float3 pixel(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* src, uint32_t x, uint32_t y) {
+    const float alpha = d->alpha != 0 ? raw_sample(d, src, 3, x, y) : 1.0f;
+    const float y0 = raw_sample(d, src, 0, x, y);
+    float3 e = make_float3(y0, y0, y0);
+    if (d->family == 2 && (d->sub_w != 0 || d->sub_h != 0)) {
+        e.y = chroma_sample(d, src, 1, x, y);
+        e.z = chroma_sample(d, src, 2, x, y);
+    } else if (d->family != 0) {
+        e.y = raw_sample(d, src, 1, x, y);
+        e.z = raw_sample(d, src, 2, x, y);
+    }
+    float3 rgb = d->mode == 1 ? cl_to_rgb(d, e) : (d->identity1 ? e : apply(d->m1, e));
+    if (d->alpha == 2) {
+        const float a = fminf(fmaxf(alpha, 0.0f), 1.0f);
+        rgb = a == 0.0f ? make_float3(0.0f, 0.0f, 0.0f) : make_float3(rgb.x / a, rgb.y / a, rgb.z / a);
+    }
+    if (d->alpha != 0) {
+        const float target = (1.0f - alpha) * d->background;
+        rgb = make_float3(alpha * rgb.x + target, alpha * rgb.y + target, alpha * rgb.z + target);
+    }
+    float3 lin = rgb;
+    if (d->mode != 1) {
+        if (d->transfer == kCurveHlg) {
+            // Per-channel inverse OETF, then the luminance OOTF and the peak.
+            const float3 c = make_float3(hlg_inverse_oetf(rgb.x), hlg_inverse_oetf(rgb.y), hlg_inverse_oetf(rgb.z));
+            const float luma = fmaxf(d->kr * c.x + d->kg * c.y + d->kb * c.z, 1.17549435e-38f);
+            const float k = powf(luma, 0.2f) * d->to_linear_scale;
+            lin = make_float3(c.x * k, c.y * k, c.z * k);
+        } else {
+            lin = make_float3(curve(d->transfer, rgb.x, d->to_linear_scale),
+                              curve(d->transfer, rgb.y, d->to_linear_scale),
+                              curve(d->transfer, rgb.z, d->to_linear_scale));
+        }
+    }
+    return d->identity2 ? lin : apply(d->m2, lin);
+}
+
+}  // namespace decode
+
 // ---- prepare ----------------------------------------------------------------
 
 // Level 1's tile is 32 x 16 samples; every later level is a quarter of the
@@ -282,7 +481,7 @@ __device__ __forceinline__ void prepare_level(ConstAS<Plan>* plan, GlobalAS<uint
     for (uint32_t c = 0; c < 3; ++c) next[c][jy * side_x + jx] = v[c];
 }
 
-template <typename Lay>
+template <typename Lay, bool General>
 __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict__ pairs,
                                                       const WorkRef* __restrict__ work,
                                                       uint32_t blocks, uint32_t tiles_per_image,
@@ -327,20 +526,40 @@ __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict
     // (color.glsl:453-462).
     float lin[2][4][3];
     const uint32_t bx = X0 + 4u * tx;
-    const bool vec = (W % 4u == 0u) && (bx + 3u < W);
+    bool table = true;
+    if constexpr (General) {
+        // Formats other than 8-bit sRGB RGB decode per pixel (block-uniform).
+        const SourceDesc* sdp = image ? pdp->source[1] : pdp->source[0];
+        if (sdp != nullptr && as_const(sdp)->kind != 0u) {
+            table = false;
+            ConstAS<SourceDesc>* sd = as_const(sdp);
+            for (uint32_t r = 0; r < 2; ++r) {
+                const uint32_t y = min(Y0 + 2u * ty + r, H - 1u);
+                for (uint32_t j = 0; j < 4; ++j) {
+                    const float3 c = decode::pixel(sd, src, min(bx + j, W - 1u), y);
+                    lin[r][j][0] = c.x;
+                    lin[r][j][1] = c.y;
+                    lin[r][j][2] = c.z;
+                }
+            }
+        }
+    }
+    if (table) {
+        const bool vec = (W % 4u == 0u) && (bx + 3u < W);
 #pragma unroll
-    for (uint32_t r = 0; r < 2; ++r) {
-        const uint32_t y = min(Y0 + 2u * ty + r, H - 1u);
+        for (uint32_t r = 0; r < 2; ++r) {
+            const uint32_t y = min(Y0 + 2u * ty + r, H - 1u);
 #pragma unroll
-        for (uint32_t c = 0; c < 3; ++c) {
-            GlobalAS<const uint8_t>* row = src + c * plane_stride + static_cast<uint64_t>(y) * W;
-            if (vec) {
-                const uint32_t v = *reinterpret_cast<GlobalAS<const uint32_t>*>(row + bx);
+            for (uint32_t c = 0; c < 3; ++c) {
+                GlobalAS<const uint8_t>* row = src + c * plane_stride + static_cast<uint64_t>(y) * W;
+                if (vec) {
+                    const uint32_t v = *reinterpret_cast<GlobalAS<const uint32_t>*>(row + bx);
 #pragma unroll
-                for (uint32_t j = 0; j < 4; ++j) lin[r][j][c] = lut[(v >> (8u * j)) & 255u];
-            } else {
+                    for (uint32_t j = 0; j < 4; ++j) lin[r][j][c] = lut[(v >> (8u * j)) & 255u];
+                } else {
 #pragma unroll
-                for (uint32_t j = 0; j < 4; ++j) lin[r][j][c] = lut[row[min(bx + j, W - 1u)]];
+                    for (uint32_t j = 0; j < 4; ++j) lin[r][j][c] = lut[row[min(bx + j, W - 1u)]];
+                }
             }
         }
     }
@@ -644,11 +863,15 @@ __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src
 }
 
 template <typename Lay>
-hipError_t launch_prepare_t(const PairDesc* pairs, const WorkRef* work, uint32_t blocks,
+hipError_t launch_prepare_t(bool general, const PairDesc* pairs, const WorkRef* work, uint32_t blocks,
                             uint32_t tiles, const float* lut, const uint8_t* src0,
                             const uint8_t* src1, hipStream_t stream) {
-    hipLaunchKernelGGL(prepare_kernel<Lay>, dim3(blocks), dim3(256), 0, stream, pairs, work,
-                       blocks, tiles, lut, src0, src1);
+    if (general)
+        hipLaunchKernelGGL((prepare_kernel<Lay, true>), dim3(blocks), dim3(256), 0, stream, pairs, work,
+                           blocks, tiles, lut, src0, src1);
+    else
+        hipLaunchKernelGGL((prepare_kernel<Lay, false>), dim3(blocks), dim3(256), 0, stream, pairs, work,
+                           blocks, tiles, lut, src0, src1);
     return hipGetLastError();
 }
 
@@ -671,19 +894,19 @@ hipError_t LaunchCopy(void* dst, const void* src, uint64_t bytes, hipStream_t st
     return hipGetLastError();
 }
 
-hipError_t LaunchPrepare(PlaneFormat format, const PairDesc* pairs, const WorkRef* work,
+hipError_t LaunchPrepare(PlaneFormat format, bool general, const PairDesc* pairs, const WorkRef* work,
                          uint32_t blocks, uint32_t tiles_per_image, const float* lut,
                          const uint8_t* src0, const uint8_t* src1, hipStream_t stream) {
     if (blocks == 0) return hipSuccess;
     switch (format) {
         case kPlaneF32:
-            return launch_prepare_t<LayoutF32>(pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
+            return launch_prepare_t<LayoutF32>(general, pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
         case kPlaneF16:
-            return launch_prepare_t<LayoutF16>(pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
+            return launch_prepare_t<LayoutF16>(general, pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
         case kPlaneBF16:
-            return launch_prepare_t<LayoutBF16>(pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
+            return launch_prepare_t<LayoutBF16>(general, pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
         case kPlaneU16:
-            return launch_prepare_t<LayoutU16>(pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
+            return launch_prepare_t<LayoutU16>(general, pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
     }
     return hipErrorInvalidValue;
 }

@@ -2,6 +2,7 @@
 
 #include "ss2_runtime.h"
 
+#include "ss2_source.h"
 #include "ss2_tune.h"
 
 #include <algorithm>
@@ -274,7 +275,7 @@ void Lane::Launch(PlaneFormat fmt, const float* lut, const LaunchEntry* entries,
     if (count == 0 || count > max_pairs_) Fail(VQM_INVALID_ARGUMENT, "lane: bad pair count");
     chunks = std::max(1u, std::min({chunks, count, kMaxChunks}));
     auto* hp = h_pairs_.as<PairDesc>();
-    bool uniform = true;
+    bool uniform = true, general = false;
     uint64_t prep_work = 0, scale_work = 0;
     std::vector<uint64_t> prep_at(count + 1, 0), scale_at(count + 1, 0);  // work-list offsets
     for (uint32_t i = 0; i < count; ++i) {
@@ -287,6 +288,9 @@ void Lane::Launch(PlaneFormat fmt, const float* lut, const LaunchEntry* entries,
         pd.norms = norms_.as<double>() + static_cast<uint64_t>(i) * kNorms;
         pd.plan = e.plan->plan.as<Plan>();
         pd.items = e.plan->items.as<ScaleItem>();
+        pd.source[0] = e.source[0];
+        pd.source[1] = e.source[1];
+        general = general || e.general;
         uniform = uniform && e.plan == entries[0].plan;
         prep_work += 2ull * e.plan->host.prep_tiles;
         scale_work += e.plan->host.plan.items;
@@ -345,7 +349,7 @@ void Lane::Launch(PlaneFormat fmt, const float* lut, const LaunchEntry* entries,
         const WorkRef* wp = uniform ? nullptr : wprep + prep_at[i0];
         const auto np = static_cast<uint32_t>(prep_at[i1] - prep_at[i0]);
         Check(hipEventRecord(Ev(4 * c).get(), s), "hipEventRecord");
-        Check(LaunchPrepare(fmt, base, wp, np, g0.prep_tiles, lut, nullptr, nullptr, s), "launch prepare");
+        Check(LaunchPrepare(fmt, general, base, wp, np, g0.prep_tiles, lut, nullptr, nullptr, s), "launch prepare");
         Check(hipEventRecord(Ev(4 * c + 1).get(), s), "hipEventRecord");
     }
     for (uint32_t c = 0; c < chunks; ++c) {
@@ -414,6 +418,8 @@ static const vqm_ss2_config& Validated(const vqm_ss2_config& config) {
     ToFormat(config.plane_format);
     if (config.resident_images > kMaxResident)
         Fail(VQM_INVALID_ARGUMENT, "gpu_resident_images is limited to 4096 slots");
+    if (config.chroma_filter > 2)
+        Fail(VQM_INVALID_ARGUMENT, "chroma_filter must be bilinear, bicubic or lanczos");
     return config;
 }
 
@@ -423,26 +429,32 @@ Pipeline::Pipeline(const vqm_ss2_config& config)
       plan_(config.width, config.height, static_cast<PlaneFormat>(config.plane_format),
             config.strip_rows),
       ws_(plan_.layout),
-      pair_(sizeof(PairDesc)) {
+      pair_(sizeof(PairDesc)),
+      sources_(2 * sizeof(SourceDesc)),
+      h_sources_(2 * sizeof(SourceDesc)) {
     const vqm_image_format srgb = Srgb8();
     image_bytes_ = RequiredBytes(srgb, config.width, config.height);
+    std::memset(h_sources_.get(), 0, 2 * sizeof(SourceDesc));
+    Check(hipMemcpy(sources_.get(), h_sources_.get(), 2 * sizeof(SourceDesc), hipMemcpyHostToDevice),
+          "upload source descriptors");
     PairDesc pd{};
     pd.planes = ws_.planes;
     pd.partials = ws_.partials;
     pd.norms = ws_.norms;
     pd.plan = plan_.plan.as<Plan>();
     pd.items = plan_.items.as<ScaleItem>();
+    pd.source[0] = sources_.as<SourceDesc>();
+    pd.source[1] = sources_.as<SourceDesc>() + 1;
     Check(hipMemcpy(pair_.get(), &pd, sizeof(pd), hipMemcpyHostToDevice), "upload pair descriptor");
     if (config.graph) {
         h_pair_ = PinnedBuffer(sizeof(PairDesc));
         std::memcpy(h_pair_.get(), &pd, sizeof(pd));
     }
+    // Slots and streaming buffers start at the size of an 8-bit RGB image and
+    // grow to the largest image they are handed.
     slots_.resize(config.resident_images);
     for (Slot& s : slots_) s.data = DeviceBuffer(image_bytes_);
-    if (config.streaming_inputs) {
-        stream_inputs_ = DeviceBuffer(2 * image_bytes_);
-        staging_ = PinnedBuffer(2 * image_bytes_);
-    }
+    if (config.streaming_inputs) EnsureStreaming(image_bytes_);
     host_norms_ = PinnedBuffer(kNorms * sizeof(double));
     if (config.input_bytes > 0) input_pool_ = DeviceBuffer(config.input_bytes);
     if (config.resident_images > 0) {
@@ -462,11 +474,36 @@ Pipeline::~Pipeline() {
     if (graph_) (void)hipGraphExecDestroy(graph_);
 }
 
+void Pipeline::EnsureStreaming(uint64_t bytes) {
+    if (bytes <= stream_image_bytes_) return;
+    const uint64_t each = (bytes + 255) / 256 * 256;
+    stream_inputs_ = DeviceBuffer(2 * each);
+    staging_ = PinnedBuffer(2 * each);
+    stream_image_bytes_ = each;
+}
+
+void Pipeline::SetSources(const vqm_image_format& a, const vqm_image_format& b, float background) {
+    auto* h = h_sources_.as<SourceDesc>();
+    const vqm_image_format* f[2] = {&a, &b};
+    bool general = false;
+    for (int k = 0; k < 2; ++k) {
+        if (IsSrgb8(*f[k])) {
+            h[k] = SourceDesc{};
+        } else {
+            h[k] = BuildSourceDesc(*f[k], config_.width, config_.height, config_.chroma_filter);
+            h[k].background = background;
+            general = true;
+        }
+    }
+    general_ = general;
+    sources_pending_ = general;
+}
+
 // Captures, on the pipeline's stream, the upload of the pinned pair
 // descriptor (its input pointers change per pair), the three kernels (the
 // prepare kernel reads its inputs through the descriptor) and the norm
 // read-back. The graph holds no memset node: the plane borders are cleared
-// once, when the workspace is created.
+// once, when the workspace is created. Graphs cover 8-bit sRGB RGB pairs.
 void Pipeline::BuildGraph() {
     const hipStream_t s = stream_.get();
     const PairDesc* pd = pair_.as<PairDesc>();
@@ -474,7 +511,7 @@ void Pipeline::BuildGraph() {
     const GeometryPlan& g = plan_.host;
     Check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin graph capture");
     hipError_t e = hipMemcpyAsync(pair_.get(), h_pair_.get(), sizeof(PairDesc), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = LaunchPrepare(fmt, pd, nullptr, 2 * g.prep_tiles, g.prep_tiles, ctx_.lut, nullptr, nullptr, s);
+    if (e == hipSuccess) e = LaunchPrepare(fmt, false, pd, nullptr, 2 * g.prep_tiles, g.prep_tiles, ctx_.lut, nullptr, nullptr, s);
     if (e == hipSuccess) e = LaunchScale(fmt, pd, nullptr, g.plan.items, g.plan.items, s);
     if (e == hipSuccess) e = LaunchReduce(pd, 1, s);
     if (e == hipSuccess)
@@ -514,7 +551,9 @@ uint64_t Pipeline::Upload(const vqm_image_desc& image, const void* data) {
     if (data == nullptr) Fail(VQM_INVALID_ARGUMENT, "image: data is null");
     for (Slot& s : slots_) {
         if (s.handle != 0) continue;
-        Check(hipMemcpy(s.data.get(), data, image_bytes_, hipMemcpyHostToDevice), "upload image");
+        if (s.data.bytes() < image.bytes) s.data = DeviceBuffer(image.bytes);
+        Check(hipMemcpy(s.data.get(), data, image.bytes, hipMemcpyHostToDevice), "upload image");
+        s.format = image.format;
         s.handle = NextId();
         return s.handle;
     }
@@ -534,7 +573,7 @@ uint64_t Pipeline::Launch(const uint8_t* src0, const uint8_t* src1, bool reuse_o
     const PlaneFormat fmt = static_cast<PlaneFormat>(config_.plane_format);
     const GeometryPlan& g = plan_.host;
     prepared_images_ = reuse_original ? 1 : 2;
-    if (config_.graph && !reuse_original) {
+    if (config_.graph && !reuse_original && !general_) {
         if (graph_ == nullptr) BuildGraph();
         auto* hp = h_pair_.as<PairDesc>();
         hp->src[0] = src0;
@@ -547,13 +586,18 @@ uint64_t Pipeline::Launch(const uint8_t* src0, const uint8_t* src1, bool reuse_o
         return in_flight_;
     }
     graph_launched_ = false;
+    if (sources_pending_) {
+        Check(hipMemcpyAsync(sources_.get(), h_sources_.get(), 2 * sizeof(SourceDesc), hipMemcpyHostToDevice, s),
+              "upload source descriptors");
+        sources_pending_ = false;
+    }
     Check(hipEventRecord(ev_[0].get(), s), "hipEventRecord");
     if (reuse_original)
-        Check(LaunchPrepare(fmt, pd, distorted_work_.as<WorkRef>(), g.prep_tiles, g.prep_tiles, ctx_.lut,
+        Check(LaunchPrepare(fmt, general_, pd, distorted_work_.as<WorkRef>(), g.prep_tiles, g.prep_tiles, ctx_.lut,
                             src0, src1, s),
               "launch prepare");
     else
-        Check(LaunchPrepare(fmt, pd, nullptr, 2 * g.prep_tiles, g.prep_tiles, ctx_.lut, src0, src1, s),
+        Check(LaunchPrepare(fmt, general_, pd, nullptr, 2 * g.prep_tiles, g.prep_tiles, ctx_.lut, src0, src1, s),
               "launch prepare");
     Check(hipEventRecord(ev_[1].get(), s), "hipEventRecord");
     Check(LaunchScale(fmt, pd, nullptr, g.plan.items, g.plan.items, s), "launch scale");
@@ -569,7 +613,7 @@ uint64_t Pipeline::Launch(const uint8_t* src0, const uint8_t* src1, bool reuse_o
 }
 
 uint64_t Pipeline::SubmitHost(const vqm_image_desc& a, const void* da, const vqm_image_desc& b,
-                              const void* db) {
+                              const void* db, float background) {
     RequireIdle();
     if (!config_.streaming_inputs)
         Fail(VQM_INVALID_ARGUMENT,
@@ -578,49 +622,57 @@ uint64_t Pipeline::SubmitHost(const vqm_image_desc& a, const void* da, const vqm
     CheckImage(a, config_.width, config_.height, "original");
     CheckImage(b, config_.width, config_.height, "distorted");
     if (da == nullptr || db == nullptr) Fail(VQM_INVALID_ARGUMENT, "image data is null");
+    EnsureStreaming(std::max(a.bytes, b.bytes));
     auto* staging = staging_.as<uint8_t>();
-    std::memcpy(staging, da, image_bytes_);
-    std::memcpy(staging + image_bytes_, db, image_bytes_);
-    Check(hipMemcpyAsync(stream_inputs_.get(), staging, 2 * image_bytes_, hipMemcpyHostToDevice,
+    std::memcpy(staging, da, a.bytes);
+    std::memcpy(staging + stream_image_bytes_, db, b.bytes);
+    Check(hipMemcpyAsync(stream_inputs_.get(), staging, 2 * stream_image_bytes_, hipMemcpyHostToDevice,
                          stream_.get()),
           "upload pair");
     const auto* base = stream_inputs_.as<uint8_t>();
+    SetSources(a.format, b.format, background);
     cached_original_ = 0;  // the workspace now holds a host image's planes
-    return Launch(base, base + image_bytes_);
+    return Launch(base, base + stream_image_bytes_);
 }
 
-uint64_t Pipeline::SubmitResident(uint64_t a, uint64_t b) {
+uint64_t Pipeline::SubmitResident(uint64_t a, uint64_t b, float background) {
     RequireIdle();
     const Slot& sa = RequireSlot(a, "original");
     const Slot& sb = RequireSlot(b, "distorted");
-    const bool reuse = a == cached_original_;
+    // vulkan.cc:716-735: the original's planes are reused when it is the
+    // previous submission's resident original with the same background.
+    const bool reuse = a == cached_original_ && background == cached_background_;
     cached_original_ = 0;
+    SetSources(sa.format, sb.format, background);
     const uint64_t token = Launch(sa.data.as<uint8_t>(), sb.data.as<uint8_t>(), reuse);
     cached_original_ = a;
+    cached_background_ = background;
     return token;
 }
 
-uint64_t Pipeline::SubmitResidentHost(uint64_t a, const vqm_image_desc& b, const void* db) {
+uint64_t Pipeline::SubmitResidentHost(uint64_t a, const vqm_image_desc& b, const void* db, float background) {
     RequireIdle();
     if (!config_.streaming_inputs)
         Fail(VQM_INVALID_ARGUMENT, "streaming_inputs is false, so the pixel overloads are unavailable");
     const Slot& sa = RequireSlot(a, "original");
     CheckImage(b, config_.width, config_.height, "distorted");
     if (db == nullptr) Fail(VQM_INVALID_ARGUMENT, "image data is null");
-    auto* staging = staging_.as<uint8_t>() + image_bytes_;
-    std::memcpy(staging, db, image_bytes_);
-    auto* dst = stream_inputs_.as<uint8_t>() + image_bytes_;
-    Check(hipMemcpyAsync(dst, staging, image_bytes_, hipMemcpyHostToDevice, stream_.get()),
-          "upload distorted");
-    const bool reuse = a == cached_original_;
+    EnsureStreaming(b.bytes);
+    auto* staging = staging_.as<uint8_t>() + stream_image_bytes_;
+    std::memcpy(staging, db, b.bytes);
+    auto* dst = stream_inputs_.as<uint8_t>() + stream_image_bytes_;
+    Check(hipMemcpyAsync(dst, staging, b.bytes, hipMemcpyHostToDevice, stream_.get()), "upload distorted");
+    const bool reuse = a == cached_original_ && background == cached_background_;
     cached_original_ = 0;
+    SetSources(sa.format, b.format, background);
     const uint64_t token = Launch(sa.data.as<uint8_t>(), dst, reuse);
     cached_original_ = a;
+    cached_background_ = background;
     return token;
 }
 
 uint64_t Pipeline::SubmitDevice(const vqm_device_image& a, const vqm_device_image& b,
-                                hipStream_t wait_stream) {
+                                hipStream_t wait_stream, float background) {
     RequireIdle();
     const vqm_image_desc da{a.width, a.height, a.format, a.bytes};
     const vqm_image_desc db{b.width, b.height, b.format, b.bytes};
@@ -632,6 +684,7 @@ uint64_t Pipeline::SubmitDevice(const vqm_device_image& a, const vqm_device_imag
         Check(hipEventRecord(ready.get(), wait_stream), "record caller stream");
         Check(hipStreamWaitEvent(stream_.get(), ready.get(), 0), "wait caller stream");
     }
+    SetSources(a.format, b.format, background);
     cached_original_ = 0;  // caller-owned memory may change between submissions
     return Launch(static_cast<const uint8_t*>(a.data), static_cast<const uint8_t*>(b.data));
 }
@@ -640,10 +693,8 @@ uint64_t Pipeline::AllocateImage(const vqm_image_format& format, uint64_t* bytes
     RequireIdle();
     if (input_pool_.bytes() == 0)
         Fail(VQM_INVALID_ARGUMENT, "no caller-owned image buffers: Ssimulacra2Config::input_bytes is 0");
-    const vqm_image_desc desc{config_.width, config_.height, format,
-                              RequiredBytes(format, config_.width, config_.height)};
-    CheckImage(desc, config_.width, config_.height, "image buffer");
-    const uint64_t need = desc.bytes;
+    ValidateFormat(format, config_.width, config_.height, "image buffer");
+    const uint64_t need = RequiredBytes(format, config_.width, config_.height);
     // First fit over the pool, 256-byte aligned, in address order.
     std::vector<InputBuffer> sorted = inputs_;
     std::sort(sorted.begin(), sorted.end(),
@@ -655,7 +706,7 @@ uint64_t Pipeline::AllocateImage(const vqm_image_format& format, uint64_t* bytes
     }
     if (cursor + need > input_pool_.bytes())
         Fail(VQM_INVALID_ARGUMENT, "the caller-owned image pool is full");
-    InputBuffer ib{NextId(), cursor, need};
+    InputBuffer ib{NextId(), cursor, need, format};
     inputs_.push_back(ib);
     if (bytes) *bytes = need;
     return ib.handle;
@@ -673,11 +724,12 @@ void Pipeline::ReleaseImage(uint64_t buffer) {
                   inputs_.end());
 }
 
-uint64_t Pipeline::SubmitBuffers(uint64_t a, uint64_t b) {
+uint64_t Pipeline::SubmitBuffers(uint64_t a, uint64_t b, float background) {
     RequireIdle();
     const InputBuffer& ba = RequireBuffer(a, "original");
     const InputBuffer& bb = RequireBuffer(b, "distorted");
     const auto* base = input_pool_.as<uint8_t>();
+    SetSources(ba.format, bb.format, background);
     cached_original_ = 0;
     return Launch(base + ba.offset, base + bb.offset);
 }
@@ -697,7 +749,7 @@ uint32_t Pipeline::Finish(uint64_t token, double* norms) {
 
 vqm_ss2_memory Pipeline::Memory() const {
     vqm_ss2_memory m{};
-    m.workspace_bytes = ws_.bytes + pair_.bytes() + plan_.plan.bytes() + plan_.items.bytes();
+    m.workspace_bytes = ws_.bytes + pair_.bytes() + sources_.bytes() + plan_.plan.bytes() + plan_.items.bytes();
     for (const Slot& s : slots_) m.resident_bytes += s.data.bytes();
     m.streaming_bytes = stream_inputs_.bytes();
     m.input_pool_bytes = input_pool_.bytes();

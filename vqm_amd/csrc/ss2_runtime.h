@@ -210,6 +210,10 @@ struct LaunchEntry {
     const uint8_t* src[2] = {nullptr, nullptr};
     const void* host[2] = {nullptr, nullptr};
     uint64_t image_bytes = 0;
+    // Device decode descriptors (null: 8-bit sRGB RGB); `general` when one
+    // of them is of kind 1.
+    const SourceDesc* source[2] = {nullptr, nullptr};
+    bool general = false;
 };
 
 // A stream with the descriptor, work-list and norm buffers of one launch of
@@ -263,16 +267,18 @@ public:
 
     uint64_t Upload(const vqm_image_desc& image, const void* data);
     void Release(uint64_t handle);
+    // Every submission takes the alpha blend target `background` (encoded
+    // RGB; opaque inputs ignore it).
     uint64_t SubmitHost(const vqm_image_desc& a, const void* da, const vqm_image_desc& b,
-                        const void* db);
-    uint64_t SubmitResident(uint64_t a, uint64_t b);
-    uint64_t SubmitResidentHost(uint64_t a, const vqm_image_desc& b, const void* db);
+                        const void* db, float background);
+    uint64_t SubmitResident(uint64_t a, uint64_t b, float background);
+    uint64_t SubmitResidentHost(uint64_t a, const vqm_image_desc& b, const void* db, float background);
     uint64_t SubmitDevice(const vqm_device_image& a, const vqm_device_image& b,
-                          hipStream_t wait_stream);
+                          hipStream_t wait_stream, float background);
     uint64_t AllocateImage(const vqm_image_format& format, uint64_t* bytes);
     void* MapImage(uint64_t buffer);
     void ReleaseImage(uint64_t buffer);
-    uint64_t SubmitBuffers(uint64_t a, uint64_t b);
+    uint64_t SubmitBuffers(uint64_t a, uint64_t b, float background);
     uint32_t Finish(uint64_t token, double* norms);
     vqm_ss2_memory Memory() const;
     std::array<float, 3> StageTimes() const { return stage_ms_; }
@@ -280,13 +286,20 @@ public:
 private:
     struct Slot {
         uint64_t handle = 0;  // 0 = free
-        DeviceBuffer data;
+        DeviceBuffer data;    // grows to the largest image it has held
+        vqm_image_format format{};
     };
     struct InputBuffer {
         uint64_t handle = 0;
         uint64_t offset = 0;
         uint64_t bytes = 0;
+        vqm_image_format format{};
     };
+    // Decode descriptors of the pair's images: kind 0 for 8-bit sRGB RGB,
+    // else built from the format; uploaded by the next launch when needed.
+    void SetSources(const vqm_image_format& a, const vqm_image_format& b, float background);
+    // Room for two host images of `bytes` each in the streaming buffers.
+    void EnsureStreaming(uint64_t bytes);
     void RequireIdle() const;
     const Slot& RequireSlot(uint64_t handle, const char* name) const;
     const InputBuffer& RequireBuffer(uint64_t handle, const char* name) const;
@@ -306,6 +319,11 @@ private:
     std::vector<Slot> slots_;
     DeviceBuffer stream_inputs_;   // two images (streaming_inputs)
     PinnedBuffer staging_;         // two images, pinned
+    uint64_t stream_image_bytes_ = 0;  // capacity per image of the two above
+    DeviceBuffer sources_;         // two SourceDesc, read through PairDesc::source
+    PinnedBuffer h_sources_;
+    bool general_ = false;         // this pair has an image of kind 1
+    bool sources_pending_ = false; // h_sources_ not yet copied to sources_
     PinnedBuffer host_norms_;
     DeviceBuffer input_pool_;      // caller-owned images (input_bytes)
     std::vector<InputBuffer> inputs_;
@@ -317,6 +335,7 @@ private:
     // Resident reference cache (vulkan.cc:238-241,716-735): the handle of
     // the resident original whose planes the workspace holds, or 0.
     uint64_t cached_original_ = 0;
+    float cached_background_ = 0.0f;
     DeviceBuffer distorted_work_;  // prepare work list of the distorted image only
     uint32_t prepared_images_ = 0; // images the last submission prepared
 

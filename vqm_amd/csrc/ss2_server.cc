@@ -2,6 +2,8 @@
 
 #include "ss2_server.h"
 
+#include "ss2_source.h"
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -87,6 +89,8 @@ uint64_t Server::SubmitHost(const vqm_image_desc& a, const void* da, const vqm_i
     PlanScales(a.width, a.height);
     CheckImage(a, a.width, a.height, "original");
     CheckImage(b, a.width, a.height, "distorted");
+    if (!IsSrgb8(a.format) || !IsSrgb8(b.format))
+        Fail(VQM_INVALID_ARGUMENT, "the server scores 8-bit full-range sRGB RGB images");
     if (da == nullptr || db == nullptr) Fail(VQM_INVALID_ARGUMENT, "image data is null");
     auto r = std::make_shared<Request>();
     r->width = a.width;

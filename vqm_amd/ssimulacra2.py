@@ -158,6 +158,34 @@ class ImageFormat:
         return f
 
 
+class ChromaLocation(enum.IntEnum):
+    Left = 0
+    Center = 1
+    TopLeft = 2
+    Top = 3
+    BottomLeft = 4
+    Bottom = 5
+
+
+class ChromaFilter(enum.IntEnum):
+    Bilinear = 0
+    Bicubic = 1
+    Lanczos = 2
+
+
+def validate_image_format(fmt: ImageFormat, size: Size) -> None:
+    """ValidateImage (input.cc:152-222): raises InvalidArgument naming the reason."""
+    _check(L.lib().vqm_validate_image_format(ctypes.byref(fmt.to_c()), size.width, size.height))
+
+
+def plane_scale_offset(fmt: ImageFormat, chroma: bool = False, alpha: bool = False):
+    """PlaneScaleOffset (input.cc:714-757): (scale, offset) as float32."""
+    scale, offset = L.f32(), L.f32()
+    _check(L.lib().vqm_plane_scale_offset(ctypes.byref(fmt.to_c()), int(chroma), int(alpha),
+                                          ctypes.byref(scale), ctypes.byref(offset)))
+    return scale.value, offset.value
+
+
 def required_bytes(fmt: ImageFormat, size: Size) -> int:
     """RequiredBytes, vqm_image.h:143-146."""
     out = L.u64()
@@ -311,11 +339,13 @@ class Ssimulacra2Config:
     plane_format: Optional[int] = None
     strip_rows: int = 0
     graph: bool = False  # replay one captured HIP graph per pair (no reference field)
+    chroma_filter: int = 0  # ChromaFilter: 0 bilinear, 1 bicubic, 2 lanczos
 
     def to_c(self) -> L.ConfigC:
         c = L.ConfigC()
         L.lib().vqm_ss2_config_init(ctypes.byref(c), self.size.width, self.size.height)
         c.graph = 1 if self.graph else 0
+        c.chroma_filter = int(self.chroma_filter)
         c.width, c.height = self.size.width, self.size.height
         c.resident_images = self.gpu_resident_images
         fmt = self.plane_format
