@@ -129,3 +129,72 @@ def test_scale_offset():
     f.depth, f.range = 7, PixelRange.Limited
     with pytest.raises(InvalidArgument):
         vqm_amd.plane_scale_offset(f)
+
+
+# ---- pins of the FP64 format reference the GPU A/B tests use -------------------
+
+def test_format_reference_closed_forms():
+    """format_ref against published constants it does not contain: the
+    luminance rows of BT.709 and BT.2020 derived from their primaries, the
+    BT.709 decode matrix, PQ and HLG anchor points, and the ICtCp and
+    YCgCo matrices on white."""
+    import format_ref as R
+    from vqm_amd import ChromaFilter, ChromaLocation, ColorPrimaries as CP, MatrixCoefficients as MC
+    from vqm_amd import TransferCharacteristics as TC
+    np.testing.assert_allclose(R.rgb_to_xyz(CP.Bt709)[1], [0.2126, 0.7152, 0.0722], atol=1e-4)
+    np.testing.assert_allclose(R.rgb_to_xyz(CP.Bt2020)[1], [0.2627, 0.6780, 0.0593], atol=1e-4)
+    np.testing.assert_allclose(R.rgb_to_xyz(CP.Bt709)[:, 0], [0.4124, 0.2126, 0.0193], atol=1e-4)
+    np.testing.assert_allclose(R.gamut(CP.Bt709), np.eye(3), atol=1e-12)
+    np.testing.assert_allclose(R.gamut(CP.Bt2020)[0], [1.6605, -0.5876, -0.0728], atol=1e-4)
+    np.testing.assert_allclose(R.ncl_to_rgb(MC.Bt709), [[1, 0, 1.5748], [1, -0.18733, -0.46813],
+                                                        [1, 1.8556, 0]], atol=1e-4)
+    kr, kb = R.derived_krkb(CP.Bt470Bg)
+    assert (kr, kb) == pytest.approx((0.2220, 0.0713), abs=2e-4)
+    assert R.decode(TC.St2084, 1.0) == pytest.approx(100.0, rel=1e-12)
+    assert R.decode(TC.St2084, 0.5080784) == pytest.approx(1.0, rel=1e-4)  # 100 cd/m2
+    assert R.hlg_inverse_oetf(0.5) == pytest.approx(1.0 / 12.0, rel=1e-12)
+    assert R.hlg_inverse_oetf(0.5 + 1e-12) == pytest.approx(1.0 / 12.0, rel=1e-6)  # continuous
+    assert R.hlg_inverse_oetf(1.0) == pytest.approx(1.0, rel=1e-6)
+    assert R.decode(TC.Srgb, 0.04045) == pytest.approx(R.decode(TC.Srgb, 0.04045 + 1e-12), rel=1e-6)
+    for v in (0.01, 0.3, 0.9):
+        assert R.bt709_inverse_oetf(R.bt709_oetf(v)) == pytest.approx(v, rel=1e-12)
+    white = R.np.linalg.inv(R.lms_to_rgb2020()) @ np.ones(3)
+    np.testing.assert_allclose(white, np.ones(3), atol=1e-12)
+    for t in (TC.St2084, TC.AribB67):
+        np.testing.assert_allclose(R.lms_to_ictcp(t) @ np.ones(3), [1, 0, 0], atol=1e-12)
+    np.testing.assert_allclose(R.ncl_to_rgb(MC.YCgCo) @ [1, 0, 0], np.ones(3), atol=1e-12)
+    # A BT.2020 constant-luminance neutral decodes to itself through the curve.
+    assert R.cl_to_rgb(0.2627, 0.0593, (0.5, 0.0, 0.0)) == pytest.approx(
+        (R.bt709_inverse_oetf(0.5),) * 3, rel=1e-12)
+    # Resampler: textbook 2x bilinear phases for centre and left siting.
+    c = R.chroma_shift(ChromaLocation.Center, True, 1)
+    assert R.axis_taps(ChromaFilter.Bilinear, 0, 2.0, c, 8) == [(0, 0.25), (0, 0.75)]
+    assert R.axis_taps(ChromaFilter.Bilinear, 1, 2.0, c, 8) == [(0, 0.75), (1, 0.25)]
+    left = R.chroma_shift(ChromaLocation.Left, True, 1)
+    assert R.axis_taps(ChromaFilter.Bilinear, 1, 2.0, left, 8) == [(0, 0.5), (1, 0.5)]
+    assert R.axis_taps(ChromaFilter.Bilinear, 15, 2.0, c, 8)[-1][0] == 7  # mirrored edge
+    for flt in ChromaFilter:
+        for out in range(16):
+            assert sum(w for _, w in R.axis_taps(flt, out, 2.0, c, 8)) == pytest.approx(1.0)
+    # Co-sited samples reproduce the source exactly under every filter.
+    for flt in ChromaFilter:
+        taps = R.axis_taps(flt, 2, 2.0, left, 8)
+        assert sum(w for i, w in taps if i == 1) == pytest.approx(1.0, abs=1e-12)
+
+
+def test_format_reference_pack_layout():
+    """format_ref.pack's plane layout agrees with the library's RequiredBytes
+    for every family, subsampling and alpha combination it packs."""
+    import format_ref as R
+    from vqm_amd import ColorPrimaries as CP, MatrixCoefficients as MC
+    for fam, mat, sub, alpha in [(ColorFamily.Rgb, MC.Rgb, (0, 0), AlphaType.NoAlpha),
+                                 (ColorFamily.Grey, MC.Unspecified, (0, 0), AlphaType.Straight),
+                                 (ColorFamily.Yuv, MC.Bt709, (1, 1), AlphaType.Premultiplied),
+                                 (ColorFamily.Yuv, MC.Bt709, (2, 0), AlphaType.NoAlpha)]:
+        for t in PixelType:
+            f = ImageFormat(type=t, family=fam, matrix=mat, alpha=alpha, subsample_w=sub[0],
+                            subsample_h=sub[1], primaries=CP.Bt709)
+            size = Size(12, 6)
+            vals = {p: np.full((h, w), 0.25) for p, w, h in R.plane_shapes(f, size)}
+            img = R.pack(f, size, vals)
+            assert len(img.data) == vqm_amd.required_bytes(f, size)

@@ -88,6 +88,13 @@ class MatrixCoefficients(enum.IntEnum):
     YCgCo = 8
     Bt2020Ncl = 9
     Bt2020Cl = 10
+    St2085YDzDx = 11
+    ChromaticityDerivedNcl = 12
+    ChromaticityDerivedCl = 13
+    Bt2100ICtCp = 14
+    St2128IptC2 = 15
+    YCgCoRe = 16
+    YCgCoRo = 17
 
 
 class TransferCharacteristics(enum.IntEnum):
@@ -119,6 +126,10 @@ class ColorPrimaries(enum.IntEnum):
     St240M = 7
     Film = 8
     Bt2020 = 9
+    St428 = 10
+    St431_2 = 11
+    St432_1 = 12
+    Ebu3213E = 22
 
 
 @dataclass(frozen=True)
@@ -129,8 +140,9 @@ class Size:
 
 @dataclass
 class ImageFormat:
-    """vqm_image.h:100-120. The default is the reference's (Float samples);
-    this path decodes Byte sRGB RGB, see ImageFormat.srgb8()."""
+    """vqm_image.h:100-120. The default is the reference's (Float sRGB RGB);
+    ImageFormat.srgb8() is the 8-bit sRGB form the table-driven fast decode
+    serves, every other accepted format takes the general per-pixel decode."""
     type: int = PixelType.Float
     family: int = ColorFamily.Rgb
     alpha: int = AlphaType.NoAlpha
