@@ -37,13 +37,30 @@ enum class PixelType : std::uint8_t { Byte = 0, Word = 1, Half = 2, Float = 3 };
 enum class PixelRange : std::uint8_t { Limited = 0, Full = 1 };
 enum class ColorFamily : std::uint8_t { Grey = 0, Rgb = 1, Yuv = 2 };
 enum class AlphaType : std::uint8_t { None = 0, Straight = 1, Premultiplied = 2 };
-enum class MatrixCoefficients : std::uint8_t { Rgb = 0, Bt709 = 1, Unspecified = 2 };
-enum class TransferCharacteristics : std::uint8_t { Bt709 = 1, Unspecified = 2, Linear = 8, Srgb = 13 };
-enum class ColorPrimaries : std::uint8_t { Bt709 = 1, Unspecified = 2, Bt2020 = 9 };
-enum class PlaneFormat : std::uint32_t { F32 = VQM_PLANES_F32, F16 = VQM_PLANES_F16, BF16 = VQM_PLANES_BF16 };
+enum class ChromaLocation : std::uint8_t { Left = 0, Center = 1, TopLeft = 2, Top = 3, BottomLeft = 4, Bottom = 5 };
+enum class MatrixCoefficients : std::uint8_t {
+    Rgb = 0, Bt709 = 1, Unspecified = 2, Fcc = 4, Bt470Bg = 5, St170M = 6, St240M = 7, YCgCo = 8,
+    Bt2020Ncl = 9, Bt2020Cl = 10, St2085YDzDx = 11, ChromaticityDerivedNcl = 12,
+    ChromaticityDerivedCl = 13, Bt2100ICtCp = 14, St2128IptC2 = 15, YCgCoRe = 16, YCgCoRo = 17
+};
+enum class TransferCharacteristics : std::uint8_t {
+    Bt709 = 1, Unspecified = 2, Bt470M = 4, Bt470Bg = 5, Bt601 = 6, St240M = 7, Linear = 8,
+    Log100 = 9, Log316 = 10, Iec61966_2_4 = 11, Bt1361 = 12, Srgb = 13, Bt2020_10 = 14,
+    Bt2020_12 = 15, St2084 = 16, St428 = 17, AribB67 = 18
+};
+enum class ColorPrimaries : std::uint8_t {
+    Bt709 = 1, Unspecified = 2, Bt470M = 4, Bt470Bg = 5, St170M = 6, St240M = 7, Film = 8,
+    Bt2020 = 9, St428 = 10, St431_2 = 11, St432_1 = 12, Ebu3213E = 22
+};
+// Upsampling kernel of subsampled chroma planes (vqm_image.h:94-99).
+enum class ChromaFilter : std::uint8_t { Bilinear = 0, Bicubic = 1, Lanczos = 2 };
+enum class PlaneFormat : std::uint32_t {
+    F32 = VQM_PLANES_F32, F16 = VQM_PLANES_F16, BF16 = VQM_PLANES_BF16, U16 = VQM_PLANES_U16
+};
 
-// vqm_image.h:100-120. The reference's default sample type is Float; this
-// path decodes 8-bit sRGB RGB (ImageFormat::Srgb8()).
+// vqm_image.h:100-120. The reference's default is Float sRGB RGB;
+// ImageFormat::Srgb8() is the 8-bit form the table-driven fast decode serves,
+// every other accepted format takes the general per-pixel decode.
 struct ImageFormat {
     PixelType type = PixelType::Float;
     ColorFamily family = ColorFamily::Rgb;
@@ -52,7 +69,7 @@ struct ImageFormat {
     MatrixCoefficients matrix = MatrixCoefficients::Rgb;
     TransferCharacteristics transfer = TransferCharacteristics::Srgb;
     ColorPrimaries primaries = ColorPrimaries::Bt709;
-    std::uint8_t chroma_location = 0;
+    ChromaLocation chroma_location = ChromaLocation::Left;
     std::uint8_t subsample_w = 0;
     std::uint8_t subsample_h = 0;
     std::uint8_t depth = 0;
@@ -71,7 +88,7 @@ struct ImageFormat {
         f.matrix = static_cast<std::uint8_t>(matrix);
         f.transfer = static_cast<std::uint8_t>(transfer);
         f.primaries = static_cast<std::uint8_t>(primaries);
-        f.chroma_location = chroma_location;
+        f.chroma_location = static_cast<std::uint8_t>(chroma_location);
         f.subsample_w = subsample_w;
         f.subsample_h = subsample_h;
         f.depth = depth;
@@ -178,7 +195,7 @@ struct Ssimulacra2Result {
     }
 };
 
-// ssimulacra2.h:35-64 (chroma_filter and memory_policy have no meaning here).
+// ssimulacra2.h:35-64 (memory_policy has no meaning here: one HBM pool).
 struct Ssimulacra2Config {
     Size size;
     std::size_t gpu_resident_images = 2;
@@ -188,6 +205,7 @@ struct Ssimulacra2Config {
     PlaneFormat plane_format = PlaneFormat::F32;  // overridden to F16 by half_planes
     std::uint32_t strip_rows = 0;
     bool graph = false;  // replay one captured HIP graph per pair (no reference field)
+    ChromaFilter chroma_filter = ChromaFilter::Bilinear;
 
     [[nodiscard]] vqm_ss2_config ToC() const {
         vqm_ss2_config c{};
@@ -200,6 +218,7 @@ struct Ssimulacra2Config {
         c.strip_rows = strip_rows;
         c.input_bytes = input_bytes;
         c.graph = graph ? 1u : 0u;
+        c.chroma_filter = static_cast<std::uint32_t>(chroma_filter);
         return c;
     }
 };

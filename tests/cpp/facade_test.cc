@@ -54,6 +54,39 @@ static vqm::Image Gradient(int variant) {
     return img;
 }
 
+// 8-bit limited-range BT.709 4:2:0, centre-sited.
+static vqm::ImageFormat Yuv420() {
+    vqm::ImageFormat f;
+    f.type = vqm::PixelType::Byte;
+    f.family = vqm::ColorFamily::Yuv;
+    f.range = vqm::PixelRange::Limited;
+    f.matrix = vqm::MatrixCoefficients::Bt709;
+    f.transfer = vqm::TransferCharacteristics::Bt709;
+    f.chroma_location = vqm::ChromaLocation::Center;
+    f.subsample_w = 1;
+    f.subsample_h = 1;
+    return f;
+}
+
+// A 4:2:0 frame from an 8-bit RGB gradient: its red plane as luma, chroma
+// from its green and blue planes, decimated.
+static vqm::Image Yuv420From(const vqm::Image& rgb) {
+    const std::size_t side = rgb.size.width, half = side / 2;
+    vqm::Image img;
+    img.size = rgb.size;
+    img.format = Yuv420();
+    img.data.assign(vqm::RequiredBytes(img.format, img.size), 0);
+    for (std::size_t i = 0; i < side * side; ++i) img.data[i] = static_cast<std::uint8_t>(16 + rgb.data[i] * 219 / 255);
+    for (std::size_t y = 0; y < half; ++y)
+        for (std::size_t x = 0; x < half; ++x) {
+            const std::size_t src = 2 * y * side + 2 * x;
+            img.data[side * side + y * half + x] = static_cast<std::uint8_t>(16 + rgb.data[side * side + src] * 224 / 255);
+            img.data[side * side + half * half + y * half + x] =
+                static_cast<std::uint8_t>(16 + rgb.data[2 * side * side + src] * 224 / 255);
+        }
+    return img;
+}
+
 static void HostContract() {
     // pipeline_test.cc:318-328: the geometry is validated before device work.
     bool named = false;
@@ -81,6 +114,7 @@ static void HostContract() {
     r.scales.resize(7);
     CHECK_THROWS_AS(r.Score(), std::invalid_argument);
     CHECK(vqm::RequiredBytes(vqm::ImageFormat::Srgb8(), {13, 9}) == 360);
+    CHECK(vqm::RequiredBytes(Yuv420(), {8, 4}) == 48);  // input_format_test.cc:122-177
 }
 
 static void DeviceContract() {
@@ -113,6 +147,14 @@ static void DeviceContract() {
     CHECK(scores.size() == 3 && scores[0] == b && scores[1] == c && scores[2] == 100.0);
     const auto usage = pipe.GetMemoryUsage();
     CHECK(usage.resident_bytes == 2 * 3 * 1024);
+    // A general format with a non-default chroma filter (input_format_chroma_test.cc).
+    vqm::Ssimulacra2Config lanczos{{32, 32}};
+    lanczos.chroma_filter = vqm::ChromaFilter::Lanczos;
+    vqm::Ssimulacra2Pipeline yuv_pipe(lanczos);
+    const vqm::Image ya = Yuv420From(orig), yb = Yuv420From(dist_b);
+    CHECK(yuv_pipe.Compute(ya, ya).Score() == 100.0);
+    const double yuv_score = yuv_pipe.Compute(ya, yb).Score();
+    CHECK(yuv_score > 50.0 && yuv_score < 100.0);
 }
 
 int main(int argc, char** argv) {
