@@ -233,6 +233,29 @@ struct PipelineMemoryUsage {
     std::uint64_t total_device_bytes = 0;
 };
 
+// GetMemoryRequirements (vqm_memory.h:63-69): the device and pinned host
+// bytes a pipeline with `config` allocates, planned before construction
+// without touching the device. Throws std::invalid_argument for a geometry
+// or configuration the pipeline would refuse.
+[[nodiscard]] inline PipelineMemoryUsage GetMemoryRequirements(const Ssimulacra2Config& config) {
+    const vqm_ss2_config c = config.ToC();
+    vqm_ss2_memory m{};
+    detail::Check(vqm_ss2_memory_plan(&c, &m));
+    return PipelineMemoryUsage{m.workspace_bytes, m.resident_bytes, m.streaming_bytes,
+                               m.input_pool_bytes, m.pinned_host_bytes, m.total_device_bytes};
+}
+
+// MaxWorkersForBudget (vqm_memory.h:110-124) for HBM, the one device heap
+// here: how many independent pipelines of `per_worker` fit `device_budget`
+// bytes. Arithmetic, not an allocation guarantee. Throws
+// std::invalid_argument for an empty requirement.
+[[nodiscard]] inline std::size_t MaxWorkersForBudget(const PipelineMemoryUsage& per_worker,
+                                                     std::uint64_t device_budget) {
+    if (per_worker.total_device_bytes == 0)
+        throw std::invalid_argument("MaxWorkersForBudget: the requirement holds no device bytes");
+    return static_cast<std::size_t>(device_budget / per_worker.total_device_bytes);
+}
+
 // ssimulacra2.h:66-215. One geometry, one submission in flight, not
 // thread-safe; use one pipeline per thread.
 class Ssimulacra2Pipeline {

@@ -115,6 +115,12 @@ static void HostContract() {
     CHECK_THROWS_AS(r.Score(), std::invalid_argument);
     CHECK(vqm::RequiredBytes(vqm::ImageFormat::Srgb8(), {13, 9}) == 360);
     CHECK(vqm::RequiredBytes(Yuv420(), {8, 4}) == 48);  // input_format_test.cc:122-177
+    // vqm_memory.h:63-124.
+    const vqm::PipelineMemoryUsage plan = vqm::GetMemoryRequirements(vqm::Ssimulacra2Config{{1920, 1080}});
+    CHECK(plan.resident_bytes == 2ull * 3 * 1920 * 1080);
+    CHECK(vqm::MaxWorkersForBudget(plan, 3 * plan.total_device_bytes) == 3);
+    CHECK_THROWS_AS(vqm::MaxWorkersForBudget(vqm::PipelineMemoryUsage{}, 1), std::invalid_argument);
+    CHECK_THROWS_AS(vqm::GetMemoryRequirements(vqm::Ssimulacra2Config{{4, 8}}), std::invalid_argument);
 }
 
 static void DeviceContract() {

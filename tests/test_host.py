@@ -146,6 +146,14 @@ def test_memory_plan_and_algorithmic_bytes():
     assert m.total_device_bytes == m.workspace_bytes + m.resident_bytes + m.streaming_bytes
     half = vqm_amd.memory_plan(Ssimulacra2Config(size=Size(1920, 1080), half_planes=True))
     assert half.workspace_bytes < 0.6 * m.workspace_bytes
+    # vqm_memory.h:63-124 under the reference's names.
+    assert vqm_amd.GetMemoryRequirements(Ssimulacra2Config(size=Size(1920, 1080))) == m
+    assert vqm_amd.MaxWorkersForBudget(m, 10 * m.total_device_bytes + 1) == 10
+    assert vqm_amd.MaxWorkersForBudget(m, m.total_device_bytes - 1) == 0
+    with pytest.raises(vqm_amd.InvalidArgument):
+        vqm_amd.MaxWorkersForBudget(vqm_amd.PipelineMemoryUsage(), 1 << 30)
+    with pytest.raises(vqm_amd.InvalidArgument):
+        vqm_amd.GetMemoryRequirements(Ssimulacra2Config(size=Size(4, 8)))
 
 
 def test_last_error_is_thread_local_and_cleared():

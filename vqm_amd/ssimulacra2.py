@@ -372,12 +372,12 @@ class Ssimulacra2Config:
 
 @dataclass
 class PipelineMemoryUsage:
-    workspace_bytes: int
-    resident_bytes: int
-    streaming_bytes: int
-    input_pool_bytes: int
-    pinned_host_bytes: int
-    total_device_bytes: int
+    workspace_bytes: int = 0
+    resident_bytes: int = 0
+    streaming_bytes: int = 0
+    input_pool_bytes: int = 0
+    pinned_host_bytes: int = 0
+    total_device_bytes: int = 0
 
     @staticmethod
     def from_c(m: L.MemoryC) -> "PipelineMemoryUsage":
@@ -390,6 +390,19 @@ def memory_plan(config: Ssimulacra2Config) -> PipelineMemoryUsage:
     m = L.MemoryC()
     _check(L.lib().vqm_ss2_memory_plan(ctypes.byref(config.to_c()), ctypes.byref(m)))
     return PipelineMemoryUsage.from_c(m)
+
+
+def GetMemoryRequirements(config: Ssimulacra2Config) -> PipelineMemoryUsage:
+    """vqm_memory.h:63-69: the plan of memory_plan under the reference's name."""
+    return memory_plan(config)
+
+
+def MaxWorkersForBudget(per_worker: PipelineMemoryUsage, device_budget: int) -> int:
+    """vqm_memory.h:110-124 for the one device heap (HBM): pipelines of
+    `per_worker` that fit `device_budget` bytes; arithmetic only."""
+    if per_worker.total_device_bytes <= 0:
+        raise InvalidArgument("MaxWorkersForBudget: the requirement holds no device bytes")
+    return int(device_budget) // per_worker.total_device_bytes
 
 
 def algorithmic_bytes(size: Size, plane_format: int = L.PLANES_F32) -> int:
