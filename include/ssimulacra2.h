@@ -213,7 +213,7 @@ struct Ssimulacra2Config {
         if (gpu_resident_images > 0xFFFFFFFFull)
             throw std::invalid_argument("gpu_resident_images is limited to 4096 slots");
         c.resident_images = static_cast<std::uint32_t>(gpu_resident_images);
-        c.plane_format = half_planes ? VQM_PLANES_F16 : static_cast<std::uint32_t>(plane_format);
+        c.plane_format = half_planes ? static_cast<std::uint32_t>(VQM_PLANES_F16) : static_cast<std::uint32_t>(plane_format);
         c.streaming_inputs = streaming_inputs ? 1u : 0u;
         c.strip_rows = strip_rows;
         c.input_bytes = input_bytes;
