@@ -270,3 +270,24 @@ def test_y4m_alpha_and_geometry_errors(load):
     assert "divisible" in load(b"YUV4MPEG2 W5 H2 F24:1 Ip C420\nFRAME\n" + bytes(64))["error"]
     assert "frame size" in load(b"YUV4MPEG2 H2 F24:1 Ip C444\nFRAME\n")["error"]
     assert "invalid y4m width" in load(b"YUV4MPEG2 Wx H2 C444\nFRAME\n")["error"]
+
+
+# ---- the command's argument handling (no device work) ------------------------------------
+
+def test_command_line_errors():
+    """Parse errors exit 2 with the usage; unreadable inputs exit 1 before
+    any device work (cli/main.cc:108-160)."""
+    cli = os.path.join(ROOT, "vqm_amd", "vqm-ss2")
+    if not os.path.exists(cli):
+        pytest.skip("vqm-ss2 is not built")
+    r = subprocess.run([cli, "--metric", "vmaf-v0", "a", "b"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "ssimulacra2 only" in r.stderr
+    r = subprocess.run([cli, "--input-matrix", "bogus", "a", "b"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "rgb|bt709" in r.stderr
+    r = subprocess.run([cli, "only-one"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "two inputs" in r.stderr
+    r = subprocess.run([cli, "--help"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "--input-transfer" in r.stdout
+    r = subprocess.run([cli, "/nonexistent/a.png", "/nonexistent/b.png"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "could not open" in r.stderr
