@@ -20,7 +20,8 @@ LIB_PATH = os.path.join(HERE, "libss2_oracle.so")
 class Mode(ctypes.Structure):
     _fields_ = [("arith", ctypes.c_int32), ("blur_order", ctypes.c_int32),
                 ("cbrt_kind", ctypes.c_int32), ("plane_format", ctypes.c_int32),
-                ("plane_round_levels", ctypes.c_uint32), ("sum_quads", ctypes.c_int32)]
+                ("plane_round_levels", ctypes.c_uint32), ("sum_quads", ctypes.c_int32),
+                ("round_linear", ctypes.c_int32)]
 
 
 # The three forms (ss2_oracle.h).
@@ -38,10 +39,13 @@ def f32_dev_planes(plane_format: int, levels_mask: int = 0x3F) -> Mode:
 def f32_dev_for(plane_format: int) -> Mode:
     """The device form of a library plane format: FP32 (0), FP16 (1) and BF16
     (2) planes at every level; the mixed form (3) rounds levels 0 and 1 to
-    unsigned 16-bit fixed point and keeps FP32 above."""
+    unsigned 16-bit fixed point and keeps FP32 above. FP16 is the reference's
+    half-plane build: its linear box averages are rounded to FP16 as well."""
     if plane_format == 0:
         return F32_DEV
-    return f32_dev_planes(plane_format, 0x03 if plane_format == 3 else 0x3F)
+    mode = f32_dev_planes(plane_format, 0x03 if plane_format == 3 else 0x3F)
+    mode.round_linear = 1 if plane_format == 1 else 0
+    return mode
 
 
 def build() -> None:
@@ -73,6 +77,8 @@ def lib() -> ctypes.CDLL:
             "ss2o_blur_f32": (None, [f32p, f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32]),
             "ss2o_blur_f64": (None, [f64p, f64p, ctypes.c_uint32, ctypes.c_uint32]),
             "ss2o_pyramid_f32": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_uint32, u8p, f32p]),
+            "ss2o_xyb_planes": (ctypes.c_int32, [ctypes.POINTER(Mode), ctypes.c_uint32, ctypes.c_uint32, u8p,
+                                                 f32p]),
             "ss2o_pyramid_f64": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_uint32, u8p, f64p]),
             "ss2o_image_bytes": (ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32]),
             "ss2o_compute": (ctypes.c_int32, [P(Mode), ctypes.c_uint32, ctypes.c_uint32, u8p, u8p,
@@ -196,6 +202,23 @@ def blur_f64(plane: np.ndarray) -> np.ndarray:
     lib().ss2o_blur_f64(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), w, h)
     return out
+
+
+def xyb_planes(mode: Mode, width: int, height: int, image: np.ndarray):
+    """Stored XYB planes under an FP32 mode: list over levels of (3, h_l, w_l)
+    float32 arrays (zeros where a level is pruned)."""
+    sizes = plan_scales(width, height)
+    total = sum(3 * w * h for w, h in sizes)
+    out = np.zeros(total, np.float32)
+    a = np.ascontiguousarray(image, np.uint8)
+    n = lib().ss2o_xyb_planes(ctypes.byref(mode), width, height, _u8(a),
+                              out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    assert n == len(sizes)
+    res, at = [], 0
+    for w, h in sizes:
+        res.append(out[at:at + 3 * w * h].reshape(3, h, w))
+        at += 3 * w * h
+    return res
 
 
 def pyramid(width: int, height: int, image: np.ndarray, f64: bool = True):

@@ -370,6 +370,62 @@ int32_t pyramid(uint32_t width, uint32_t height, const uint8_t* image, T* out) {
     return levels;
 }
 
+// One image's stored XYB planes [level][channel] under `mode`: the linear
+// pyramid (box averages rounded to plane_format first when round_linear),
+// XYB of every level with a stored channel, stored planes rounded to
+// plane_format at the levels in plane_round_levels. Pruned levels stay empty.
+template <typename T>
+void xyb_pyramid(const ss2o_mode& mode, uint32_t width, uint32_t height, int32_t levels,
+                 const uint32_t sizes[12], const uint8_t* image, const T lut[256],
+                 std::vector<T> xyb[kMaxScales][3]) {
+    const uint64_t plane_stride = align4(static_cast<uint64_t>(width) * height);
+    std::vector<T> lin[3];
+    const size_t n0 = static_cast<size_t>(width) * height;
+    for (int c = 0; c < 3; ++c) {
+        lin[c].resize(n0);
+        const uint8_t* p = image + c * plane_stride;
+        for (size_t i = 0; i < n0; ++i) lin[c][i] = lut[p[i]];
+    }
+    uint32_t w = width, h = height;
+    for (int l = 0; l < levels; ++l) {
+        const uint32_t lw = sizes[2 * l], lh = sizes[2 * l + 1];
+        if (l > 0) {
+            downsample(lin, w, h, lw, lh);
+            w = lw;
+            h = lh;
+            if constexpr (sizeof(T) == 4) {
+                if (mode.round_linear && mode.plane_format != 0)
+                    for (int c = 0; c < 3; ++c)
+                        for (T& v : lin[c]) v = ss2o_round_plane(mode.plane_format, v);
+            }
+        }
+        bool any = false;
+        for (int c = 0; c < 3; ++c) any |= !ss2o_channel_pruned(c, l, levels);
+        if (!any) continue;
+        const size_t n = static_cast<size_t>(w) * h;
+        for (int c = 0; c < 3; ++c) xyb[l][c].assign(n, T(0));
+        const bool round_plane =
+            sizeof(T) == 4 && mode.plane_format != 0 &&
+            ((mode.plane_round_levels >> l) & 1u) != 0;
+        for (size_t i = 0; i < n; ++i) {
+            T v[3];
+            if constexpr (sizeof(T) == 4) {
+                xyb_f32(lin[0][i], lin[1][i], lin[2][i], mode.cbrt_kind,
+                        reinterpret_cast<float*>(v));
+                if (round_plane) {
+                    for (int c = 0; c < 3; ++c)
+                        v[c] = mode.plane_format == 1   ? round_half(v[c])
+                               : mode.plane_format == 2 ? round_bf16(v[c])
+                                                        : round_unorm16(v[c]);
+                }
+            } else {
+                xyb_f64(lin[0][i], lin[1][i], lin[2][i], reinterpret_cast<double*>(v));
+            }
+            for (int c = 0; c < 3; ++c) xyb[l][c][i] = v[c];
+        }
+    }
+}
+
 template <typename T>
 int32_t compute(const ss2o_mode& mode, uint32_t width, uint32_t height,
                 const uint8_t* img[2], double norms[108], double sums_out[108]) {
@@ -399,51 +455,9 @@ int32_t compute(const ss2o_mode& mode, uint32_t width, uint32_t height,
         taps_as(wt);
     }
 
-    const uint64_t plane_stride = align4(static_cast<uint64_t>(width) * height);
     // XYB planes [image][level][channel].
     std::vector<T> xyb[2][kMaxScales][3];
-    for (int im = 0; im < 2; ++im) {
-        std::vector<T> lin[3];
-        const size_t n0 = static_cast<size_t>(width) * height;
-        for (int c = 0; c < 3; ++c) {
-            lin[c].resize(n0);
-            const uint8_t* p = img[im] + c * plane_stride;
-            for (size_t i = 0; i < n0; ++i) lin[c][i] = lut[p[i]];
-        }
-        uint32_t w = width, h = height;
-        for (int l = 0; l < levels; ++l) {
-            const uint32_t lw = sizes[2 * l], lh = sizes[2 * l + 1];
-            if (l > 0) {
-                downsample(lin, w, h, lw, lh);
-                w = lw;
-                h = lh;
-            }
-            bool any = false;
-            for (int c = 0; c < 3; ++c) any |= !ss2o_channel_pruned(c, l, levels);
-            if (!any) continue;
-            const size_t n = static_cast<size_t>(w) * h;
-            for (int c = 0; c < 3; ++c) xyb[im][l][c].assign(n, T(0));
-            const bool round_plane =
-                sizeof(T) == 4 && mode.plane_format != 0 &&
-                ((mode.plane_round_levels >> l) & 1u) != 0;
-            for (size_t i = 0; i < n; ++i) {
-                T v[3];
-                if constexpr (sizeof(T) == 4) {
-                    xyb_f32(lin[0][i], lin[1][i], lin[2][i], mode.cbrt_kind,
-                            reinterpret_cast<float*>(v));
-                    if (round_plane) {
-                        for (int c = 0; c < 3; ++c)
-                            v[c] = mode.plane_format == 1   ? round_half(v[c])
-                                   : mode.plane_format == 2 ? round_bf16(v[c])
-                                                            : round_unorm16(v[c]);
-                    }
-                } else {
-                    xyb_f64(lin[0][i], lin[1][i], lin[2][i], reinterpret_cast<double*>(v));
-                }
-                for (int c = 0; c < 3; ++c) xyb[im][l][c][i] = v[c];
-            }
-        }
-    }
+    for (int im = 0; im < 2; ++im) xyb_pyramid(mode, width, height, levels, sizes, img[im], lut, xyb[im]);
 
     for (int l = 0; l < levels; ++l) {
         const uint32_t w = sizes[2 * l], h = sizes[2 * l + 1];
@@ -592,6 +606,29 @@ int32_t ss2o_pyramid_f32(uint32_t width, uint32_t height, const uint8_t* image, 
 
 int32_t ss2o_pyramid_f64(uint32_t width, uint32_t height, const uint8_t* image, double* out) {
     return pyramid(width, height, image, out);
+}
+
+int32_t ss2o_xyb_planes(const ss2o_mode* mode, uint32_t width, uint32_t height, const uint8_t* image,
+                        float* out) {
+    if (mode == nullptr || mode->arith != 1) return -1;
+    uint32_t sizes[12];
+    const int32_t levels = ss2o_plan_scales(width, height, sizes);
+    if (levels < 0) return -1;
+    float lut[256];
+    ss2o_srgb_lut_f32(lut);
+    std::vector<float> xyb[kMaxScales][3];
+    xyb_pyramid(*mode, width, height, levels, sizes, image, lut, xyb);
+    for (int l = 0; l < levels; ++l) {
+        const size_t n = static_cast<size_t>(sizes[2 * l]) * sizes[2 * l + 1];
+        for (int c = 0; c < 3; ++c) {
+            if (xyb[l][c].empty()) {
+                out = std::fill_n(out, n, 0.0f);
+            } else {
+                out = std::copy(xyb[l][c].begin(), xyb[l][c].end(), out);
+            }
+        }
+    }
+    return levels;
 }
 
 uint64_t ss2o_image_bytes(uint32_t width, uint32_t height) {

@@ -47,6 +47,9 @@ typedef struct ss2o_mode {
     int32_t plane_format; /* FP32 arith only: 0 = FP32, 1 = FP16, 2 = BF16, 3 = unorm16 stored planes */
     uint32_t plane_round_levels; /* levels whose stored planes use plane_format */
     int32_t sum_quads;    /* 1 = FP32 sums of 4 adjacent pixels, then FP64; 0 = FP64 per pixel */
+    int32_t round_linear; /* 1 = the linear box averages of levels >= 1 are rounded to
+                             plane_format as well, as the reference's half-plane build
+                             does (prepare.comp:100-111, ssimu2_common.glsl:84-96) */
 } ss2o_mode;
 
 /* One stored plane value as plane_format keeps it: 1 FP16 and 2 BF16 (round
@@ -89,6 +92,12 @@ void ss2o_blur_f64(const double* in, double* out, uint32_t w, uint32_t h);
 /* The linear RGB pyramid of one packed u8 image: level by level, channel
  * by channel, w_l*h_l values each, FP32 (device and FP32 forms) or FP64.
  * prepare.comp:100-111,149-168. Returns the level count or -1. */
+/* The stored XYB planes of one packed u8 image under an FP32 `mode` (arith 1):
+ * level by level, channel X, Y, B, each w_l * h_l floats; levels where every
+ * channel is pruned are written as zeros. Returns the level count, -1 below
+ * 8x8 or for an FP64 mode. */
+int32_t ss2o_xyb_planes(const ss2o_mode* mode, uint32_t width, uint32_t height, const uint8_t* image,
+                        float* out);
 int32_t ss2o_pyramid_f32(uint32_t width, uint32_t height, const uint8_t* image, float* out);
 int32_t ss2o_pyramid_f64(uint32_t width, uint32_t height, const uint8_t* image, double* out);
 

@@ -306,3 +306,61 @@ def test_plane_roundings(oracle_lib):
         assert oracle_lib.round_plane(3, float(v)) == c, v
     assert oracle_lib.round_plane(3, float("nan")) == 0.0
     assert oracle_lib.round_plane(0, 0.123) == np.float32(0.123)
+
+
+def test_xyb_planes_and_the_half_plane_build(oracle_lib):
+    """The oracle's stored planes against a numpy restatement of the pyramid:
+    clamped 2x2 boxes summed ((((a + b) + c) + d) * 0.25 in float32, numpy's
+    IEEE binary16 rounding of the linear averages for the reference's
+    half-plane build (prepare.comp:100-111, ssimu2_common.glsl:84-96) and of
+    the stored XYB values, XYB per sample through the oracle's own xyb_f32
+    (pinned by test_xyb_closed_form)."""
+    w, h = 37, 22
+    rng = np.random.default_rng(21)
+    stride = (w * h + 3) // 4 * 4
+    img = np.zeros(3 * stride, np.uint8)
+    for c in range(3):
+        img[c * stride:c * stride + w * h] = rng.integers(0, 256, w * h)
+    lut = oracle_lib.srgb_lut_f32()
+    sizes = oracle_lib.plan_scales(w, h)
+
+    def restated(round_linear: bool, round_planes: bool):
+        lin = np.stack([lut[img[c * stride:c * stride + w * h]].reshape(h, w) for c in range(3)]).astype(np.float32)
+        out = []
+        for l, (lw, lh) in enumerate(sizes):
+            if l > 0:
+                ph, pw = lin.shape[1:]
+                ys = np.minimum(np.arange(lh)[:, None] * 2 + np.array([0, 1])[None, :], ph - 1)
+                xs = np.minimum(np.arange(lw)[:, None] * 2 + np.array([0, 1])[None, :], pw - 1)
+                a = lin[:, ys[:, 0][:, None], xs[:, 0][None, :]]
+                b = lin[:, ys[:, 0][:, None], xs[:, 1][None, :]]
+                c = lin[:, ys[:, 1][:, None], xs[:, 0][None, :]]
+                d = lin[:, ys[:, 1][:, None], xs[:, 1][None, :]]
+                lin = ((((a + b).astype(np.float32) + c).astype(np.float32) + d).astype(np.float32)
+                       * np.float32(0.25)).astype(np.float32)
+                if round_linear:
+                    lin = lin.astype(np.float16).astype(np.float32)
+            stored = any(not oracle_lib.channel_pruned(c, l, len(sizes)) for c in range(3))
+            x = np.zeros((3, lh, lw), np.float32)
+            if stored:
+                for yy in range(lh):
+                    for xx in range(lw):
+                        x[:, yy, xx] = oracle_lib.xyb_f32(*lin[:, yy, xx])
+                if round_planes:
+                    x = x.astype(np.float16).astype(np.float32)
+            out.append(x)
+        return out
+
+    for mode, rl, rp in [(oracle_lib.F32_DEV, False, False),
+                         (oracle_lib.f32_dev_planes(1), False, True),
+                         (oracle_lib.f32_dev_for(1), True, True)]:
+        got = oracle_lib.xyb_planes(mode, w, h, img)
+        for g, e in zip(got, restated(rl, rp)):
+            np.testing.assert_array_equal(g, e)
+    assert oracle_lib.f32_dev_for(1).round_linear == 1
+    assert oracle_lib.f32_dev_for(3).round_linear == 0 and oracle_lib.f32_dev_for(2).round_linear == 0
+    # Rounding the linear samples moves levels >= 1 only.
+    a = oracle_lib.xyb_planes(oracle_lib.f32_dev_planes(1), w, h, img)
+    b = oracle_lib.xyb_planes(oracle_lib.f32_dev_for(1), w, h, img)
+    np.testing.assert_array_equal(a[0], b[0])
+    assert any(not np.array_equal(x, y) for x, y in zip(a[1:], b[1:]))

@@ -203,18 +203,33 @@ using BF16 = Narrow<to_bf16, from_bf16>;
 using U16 = Narrow<to_unorm16, from_unorm16>;
 
 // Element formats per pyramid level: Fine below kFineLevels, Coarse above.
-template <typename FineT, typename CoarseT>
+// RoundLinear: the linear box averages of levels >= 1 are rounded to the
+// element format too, before their XYB and before the next level averages
+// them -- the reference's half-plane build (prepare.comp:100-111,
+// ssimu2_common.glsl:84-96), which FP16 planes reproduce.
+template <typename FineT, typename CoarseT, bool RoundLinear = false>
 struct Layout {
     using Fine = FineT;
     using Coarse = CoarseT;
     template <uint32_t L>
     using At = std::conditional_t<(L < kFineLevels), Fine, Coarse>;
     static constexpr bool kUniform = std::is_same_v<Fine, Coarse>;
+    static constexpr bool kRoundLinear = RoundLinear;
 };
 using LayoutF32 = Layout<F32, F32>;
-using LayoutF16 = Layout<F16, F16>;
+using LayoutF16 = Layout<F16, F16, true>;
 using LayoutBF16 = Layout<BF16, BF16>;
 using LayoutU16 = Layout<U16, F32>;
+
+// A level's linear box average as the layout hands it on.
+template <typename Lay>
+__device__ __forceinline__ float linear_sample(float v) {
+    if constexpr (Lay::kRoundLinear) {
+        return from_f16(to_f16(v));
+    } else {
+        return v;
+    }
+}
 
 // Element pointer of a plane: planes of one image start at `base` and
 // `offset` is in bytes.
@@ -465,7 +480,7 @@ __device__ __forceinline__ void prepare_level(ConstAS<Plan>* plan, GlobalAS<uint
             s = s + prev[c][cy0 * prev_x + cx1];
             s = s + prev[c][cy1 * prev_x + cx0];
             s = s + prev[c][cy1 * prev_x + cx1];
-            v[c] = s * 0.25f;
+            v[c] = linear_sample<Lay>(s * 0.25f);
         }
         const uint32_t mask = plan->mask[L];
         if (mask != 0u) {
@@ -607,7 +622,7 @@ __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict
                 s = s + lin[0][2 * jj + 1][c];
                 s = s + lin[1][2 * jj][c];
                 s = s + lin[1][2 * jj + 1][c];
-                v1[jj][c] = s * 0.25f;
+                v1[jj][c] = linear_sample<Lay>(s * 0.25f);
             }
             *reinterpret_cast<float2*>(&ladder[1][c][ty * 32u + 2u * tx]) = make_float2(v1[0][c], v1[1][c]);
         }
