@@ -54,6 +54,9 @@ enum class ColorPrimaries : std::uint8_t {
 };
 // Upsampling kernel of subsampled chroma planes (vqm_image.h:94-99).
 enum class ChromaFilter : std::uint8_t { Bilinear = 0, Bicubic = 1, Lanczos = 2 };
+// The blur of the maps: the reference's sampled Gaussian, or the kernel of
+// Cloudinary ssimulacra2's recursive Gaussian (VQM_BLUR_*; no reference field).
+enum class Blur : std::uint32_t { Sampled = VQM_BLUR_SAMPLED, Recursive = VQM_BLUR_RECURSIVE };
 enum class PlaneFormat : std::uint32_t {
     F32 = VQM_PLANES_F32, F16 = VQM_PLANES_F16, BF16 = VQM_PLANES_BF16, U16 = VQM_PLANES_U16
 };
@@ -206,6 +209,7 @@ struct Ssimulacra2Config {
     std::uint32_t strip_rows = 0;
     bool graph = false;  // replay one captured HIP graph per pair (no reference field)
     ChromaFilter chroma_filter = ChromaFilter::Bilinear;
+    Blur blur = Blur::Sampled;  // no reference field
 
     [[nodiscard]] vqm_ss2_config ToC() const {
         vqm_ss2_config c{};
@@ -219,6 +223,7 @@ struct Ssimulacra2Config {
         c.input_bytes = input_bytes;
         c.graph = graph ? 1u : 0u;
         c.chroma_filter = static_cast<std::uint32_t>(chroma_filter);
+        c.blur = static_cast<std::uint32_t>(blur);
         return c;
     }
 };

@@ -64,6 +64,16 @@ enum {
  *         FP32 plane bytes. An addition with no reference counterpart. */
 enum { VQM_PLANES_F32 = 0, VQM_PLANES_F16 = 1, VQM_PLANES_BF16 = 2, VQM_PLANES_U16 = 3 };
 
+/* The blur of the maps (vqm_ss2_config.blur, vqm_ss2_batch_config.blur):
+ *   SAMPLED    the reference's 13-tap sampled Gaussian, sigma 1.5
+ *              (pipeline.cc:274-291); the default.
+ *   RECURSIVE  the filter of Cloudinary's ssimulacra2 2.1, libjxl's recursive
+ *              Gaussian (ssimulacra2.cc:82-106; Charalampidis 2016), applied
+ *              as the zero-padded 9-tap FIR its recursions compute. The
+ *              reference dropped it (docs/divergence.md:22); no reference
+ *              counterpart. */
+enum { VQM_BLUR_SAMPLED = 0, VQM_BLUR_RECURSIVE = 1 };
+
 /* ImageFormat, include/vqm_image.h:100-120; numeric enum values are zimg's
  * as in the reference (vqm_image.h:25-96). Validation follows the
  * reference's ValidateImage (input.cc:152-222): an unknown or unsupported
@@ -122,6 +132,10 @@ typedef struct vqm_ss2_config {
     uint32_t chroma_filter;     /* subsampled chroma upsampling: 0 bilinear
                                    (default), 1 bicubic, 2 lanczos
                                    (Ssimulacra2Config::chroma_filter) */
+    uint32_t blur;              /* VQM_BLUR_SAMPLED (default): the reference's
+                                   13-tap sampled Gaussian; VQM_BLUR_RECURSIVE:
+                                   the kernel of Cloudinary ssimulacra2's
+                                   recursive Gaussian. No reference field. */
 } vqm_ss2_config;
 
 /* PipelineMemoryUsage, include/vqm_memory_types.h:29-38: device bytes the
@@ -168,6 +182,10 @@ int vqm_ss2_plan_scales(uint32_t width, uint32_t height, uint32_t sizes[12],
 
 /* MakeGaussian(1.5), pipeline.cc:274-291. */
 void vqm_ss2_gaussian_taps(float taps[13]);
+
+/* The 13 FP32 taps the scale kernel applies for `blur` (VQM_BLUR_*), centre
+ * at index 6; the recursive kernel's outer four are zero. */
+int vqm_ss2_blur_taps(uint32_t blur, float taps[13]);
 
 /* Ssimulacra2ChannelPruned, score.cc:103-114: 1 when pruned. */
 int vqm_ss2_channel_pruned(uint32_t channel, uint32_t scale, uint32_t scales);
@@ -283,7 +301,7 @@ typedef struct vqm_ss2_batch_config {
     uint64_t hbm_budget;     /* device bytes the workspace pool may hold, 0 = unlimited */
     uint32_t autotune;       /* 1: strip_rows and chunks come from vqm_ss2_autotune
                                 for each geometry the first time it is seen */
-    uint32_t reserved;
+    uint32_t blur;           /* VQM_BLUR_SAMPLED or VQM_BLUR_RECURSIVE */
 } vqm_ss2_batch_config;
 
 /* One pair of caller-owned device images (packed u8 sRGB RGB). */

@@ -21,7 +21,7 @@ class Mode(ctypes.Structure):
     _fields_ = [("arith", ctypes.c_int32), ("blur_order", ctypes.c_int32),
                 ("cbrt_kind", ctypes.c_int32), ("plane_format", ctypes.c_int32),
                 ("plane_round_levels", ctypes.c_uint32), ("sum_quads", ctypes.c_int32),
-                ("round_linear", ctypes.c_int32)]
+                ("round_linear", ctypes.c_int32), ("blur_kind", ctypes.c_int32)]
 
 
 # The three forms (ss2_oracle.h).
@@ -77,6 +77,7 @@ def lib() -> ctypes.CDLL:
             "ss2o_blur_f32": (None, [f32p, f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32]),
             "ss2o_blur_f64": (None, [f64p, f64p, ctypes.c_uint32, ctypes.c_uint32]),
             "ss2o_pyramid_f32": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_uint32, u8p, f32p]),
+            "ss2o_cosine_taps": (None, [f64p]),
             "ss2o_xyb_planes": (ctypes.c_int32, [ctypes.POINTER(Mode), ctypes.c_uint32, ctypes.c_uint32, u8p,
                                                  f32p]),
             "ss2o_pyramid_f64": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_uint32, u8p, f64p]),
@@ -146,6 +147,20 @@ def plan_scales(width: int, height: int):
     if n < 0:
         raise ValueError("minimum image size is 8x8 pixels")
     return [(sizes[2 * i], sizes[2 * i + 1]) for i in range(n)]
+
+
+def cosine_taps() -> np.ndarray:
+    """The recursive Gaussian's truncated-cosine kernel (FP64, 13-tap frame)."""
+    out = np.zeros(13)
+    lib().ss2o_cosine_taps(out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out
+
+
+def with_blur(mode: Mode, blur_kind: int) -> Mode:
+    """A copy of `mode` with another blur kernel (0 sampled, 1 recursive)."""
+    m = Mode.from_buffer_copy(mode)
+    m.blur_kind = blur_kind
+    return m
 
 
 def gaussian_taps() -> np.ndarray:

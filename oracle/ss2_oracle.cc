@@ -206,6 +206,54 @@ template <typename T> void taps_as(T taps[kTaps]) {
     for (int i = 0; i < kTaps; ++i) taps[i] = static_cast<T>(f[i]);
 }
 
+// The kernel of libjxl's recursive Gaussian (CreateRecursiveGaussian(1.5),
+// Cloudinary ssimulacra2.cc:82-106; third-party, libjxl 0.8/0.9 as vendored
+// by ssimulacra2 2.1), restated from its published algorithm: Charalampidis,
+// "Recursive implementation of the Gaussian filter using truncated cosine
+// functions", IEEE Trans. Signal Process. 64(14), 2016. The filter is
+//     K[m] = sum_{k=1,3,5} beta_k cos(w_k m),  |m| <= N,  w_k = k pi / (2N),
+// N = round(3.2795 sigma + 0.2546) (eq. 57) = 5 at sigma 1.5. Its three
+// second-order recursions with zero input outside the row compute exactly
+// this zero-padded FIR: with cos(w_k N) = 0 the window edges enter the
+// recursion only through x[n+N-1] + x[n-N-1], and K[+-N] = 0, so 9 taps
+// remain. beta is the cosine-series coefficient of the Gaussian,
+// rho_k = exp(-sigma^2 w_k^2 / 2) / N (eq. 50), corrected by the smallest
+// change that restores unit gain (sum K = 1, eq. 39) and variance
+// (sum m^2 K = sigma^2, eq. 55): beta = rho + a p + b r with
+// p_k = sum_m cos(w_k m), r_k = sum_m (N^2 - m^2) cos(w_k m) (eq. 37, 44);
+// the paper's eq. (52)-(56) solve the same system. Returned in the 13-tap
+// frame of the sampled kernel, centre at index 6, zeros at |m| = 5, 6.
+void cosine_taps_f64(double taps[kTaps]) {
+    constexpr double kPi = 3.14159265358979323846;
+    const double sigma = 1.5;
+    const int n = static_cast<int>(std::lround(3.2795 * sigma + 0.2546));
+    double w[3], p[3] = {0, 0, 0}, r[3] = {0, 0, 0}, rho[3];
+    for (int k = 0; k < 3; ++k) {
+        w[k] = (2 * k + 1) * kPi / (2.0 * n);
+        for (int m = -n; m <= n; ++m) {
+            const double c = std::cos(w[k] * m);
+            p[k] += c;
+            r[k] += (static_cast<double>(n) * n - static_cast<double>(m) * m) * c;
+        }
+        rho[k] = std::exp(-0.5 * sigma * sigma * w[k] * w[k]) / n;
+    }
+    // Solve [p.p p.r; r.p r.r] [a b]^T = [1 - p.rho, N^2 - sigma^2 - r.rho].
+    auto dot = [](const double* x, const double* y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+    const double m00 = dot(p, p), m01 = dot(p, r), m11 = dot(r, r);
+    const double b0 = 1.0 - dot(p, rho), b1 = static_cast<double>(n) * n - sigma * sigma - dot(r, rho);
+    const double det = m00 * m11 - m01 * m01;
+    const double a = (b0 * m11 - m01 * b1) / det, b = (m00 * b1 - m01 * b0) / det;
+    double beta[3];
+    for (int k = 0; k < 3; ++k) beta[k] = rho[k] + a * p[k] + b * r[k];
+    for (int i = 0; i < kTaps; ++i) {
+        const int m = i - kRadius;
+        double v = 0.0;
+        if (m > -n && m < n)
+            for (int k = 0; k < 3; ++k) v += beta[k] * std::cos(w[k] * m);
+        taps[i] = v;
+    }
+}
+
 // One 13-tap pass along rows (dir 0) or columns (dir 1) with zeros outside.
 // The chain is acc = w0*x0, then acc = fma(wk, xk, acc) in ascending k: the
 // tap order of scale.comp:187-199 and :243-256 and of the HIP kernels.
@@ -442,7 +490,11 @@ int32_t compute(const ss2o_mode& mode, uint32_t width, uint32_t height,
         ss2o_srgb_lut_f64(reinterpret_cast<double*>(lut));
     }
     T wt[kTaps];
-    if constexpr (sizeof(T) == 8) {
+    if (mode.blur_kind == 1) {
+        double d[kTaps];
+        cosine_taps_f64(d);
+        for (int i = 0; i < kTaps; ++i) wt[i] = static_cast<T>(d[i]);
+    } else if constexpr (sizeof(T) == 8) {
         // FP64: the taps before their FP32 rounding (pipeline.cc:279-285).
         double s = 0.0, d[kTaps];
         for (int i = 0; i < kTaps; ++i) {
@@ -579,6 +631,8 @@ void ss2o_srgb_lut_f32(float lut[256]) {
 void ss2o_srgb_lut_f64(double lut[256]) {
     for (int i = 0; i < 256; ++i) lut[i] = srgb_f64(i);
 }
+
+void ss2o_cosine_taps(double taps[13]) { cosine_taps_f64(taps); }
 
 float ss2o_cbrt_dev(float x) { return cbrt_dev(x); }
 

@@ -50,12 +50,19 @@ typedef struct ss2o_mode {
     int32_t round_linear; /* 1 = the linear box averages of levels >= 1 are rounded to
                              plane_format as well, as the reference's half-plane build
                              does (prepare.comp:100-111, ssimu2_common.glsl:84-96) */
+    int32_t blur_kind;    /* 0 = the 13-tap sampled Gaussian (pipeline.cc:274-291);
+                             1 = the 9-tap truncated-cosine kernel that libjxl's recursive
+                             Gaussian computes (Cloudinary ssimulacra2.cc:82-106) */
 } ss2o_mode;
 
 /* One stored plane value as plane_format keeps it: 1 FP16 and 2 BF16 (round
  * to nearest even), 3 unsigned 16-bit fixed point q / 65536 (round to
  * nearest even, clamped to 0..65535, NaN to 0); 0 returns v. */
 float ss2o_round_plane(int32_t plane_format, float v);
+
+/* The truncated-cosine kernel of the recursive Gaussian at sigma 1.5, in FP64,
+ * in the 13-tap frame (centre at index 6, zeros at |m| >= 5). */
+void ss2o_cosine_taps(double taps[13]);
 
 /* Scale plan: returns the level count (1..6) and writes width,height pairs,
  * or -1 below 8x8. pipeline.cc:28-48. */

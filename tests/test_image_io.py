@@ -284,6 +284,8 @@ def test_command_line_errors():
     assert r.returncode == 2 and "ssimulacra2 only" in r.stderr
     r = subprocess.run([cli, "--input-matrix", "bogus", "a", "b"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "rgb|bt709" in r.stderr
+    r = subprocess.run([cli, "--blur", "iir", "a", "b"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "sampled|recursive" in r.stderr
     r = subprocess.run([cli, "only-one"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "two inputs" in r.stderr
     r = subprocess.run([cli, "--help"], capture_output=True, text=True, timeout=60)

@@ -49,7 +49,7 @@ class DeviceImageC(ctypes.Structure):
 class ConfigC(ctypes.Structure):
     _fields_ = [("width", u32), ("height", u32), ("resident_images", u32),
                 ("plane_format", u32), ("streaming_inputs", u32), ("strip_rows", u32),
-                ("input_bytes", u64), ("graph", u32), ("chroma_filter", u32)]
+                ("input_bytes", u64), ("graph", u32), ("chroma_filter", u32), ("blur", u32)]
 
 
 class MemoryC(ctypes.Structure):
@@ -60,7 +60,7 @@ class MemoryC(ctypes.Structure):
 
 class BatchConfigC(ctypes.Structure):
     _fields_ = [("plane_format", u32), ("strip_rows", u32), ("max_pairs", u32),
-                ("chunks", u32), ("hbm_budget", u64), ("autotune", u32), ("reserved", u32)]
+                ("chunks", u32), ("hbm_budget", u64), ("autotune", u32), ("blur", u32)]
 
 
 class TuneResultC(ctypes.Structure):
@@ -105,6 +105,7 @@ SIGNATURES = {
                                               ctypes.POINTER(f32), ctypes.POINTER(f32)]),
     "vqm_ss2_plan_scales": (ctypes.c_int, [u32, u32, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
     "vqm_ss2_gaussian_taps": (None, [ctypes.POINTER(f32)]),
+    "vqm_ss2_blur_taps": (ctypes.c_int, [u32, ctypes.POINTER(f32)]),
     "vqm_ss2_channel_pruned": (ctypes.c_int, [u32, u32, u32]),
     "vqm_ss2_score": (ctypes.c_int, [u32, ctypes.POINTER(f64), ctypes.POINTER(f64)]),
     "vqm_ss2_srgb_table": (None, [ctypes.POINTER(f32)]),

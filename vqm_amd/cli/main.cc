@@ -26,6 +26,7 @@ struct Options {
     bool raw = false;
     bool half_planes = false;
     std::optional<PlaneFormat> planes;
+    Blur blur = Blur::Sampled;
     image_io::InputColorOptions input;
     std::string original, distorted;
 };
@@ -33,7 +34,7 @@ struct Options {
 const char* kUsage =
     "vqm-ss2 - SSIMULACRA 2.1 on AMD Instinct (HIP)\n"
     "Usage: %s [--metric ssimulacra2] [--raw] [--half-planes] [--planes f32|f16|bf16|u16]\n"
-    "          [--chroma-filter bilinear|bicubic|lanczos]\n"
+    "          [--chroma-filter bilinear|bicubic|lanczos] [--blur sampled|recursive]\n"
     "          [--input-matrix M] [--input-transfer T] [--input-primaries P] [--input-range R]\n"
     "          original distorted\n"
     "\n"
@@ -46,6 +47,8 @@ const char* kUsage =
     "--raw prints the 108 norms, channel-major, before the score.\n"
     "--half-planes stores intermediate planes as FP16 (--planes f16); --planes u16\n"
     "is the mixed form (16-bit fixed point at the two finest scales, FP32 above).\n"
+    "--blur recursive uses the kernel of Cloudinary ssimulacra2's recursive Gaussian\n"
+    "instead of the 13-tap sampled Gaussian (an addition of this build).\n"
     "--input-matrix     rgb|bt709|fcc|bt470bg|st170m|st240m|ycgco|bt2020ncl|\n"
     "                   bt2020cl|chromaticity-derived-ncl|chromaticity-derived-cl|ictcp\n"
     "--input-transfer   bt709|bt601|bt470m|bt470bg|st240m|linear|log100|log316|xvycc|\n"
@@ -92,6 +95,11 @@ std::optional<Options> Parse(int argc, char** argv) {
             else if (v == "bf16") o.planes = PlaneFormat::BF16;
             else if (v == "u16") o.planes = PlaneFormat::U16;
             else throw ParseError{"invalid --planes '" + std::string(v) + "' (expected f32|f16|bf16|u16)"};
+        } else if (a == "--blur") {
+            const std::string_view v = value();
+            if (v == "sampled") o.blur = Blur::Sampled;
+            else if (v == "recursive") o.blur = Blur::Recursive;
+            else throw ParseError{"invalid --blur '" + std::string(v) + "' (expected sampled|recursive)"};
         } else if (a == "--chroma-filter") {
             const std::string_view v = value();
             if (!image_io::ParseChromaFilter(v, o.input.chroma_filter, accepted)) throw bad(v);
@@ -135,6 +143,7 @@ Ssimulacra2Config MakeConfig(const Options& o, Size size) {
     c.half_planes = o.half_planes;
     if (o.planes) c.plane_format = *o.planes;
     c.chroma_filter = o.input.chroma_filter;
+    c.blur = o.blur;
     return c;
 }
 

@@ -134,6 +134,16 @@ void vqm_ss2_gaussian_taps(float taps[13]) {
     std::memcpy(taps, t.data(), sizeof(float) * 13);
 }
 
+int vqm_ss2_blur_taps(uint32_t blur, float taps[13]) {
+    return Guard([&] {
+        if (taps == nullptr) vqm::ss2::Fail(VQM_INVALID_ARGUMENT, "taps is null");
+        if (blur > vqm::ss2::kBlurRecursive)
+            vqm::ss2::Fail(VQM_INVALID_ARGUMENT, "blur must be sampled (0) or recursive (1)");
+        const float* w = blur == vqm::ss2::kBlurRecursive ? vqm::ss2::CosineTaps::w : vqm::ss2::SampledTaps::w;
+        std::memcpy(taps, w, sizeof(float) * 13);
+    });
+}
+
 int vqm_ss2_channel_pruned(uint32_t channel, uint32_t scale, uint32_t scales) {
     return vqm::ss2::ChannelPruned(channel, scale, scales) ? 1 : 0;
 }

@@ -82,6 +82,31 @@ struct Plan {
 // (BT.709, BT.601, BT.2020, SMPTE 240M); 2.2 and 2.8 powers (BT.470M/BG);
 // IEC 61966-2-1 sRGB; the two log curves; xvYCC; BT.1361; SMPTE ST 2084 (PQ);
 // SMPTE ST 428; ARIB STD-B67 (HLG).
+// The blur of the maps: the reference's 13-tap sampled Gaussian
+// (pipeline.cc:274-291), or the kernel Cloudinary's recursive Gaussian
+// computes (ssimulacra2.cc:82-106).
+enum BlurKind : uint32_t { kBlurSampled = 0, kBlurRecursive = 1 };
+
+// The 13 taps of MakeGaussian(1.5) (pipeline.cc:274-291), exact FP32 values.
+struct SampledTaps {
+    static constexpr float w[kTaps] = {
+        0x1.763864p-14f, 0x1.0d891ap-10f, 0x1.f1e742p-8f, 0x1.26dd9ep-5f,
+        0x1.bfdc88p-4f,  0x1.b42852p-3f,  0x1.1058fp-2f,  0x1.b42852p-3f,
+        0x1.bfdc88p-4f,  0x1.26dd9ep-5f,  0x1.f1e742p-8f, 0x1.0d891ap-10f,
+        0x1.763864p-14f};
+};
+
+// The recursive Gaussian of Cloudinary's ssimulacra2 (libjxl
+// CreateRecursiveGaussian(1.5), ssimulacra2.cc:82-106) as the zero-padded FIR
+// it computes: K[m] = sum_{k=1,3,5} beta_k cos(k pi m / 10), |m| <= 4, in the
+// 13-tap frame (zeros at |m| = 5, 6 leave the FMA chain bit-identical to a
+// 9-tap one). FP32 roundings of the FP64 kernel.
+struct CosineTaps {
+    static constexpr float w[kTaps] = {
+        0.0f, 0.0f, 0x1.347d7p-7f, 0x1.2700c8p-5f, 0x1.bfd672p-4f, 0x1.b413e8p-3f, 0x1.0ef8d6p-2f,
+        0x1.b413e8p-3f, 0x1.bfd672p-4f, 0x1.2700c8p-5f, 0x1.347d7p-7f, 0.0f, 0.0f};
+};
+
 enum TransferId : uint32_t {
     kCurveLinear = 0, kCurvePow24 = 1, kCurvePow22 = 2, kCurvePow28 = 3, kCurveSrgb = 4,
     kCurveLog100 = 5, kCurveLog316 = 6, kCurveXvycc = 7, kCurveBt1361 = 8, kCurvePq = 9,

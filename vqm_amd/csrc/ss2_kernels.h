@@ -30,8 +30,10 @@ hipError_t LaunchPrepare(PlaneFormat format, bool general, const PairDesc* pairs
                          const uint8_t* src0, const uint8_t* src1, hipStream_t stream);
 
 // One 64-lane wave per scale work item (level, channel, 128-column strip).
-// Uniform geometry: blocks = pairs * items_per_pair, work = nullptr.
-hipError_t LaunchScale(PlaneFormat format, const PairDesc* pairs, const WorkRef* work,
+// Uniform geometry: blocks = pairs * items_per_pair, work = nullptr. `blur`
+// picks the kernel: kBlurSampled (13-tap sampled Gaussian) or kBlurRecursive
+// (the recursive Gaussian's 9-tap truncated-cosine kernel).
+hipError_t LaunchScale(PlaneFormat format, uint32_t blur, const PairDesc* pairs, const WorkRef* work,
                        uint32_t blocks, uint32_t items_per_pair, hipStream_t stream);
 
 // Sums each (level, channel) norm's partials in FP64 and writes the 108 norms.

@@ -55,14 +55,7 @@ __device__ __forceinline__ GlobalAS<T>* as_global(T* p) {
     return (GlobalAS<T>*)p;
 }
 
-// The 13 taps of MakeGaussian(1.5) (pipeline.cc:274-291), exact FP32 values.
-struct Taps {
-    static constexpr float w[kTaps] = {
-        0x1.763864p-14f, 0x1.0d891ap-10f, 0x1.f1e742p-8f, 0x1.26dd9ep-5f,
-        0x1.bfdc88p-4f,  0x1.b42852p-3f,  0x1.1058fp-2f,  0x1.b42852p-3f,
-        0x1.bfdc88p-4f,  0x1.26dd9ep-5f,  0x1.f1e742p-8f, 0x1.0d891ap-10f,
-        0x1.763864p-14f};
-};
+// The tap tables SampledTaps and CosineTaps live in ss2_common.h.
 
 // XYB constants of prepare.comp:40-54 rounded to FP32.
 constexpr float kM00 = 0x1.333334p-2f, kM01 = 0x1.3e76c8p-1f, kM02 = 0x1.3f7ceep-4f;
@@ -670,7 +663,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // One scale work item: the strip of `level`, `channel` starting at (x0, y0),
 // with the plane elements in format Fmt.
-template <typename Fmt>
+template <typename Fmt, typename TapsT>
 __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>* plan, uint32_t item_idx,
                                            uint32_t level, uint32_t channel, uint32_t x0, uint32_t y0,
                                            float (*ring_a)[kSrcCols], float (*ring_b)[kSrcCols],
@@ -749,11 +742,11 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
                     v[2 * k] = t.x;
                     v[2 * k + 1] = t.y;
                 }
-                float h0 = Taps::w[0] * v[0];
-                float h1 = Taps::w[0] * v[1];
+                float h0 = TapsT::w[0] * v[0];
+                float h1 = TapsT::w[0] * v[1];
                 for (int k = 1; k < kTaps; ++k) {
-                    h0 = fma_(Taps::w[k], v[k], h0);
-                    h1 = fma_(Taps::w[k], v[k + 1], h1);
+                    h0 = fma_(TapsT::w[k], v[k], h0);
+                    h1 = fma_(TapsT::w[k], v[k + 1], h1);
                 }
                 win[P][q][0] = h0;
                 win[P][q][1] = h1;
@@ -765,9 +758,9 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
                 float bl[5][2];
                 for (int q = 0; q < 5; ++q) {
                     for (int c = 0; c < 2; ++c) {
-                        float acc = Taps::w[0] * win[(P + 1) % kTaps][q][c];
+                        float acc = TapsT::w[0] * win[(P + 1) % kTaps][q][c];
                         for (int k = 1; k < kTaps; ++k)
-                            acc = fma_(Taps::w[k], win[(P + 1 + k) % kTaps][q][c], acc);
+                            acc = fma_(TapsT::w[k], win[(P + 1 + k) % kTaps][q][c], acc);
                         bl[q][c] = acc;
                     }
                 }
@@ -812,7 +805,7 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
     }
 }
 
-template <typename Lay>
+template <typename Lay, typename TapsT>
 __global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ pairs,
                                                    const WorkRef* __restrict__ work,
                                                    uint32_t blocks, uint32_t items_per_pair) {
@@ -837,9 +830,10 @@ __global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ 
     const uint32_t x0 = item->x0, y0 = item->y0;
     // The item's level picks the element format (wave-uniform).
     if (Lay::kUniform || level < kFineLevels)
-        scale_item<typename Lay::Fine>(pdp, plan, item_idx, level, channel, x0, y0, ring_a, ring_b, prod);
+        scale_item<typename Lay::Fine, TapsT>(pdp, plan, item_idx, level, channel, x0, y0, ring_a, ring_b, prod);
     else
-        scale_item<typename Lay::Coarse>(pdp, plan, item_idx, level, channel, x0, y0, ring_a, ring_b, prod);
+        scale_item<typename Lay::Coarse, TapsT>(pdp, plan, item_idx, level, channel, x0, y0, ring_a, ring_b,
+                                                prod);
 }
 
 // ---- reduce -----------------------------------------------------------------
@@ -891,10 +885,14 @@ hipError_t launch_prepare_t(bool general, const PairDesc* pairs, const WorkRef* 
 }
 
 template <typename Lay>
-hipError_t launch_scale_t(const PairDesc* pairs, const WorkRef* work, uint32_t blocks,
+hipError_t launch_scale_t(uint32_t blur, const PairDesc* pairs, const WorkRef* work, uint32_t blocks,
                           uint32_t items, hipStream_t stream) {
-    hipLaunchKernelGGL(scale_kernel<Lay>, dim3(blocks), dim3(64), 0, stream, pairs, work,
-                       blocks, items);
+    if (blur == kBlurRecursive)
+        hipLaunchKernelGGL((scale_kernel<Lay, CosineTaps>), dim3(blocks), dim3(64), 0, stream, pairs, work,
+                           blocks, items);
+    else
+        hipLaunchKernelGGL((scale_kernel<Lay, SampledTaps>), dim3(blocks), dim3(64), 0, stream, pairs, work,
+                           blocks, items);
     return hipGetLastError();
 }
 
@@ -926,14 +924,15 @@ hipError_t LaunchPrepare(PlaneFormat format, bool general, const PairDesc* pairs
     return hipErrorInvalidValue;
 }
 
-hipError_t LaunchScale(PlaneFormat format, const PairDesc* pairs, const WorkRef* work,
+hipError_t LaunchScale(PlaneFormat format, uint32_t blur, const PairDesc* pairs, const WorkRef* work,
                        uint32_t blocks, uint32_t items_per_pair, hipStream_t stream) {
     if (blocks == 0) return hipSuccess;
+    if (blur > kBlurRecursive) return hipErrorInvalidValue;
     switch (format) {
-        case kPlaneF32: return launch_scale_t<LayoutF32>(pairs, work, blocks, items_per_pair, stream);
-        case kPlaneF16: return launch_scale_t<LayoutF16>(pairs, work, blocks, items_per_pair, stream);
-        case kPlaneBF16: return launch_scale_t<LayoutBF16>(pairs, work, blocks, items_per_pair, stream);
-        case kPlaneU16: return launch_scale_t<LayoutU16>(pairs, work, blocks, items_per_pair, stream);
+        case kPlaneF32: return launch_scale_t<LayoutF32>(blur, pairs, work, blocks, items_per_pair, stream);
+        case kPlaneF16: return launch_scale_t<LayoutF16>(blur, pairs, work, blocks, items_per_pair, stream);
+        case kPlaneBF16: return launch_scale_t<LayoutBF16>(blur, pairs, work, blocks, items_per_pair, stream);
+        case kPlaneU16: return launch_scale_t<LayoutU16>(blur, pairs, work, blocks, items_per_pair, stream);
     }
     return hipErrorInvalidValue;
 }

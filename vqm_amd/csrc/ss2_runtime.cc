@@ -270,7 +270,7 @@ Lane::~Lane() {
 }
 
 void Lane::Launch(PlaneFormat fmt, const float* lut, const LaunchEntry* entries, uint32_t count,
-                  uint32_t chunks) {
+                  uint32_t chunks, uint32_t blur) {
     if (pending_) Fail(VQM_LOGIC_ERROR, "lane: the previous launch has not finished");
     if (count == 0 || count > max_pairs_) Fail(VQM_INVALID_ARGUMENT, "lane: bad pair count");
     chunks = std::max(1u, std::min({chunks, count, kMaxChunks}));
@@ -360,7 +360,7 @@ void Lane::Launch(PlaneFormat fmt, const float* lut, const LaunchEntry* entries,
         const auto ns = static_cast<uint32_t>(scale_at[i1] - scale_at[i0]);
         if (t != s) Check(hipStreamWaitEvent(t, Ev(4 * c + 1).get(), 0), "order scale after prepare");
         Check(hipEventRecord(Ev(4 * c + 2).get(), t), "hipEventRecord");
-        Check(LaunchScale(fmt, base, ws, ns, g0.plan.items, t), "launch scale");
+        Check(LaunchScale(fmt, blur, base, ws, ns, g0.plan.items, t), "launch scale");
         Check(hipEventRecord(Ev(4 * c + 3).get(), t), "hipEventRecord");
     }
     const uint32_t r = 4 * chunks;
@@ -420,6 +420,7 @@ static const vqm_ss2_config& Validated(const vqm_ss2_config& config) {
         Fail(VQM_INVALID_ARGUMENT, "gpu_resident_images is limited to 4096 slots");
     if (config.chroma_filter > 2)
         Fail(VQM_INVALID_ARGUMENT, "chroma_filter must be bilinear, bicubic or lanczos");
+    if (config.blur > kBlurRecursive) Fail(VQM_INVALID_ARGUMENT, "blur must be sampled (0) or recursive (1)");
     return config;
 }
 
@@ -512,7 +513,7 @@ void Pipeline::BuildGraph() {
     Check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin graph capture");
     hipError_t e = hipMemcpyAsync(pair_.get(), h_pair_.get(), sizeof(PairDesc), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = LaunchPrepare(fmt, false, pd, nullptr, 2 * g.prep_tiles, g.prep_tiles, ctx_.lut, nullptr, nullptr, s);
-    if (e == hipSuccess) e = LaunchScale(fmt, pd, nullptr, g.plan.items, g.plan.items, s);
+    if (e == hipSuccess) e = LaunchScale(fmt, config_.blur, pd, nullptr, g.plan.items, g.plan.items, s);
     if (e == hipSuccess) e = LaunchReduce(pd, 1, s);
     if (e == hipSuccess)
         e = hipMemcpyAsync(host_norms_.get(), ws_.norms, kNorms * sizeof(double), hipMemcpyDeviceToHost, s);
@@ -600,7 +601,7 @@ uint64_t Pipeline::Launch(const uint8_t* src0, const uint8_t* src1, bool reuse_o
         Check(LaunchPrepare(fmt, general_, pd, nullptr, 2 * g.prep_tiles, g.prep_tiles, ctx_.lut, src0, src1, s),
               "launch prepare");
     Check(hipEventRecord(ev_[1].get(), s), "hipEventRecord");
-    Check(LaunchScale(fmt, pd, nullptr, g.plan.items, g.plan.items, s), "launch scale");
+    Check(LaunchScale(fmt, config_.blur, pd, nullptr, g.plan.items, g.plan.items, s), "launch scale");
     Check(hipEventRecord(ev_[2].get(), s), "hipEventRecord");
     Check(LaunchReduce(pd, 1, s), "launch reduce");
     Check(hipEventRecord(ev_[3].get(), s), "hipEventRecord");
@@ -765,6 +766,7 @@ const vqm_ss2_batch_config& ValidatedBatch(const vqm_ss2_batch_config& config) {
     ToFormat(config.plane_format);
     if (config.max_pairs == 0) Fail(VQM_INVALID_ARGUMENT, "max_pairs must be at least 1");
     if (config.chunks > Lane::kMaxChunks) Fail(VQM_INVALID_ARGUMENT, "chunks must be at most 16");
+    if (config.blur > kBlurRecursive) Fail(VQM_INVALID_ARGUMENT, "blur must be sampled (0) or recursive (1)");
     return config;
 }
 }  // namespace
@@ -820,7 +822,7 @@ uint64_t Batch::Submit(const vqm_ss2_device_pair* pairs, uint32_t count) {
             taken.push_back(Taken{pairs[i].width, pairs[i].height, std::move(ws)});
         }
         lane_.Launch(static_cast<PlaneFormat>(config_.plane_format), ctx_.lut, entries.data(), count,
-                     chunks);
+                     chunks, config_.blur);
     } catch (...) {
         for (Taken& t : taken) pools_.Return(t.width, t.height, false, std::move(t.ws));
         throw;

@@ -185,6 +185,21 @@ class ChromaFilter(enum.IntEnum):
     Lanczos = 2
 
 
+class Blur(enum.IntEnum):
+    """The blur of the maps: the reference's 13-tap sampled Gaussian, or the
+    kernel of Cloudinary ssimulacra2's recursive Gaussian (no reference
+    counterpart; include/vqm_ss2.h VQM_BLUR_*)."""
+    Sampled = 0
+    Recursive = 1
+
+
+def blur_taps(blur: int = Blur.Sampled) -> np.ndarray:
+    """The 13 FP32 taps the scale kernel applies for `blur`."""
+    t = (L.f32 * 13)()
+    _check(L.lib().vqm_ss2_blur_taps(int(blur), t))
+    return np.array(t, np.float32)
+
+
 def validate_image_format(fmt: ImageFormat, size: Size) -> None:
     """ValidateImage (input.cc:152-222): raises InvalidArgument naming the reason."""
     _check(L.lib().vqm_validate_image_format(ctypes.byref(fmt.to_c()), size.width, size.height))
@@ -352,12 +367,14 @@ class Ssimulacra2Config:
     strip_rows: int = 0
     graph: bool = False  # replay one captured HIP graph per pair (no reference field)
     chroma_filter: int = 0  # ChromaFilter: 0 bilinear, 1 bicubic, 2 lanczos
+    blur: int = 0  # Blur: 0 sampled (the reference), 1 recursive (no reference field)
 
     def to_c(self) -> L.ConfigC:
         c = L.ConfigC()
         L.lib().vqm_ss2_config_init(ctypes.byref(c), self.size.width, self.size.height)
         c.graph = 1 if self.graph else 0
         c.chroma_filter = int(self.chroma_filter)
+        c.blur = int(self.blur)
         c.width, c.height = self.size.width, self.size.height
         c.resident_images = self.gpu_resident_images
         fmt = self.plane_format
@@ -659,8 +676,9 @@ class Ssimulacra2Batch:
     Pairs are device-resident (DeviceImage pointers)."""
 
     def __init__(self, max_pairs: int, plane_format: int = L.PLANES_F32, hbm_budget: int = 0,
-                 strip_rows: int = 0, chunks: int = 1, autotune: bool = False):
+                 strip_rows: int = 0, chunks: int = 1, autotune: bool = False, blur: int = 0):
         cfg = L.BatchConfigC()
+        cfg.blur = int(blur)
         cfg.plane_format = plane_format
         cfg.strip_rows = strip_rows
         cfg.chunks = chunks
