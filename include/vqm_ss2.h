@@ -54,7 +54,9 @@ enum {
 /* Stored plane precision; arithmetic is FP32 in every format.
  *   F32   FP32 planes.
  *   F16   FP16 planes: the reference's half_planes (include/ssimulacra2.h:
- *         41-46); scores move by up to ~0.06 against FP64.
+ *         41-46), with its rounding of each level's linear box average to
+ *         FP16 as well (ssimu2_common.glsl:84-96); scores move by up to
+ *         ~0.06 against FP64.
  *   BF16  BF16 planes: 8 mantissa bits are too few for the metric (scores
  *         move by points); kept for comparison only.
  *   U16   the mixed form: unsigned 16-bit fixed point (q / 65536, exact zero,
