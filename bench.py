@@ -13,8 +13,10 @@ engine; the norms of all pairs come back to the host before the step ends.
 Printed: one JSON line with pairs/s (`value`), the roofline of the dominant
 kernel (algorithmic bytes per launch over its HIP-event duration on the batch
 stream, against 8 TB/s), the CPU baseline (the oracle's FP32 device form on
-the host's cores, one pair per thread), and per-mode extras (streaming and
-single-pair latency).
+the host's cores, one pair per thread), and extras: the other plane form and
+the 4K batch (`plane_forms`, `workload_4k`), single-pair and streaming
+latency (`modes`), the uneven-size batch (`ladder_batch`) and the serving
+loop (`serving`).
 """
 from __future__ import annotations
 
@@ -189,6 +191,35 @@ def ladder_batch(args, pool):
     return out
 
 
+def resident_batch(refs, w, h, fmt, steps, warmup, chunks=1, strip_rows=0):
+    """Pairs/s of the batch engine over resident pairs, with the per-stage
+    HIP-event times and the scale kernel's HBM fraction."""
+    from vqm_amd import Ssimulacra2Batch
+    batch = Ssimulacra2Batch(max_pairs=len(refs), plane_format=fmt, strip_rows=strip_rows, chunks=chunks)
+    for _ in range(warmup):
+        batch.Finish(batch.Submit(refs), raw=True)
+    stages = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, norms = batch.Finish(batch.Submit(refs), raw=True)
+        stages.append(batch.StageTimes())
+    total = time.perf_counter() - t0
+    batch.close()
+    if not np.all(np.isfinite(norms)):
+        raise RuntimeError("non-finite norms")
+    stage = [statistics.mean(s[k] for s in stages) for k in range(3)]
+    planes = plane_bytes(w, h, fmt)
+    prep_gbps = len(refs) * (6 * w * h + 2 * planes) / (stage[0] * 1e-3) / 1e9
+    scale_gbps = len(refs) * 2 * planes / (stage[1] * 1e-3) / 1e9
+    return {"pairs_per_s": round(len(refs) * steps / total, 1),
+            "ms_per_step": round(total / steps * 1e3, 3),
+            "pairs_per_step": len(refs),
+            "stage_ms": {"prepare": round(stage[0], 4), "scale": round(stage[1], 4),
+                         "reduce": round(stage[2], 4)},
+            "hbm_frac": {"prepare": round(prep_gbps / HBM_PEAK_GBPS, 4),
+                         "scale": round(scale_gbps / HBM_PEAK_GBPS, 4)}}
+
+
 def cpu_baseline(w, h, pairs, sample):
     import oracle
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -227,6 +258,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-modes", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the 4K batch and the other plane forms at the main workload")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run the plain copy kernel over MIB MiB (traffic counter reference)")
     args = ap.parse_args()
@@ -332,6 +365,31 @@ def main() -> int:
         "roofline": roofline,
         "vs_baseline_note": "value / 3836.7 pairs/s (vqm 1080p resident, 3 workers, RTX 5070, BASELINE.md)",
     }
+
+    if not args.no_secondary:
+        # The other plane forms on the same resident pairs, then the 4K batch.
+        forms = {}
+        for name, f in sorted(PLANES.items()):
+            if f != fmt and name in ("f32", "u16"):
+                forms[name] = resident_batch(refs, w, h, f, max(3, args.steps // 4), 1, args.chunks)
+        out["plane_forms"] = forms
+        log(f"plane forms: {forms}")
+        if args.workload == "1080p":
+            w4, h4, cfg4 = WORKLOADS["4k"]
+            pairs4 = make_pairs(w4, h4, cfg4, 10)
+            slot4 = (pairs4[0][0].nbytes + 255) // 256 * 256
+            dev4 = DeviceBuffer(2 * slot4 * len(pairs4))
+            refs4 = []
+            for i, (a, b) in enumerate(pairs4):
+                dev4.upload(a, 2 * i * slot4)
+                dev4.upload(b, (2 * i + 1) * slot4)
+                refs4.append((dev4.ptr + 2 * i * slot4, dev4.ptr + (2 * i + 1) * slot4, w4, h4))
+            out["workload_4k"] = {"workload": f"{w4}x{h4} 8-bit sRGB RGB, {len(refs4)} resident pairs per step "
+                                              f"(3 strengths x 10), {args.planes} planes",
+                                  **resident_batch(refs4, w4, h4, fmt, max(3, args.steps // 4), 1, args.chunks)}
+            dev4.free()
+            del pairs4
+            log(f"4k: {out['workload_4k']}")
 
     if not args.no_modes:
         modes = {}
