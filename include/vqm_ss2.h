@@ -375,6 +375,8 @@ typedef struct vqm_ss2_server_config {
     uint32_t strip_rows;     /* 0 = default */
     uint64_t hbm_budget;     /* device bytes for workspaces and host-input
                                 staging; 0 = unlimited */
+    uint32_t blur;           /* VQM_BLUR_SAMPLED (default) or VQM_BLUR_RECURSIVE */
+    uint32_t reserved;       /* 0 */
 } vqm_ss2_server_config;
 
 /* The outcome of one request. Times are vqm_now_ns() readings. */

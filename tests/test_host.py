@@ -168,9 +168,13 @@ def test_last_error_is_thread_local_and_cleared():
 
 def test_server_config_validation_before_device_work():
     """The server rejects a bad configuration before touching the device."""
-    for kwargs in ({"lanes": 17}, {"max_batch": 5000}, {"plane_format": 4}, {"strip_rows": 6}):
+    for kwargs in ({"lanes": 17}, {"max_batch": 5000}, {"plane_format": 4}, {"strip_rows": 6}, {"blur": 2}):
         with pytest.raises(InvalidArgument):
             vqm_amd.Ssimulacra2Server(**kwargs)
+    with pytest.raises(InvalidArgument):
+        vqm_amd.Ssimulacra2Batch(max_pairs=1, blur=2)
+    with pytest.raises(InvalidArgument):
+        vqm_amd.Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(32, 32), blur=2))
     cfg = L.ServerConfigC()
     L.lib().vqm_ss2_server_config_init(ctypes.byref(cfg))
     assert (cfg.plane_format, cfg.lanes, cfg.max_batch, cfg.strip_rows, cfg.hbm_budget) == (0, 3, 64, 0, 0)

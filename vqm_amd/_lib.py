@@ -74,7 +74,7 @@ class DevicePairC(ctypes.Structure):
 
 class ServerConfigC(ctypes.Structure):
     _fields_ = [("plane_format", u32), ("lanes", u32), ("max_batch", u32), ("strip_rows", u32),
-                ("hbm_budget", u64)]
+                ("hbm_budget", u64), ("blur", u32), ("reserved", u32)]
 
 
 class RequestResultC(ctypes.Structure):

@@ -23,6 +23,7 @@ const vqm_ss2_server_config& ValidatedServer(const vqm_ss2_server_config& c) {
     ToFormat(c.plane_format);
     if (c.lanes > 16) Fail(VQM_INVALID_ARGUMENT, "lanes must be between 1 and 16");
     if (c.max_batch > 4096) Fail(VQM_INVALID_ARGUMENT, "max_batch must be between 1 and 4096");
+    if (c.blur > kBlurRecursive) Fail(VQM_INVALID_ARGUMENT, "blur must be sampled (0) or recursive (1)");
     BuildPlan(8, 8, static_cast<PlaneFormat>(c.plane_format), c.strip_rows);  // validates strip_rows
     return c;
 }
@@ -192,7 +193,7 @@ void Server::DispatchLoop() {
                 }
                 e.image_bytes = r.image_bytes;
             }
-            lanes_[lane]->Launch(format_, ctx_.lut, entries.data(), n);
+            lanes_[lane]->Launch(format_, ctx_.lut, entries.data(), n, 1, config_.blur);
         } catch (const Error& e) {
             status = e.status;
             message = e.message;

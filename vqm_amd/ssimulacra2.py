@@ -809,9 +809,10 @@ class Ssimulacra2Server:
     against `hbm_budget`, and overlap on the device. Thread-safe."""
 
     def __init__(self, plane_format: int = L.PLANES_F32, lanes: int = 3, max_batch: int = 64,
-                 hbm_budget: int = 0, strip_rows: int = 0):
+                 hbm_budget: int = 0, strip_rows: int = 0, blur: int = 0):
         cfg = L.ServerConfigC()
         L.lib().vqm_ss2_server_config_init(ctypes.byref(cfg))
+        cfg.blur = int(blur)
         cfg.plane_format = plane_format
         cfg.lanes = lanes
         cfg.max_batch = max_batch
