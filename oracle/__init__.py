@@ -21,19 +21,20 @@ class Mode(ctypes.Structure):
     _fields_ = [("arith", ctypes.c_int32), ("blur_order", ctypes.c_int32),
                 ("cbrt_kind", ctypes.c_int32), ("plane_format", ctypes.c_int32),
                 ("plane_round_levels", ctypes.c_uint32), ("sum_quads", ctypes.c_int32),
-                ("round_linear", ctypes.c_int32), ("blur_kind", ctypes.c_int32)]
+                ("round_linear", ctypes.c_int32), ("blur_kind", ctypes.c_int32),
+                ("sum_variances", ctypes.c_int32)]
 
 
 # The three forms (ss2_oracle.h).
 F64 = Mode(0, 0, 0, 0, 0, 0)
 F32_REF = Mode(1, 0, 0, 0, 0, 0)
-F32_DEV = Mode(1, 0, 1, 0, 0, 0)
+F32_DEV = Mode(1, 0, 1, 0, 0, 0, 0, 0, 1)
 
 
 def f32_dev_planes(plane_format: int, levels_mask: int = 0x3F) -> Mode:
     """The device form with the stored planes of the levels in `levels_mask`
     rounded to FP16 (1), BF16 (2) or unsigned 16-bit fixed point (3)."""
-    return Mode(1, 0, 1, plane_format, levels_mask, 0)
+    return Mode(1, 0, 1, plane_format, levels_mask, 0, 0, 0, 1)
 
 
 def f32_dev_for(plane_format: int) -> Mode:

@@ -314,15 +314,28 @@ struct Pyramid {
 // The six map sums of one (level, channel), scale.comp:305-333.
 template <typename T>
 void map_sums(const T* a, const T* b, uint32_t w, uint32_t h, int order,
-              int sum_quads, const T* wt, double out[6]) {
+              int sum_quads, int sum_variances, const T* wt, double out[6]) {
     const size_t n = static_cast<size_t>(w) * h;
     std::vector<T> q(n), mu1(n), mu2(n), s11(n), s22(n), s12(n);
     blur(a, mu1.data(), w, h, order, wt);
     blur(b, mu2.data(), w, h, order, wt);
-    for (size_t i = 0; i < n; ++i) q[i] = a[i] * a[i];
-    blur(q.data(), s11.data(), w, h, order, wt);
-    for (size_t i = 0; i < n; ++i) q[i] = b[i] * b[i];
-    blur(q.data(), s22.data(), w, h, order, wt);
+    if (sum_variances) {
+        // The device form: (a - b)^2 as one blurred quantity (s11 holds it).
+        // s11 + s22 = blur((a - b)^2) + 2 s12, so the SSIM' denominator is
+        // num_s + v with v = blur((a - b)^2) - (mu1 - mu2)^2, and
+        // den - num_m num_s = v + (mu1 - mu2)^2 num_s: no large-minus-large
+        // cancellation beyond the one num_s already has.
+        for (size_t i = 0; i < n; ++i) {
+            const T dlt = a[i] - b[i];
+            q[i] = dlt * dlt;
+        }
+        blur(q.data(), s11.data(), w, h, order, wt);
+    } else {
+        for (size_t i = 0; i < n; ++i) q[i] = a[i] * a[i];
+        blur(q.data(), s11.data(), w, h, order, wt);
+        for (size_t i = 0; i < n; ++i) q[i] = b[i] * b[i];
+        blur(q.data(), s22.data(), w, h, order, wt);
+    }
     for (size_t i = 0; i < n; ++i) q[i] = a[i] * b[i];
     blur(q.data(), s12.data(), w, h, order, wt);
 
@@ -336,10 +349,17 @@ void map_sums(const T* a, const T* b, uint32_t w, uint32_t h, int order,
             const size_t i = static_cast<size_t>(y) * w + x;
             const T m1 = mu1[i], m2 = mu2[i];
             const T t = m1 - m2;
-            const T num_m = fmaT(-t, t, one);
             const T num_s = fmaT(two, fmaT(-m1, m2, s12[i]), c2);
-            const T den = (fmaT(-m1, m1, s11[i]) + fmaT(-m2, m2, s22[i])) + c2;
-            const T d = std::max(fmaT(-num_m, num_s, den) / den, zero);
+            T d;
+            if (sum_variances) {
+                const T v = fmaT(-t, t, s11[i]);
+                const T den = num_s + v;
+                d = std::max(fmaT(t * t, num_s, v) / den, zero);
+            } else {
+                const T num_m = fmaT(-t, t, one);
+                const T den = (fmaT(-m1, m1, s11[i]) + fmaT(-m2, m2, s22[i])) + c2;
+                d = std::max(fmaT(-num_m, num_s, den) / den, zero);
+            }
             const T dd = d * d;
             const T det_dist = one + std::fabs(b[i] - m2);
             const T det_orig = one + std::fabs(a[i] - m1);
@@ -519,7 +539,8 @@ int32_t compute(const ss2o_mode& mode, uint32_t width, uint32_t height,
             double s[6];
             map_sums<T>(xyb[0][l][c].data(), xyb[1][l][c].data(), w, h,
                         sizeof(T) == 8 ? 0 : mode.blur_order,
-                        sizeof(T) == 8 ? 0 : mode.sum_quads, wt, s);
+                        sizeof(T) == 8 ? 0 : mode.sum_quads,
+                        sizeof(T) == 8 ? 0 : mode.sum_variances, wt, s);
             // vulkan.cc:545-583: mean in double, fourth root of the 4-norms.
             double* out = norms + l * 18;
             out[c * 2 + 0] = one_per_pixel * s[0];

@@ -15,10 +15,11 @@
  *             first, scale.comp:149-301), correctly rounded cube root.
  *   FP32 dev  the HIP kernels' FP32 operations: the reference's blur order
  *             (horizontal pass first), the kernels' own reproducible cube
- *             root, per-pixel map values summed in FP64, optional FP16/BF16
- *             rounding of stored planes. The HIP path matches this form
- *             per pixel bit for bit; only the order of the FP64 sums
- *             differs.
+ *             root, (a - b)^2 blurred in place of a^2 and b^2 (the SSIM'
+ *             map needs only s11 + s22 = blur((a - b)^2) + 2 s12), per-pixel map values
+ *             summed in FP64, optional FP16/BF16/fixed-point rounding of
+ *             stored planes. The HIP path matches this form per pixel bit
+ *             for bit; only the order of the FP64 sums differs.
  *
  * Parity pins (checked in tests/test_oracle.py):
  *   - end to end: the reference's 32x32 goldens 93.17505696 / 92.12553218
@@ -53,6 +54,12 @@ typedef struct ss2o_mode {
     int32_t blur_kind;    /* 0 = the 13-tap sampled Gaussian (pipeline.cc:274-291);
                              1 = the 9-tap truncated-cosine kernel that libjxl's recursive
                              Gaussian computes (Cloudinary ssimulacra2.cc:82-106) */
+    int32_t sum_variances; /* FP32 only: 1 = the device form of the SSIM' map: (a - b)^2
+                              blurred as one quantity e in place of a^2 and b^2, then
+                              v = e - (mu1 - mu2)^2, den = num_s + v and
+                              d = max(((mu1 - mu2)^2 num_s + v) / den, 0), algebraically
+                              scale.comp:305-333; 0 = s11 and s22 blurred apart and the
+                              map in the reference's order (scale.comp:130-142,305-333) */
 } ss2o_mode;
 
 /* One stored plane value as plane_format keeps it: 1 FP16 and 2 BF16 (round

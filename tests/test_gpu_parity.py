@@ -47,7 +47,11 @@ def check_against_oracle(oracle_lib, w, h, a, b, dev_mode=None, **cfg):
 
 
 def test_reference_goldens_on_gpu(oracle_lib):
-    """pipeline_test.cc:146-174."""
+    """pipeline_test.cc:146-174: the reference's goldens within its own 5e-3,
+    and FP64 within 1e-3. On these 32x32 pairs the reference's FP32 operation
+    order is itself 2.5e-3 from FP64 (gradient C); the kernels' SSIM' form
+    stays within 1e-4 of FP64, so FP64 is the 1e-3 comparator here and the
+    reference's FP32 order is the comparator on the workload sizes."""
     orig = gradient_image(0)
     for variant, golden in ((1, GOLDEN_B), (2, GOLDEN_C)):
         dist = gradient_image(variant)
@@ -55,8 +59,9 @@ def test_reference_goldens_on_gpu(oracle_lib):
         assert n == 4
         score = res.Score()
         assert abs(score - golden) <= GOLDEN_TOL
-        _, ref, _ = oracle_lib.compute(oracle_lib.F32_REF, 32, 32, orig, dist)
-        assert abs(score - oracle_lib.score(n, ref)) <= SCORE_ABS
+        _, f64, _ = oracle_lib.compute(oracle_lib.F64, 32, 32, orig, dist)
+        assert abs(score - oracle_lib.score(n, f64)) <= SCORE_ABS
+        assert_features_close(norms, f64, what="32x32 golden pair vs FP64")
 
 
 @pytest.mark.parametrize("size,scales", [((8, 8), 2), ((13, 9), 2), ((32, 32), 4), ((64, 64), 5),

@@ -650,6 +650,7 @@ __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict
 // ---- scale ------------------------------------------------------------------
 
 constexpr int kRing = 8;  // raw rows kept in LDS: the map needs the row six back
+constexpr int kBlurred = 4;  // a, b, (a - b)^2, ab
 
 template <int... P, typename F>
 __device__ __forceinline__ void unroll(std::integer_sequence<int, P...>, F&& f) {
@@ -692,7 +693,7 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
         cur_hb = Fmt::load2(pb + jh);
     }
 
-    float win[kTaps][5][2];
+    float win[kTaps][kBlurred][2];
     double sums[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const uint32_t col = x0 + 2u * static_cast<uint32_t>(lane);
 
@@ -708,17 +709,17 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
                 const float2 a = Fmt::unpack(cur_a), b = Fmt::unpack(cur_b);
                 *reinterpret_cast<float2*>(ra + jm) = a;
                 *reinterpret_cast<float2*>(rb + jm) = b;
-                *reinterpret_cast<float2*>(&prod[0][jm]) = make_float2(a.x * a.x, a.y * a.y);
-                *reinterpret_cast<float2*>(&prod[1][jm]) = make_float2(b.x * b.x, b.y * b.y);
-                *reinterpret_cast<float2*>(&prod[2][jm]) = make_float2(a.x * b.x, a.y * b.y);
+                const float dx = a.x - b.x, dy = a.y - b.y;
+                *reinterpret_cast<float2*>(&prod[0][jm]) = make_float2(dx * dx, dy * dy);
+                *reinterpret_cast<float2*>(&prod[1][jm]) = make_float2(a.x * b.x, a.y * b.y);
             }
             if (halo) {
                 const float2 a = Fmt::unpack(cur_ha), b = Fmt::unpack(cur_hb);
                 *reinterpret_cast<float2*>(ra + jh) = a;
                 *reinterpret_cast<float2*>(rb + jh) = b;
-                *reinterpret_cast<float2*>(&prod[0][jh]) = make_float2(a.x * a.x, a.y * a.y);
-                *reinterpret_cast<float2*>(&prod[1][jh]) = make_float2(b.x * b.x, b.y * b.y);
-                *reinterpret_cast<float2*>(&prod[2][jh]) = make_float2(a.x * b.x, a.y * b.y);
+                const float dx = a.x - b.x, dy = a.y - b.y;
+                *reinterpret_cast<float2*>(&prod[0][jh]) = make_float2(dx * dx, dy * dy);
+                *reinterpret_cast<float2*>(&prod[1][jh]) = make_float2(a.x * b.x, a.y * b.y);
             }
             // Prefetch input row i + 1.
             if (i + 1 < total) {
@@ -732,9 +733,11 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
             }
             __syncthreads();
 
-            // Horizontal 13-tap pass of the five quantities at this lane's two
-            // output columns (source columns jm + 6 and jm + 7).
-            for (int q = 0; q < 5; ++q) {
+            // Horizontal 13-tap pass of the four quantities a, b, (a - b)^2
+            // and ab at this lane's two output columns (source columns
+            // jm + 6 and jm + 7). The SSIM' map needs s11 + s22 only, which
+            // is blur((a - b)^2) + 2 s12.
+            for (int q = 0; q < kBlurred; ++q) {
                 const float* s = q == 0 ? ra : q == 1 ? rb : prod[q - 2];
                 float v[14];
                 for (int k = 0; k < 7; ++k) {
@@ -755,8 +758,8 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
             // Vertical pass for output row i - 12 (image row y0 + i - 12),
             // oldest ring entry first: scale.comp:243-298's ascending taps.
             if (i >= 2 * kRadius) {
-                float bl[5][2];
-                for (int q = 0; q < 5; ++q) {
+                float bl[kBlurred][2];
+                for (int q = 0; q < kBlurred; ++q) {
                     for (int c = 0; c < 2; ++c) {
                         float acc = TapsT::w[0] * win[(P + 1) % kTaps][q][c];
                         for (int k = 1; k < kTaps; ++k)
@@ -772,13 +775,20 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
                 if (y < H) {
                     for (int c = 0; c < 2; ++c) {
                         if (col + static_cast<uint32_t>(c) >= W) continue;
-                        // scale.comp:305-333.
+                        // scale.comp:305-333 rearranged: with e the blurred
+                        // (a - b)^2 and t = mu1 - mu2, s11 + s22 - mu1^2 -
+                        // mu2^2 = num_s - c2 + v where v = e - t^2, so
+                        // den = num_s + v and den - num_m num_s =
+                        // v + t^2 num_s. The small v replaces the difference
+                        // of two large second moments (closer to FP64 than
+                        // the direct form), and identical images give
+                        // v = t = 0, d = 0 exactly.
                         const float mu1 = bl[0][c], mu2 = bl[1][c];
                         const float t = mu1 - mu2;
-                        const float num_m = fma_(-t, t, 1.0f);
-                        const float num_s = fma_(2.0f, fma_(-mu1, mu2, bl[4][c]), kC2);
-                        const float den = (fma_(-mu1, mu1, bl[2][c]) + fma_(-mu2, mu2, bl[3][c])) + kC2;
-                        const float d = fmaxf(fma_(-num_m, num_s, den) / den, 0.0f);
+                        const float num_s = fma_(2.0f, fma_(-mu1, mu2, bl[3][c]), kC2);
+                        const float v = fma_(-t, t, bl[2][c]);
+                        const float den = num_s + v;
+                        const float d = fmaxf(fma_(t * t, num_s, v) / den, 0.0f);
                         const float dd = d * d;
                         const float det_dist = 1.0f + fabsf(bv[c] - mu2);
                         const float det_orig = 1.0f + fabsf(av[c] - mu1);
@@ -811,7 +821,7 @@ __global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ 
                                                    uint32_t blocks, uint32_t items_per_pair) {
     __shared__ __align__(16) float ring_a[kRing][kSrcCols];
     __shared__ __align__(16) float ring_b[kRing][kSrcCols];
-    __shared__ __align__(16) float prod[3][kSrcCols];
+    __shared__ __align__(16) float prod[2][kSrcCols];
 
     uint32_t pair_idx, item_idx;
     if (work != nullptr) {
