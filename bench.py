@@ -356,7 +356,7 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": round(value / RTX5070_1080P_RESIDENT, 3) if args.workload == "1080p" else None,
         "dtype": {0: "f32", 1: "f32 (f16 planes)", 2: "f32 (bf16 planes)",
-                  3: "f32 (u16 fixed-point planes at levels 0-1)"}[fmt],
+                  3: "f32 (u16 fixed-point planes)"}[fmt],
         "data": "synthetic (vqm_amd/synth.py: sinusoid field + block quantisation + Gaussian noise)",
         "config": {"workload": f"{w}x{h} 8-bit sRGB RGB, 6 scales, {len(refs)} pairs per step "
                                f"(3 strengths x {args.pairs_per_strength}), resident in HBM, "

@@ -59,11 +59,11 @@ enum {
  *         ~0.06 against FP64.
  *   BF16  BF16 planes: 8 mantissa bits are too few for the metric (scores
  *         move by points); kept for comparison only.
- *   U16   the mixed form: unsigned 16-bit fixed point (q / 65536, exact zero,
- *         uniform 2^-16 step over the XYB range [0, 0.94]) at levels 0 and 1,
- *         which hold 89% of the plane elements, and FP32 at levels 2..5.
- *         Scores stay within ~2e-4 of FP64 (FP32: ~1.7e-4) with 55% of the
- *         FP32 plane bytes. An addition with no reference counterpart. */
+ *   U16   the mixed form: planes stored as unsigned 16-bit fixed point
+ *         (q / 65536, exact zero, uniform 2^-16 step over the XYB range
+ *         [0, 0.94]), arithmetic in FP32, sums in FP64. Scores stay within
+ *         ~2e-4 of FP64 (FP32: ~1.5e-4) with half the FP32 plane bytes. An
+ *         addition with no reference counterpart. */
 enum { VQM_PLANES_F32 = 0, VQM_PLANES_F16 = 1, VQM_PLANES_BF16 = 2, VQM_PLANES_U16 = 3 };
 
 /* The blur of the maps (vqm_ss2_config.blur, vqm_ss2_batch_config.blur):

@@ -38,13 +38,13 @@ def f32_dev_planes(plane_format: int, levels_mask: int = 0x3F) -> Mode:
 
 
 def f32_dev_for(plane_format: int) -> Mode:
-    """The device form of a library plane format: FP32 (0), FP16 (1) and BF16
-    (2) planes at every level; the mixed form (3) rounds levels 0 and 1 to
-    unsigned 16-bit fixed point and keeps FP32 above. FP16 is the reference's
-    half-plane build: its linear box averages are rounded to FP16 as well."""
+    """The device form of a library plane format: FP32 (0), FP16 (1), BF16 (2)
+    or unsigned 16-bit fixed point (3, the mixed form) planes at every level.
+    FP16 is the reference's half-plane build: its linear box averages are
+    rounded to FP16 as well."""
     if plane_format == 0:
         return F32_DEV
-    mode = f32_dev_planes(plane_format, 0x03 if plane_format == 3 else 0x3F)
+    mode = f32_dev_planes(plane_format, 0x3F)
     mode.round_linear = 1 if plane_format == 1 else 0
     return mode
 

@@ -36,8 +36,8 @@ static void check(uint32_t width, uint32_t height, PlaneFormat fmt, uint32_t str
     // element size per level).
     uint64_t prev_end = 0;
     for (uint32_t l = 0; l < p.levels; ++l) {
-        const uint32_t e = ElementBytes(fmt, l);
-        EXPECT(e == (fmt == kPlaneF32 || (fmt == kPlaneU16 && l >= kFineLevels) ? 4u : 2u), "element size");
+        const uint32_t e = ElementBytes(fmt);
+        EXPECT(e == (fmt == kPlaneF32 ? 4u : 2u), "element size");
         EXPECT(p.pitch[l] % 4 == 0, "pitch %u not a multiple of 4", p.pitch[l]);
         const uint32_t strips = (p.w[l] + kStripCols - 1) / kStripCols;
         EXPECT(p.pitch[l] >= strips * kStripCols + 14, "pitch too small at level %u", l);

@@ -146,8 +146,8 @@ def test_narrow_planes_match_oracle(oracle_lib, fmt):
 
 @pytest.mark.parametrize("strength", synth.STRENGTHS)
 def test_mixed_form_keeps_the_fp32_contract_at_1080p(oracle_lib, strength):
-    """The mixed form (unsigned 16-bit fixed-point planes at levels 0-1, FP32
-    above) is bit-exact with its oracle form and meets the same score and
+    """The mixed form (unsigned 16-bit fixed-point planes, FP32 arithmetic,
+    FP64 sums) is bit-exact with its oracle form and meets the same score and
     feature bounds against FP64 as the FP32 form."""
     a, b = synth.pair(1, 7, 1920, 1080, strength)
     n, got, res = check_against_oracle(oracle_lib, 1920, 1080, a, b, dev_mode=oracle_lib.f32_dev_for(3),

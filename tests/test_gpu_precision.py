@@ -1,5 +1,5 @@
 """Plane precision at 3840x2160: the FP32 form and the mixed form (unsigned
-16-bit fixed-point planes at levels 0-1, FP32 above) against the FP64 oracle
+16-bit fixed-point planes) against the FP64 oracle
 on 50 synthetic pairs (three strengths), with FP16 and BF16 planes reported
 beside them. Every pair of every form is scored in one batched launch.
 

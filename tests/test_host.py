@@ -132,12 +132,11 @@ def test_memory_plan_and_algorithmic_bytes():
     assert b32 / n == pytest.approx(32.94, abs=0.01)
     assert b16 / n == pytest.approx(19.47, abs=0.01)
     assert b32 == pytest.approx(68.30e6, rel=1e-3)
-    # The mixed form: 2-byte elements at levels 0-1, 4-byte above.
+    # The mixed form: 2-byte fixed-point elements at every level, like FP16.
     sizes = vqm_amd.plan_scales(Size(1920, 1080))
     stored = [sum(not vqm_amd.channel_pruned(c, l, len(sizes)) for c in range(3)) for l in range(len(sizes))]
-    mixed_planes = sum(k * s.width * s.height * (2 if l < 2 else 4) for l, (k, s) in enumerate(zip(stored, sizes)))
-    assert vqm_amd.algorithmic_bytes(Size(1920, 1080), L.PLANES_U16) == 6 * n + 4 * mixed_planes
-    assert b16 < 6 * n + 4 * mixed_planes < 0.65 * b32
+    mixed_planes = sum(k * s.width * s.height * 2 for k, s in zip(stored, sizes))
+    assert vqm_amd.algorithmic_bytes(Size(1920, 1080), L.PLANES_U16) == 6 * n + 4 * mixed_planes == b16
     m = vqm_amd.memory_plan(Ssimulacra2Config(size=Size(1920, 1080)))
     assert m.resident_bytes == 2 * 3 * n
     assert m.streaming_bytes == 2 * 3 * n

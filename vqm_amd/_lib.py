@@ -23,7 +23,7 @@ VQM_OVERFLOW = 4
 PLANES_F32 = 0
 PLANES_F16 = 1
 PLANES_BF16 = 2
-PLANES_U16 = 3  # unorm16 at levels 0-1, FP32 at levels 2-5 (the mixed form)
+PLANES_U16 = 3  # unsigned 16-bit fixed point at every level (the mixed form)
 
 u8, u32, u64, i32, f32, f64 = (ctypes.c_uint8, ctypes.c_uint32, ctypes.c_uint64,
                                ctypes.c_int32, ctypes.c_float, ctypes.c_double)

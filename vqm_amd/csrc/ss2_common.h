@@ -4,9 +4,8 @@
 //
 // HBM layout of one pair's workspace (one "slot"):
 //   planes   2 images x (sum over stored (level, channel) planes of
-//            pitch_l * rows_l elements). The element type is per level:
-//            FP32, FP16 or BF16 at every level, or unsigned 16-bit fixed
-//            point (q / 65536) at levels 0 and 1 with FP32 at levels 2..5.
+//            pitch_l * rows_l elements), every element FP32, FP16, BF16 or
+//            unsigned 16-bit fixed point (q / 65536).
 //            Every plane carries a zero border: element (x, y) of a level
 //            lives at (y + kPadTop) * pitch + (x + kPadLeft), and the border
 //            (never written after the workspace is cleared) is the zero
@@ -43,14 +42,8 @@ inline constexpr int kPrepTileH = 32;
 
 enum PlaneFormat : uint32_t { kPlaneF32 = 0, kPlaneF16 = 1, kPlaneBF16 = 2, kPlaneU16 = 3 };
 
-// Levels below this one store the format's fine element type; the U16 format
-// stores FP32 from here on.
-inline constexpr uint32_t kFineLevels = 2;
-
-// Bytes of one stored element of `level` in `format`.
-constexpr uint32_t ElementBytes(PlaneFormat format, uint32_t level) {
-    return format == kPlaneF32 ? 4u : format == kPlaneU16 ? (level < kFineLevels ? 2u : 4u) : 2u;
-}
+// Bytes of one stored plane element in `format`.
+constexpr uint32_t ElementBytes(PlaneFormat format) { return format == kPlaneF32 ? 4u : 2u; }
 
 struct ScaleItem {
     uint32_t level;

@@ -209,7 +209,7 @@ GeometryPlan BuildPlan(uint32_t width, uint32_t height, PlaneFormat format, uint
             p.mask[l] |= 1u << c;
             p.plane_off[l][c] = cursor;
             // 256-byte alignment keeps every row start and vector store aligned.
-            cursor = Align(cursor + CheckedMul(CheckedMul(p.pitch[l], p.rows[l]), ElementBytes(format, l)), 256);
+            cursor = Align(cursor + CheckedMul(CheckedMul(p.pitch[l], p.rows[l]), ElementBytes(format)), 256);
         }
     }
     p.image_bytes = cursor;
@@ -248,7 +248,7 @@ uint64_t AlgorithmicBytes(const GeometryPlan& plan) {
     for (uint32_t l = 0; l < p.levels; ++l)
         for (int c = 0; c < kChannels; ++c)
             if (p.mask[l] & (1u << c))
-                bytes += static_cast<uint64_t>(p.w[l]) * p.h[l] * ElementBytes(plan.format, l);
+                bytes += static_cast<uint64_t>(p.w[l]) * p.h[l] * ElementBytes(plan.format);
     const uint64_t input = 3ull * p.width * p.height;
     return 2 * input + 2 * (2 * bytes);
 }

@@ -195,24 +195,20 @@ using F16 = Narrow<to_f16, from_f16>;
 using BF16 = Narrow<to_bf16, from_bf16>;
 using U16 = Narrow<to_unorm16, from_unorm16>;
 
-// Element formats per pyramid level: Fine below kFineLevels, Coarse above.
-// RoundLinear: the linear box averages of levels >= 1 are rounded to the
-// element format too, before their XYB and before the next level averages
-// them -- the reference's half-plane build (prepare.comp:100-111,
-// ssimu2_common.glsl:84-96), which FP16 planes reproduce.
-template <typename FineT, typename CoarseT, bool RoundLinear = false>
+// The element format of every stored plane, and RoundLinear: the linear box
+// averages of levels >= 1 are rounded to the element format too, before
+// their XYB and before the next level averages them -- the reference's
+// half-plane build (prepare.comp:100-111, ssimu2_common.glsl:84-96), which
+// FP16 planes reproduce.
+template <typename FmtT, bool RoundLinear = false>
 struct Layout {
-    using Fine = FineT;
-    using Coarse = CoarseT;
-    template <uint32_t L>
-    using At = std::conditional_t<(L < kFineLevels), Fine, Coarse>;
-    static constexpr bool kUniform = std::is_same_v<Fine, Coarse>;
+    using Fmt = FmtT;
     static constexpr bool kRoundLinear = RoundLinear;
 };
-using LayoutF32 = Layout<F32, F32>;
-using LayoutF16 = Layout<F16, F16, true>;
-using LayoutBF16 = Layout<BF16, BF16>;
-using LayoutU16 = Layout<U16, F32>;
+using LayoutF32 = Layout<F32>;
+using LayoutF16 = Layout<F16, true>;
+using LayoutBF16 = Layout<BF16>;
+using LayoutU16 = Layout<U16>;
 
 // A level's linear box average as the layout hands it on.
 template <typename Lay>
@@ -450,7 +446,7 @@ template <typename Lay, uint32_t L>
 __device__ __forceinline__ void prepare_level(ConstAS<Plan>* plan, GlobalAS<uint8_t>* planes,
                                               uint32_t X0, uint32_t Y0, uint32_t tid,
                                               float (*ladder)[3][kLadder]) {
-    using Fmt = typename Lay::template At<L>;
+    using Fmt = typename Lay::Fmt;
     // Level L from the ladder level L-1 published, with the clamp of level
     // L-1's last row and column applied in this tile's coordinates
     // (prepare.comp:146-168).
@@ -586,7 +582,7 @@ __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict
 #pragma unroll
             for (uint32_t c = 0; c < 3; ++c) {
                 if (!(mask0 & (1u << c))) continue;
-                using Fmt = typename Lay::template At<0>;
+                using Fmt = typename Lay::Fmt;
                 auto* p = plane_at<Fmt>(planes, plan->plane_off[0][c]) + at;
                 if (bx + 3u < W) {
                     const float q[4] = {v[0][c], v[1][c], v[2][c], v[3][c]};
@@ -629,7 +625,7 @@ __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict
 #pragma unroll
             for (uint32_t c = 0; c < 3; ++c) {
                 if (!(mask1 & (1u << c))) continue;
-                using Fmt = typename Lay::template At<1>;
+                using Fmt = typename Lay::Fmt;
                 auto* p = plane_at<Fmt>(planes, plan->plane_off[1][c]) + at;
                 if (gx + 1u < W1) {
                     Fmt::store2(p, x0[c], x1[c]);
@@ -838,12 +834,7 @@ __global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ 
     ConstAS<ScaleItem>* item = as_const(pdp->items) + item_idx;
     const uint32_t level = item->level, channel = item->channel;
     const uint32_t x0 = item->x0, y0 = item->y0;
-    // The item's level picks the element format (wave-uniform).
-    if (Lay::kUniform || level < kFineLevels)
-        scale_item<typename Lay::Fine, TapsT>(pdp, plan, item_idx, level, channel, x0, y0, ring_a, ring_b, prod);
-    else
-        scale_item<typename Lay::Coarse, TapsT>(pdp, plan, item_idx, level, channel, x0, y0, ring_a, ring_b,
-                                                prod);
+    scale_item<typename Lay::Fmt, TapsT>(pdp, plan, item_idx, level, channel, x0, y0, ring_a, ring_b, prod);
 }
 
 // ---- reduce -----------------------------------------------------------------
