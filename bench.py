@@ -191,11 +191,12 @@ def ladder_batch(args, pool):
     return out
 
 
-def resident_batch(refs, w, h, fmt, steps, warmup, chunks=1, strip_rows=0):
+def resident_batch(refs, w, h, fmt, steps, warmup, chunks=1, strip_rows=0, persistent=False):
     """Pairs/s of the batch engine over resident pairs, with the per-stage
     HIP-event times and the scale kernel's HBM fraction."""
     from vqm_amd import Ssimulacra2Batch
-    batch = Ssimulacra2Batch(max_pairs=len(refs), plane_format=fmt, strip_rows=strip_rows, chunks=chunks)
+    batch = Ssimulacra2Batch(max_pairs=len(refs), plane_format=fmt, strip_rows=strip_rows, chunks=chunks,
+                             persistent=persistent)
     for _ in range(warmup):
         batch.Finish(batch.Submit(refs), raw=True)
     stages = []
@@ -245,6 +246,8 @@ def main() -> int:
                     help="output rows per scale work item of the batch (0 = the library default)")
     ap.add_argument("--chunks", type=int, default=1,
                     help="runs of pairs per batch whose scale overlaps the next run's prepare")
+    ap.add_argument("--persistent", action="store_true",
+                    help="size the kernels' grids to the resident occupancy (blocks loop over the work)")
     ap.add_argument("--single-strip-rows", type=int, default=0,
                     help="output rows per scale work item of the single-pair modes (0 = the library default)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
@@ -305,7 +308,7 @@ def main() -> int:
         dst_buf.free()
 
     batch = Ssimulacra2Batch(max_pairs=len(refs), plane_format=fmt, strip_rows=args.strip_rows,
-                             chunks=args.chunks)
+                             chunks=args.chunks, persistent=args.persistent)
     for _ in range(args.warmup):
         scales, norms = batch.Finish(batch.Submit(refs), raw=True)
     step_ms, stages = [], []
@@ -371,7 +374,8 @@ def main() -> int:
         forms = {}
         for name, f in sorted(PLANES.items()):
             if f != fmt and name in ("f32", "u16"):
-                forms[name] = resident_batch(refs, w, h, f, max(3, args.steps // 4), 1, args.chunks)
+                forms[name] = resident_batch(refs, w, h, f, max(3, args.steps // 4), 1, args.chunks,
+                                             persistent=args.persistent)
         out["plane_forms"] = forms
         log(f"plane forms: {forms}")
         if args.workload == "1080p":
@@ -386,7 +390,8 @@ def main() -> int:
                 refs4.append((dev4.ptr + 2 * i * slot4, dev4.ptr + (2 * i + 1) * slot4, w4, h4))
             out["workload_4k"] = {"workload": f"{w4}x{h4} 8-bit sRGB RGB, {len(refs4)} resident pairs per step "
                                               f"(3 strengths x 10), {args.planes} planes",
-                                  **resident_batch(refs4, w4, h4, fmt, max(3, args.steps // 4), 1, args.chunks)}
+                                  **resident_batch(refs4, w4, h4, fmt, max(3, args.steps // 4), 1, args.chunks,
+                                                   persistent=args.persistent)}
             dev4.free()
             del pairs4
             log(f"4k: {out['workload_4k']}")

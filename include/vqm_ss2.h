@@ -304,6 +304,11 @@ typedef struct vqm_ss2_batch_config {
     uint32_t autotune;       /* 1: strip_rows and chunks come from vqm_ss2_autotune
                                 for each geometry the first time it is seen */
     uint32_t blur;           /* VQM_BLUR_SAMPLED or VQM_BLUR_RECURSIVE */
+    uint32_t persistent;     /* 1: each kernel launches only the blocks that stay
+                                resident at once (occupancy x compute units), each
+                                looping over the batch's tiles and strips in a
+                                fixed stride; 0: one block per tile or strip */
+    uint32_t reserved;       /* 0 */
 } vqm_ss2_batch_config;
 
 /* One pair of caller-owned device images (packed u8 sRGB RGB). */

@@ -115,3 +115,23 @@ def test_autotune_picks_a_shape_and_keeps_results():
     assert np.array_equal(a, b)
     with pytest.raises(InvalidArgument):
         vqm_amd.autotune(Size(4, 4))
+
+
+@pytest.mark.parametrize("fmt", [vqm_amd.PLANES_F32, vqm_amd.PLANES_U16])
+def test_persistent_grid_matches_one_block_per_item(fmt):
+    """A persistent launch (blocks = resident occupancy, each looping over
+    tiles and strips) gives the same norms bit for bit as one block per work
+    item, for a uniform batch larger than the resident grid and for a mixed
+    one; chunked persistent launches too."""
+    for shapes in ([(1920, 1080)] * 6, [(64, 64), (1280, 720), (33, 17), (640, 360), (8, 8), (1000, 37)]):
+        pairs, keep, host = upload_pairs(shapes, seed=31)
+        plain = Ssimulacra2Batch(max_pairs=8, plane_format=fmt)
+        _, want = plain.Finish(plain.Submit(pairs), raw=True)
+        plain.close()
+        for chunks in (1, 3):
+            pers = Ssimulacra2Batch(max_pairs=8, plane_format=fmt, persistent=True, chunks=chunks)
+            _, got = pers.Finish(pers.Submit(pairs), raw=True)
+            _, again = pers.Finish(pers.Submit(pairs), raw=True)
+            pers.close()
+            assert np.array_equal(got, want), (shapes[0], chunks)
+            assert np.array_equal(again, want)

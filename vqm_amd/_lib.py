@@ -60,7 +60,8 @@ class MemoryC(ctypes.Structure):
 
 class BatchConfigC(ctypes.Structure):
     _fields_ = [("plane_format", u32), ("strip_rows", u32), ("max_pairs", u32),
-                ("chunks", u32), ("hbm_budget", u64), ("autotune", u32), ("blur", u32)]
+                ("chunks", u32), ("hbm_budget", u64), ("autotune", u32), ("blur", u32),
+                ("persistent", u32), ("reserved", u32)]
 
 
 class TuneResultC(ctypes.Structure):

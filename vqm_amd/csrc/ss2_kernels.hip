@@ -485,27 +485,24 @@ __device__ __forceinline__ void prepare_level(ConstAS<Plan>* plan, GlobalAS<uint
     for (uint32_t c = 0; c < 3; ++c) next[c][jy * side_x + jx] = v[c];
 }
 
+// One prepare work item: tile `b` of the launch's work (a 64 x 32 base-pixel
+// tile of one image of one pair). Every thread of the block calls it with the
+// same `b`; the LUT in LDS is already loaded.
 template <typename Lay, bool General>
-__global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict__ pairs,
-                                                      const WorkRef* __restrict__ work,
-                                                      uint32_t blocks, uint32_t tiles_per_image,
-                                                      const float* __restrict__ lut_g,
-                                                      const uint8_t* src0, const uint8_t* src1) {
-    __shared__ float lut[256];
-    // Two ladders alternate between levels: level l publishes into l & 1.
-    __shared__ __align__(16) float ladder[2][3][kLadder];
-
+__device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restrict__ pairs,
+                                             const WorkRef* __restrict__ work, uint32_t blocks,
+                                             uint32_t tiles_per_image, const float* lut,
+                                             float (*ladder)[3][kLadder], const uint8_t* src0,
+                                             const uint8_t* src1) {
     const uint32_t tid = threadIdx.x;
-    lut[tid] = as_global(lut_g)[tid];
-
     uint32_t pair_idx, image, tile;
     if (work != nullptr) {
-        ConstAS<WorkRef>* w = as_const(work) + blockIdx.x;
+        ConstAS<WorkRef>* w = as_const(work) + b;
         pair_idx = w->pair;
         image = w->index >> 31;
         tile = w->index & 0x7FFFFFFFu;
     } else {
-        const uint32_t l = xcd_remap(blockIdx.x, blocks);
+        const uint32_t l = xcd_remap(b, blocks);
         tile = l % tiles_per_image;
         image = (l / tiles_per_image) & 1u;
         pair_idx = l / (2 * tiles_per_image);
@@ -523,7 +520,6 @@ __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict
                                    : (src0 != nullptr ? src0 : pdp->src[0]);
     GlobalAS<const uint8_t>* src = as_global(src_ptr);
     const uint64_t plane_stride = plan->plane_stride;
-    __syncthreads();
 
     // Level 0: this thread's 4 x 2 base pixels, coordinates clamped to the
     // image so the level-1 box below sees the clamped samples
@@ -641,6 +637,27 @@ __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict
     if (levels > 3) { __syncthreads(); prepare_level<Lay, 3>(plan, planes, X0, Y0, tid, ladder); }
     if (levels > 4) { __syncthreads(); prepare_level<Lay, 4>(plan, planes, X0, Y0, tid, ladder); }
     if (levels > 5) { __syncthreads(); prepare_level<Lay, 5>(plan, planes, X0, Y0, tid, ladder); }
+}
+
+// Work items blockIdx.x, blockIdx.x + gridDim.x, ... of `blocks`: one each
+// when the grid covers the work, several per block for a persistent grid
+// sized to the resident occupancy. The grid is a multiple of 8, so item b
+// stays on XCD b % 8 and xcd_remap keeps neighbouring tiles on one L2.
+template <typename Lay, bool General>
+__global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict__ pairs,
+                                                      const WorkRef* __restrict__ work,
+                                                      uint32_t blocks, uint32_t tiles_per_image,
+                                                      const float* __restrict__ lut_g,
+                                                      const uint8_t* src0, const uint8_t* src1) {
+    __shared__ float lut[256];
+    // Two ladders alternate between levels: level l publishes into l & 1.
+    __shared__ __align__(16) float ladder[2][3][kLadder];
+    lut[threadIdx.x] = as_global(lut_g)[threadIdx.x];
+    __syncthreads();
+    for (uint32_t b = blockIdx.x; b < blocks; b += gridDim.x) {
+        prepare_tile<Lay, General>(b, pairs, work, blocks, tiles_per_image, lut, ladder, src0, src1);
+        __syncthreads();  // the next tile reuses the ladder
+    }
 }
 
 // ---- scale ------------------------------------------------------------------
@@ -811,6 +828,8 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
     }
 }
 
+// Work items blockIdx.x, blockIdx.x + gridDim.x, ... of `blocks`, as in
+// prepare_kernel.
 template <typename Lay, typename TapsT>
 __global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ pairs,
                                                    const WorkRef* __restrict__ work,
@@ -819,22 +838,25 @@ __global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ 
     __shared__ __align__(16) float ring_b[kRing][kSrcCols];
     __shared__ __align__(16) float prod[2][kSrcCols];
 
-    uint32_t pair_idx, item_idx;
-    if (work != nullptr) {
-        ConstAS<WorkRef>* w = as_const(work) + blockIdx.x;
-        pair_idx = w->pair;
-        item_idx = w->index;
-    } else {
-        const uint32_t l = xcd_remap(blockIdx.x, blocks);
-        pair_idx = l / items_per_pair;
-        item_idx = l % items_per_pair;
+    for (uint32_t b = blockIdx.x; b < blocks; b += gridDim.x) {
+        uint32_t pair_idx, item_idx;
+        if (work != nullptr) {
+            ConstAS<WorkRef>* w = as_const(work) + b;
+            pair_idx = w->pair;
+            item_idx = w->index;
+        } else {
+            const uint32_t l = xcd_remap(b, blocks);
+            pair_idx = l / items_per_pair;
+            item_idx = l % items_per_pair;
+        }
+        ConstAS<PairDesc>* pdp = as_const(pairs) + pair_idx;
+        ConstAS<Plan>* plan = as_const(pdp->plan);
+        ConstAS<ScaleItem>* item = as_const(pdp->items) + item_idx;
+        const uint32_t level = item->level, channel = item->channel;
+        const uint32_t x0 = item->x0, y0 = item->y0;
+        scale_item<typename Lay::Fmt, TapsT>(pdp, plan, item_idx, level, channel, x0, y0, ring_a, ring_b, prod);
+        __syncthreads();  // the next item reuses the row ring
     }
-    ConstAS<PairDesc>* pdp = as_const(pairs) + pair_idx;
-    ConstAS<Plan>* plan = as_const(pdp->plan);
-    ConstAS<ScaleItem>* item = as_const(pdp->items) + item_idx;
-    const uint32_t level = item->level, channel = item->channel;
-    const uint32_t x0 = item->x0, y0 = item->y0;
-    scale_item<typename Lay::Fmt, TapsT>(pdp, plan, item_idx, level, channel, x0, y0, ring_a, ring_b, prod);
 }
 
 // ---- reduce -----------------------------------------------------------------
@@ -872,29 +894,59 @@ __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src
         dst[i] = src[i];
 }
 
+// Blocks of `kernel` the device holds resident at once: occupancy per
+// compute unit x compute units, rounded down to a multiple of 8 (at least
+// 8). A persistent launch uses min(work items, this) blocks, each looping
+// over its share; 0 when the runtime cannot say.
+uint32_t ResidentGrid(const void* kernel, int block_threads) {
+    int per_cu = 0, device = 0;
+    hipDeviceProp_t prop{};
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block_threads, 0) != hipSuccess ||
+        hipGetDevice(&device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
+        per_cu <= 0 || prop.multiProcessorCount <= 0)
+        return 0;
+    const uint64_t resident = static_cast<uint64_t>(per_cu) * static_cast<uint64_t>(prop.multiProcessorCount);
+    return static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(8, resident / 8 * 8), 0x7FFFFFF8u));
+}
+
+template <typename K>
+uint32_t GridOf(K kernel, int block_threads, uint32_t blocks, bool persistent) {
+    if (!persistent) return blocks;
+    static const uint32_t resident = ResidentGrid(reinterpret_cast<const void*>(kernel), block_threads);
+    return resident == 0 ? blocks : std::min(blocks, resident);
+}
+
+template <typename Lay, bool General>
+hipError_t launch_prepare_g(const PairDesc* pairs, const WorkRef* work, uint32_t blocks, uint32_t tiles,
+                            const float* lut, const uint8_t* src0, const uint8_t* src1, bool persistent,
+                            hipStream_t stream) {
+    const uint32_t grid = GridOf(&prepare_kernel<Lay, General>, 256, blocks, persistent);
+    hipLaunchKernelGGL((prepare_kernel<Lay, General>), dim3(grid), dim3(256), 0, stream, pairs, work, blocks,
+                       tiles, lut, src0, src1);
+    return hipGetLastError();
+}
+
 template <typename Lay>
 hipError_t launch_prepare_t(bool general, const PairDesc* pairs, const WorkRef* work, uint32_t blocks,
                             uint32_t tiles, const float* lut, const uint8_t* src0,
-                            const uint8_t* src1, hipStream_t stream) {
-    if (general)
-        hipLaunchKernelGGL((prepare_kernel<Lay, true>), dim3(blocks), dim3(256), 0, stream, pairs, work,
-                           blocks, tiles, lut, src0, src1);
-    else
-        hipLaunchKernelGGL((prepare_kernel<Lay, false>), dim3(blocks), dim3(256), 0, stream, pairs, work,
-                           blocks, tiles, lut, src0, src1);
+                            const uint8_t* src1, bool persistent, hipStream_t stream) {
+    return general ? launch_prepare_g<Lay, true>(pairs, work, blocks, tiles, lut, src0, src1, persistent, stream)
+                   : launch_prepare_g<Lay, false>(pairs, work, blocks, tiles, lut, src0, src1, persistent, stream);
+}
+
+template <typename Lay, typename TapsT>
+hipError_t launch_scale_g(const PairDesc* pairs, const WorkRef* work, uint32_t blocks, uint32_t items,
+                          bool persistent, hipStream_t stream) {
+    const uint32_t grid = GridOf(&scale_kernel<Lay, TapsT>, 64, blocks, persistent);
+    hipLaunchKernelGGL((scale_kernel<Lay, TapsT>), dim3(grid), dim3(64), 0, stream, pairs, work, blocks, items);
     return hipGetLastError();
 }
 
 template <typename Lay>
 hipError_t launch_scale_t(uint32_t blur, const PairDesc* pairs, const WorkRef* work, uint32_t blocks,
-                          uint32_t items, hipStream_t stream) {
-    if (blur == kBlurRecursive)
-        hipLaunchKernelGGL((scale_kernel<Lay, CosineTaps>), dim3(blocks), dim3(64), 0, stream, pairs, work,
-                           blocks, items);
-    else
-        hipLaunchKernelGGL((scale_kernel<Lay, SampledTaps>), dim3(blocks), dim3(64), 0, stream, pairs, work,
-                           blocks, items);
-    return hipGetLastError();
+                          uint32_t items, bool persistent, hipStream_t stream) {
+    return blur == kBlurRecursive ? launch_scale_g<Lay, CosineTaps>(pairs, work, blocks, items, persistent, stream)
+                                  : launch_scale_g<Lay, SampledTaps>(pairs, work, blocks, items, persistent, stream);
 }
 
 }  // namespace
@@ -910,30 +962,32 @@ hipError_t LaunchCopy(void* dst, const void* src, uint64_t bytes, hipStream_t st
 
 hipError_t LaunchPrepare(PlaneFormat format, bool general, const PairDesc* pairs, const WorkRef* work,
                          uint32_t blocks, uint32_t tiles_per_image, const float* lut,
-                         const uint8_t* src0, const uint8_t* src1, hipStream_t stream) {
+                         const uint8_t* src0, const uint8_t* src1, hipStream_t stream, bool persistent) {
     if (blocks == 0) return hipSuccess;
+    const uint32_t t = tiles_per_image;
     switch (format) {
         case kPlaneF32:
-            return launch_prepare_t<LayoutF32>(general, pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
+            return launch_prepare_t<LayoutF32>(general, pairs, work, blocks, t, lut, src0, src1, persistent, stream);
         case kPlaneF16:
-            return launch_prepare_t<LayoutF16>(general, pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
+            return launch_prepare_t<LayoutF16>(general, pairs, work, blocks, t, lut, src0, src1, persistent, stream);
         case kPlaneBF16:
-            return launch_prepare_t<LayoutBF16>(general, pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
+            return launch_prepare_t<LayoutBF16>(general, pairs, work, blocks, t, lut, src0, src1, persistent, stream);
         case kPlaneU16:
-            return launch_prepare_t<LayoutU16>(general, pairs, work, blocks, tiles_per_image, lut, src0, src1, stream);
+            return launch_prepare_t<LayoutU16>(general, pairs, work, blocks, t, lut, src0, src1, persistent, stream);
     }
     return hipErrorInvalidValue;
 }
 
 hipError_t LaunchScale(PlaneFormat format, uint32_t blur, const PairDesc* pairs, const WorkRef* work,
-                       uint32_t blocks, uint32_t items_per_pair, hipStream_t stream) {
+                       uint32_t blocks, uint32_t items_per_pair, hipStream_t stream, bool persistent) {
     if (blocks == 0) return hipSuccess;
     if (blur > kBlurRecursive) return hipErrorInvalidValue;
+    const uint32_t n = items_per_pair;
     switch (format) {
-        case kPlaneF32: return launch_scale_t<LayoutF32>(blur, pairs, work, blocks, items_per_pair, stream);
-        case kPlaneF16: return launch_scale_t<LayoutF16>(blur, pairs, work, blocks, items_per_pair, stream);
-        case kPlaneBF16: return launch_scale_t<LayoutBF16>(blur, pairs, work, blocks, items_per_pair, stream);
-        case kPlaneU16: return launch_scale_t<LayoutU16>(blur, pairs, work, blocks, items_per_pair, stream);
+        case kPlaneF32: return launch_scale_t<LayoutF32>(blur, pairs, work, blocks, n, persistent, stream);
+        case kPlaneF16: return launch_scale_t<LayoutF16>(blur, pairs, work, blocks, n, persistent, stream);
+        case kPlaneBF16: return launch_scale_t<LayoutBF16>(blur, pairs, work, blocks, n, persistent, stream);
+        case kPlaneU16: return launch_scale_t<LayoutU16>(blur, pairs, work, blocks, n, persistent, stream);
     }
     return hipErrorInvalidValue;
 }

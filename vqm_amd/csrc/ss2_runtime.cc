@@ -270,7 +270,7 @@ Lane::~Lane() {
 }
 
 void Lane::Launch(PlaneFormat fmt, const float* lut, const LaunchEntry* entries, uint32_t count,
-                  uint32_t chunks, uint32_t blur) {
+                  uint32_t chunks, uint32_t blur, bool persistent) {
     if (pending_) Fail(VQM_LOGIC_ERROR, "lane: the previous launch has not finished");
     if (count == 0 || count > max_pairs_) Fail(VQM_INVALID_ARGUMENT, "lane: bad pair count");
     chunks = std::max(1u, std::min({chunks, count, kMaxChunks}));
@@ -349,7 +349,8 @@ void Lane::Launch(PlaneFormat fmt, const float* lut, const LaunchEntry* entries,
         const WorkRef* wp = uniform ? nullptr : wprep + prep_at[i0];
         const auto np = static_cast<uint32_t>(prep_at[i1] - prep_at[i0]);
         Check(hipEventRecord(Ev(4 * c).get(), s), "hipEventRecord");
-        Check(LaunchPrepare(fmt, general, base, wp, np, g0.prep_tiles, lut, nullptr, nullptr, s), "launch prepare");
+        Check(LaunchPrepare(fmt, general, base, wp, np, g0.prep_tiles, lut, nullptr, nullptr, s, persistent),
+              "launch prepare");
         Check(hipEventRecord(Ev(4 * c + 1).get(), s), "hipEventRecord");
     }
     for (uint32_t c = 0; c < chunks; ++c) {
@@ -360,7 +361,7 @@ void Lane::Launch(PlaneFormat fmt, const float* lut, const LaunchEntry* entries,
         const auto ns = static_cast<uint32_t>(scale_at[i1] - scale_at[i0]);
         if (t != s) Check(hipStreamWaitEvent(t, Ev(4 * c + 1).get(), 0), "order scale after prepare");
         Check(hipEventRecord(Ev(4 * c + 2).get(), t), "hipEventRecord");
-        Check(LaunchScale(fmt, blur, base, ws, ns, g0.plan.items, t), "launch scale");
+        Check(LaunchScale(fmt, blur, base, ws, ns, g0.plan.items, t, persistent), "launch scale");
         Check(hipEventRecord(Ev(4 * c + 3).get(), t), "hipEventRecord");
     }
     const uint32_t r = 4 * chunks;
@@ -767,6 +768,7 @@ const vqm_ss2_batch_config& ValidatedBatch(const vqm_ss2_batch_config& config) {
     if (config.max_pairs == 0) Fail(VQM_INVALID_ARGUMENT, "max_pairs must be at least 1");
     if (config.chunks > Lane::kMaxChunks) Fail(VQM_INVALID_ARGUMENT, "chunks must be at most 16");
     if (config.blur > kBlurRecursive) Fail(VQM_INVALID_ARGUMENT, "blur must be sampled (0) or recursive (1)");
+    if (config.persistent > 1) Fail(VQM_INVALID_ARGUMENT, "persistent must be 0 or 1");
     return config;
 }
 }  // namespace
@@ -822,7 +824,7 @@ uint64_t Batch::Submit(const vqm_ss2_device_pair* pairs, uint32_t count) {
             taken.push_back(Taken{pairs[i].width, pairs[i].height, std::move(ws)});
         }
         lane_.Launch(static_cast<PlaneFormat>(config_.plane_format), ctx_.lut, entries.data(), count,
-                     chunks, config_.blur);
+                     chunks, config_.blur, config_.persistent != 0);
     } catch (...) {
         for (Taken& t : taken) pools_.Return(t.width, t.height, false, std::move(t.ws));
         throw;

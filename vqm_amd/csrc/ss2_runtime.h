@@ -230,9 +230,10 @@ public:
     // one another on the lane's first stream, and each run's scale launch
     // follows on a second stream as soon as its prepare is done, so the
     // scale of one run overlaps the prepare of the next.
-    // `blur` is the scale kernel's BlurKind.
+    // `blur` is the scale kernel's BlurKind; `persistent` sizes both kernels'
+    // grids to the resident occupancy (LaunchPrepare, LaunchScale).
     void Launch(PlaneFormat format, const float* lut, const LaunchEntry* entries, uint32_t count,
-                uint32_t chunks = 1, uint32_t blur = kBlurSampled);
+                uint32_t chunks = 1, uint32_t blur = kBlurSampled, bool persistent = false);
     bool Done() const;
     void Wait();
     // Forgets a launch that failed part-way (its streams already synchronized).

@@ -676,9 +676,11 @@ class Ssimulacra2Batch:
     Pairs are device-resident (DeviceImage pointers)."""
 
     def __init__(self, max_pairs: int, plane_format: int = L.PLANES_F32, hbm_budget: int = 0,
-                 strip_rows: int = 0, chunks: int = 1, autotune: bool = False, blur: int = 0):
+                 strip_rows: int = 0, chunks: int = 1, autotune: bool = False, blur: int = 0,
+                 persistent: bool = False):
         cfg = L.BatchConfigC()
         cfg.blur = int(blur)
+        cfg.persistent = 1 if persistent else 0
         cfg.plane_format = plane_format
         cfg.strip_rows = strip_rows
         cfg.chunks = chunks
