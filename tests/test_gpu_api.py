@@ -103,10 +103,18 @@ def test_input_validation():
     bad.subsample_w = 1
     with pytest.raises(InvalidArgument):
         p.Compute(Image(S, bad, orig), img(orig))
-    for fmt in (ImageFormat(), ImageFormat(type=0, alpha=1), ImageFormat(type=0, transfer=16),
-                ImageFormat(type=0, range=0), ImageFormat(type=0, family=2, matrix=1)):
+    # Formats the reference refuses (input_format_chroma_test.cc:430-510).
+    for fmt in (ImageFormat(family=2, matrix=0), ImageFormat(type=0, depth=12),
+                ImageFormat(type=0, depth=7, range=0), ImageFormat(family=0, subsample_h=1, matrix=1),
+                ImageFormat(transfer=2), ImageFormat(family=2, matrix=10, primaries=1)):
         with pytest.raises(InvalidArgument):
             p.Upload(Image(S, fmt, np.zeros(vqm_amd.required_bytes(fmt, S), np.uint8)))
+    # Formats it accepts upload, each slot growing to its image.
+    slots = []
+    for fmt in (ImageFormat(), ImageFormat(type=0, alpha=1), ImageFormat(type=0, transfer=16),
+                ImageFormat(type=0, range=0), ImageFormat(type=0, family=2, matrix=1)):
+        slots.append(p.Upload(Image(S, fmt, np.zeros(vqm_amd.required_bytes(fmt, S), np.uint8))))
+        p.Release(slots[-1])
     # Nothing in flight after the failures: a valid pair still scores.
     assert p.Compute(img(orig), img(orig)).Score() == 100.0
 

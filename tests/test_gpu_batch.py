@@ -99,7 +99,8 @@ def test_chunked_batch_equals_each_pair_alone(chunks):
 
 def test_autotune_picks_a_shape_and_keeps_results():
     """The tuner times every (strip height, chunks) shape and caches the
-    fastest; launch shapes never change a result bit."""
+    fastest; a tuned shape keeps every pixel's arithmetic, so the norms agree
+    to the order of the FP64 sums over strips (relative 1e-10)."""
     first = vqm_amd.autotune(Size(320, 180), vqm_amd.PLANES_F32, pairs=6)
     assert first["strip_rows"] in (16, 32, 64, 128) and first["chunks"] in (1, 2, 4, 8)
     assert first["candidates"] == 4 * 4 and not first["cached"]
@@ -112,7 +113,7 @@ def test_autotune_picks_a_shape_and_keeps_results():
     tuned = Ssimulacra2Batch(max_pairs=8, autotune=True)
     _, a = plain.Finish(plain.Submit(pairs), raw=True)
     _, b = tuned.Finish(tuned.Submit(pairs), raw=True)
-    assert np.array_equal(a, b)
+    np.testing.assert_allclose(b, a, rtol=1e-10, atol=0)
     with pytest.raises(InvalidArgument):
         vqm_amd.autotune(Size(4, 4))
 

@@ -487,7 +487,7 @@ __device__ __forceinline__ void prepare_level(ConstAS<Plan>* plan, GlobalAS<uint
 
 // One prepare work item: tile `b` of the launch's work (a 64 x 32 base-pixel
 // tile of one image of one pair). Every thread of the block calls it with the
-// same `b`; the LUT in LDS is already loaded.
+// same `b`; the LUT's LDS stores are issued before the first call.
 template <typename Lay, bool General>
 __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restrict__ pairs,
                                              const WorkRef* __restrict__ work, uint32_t blocks,
@@ -520,6 +520,9 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
                                    : (src0 != nullptr ? src0 : pdp->src[0]);
     GlobalAS<const uint8_t>* src = as_global(src_ptr);
     const uint64_t plane_stride = plan->plane_stride;
+    // The decode table's load (issued before the first tile) overlaps the
+    // descriptor loads above; this barrier publishes it.
+    __syncthreads();
 
     // Level 0: this thread's 4 x 2 base pixels, coordinates clamped to the
     // image so the level-1 box below sees the clamped samples
@@ -652,8 +655,7 @@ __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict
     __shared__ float lut[256];
     // Two ladders alternate between levels: level l publishes into l & 1.
     __shared__ __align__(16) float ladder[2][3][kLadder];
-    lut[threadIdx.x] = as_global(lut_g)[threadIdx.x];
-    __syncthreads();
+    lut[threadIdx.x] = as_global(lut_g)[threadIdx.x];  // published inside prepare_tile
     for (uint32_t b = blockIdx.x; b < blocks; b += gridDim.x) {
         prepare_tile<Lay, General>(b, pairs, work, blocks, tiles_per_image, lut, ladder, src0, src1);
         __syncthreads();  // the next tile reuses the ladder
