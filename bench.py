@@ -258,7 +258,7 @@ def main() -> int:
     ap.add_argument("--serve-budget-gib", type=int, default=8)
     ap.add_argument("--ladder-variants", type=int, default=2,
                     help="distinct pairs per ladder size and strength")
-    ap.add_argument("--cpu-sample", type=int, default=48)
+    ap.add_argument("--cpu-sample", type=int, default=160)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-modes", action="store_true")
     ap.add_argument("--no-secondary", action="store_true",
