@@ -87,7 +87,7 @@ def decode(transfer: int, v: float) -> float:
         m1, m2 = 2610.0 / 16384.0, 2523.0 / 4096.0 * 128.0
         c1, c2, c3 = 3424.0 / 4096.0, 2413.0 / 4096.0 * 32.0, 2392.0 / 4096.0 * 32.0
         p = v ** (1.0 / m2)
-        return 100.0 * (max(p - c1, 0.0) / (c2 - c3 * p)) ** (1.0 / m1)
+        return 100.0 * (max(p - c1, 0.0) / max(c2 - c3 * p, 1.17549435e-38)) ** (1.0 / m1)
     if t == TC.St428:
         return 0.0 if v < 0.0 else 52.37 / 48.0 * v ** 2.6
     if t == TC.AribB67:

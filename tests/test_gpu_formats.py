@@ -410,3 +410,37 @@ def test_mixed_table_and_general_pair_and_back():
     assert max_norm_delta(mixed, u8) <= ROUNDING
     again = p.Compute(Image(S, ImageFormat.srgb8(), orig), Image(S, ImageFormat.srgb8(), dist))
     assert np.array_equal(again.norms(), u8.norms())
+
+
+def test_hdr_yuv420_at_360p():
+    """A 640x360 10-bit limited-range BT.2020 PQ 4:2:0 frame (Lanczos chroma,
+    top-left siting) against its FP64 decode in linear BT.709: the A/B bar of
+    the PQ curve at a real frame size, on structured content."""
+    w, h = 640, 360
+    size = Size(w, h)
+    y, x = np.mgrid[0:h, 0:w]
+    luma = 0.3 + 0.15 * np.sin(x / 23.0) * np.cos(y / 17.0) + 0.08 * ((x // 40 + y // 30) % 2)
+    cy, cx = np.mgrid[0:h // 2, 0:w // 2]
+    cb = 0.08 * np.sin(cx / 11.0 + cy / 29.0)
+    cr = -0.06 * np.cos(cx / 13.0 - cy / 7.0)
+    fmt = ImageFormat(type=PixelType.Word, depth=10, range=PixelRange.Limited, family=ColorFamily.Yuv,
+                      matrix=MC.Bt2020Ncl, transfer=TC.St2084, primaries=CP.Bt2020, subsample_w=1,
+                      subsample_h=1, chroma_location=ChromaLocation.TopLeft)
+    a = R.pack(fmt, size, {0: luma, 1: cb, 2: cr})
+    # The FP64 decode of the packed codes: codes -> normalised, upsample,
+    # matrix, PQ, gamut.
+    raw = np.frombuffer(a.data.tobytes(), np.uint16).astype(np.float64)
+    n_l, n_c = w * h, (w // 2) * (h // 2)
+    stride_l = (n_l * 2 + 3) // 4 * 4 // 2
+    stride_c = (n_c * 2 + 3) // 4 * 4 // 2
+    yl = (raw[:n_l].reshape(h, w) - 64) / 876.0
+    u = (raw[stride_l:stride_l + n_c].reshape(h // 2, w // 2) - 512) / 896.0
+    v = (raw[stride_l + stride_c:stride_l + stride_c + n_c].reshape(h // 2, w // 2) - 512) / 896.0
+    uu = R.upsample(u, 1, 1, ChromaFilter.Lanczos, ChromaLocation.TopLeft, w, h)
+    vv = R.upsample(v, 1, 1, ChromaFilter.Lanczos, ChromaLocation.TopLeft, w, h)
+    rgb = np.einsum("ij,jyx->iyx", R.ncl_to_rgb(MC.Bt2020Ncl), np.stack([yl, uu, vv]))
+    lin = np.vectorize(lambda t: R.decode(TC.St2084, t))(rgb)
+    out = np.einsum("ij,jyx->iyx", R.gamut(CP.Bt2020), lin)
+    b = R.pack(R.linear_rgb_format(), size, {0: out[0], 1: out[1], 2: out[2]})
+    with Ssimulacra2Pipeline(Ssimulacra2Config(size=size, chroma_filter=ChromaFilter.Lanczos)) as p:
+        check_agrees(p, a, b, PQ_CURVE, "360p HDR 4:2:0")
