@@ -96,6 +96,21 @@ class LadderPool:
         return w, h, self.entries[(w, h, sk, v)]
 
 
+def h2d_gbps(nbytes: int = 256 << 20, reps: int = 5) -> float:
+    """Pinned host-to-device copy rate (hipMemcpy from page-locked memory)."""
+    from vqm_amd import DeviceBuffer, PinnedBuffer, memcpy_htod
+    src, dst = PinnedBuffer(nbytes), DeviceBuffer(nbytes)
+    src.array[:] = 1
+    memcpy_htod(dst.ptr, src.array)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        memcpy_htod(dst.ptr, src.array)
+    dt = time.perf_counter() - t0
+    src.free()
+    dst.free()
+    return nbytes * reps / dt / 1e9
+
+
 def serving(args, pool):
     """Poisson arrivals over the five-size ladder (weights 0.1/0.2/0.35/0.2/
     0.15) replayed open-loop against the continuous-batching server: the
@@ -126,6 +141,12 @@ def serving(args, pool):
 
     report = {"requests": n, "hbm_budget_bytes": budget, "lanes": args.serve_lanes,
               "max_batch": args.serve_max_batch}
+    # The streaming forms' roof: host inputs cross PCIe, 6 bytes per pixel of
+    # each pair, at the pinned host-to-device copy rate measured here.
+    h2d = h2d_gbps()
+    mean_px = sum(w * h for w, h, _ in picks) / n
+    report["h2d_gbps"] = round(h2d, 1)
+    report["streaming_roof_pairs_per_s"] = round(h2d * 1e9 / (6 * mean_px), 1)
     # The plain FP32 single-pair form (one pair on the device at a time, no
     # batching) faces the same arrivals at the same offered rates.
     forms = (("f32", args.serve_lanes, args.serve_max_batch), ("u16", args.serve_lanes, args.serve_max_batch),
