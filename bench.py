@@ -290,7 +290,7 @@ def main() -> int:
 
     if args.gpus != 1:
         log(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (pairs are independent; "
-            "several GPUs would each score their own share, see DESIGN.md)")
+            "several GPUs would each score their own share of the pairs)")
         return 2
 
     from vqm_amd import (DeviceBuffer, Image, ImageFormat, Size, Ssimulacra2Batch,
