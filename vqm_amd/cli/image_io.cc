@@ -585,7 +585,7 @@ std::optional<Image> LoadImage(std::string_view path, std::string& error) {
     }
     if (got >= 3 && head[0] == 0xff && head[1] == 0xd8 && head[2] == 0xff) {
         std::fclose(f);
-        error = "JPEG input is not available in this build (no libjpeg-compatible decoder)";
+        error = "JPEG input is not supported (no libjpeg-compatible decoder)";
         return std::nullopt;
     }
     if (got < 8 || std::memcmp(head, kPngSignature, 8) != 0) {

@@ -48,7 +48,7 @@ const char* kUsage =
     "--half-planes stores intermediate planes as FP16 (--planes f16); --planes u16\n"
     "is the mixed form (16-bit fixed point at the two finest scales, FP32 above).\n"
     "--blur recursive uses the kernel of Cloudinary ssimulacra2's recursive Gaussian\n"
-    "instead of the 13-tap sampled Gaussian (an addition of this build).\n"
+    "instead of the 13-tap sampled Gaussian (not in the reference command).\n"
     "--input-matrix     rgb|bt709|fcc|bt470bg|st170m|st240m|ycgco|bt2020ncl|\n"
     "                   bt2020cl|chromaticity-derived-ncl|chromaticity-derived-cl|ictcp\n"
     "--input-transfer   bt709|bt601|bt470m|bt470bg|st240m|linear|log100|log316|xvycc|\n"
@@ -87,7 +87,7 @@ std::optional<Options> Parse(int argc, char** argv) {
         } else if (a == "--metric") {
             const std::string_view v = value();
             if (v != "ssimulacra2")
-                throw ParseError{"metric '" + std::string(v) + "' is not part of this build (ssimulacra2 only)"};
+                throw ParseError{"metric '" + std::string(v) + "' is not provided by vqm-ss2 (ssimulacra2 only)"};
         } else if (a == "--planes") {
             const std::string_view v = value();
             if (v == "f32") o.planes = PlaneFormat::F32;
