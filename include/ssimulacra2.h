@@ -77,6 +77,9 @@ struct ImageFormat {
     std::uint8_t subsample_h = 0;
     std::uint8_t depth = 0;
 
+    // SameImageFormat (input.h:21-30): every field equal.
+    [[nodiscard]] constexpr bool operator==(const ImageFormat&) const noexcept = default;
+
     static ImageFormat Srgb8() {
         ImageFormat f;
         f.type = PixelType::Byte;
@@ -118,6 +121,19 @@ struct ImageBuffer {  // vqm_image.h:162-166
     std::size_t bytes = 0;
 };
 
+// vqm_views.h:18-30: a non-owning planar host image. Each present plane
+// (Y/R, U/G, V/B, alpha) has its own address and a row stride of at least
+// one row; padding at row ends is skipped. Absent planes are ignored.
+struct PlaneView {
+    const std::byte* data = nullptr;
+    std::size_t stride_bytes = 0;
+};
+struct ImageView {
+    Size size;
+    ImageFormat format;
+    std::array<PlaneView, 4> planes{};
+};
+
 // Caller-owned device image (replaces VulkanBufferView, vqm_vulkan.h:74-101):
 // must stay alive and unmodified until Finish returns.
 struct DeviceImageView {
@@ -152,6 +168,18 @@ inline vqm_image_desc Desc(const Image& image) {
                           image.data.size()};
 }
 
+inline vqm_host_view HostView(const ImageView& view) {
+    vqm_host_view v{};
+    v.width = Dim(view.size.width);
+    v.height = Dim(view.size.height);
+    v.format = view.format.ToC();
+    for (std::size_t p = 0; p < 4; ++p) {
+        v.planes[p] = view.planes[p].data;
+        v.strides[p] = view.planes[p].stride_bytes;
+    }
+    return v;
+}
+
 }  // namespace detail
 
 // RequiredBytes, vqm_image.h:143-146.
@@ -160,6 +188,19 @@ inline vqm_image_desc Desc(const Image& image) {
     const auto f = format.ToC();
     detail::Check(vqm_required_bytes(&f, detail::Dim(size.width), detail::Dim(size.height), &bytes));
     return static_cast<std::size_t>(bytes);
+}
+
+// CopyImage, vqm_views.h:32-36: packs a host view into an owning Image (a
+// CPU copy). std::invalid_argument for an invalid image or plane,
+// std::overflow_error when a plane's geometry or stride is not addressable.
+[[nodiscard]] inline Image CopyImage(const ImageView& view) {
+    const vqm_host_view v = detail::HostView(view);
+    std::uint64_t bytes = 0;
+    detail::Check(vqm_copy_image(&v, nullptr, &bytes));
+    Image image{view.size, view.format, {}};
+    image.data.resize(static_cast<std::size_t>(bytes));
+    detail::Check(vqm_copy_image(&v, image.data.data(), &bytes));
+    return image;
 }
 
 // ssimulacra2.h:17-25.
@@ -261,6 +302,8 @@ struct PipelineMemoryUsage {
     return static_cast<std::size_t>(device_budget / per_worker.total_device_bytes);
 }
 
+class Metrics;
+
 // ssimulacra2.h:66-215. One geometry, one submission in flight, not
 // thread-safe; use one pipeline per thread.
 class Ssimulacra2Pipeline {
@@ -284,6 +327,10 @@ public:
     Ssimulacra2Pipeline& operator=(const Ssimulacra2Pipeline&) = delete;
 
     [[nodiscard]] Ssimulacra2Result Compute(const Image& a, const Image& b, float background = 0.5f) {
+        return Finish(Submit(a, b, background));
+    }
+    // Both views are packed straight into the pinned staging (vqm_views.h).
+    [[nodiscard]] Ssimulacra2Result Compute(const ImageView& a, const ImageView& b, float background = 0.5f) {
         return Finish(Submit(a, b, background));
     }
     [[nodiscard]] Ssimulacra2Result Compute(const UploadedImage& a, const UploadedImage& b,
@@ -315,6 +362,12 @@ public:
         const vqm_image_desc da = detail::Desc(a), db = detail::Desc(b);
         std::uint64_t token = 0;
         detail::Check(vqm_ss2_submit_host(p_, &da, a.data.data(), &db, b.data.data(), background, &token));
+        return token;
+    }
+    [[nodiscard]] std::uint64_t Submit(const ImageView& a, const ImageView& b, float background = 0.5f) {
+        const vqm_host_view va = detail::HostView(a), vb = detail::HostView(b);
+        std::uint64_t token = 0;
+        detail::Check(vqm_ss2_submit_views(p_, &va, &vb, background, &token));
         return token;
     }
     [[nodiscard]] std::uint64_t Submit(const UploadedImage& a, const UploadedImage& b,
@@ -375,6 +428,13 @@ public:
     }
 
 private:
+    friend class Metrics;  // scores tightly packed byte spans without a copy
+    [[nodiscard]] std::uint64_t SubmitPacked(const vqm_image_desc& a, const void* da, const vqm_image_desc& b,
+                                             const void* db, float background) {
+        std::uint64_t token = 0;
+        detail::Check(vqm_ss2_submit_host(p_, &a, da, &b, db, background, &token));
+        return token;
+    }
     vqm_device_image View(const DeviceImageView& v) const {
         return vqm_device_image{v.data, detail::Dim(v.size.width), detail::Dim(v.size.height),
                                 v.format.ToC(), RequiredBytes(v.format, v.size)};
@@ -393,6 +453,14 @@ template <typename Frame>
     auto dark = pipeline.Compute(original, distorted, 0.1f);
     auto bright = pipeline.Compute(original, distorted, 0.9f);
     return dark.Score() <= bright.Score() ? dark : bright;
+}
+
+// Score, vqm_views.h:38-46 (image_view.cc:97-105): one pair of views at
+// one background; the two-background alpha policy is not applied.
+[[nodiscard]] inline double Score(Ssimulacra2Pipeline& pipeline, const ImageView& original,
+                                  const ImageView& distorted, float background = 0.5f) {
+    if (original.size != distorted.size) throw std::invalid_argument("image views must have the same size");
+    return pipeline.Compute(original, distorted, background).Score();
 }
 
 struct ImagePair {  // vqm_pool.h:15-18

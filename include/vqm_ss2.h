@@ -106,6 +106,19 @@ typedef struct vqm_image_desc {
     uint64_t bytes;
 } vqm_image_desc;
 
+/* A non-owning planar host image, ImageView/PlaneView
+ * (include/vqm_views.h:18-30): the planes in Image order (Y/R, U/G, V/B,
+ * alpha), each at its own address with its own row stride in bytes (at
+ * least the row's length; padding at row ends is skipped). Planes the
+ * format does not have are ignored. */
+typedef struct vqm_host_view {
+    uint32_t width;
+    uint32_t height;
+    vqm_image_format format;
+    const void* planes[4];
+    uint64_t strides[4];
+} vqm_host_view;
+
 /* A device-resident image owned by the caller: replaces VulkanBufferView
  * (include/vqm_vulkan.h:74-101). `data` is a device pointer to the packed
  * planes and must stay alive and unmodified until vqm_ss2_finish returns. */
@@ -172,6 +185,16 @@ int vqm_required_bytes(const vqm_image_format* format, uint32_t width, uint32_t 
  * `format` at this geometry, else INVALID_ARGUMENT naming the reason. */
 int vqm_validate_image_format(const vqm_image_format* format, uint32_t width, uint32_t height);
 
+/* CopyImage (include/vqm_views.h:32-36, image_view.cc:16-95): validates a
+ * view and packs it into the Image layout at `out`. On entry `*bytes` is
+ * the capacity of `out`; on return it is the packed length (RequiredBytes
+ * of the view's format and size). With `out` NULL the view is only
+ * validated and its length reported. INVALID_ARGUMENT for an empty size,
+ * an invalid format, a missing plane, a short stride or a capacity other
+ * than the packed length; OVERFLOW when a plane's last row is not
+ * addressable. A CPU copy. */
+int vqm_copy_image(const vqm_host_view* view, void* out, uint64_t* bytes);
+
 /* PlaneScaleOffset (input.cc:714-757): the decode scale and offset of a
  * sample class (luma/RGB, chroma when `chroma`, alpha when `alpha`). */
 int vqm_plane_scale_offset(const vqm_image_format* format, int chroma, int alpha, float* scale,
@@ -234,6 +257,14 @@ int vqm_ss2_release(vqm_ss2_pipeline* pipeline, uint64_t handle);
 int vqm_ss2_submit_host(vqm_ss2_pipeline* pipeline, const vqm_image_desc* original,
                         const void* original_data, const vqm_image_desc* distorted,
                         const void* distorted_data, float background, uint64_t* token);
+
+/* Submit(ImageView, ImageView) (include/ssimulacra2.h:148-151,
+ * vulkan_runtime_inputs.cc:156-183): both views are validated, then packed
+ * straight into the pipeline's pinned staging (no intermediate copy) before
+ * this returns. INVALID_ARGUMENT when a view's size differs from the
+ * pipeline's or as vqm_copy_image. */
+int vqm_ss2_submit_views(vqm_ss2_pipeline* pipeline, const vqm_host_view* original,
+                         const vqm_host_view* distorted, float background, uint64_t* token);
 
 /* Submit(UploadedImage, UploadedImage) (include/ssimulacra2.h:152-155).
  * INVALID_ARGUMENT for a stale or unknown handle. */

@@ -3,14 +3,18 @@
 // score_test.cc, pool_test.cc). Without arguments only the host-side
 // contract runs; with --gpu the pipeline tests run on the device.
 
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <limits>
 #include <stdexcept>
+#include <span>
 #include <string>
+#include <vector>
 
 #include "ssimulacra2.h"
+#include "vqm_metrics.h"
 
 static int failures = 0;
 #define CHECK(cond)                                                          \
@@ -87,6 +91,105 @@ static vqm::Image Yuv420From(const vqm::Image& rgb) {
     return img;
 }
 
+// A view of `img` whose rows sit `pad` bytes apart beyond their length, in
+// buffers of its own (`store` keeps them alive).
+static vqm::ImageView PaddedView(const vqm::Image& img, std::size_t pad, std::vector<std::vector<std::byte>>& store) {
+    vqm::ImageView view;
+    view.size = img.size;
+    view.format = img.format;
+    const bool yuv = img.format.family == vqm::ColorFamily::Yuv;
+    std::size_t offset = 0;
+    for (std::size_t p = 0; p < 3; ++p) {
+        const std::size_t w = (yuv && p > 0) ? img.size.width >> img.format.subsample_w : img.size.width;
+        const std::size_t h = (yuv && p > 0) ? img.size.height >> img.format.subsample_h : img.size.height;
+        std::vector<std::byte> plane((w + pad) * h, std::byte{0xA5});
+        for (std::size_t y = 0; y < h; ++y)
+            for (std::size_t x = 0; x < w; ++x) plane[y * (w + pad) + x] = std::byte{img.data[offset + y * w + x]};
+        offset = (offset + w * h + 3) / 4 * 4;
+        store.push_back(std::move(plane));
+        view.planes[p] = {store.back().data(), w + pad};
+    }
+    return view;
+}
+
+static vqm::MetricsConfig Srgb8Metrics(vqm::Size size) {
+    vqm::MetricsConfig c;
+    c.size = size;
+    c.original_format = vqm::ImageFormat::Srgb8();
+    c.distorted_format = c.original_format;
+    c.ssimulacra2 = vqm::Ssimulacra2Config{size};
+    return c;
+}
+
+// vqm_views.h:32-36 and tests/core/metrics_test.cc:98-240: everything that
+// needs no device.
+static void ViewAndMetricsHostContract() {
+    const vqm::Image rgb = Gradient(1), yuv = Yuv420From(rgb);
+    for (const vqm::Image* img : {&rgb, &yuv}) {
+        std::vector<std::vector<std::byte>> store;
+        const vqm::Image copy = vqm::CopyImage(PaddedView(*img, 7, store));
+        CHECK(copy.size == img->size && copy.format == img->format && copy.data == img->data);
+        vqm::ImageView short_stride = PaddedView(*img, 0, store);
+        short_stride.planes[1].stride_bytes -= 1;
+        CHECK_THROWS_AS(vqm::CopyImage(short_stride), std::invalid_argument);
+        vqm::ImageView missing = PaddedView(*img, 0, store);
+        missing.planes[2].data = nullptr;
+        CHECK_THROWS_AS(vqm::CopyImage(missing), std::invalid_argument);
+        vqm::ImageView huge = PaddedView(*img, 0, store);
+        huge.planes[0].stride_bytes = std::numeric_limits<std::size_t>::max() / 2;
+        CHECK_THROWS_AS(vqm::CopyImage(huge), std::overflow_error);
+    }
+    vqm::ImageView empty;
+    CHECK_THROWS_AS(vqm::CopyImage(empty), std::invalid_argument);
+
+    // Selection and configuration checks come before any device work.
+    CHECK_THROWS_AS(vqm::MetricPipelines(vqm::MetricsConfig{}), std::invalid_argument);
+    CHECK_THROWS_AS(vqm::Metrics(vqm::MetricsConfig{}), std::invalid_argument);
+    vqm::MetricsConfig config = Srgb8Metrics({64, 48});
+    config.background = 1.5f;
+    CHECK_THROWS_AS(vqm::Metrics(config), std::invalid_argument);
+    config.background = std::nanf("");
+    CHECK_THROWS_AS(vqm::GetMemoryRequirements(config), std::invalid_argument);
+    config.background = 0.5f;
+    config.original_format.transfer = vqm::TransferCharacteristics::Unspecified;
+    bool named = false;
+    try {
+        (void)vqm::GetMemoryRequirements(config);
+    } catch (const std::invalid_argument& e) {
+        named = std::string(e.what()).rfind("original format: ", 0) == 0;
+    }
+    CHECK(named);
+    config.original_format = vqm::ImageFormat::Srgb8();
+
+    // The metric's own geometry, resident slots and pool are overridden.
+    config.ssimulacra2->size = {8, 8};
+    config.ssimulacra2->gpu_resident_images = 5;
+    config.ssimulacra2->input_bytes = 1 << 20;
+    const vqm::MetricsMemoryRequirements req = vqm::GetMemoryRequirements(config);
+    vqm::Ssimulacra2Config single{{64, 48}};
+    single.gpu_resident_images = 0;
+    const vqm::PipelineMemoryUsage expected = vqm::GetMemoryRequirements(single);
+    CHECK(req.spatial.has_value() && !req.scratch.has_value());
+    CHECK(req.spatial->total_device_bytes == expected.total_device_bytes);
+    CHECK(req.spatial->resident_bytes == 0 && req.spatial->input_pool_bytes == 0);
+    CHECK(req.spatial->pinned_host_bytes == expected.pinned_host_bytes);
+
+    // metrics_test.cc:187-240: per-heap worker counts.
+    const std::uint64_t dev = req.spatial->total_device_bytes, host = req.spatial->pinned_host_bytes;
+    const std::array<std::uint64_t, 2> roomy{5 * dev + 1, 9 * host};
+    CHECK(vqm::MaxWorkersForBudgetWorstCase(req, roomy) == 5);
+    const std::array<std::uint64_t, 2> host_bound{5 * dev, 2 * host + host / 2};
+    CHECK(vqm::MaxWorkersForBudgetWorstCase(req, host_bound) == 2);
+    const std::array<std::uint64_t, 1> one_heap{dev};
+    CHECK_THROWS_AS(vqm::MaxWorkersForBudgetWorstCase(req, one_heap), std::invalid_argument);
+    CHECK_THROWS_AS(vqm::MaxWorkersForBudgetWorstCase(vqm::MetricsMemoryRequirements{}, roomy),
+                    std::invalid_argument);
+    vqm::MetricsMemoryRequirements huge_req = req;
+    huge_req.scratch = vqm::PipelineMemoryUsage{};
+    huge_req.scratch->total_device_bytes = std::numeric_limits<std::uint64_t>::max();
+    CHECK_THROWS_AS(vqm::MaxWorkersForBudgetWorstCase(huge_req, roomy), std::overflow_error);
+}
+
 static void HostContract() {
     // pipeline_test.cc:318-328: the geometry is validated before device work.
     bool named = false;
@@ -121,6 +224,102 @@ static void HostContract() {
     CHECK(vqm::MaxWorkersForBudget(plan, 3 * plan.total_device_bytes) == 3);
     CHECK_THROWS_AS(vqm::MaxWorkersForBudget(vqm::PipelineMemoryUsage{}, 1), std::invalid_argument);
     CHECK_THROWS_AS(vqm::GetMemoryRequirements(vqm::Ssimulacra2Config{{4, 8}}), std::invalid_argument);
+    ViewAndMetricsHostContract();
+}
+
+// tests/core/metrics_test.cc:241-575, the SSIMULACRA 2 parts: every input
+// form scores what the pipeline scores, frames are numbered, rejected pairs
+// consume no frame, and the sequence states follow the reference.
+static void MetricsDeviceContract() {
+    const vqm::Image orig = Gradient(0), dist = Gradient(1);
+    vqm::Ssimulacra2Pipeline pipe(vqm::Ssimulacra2Config{{32, 32}});
+    const double want = pipe.Compute(orig, dist).Score();
+    std::vector<std::vector<std::byte>> store;
+    const vqm::ImageView va = PaddedView(orig, 5, store), vb = PaddedView(dist, 3, store);
+    CHECK(pipe.Compute(va, vb).Score() == want);
+    CHECK(vqm::Score(pipe, va, vb) == want);
+
+    vqm::Metrics metrics(Srgb8Metrics({32, 32}));
+    auto first = metrics.Compute(orig, dist);
+    CHECK(first && first->frame_index == 0 && first->ssimulacra2 && *first->ssimulacra2 == want);
+    vqm::Image wrong = dist;
+    wrong.size.width -= 1;
+    CHECK_THROWS_AS(metrics.Compute(orig, wrong), std::invalid_argument);
+    wrong = dist;
+    wrong.format.transfer = vqm::TransferCharacteristics::Bt709;
+    CHECK_THROWS_AS(metrics.Compute(orig, wrong), std::invalid_argument);
+    wrong = dist;
+    wrong.data.pop_back();
+    CHECK_THROWS_AS(metrics.Compute(orig, wrong), std::invalid_argument);
+    auto second = metrics.Compute(va, vb);
+    CHECK(second && second->frame_index == 1 && *second->ssimulacra2 == want);
+    const auto a = std::as_bytes(std::span(orig.data)), b = std::as_bytes(std::span(dist.data));
+    CHECK_THROWS_AS(metrics.Compute(a.first(a.size() - 1), b), std::invalid_argument);
+    CHECK_THROWS_AS(metrics.Compute(a, b.first(b.size() - 1)), std::invalid_argument);
+    auto third = metrics.Compute(a, b);
+    CHECK(third && third->frame_index == 2 && *third->ssimulacra2 == want);
+    CHECK_THROWS_AS(metrics.ComputeImage(orig, dist), std::logic_error);  // sequence active
+    CHECK(!metrics.Flush());
+    CHECK_THROWS_AS(metrics.Compute(orig, dist), std::logic_error);  // sequence finished
+    const vqm::MetricsFinalResult fin = metrics.Finalize();
+    CHECK(!fin.last_frame && !fin.penultimate_frame && fin.frames == 3);
+    metrics.Reset();
+    const vqm::MetricsResult still = metrics.ComputeImage(a, b);
+    CHECK(still.frame_index == 0 && still.ssimulacra2 && *still.ssimulacra2 == want);
+    const vqm::MetricsResult again = metrics.ComputeImage(orig, dist);  // ComputeImage resets
+    CHECK(again.frame_index == 0 && *again.ssimulacra2 == want);
+    const auto usage = metrics.GetMemoryUsage();
+    CHECK(usage.spatial && !usage.scratch);
+    CHECK(usage.spatial->total_device_bytes ==
+          vqm::GetMemoryRequirements(Srgb8Metrics({32, 32})).spatial->total_device_bytes);
+
+    // Device inputs, and streaming disabled: pixel overloads are refused
+    // before any work, device views still score.
+    vqm::MetricsConfig device_only = Srgb8Metrics({32, 32});
+    device_only.streaming_inputs = false;
+    vqm::Metrics dm(device_only);
+    CHECK_THROWS_AS(dm.Compute(orig, dist), std::invalid_argument);
+    void* d[2] = {nullptr, nullptr};
+    for (int i = 0; i < 2; ++i) {
+        const vqm::Image& src = i ? dist : orig;
+        CHECK(vqm_device_malloc(src.data.size(), &d[i]) == VQM_OK);
+        CHECK(vqm_memcpy_htod(d[i], src.data.data(), src.data.size()) == VQM_OK);
+    }
+    const vqm::DeviceImageView da{d[0], {32, 32}, vqm::ImageFormat::Srgb8()};
+    const vqm::DeviceImageView db{d[1], {32, 32}, vqm::ImageFormat::Srgb8()};
+    auto dev = dm.Compute(da, db);
+    CHECK(dev && dev->frame_index == 0 && *dev->ssimulacra2 == want);
+    CHECK_THROWS_AS(dm.Compute(da, vqm::DeviceImageView{nullptr, {32, 32}, vqm::ImageFormat::Srgb8()}),
+                    std::invalid_argument);
+    auto dev2 = dm.Compute(da, db);
+    CHECK(dev2 && dev2->frame_index == 1);
+    for (void* p : d) CHECK(vqm_device_free(p) == VQM_OK);
+
+    // MetricPipelines: the getter's pipeline survives a move and scores as
+    // a standalone one.
+    vqm::MetricPipelines pipelines(Srgb8Metrics({32, 32}));
+    vqm::Ssimulacra2Pipeline* held = pipelines.Ssimulacra2();
+    vqm::MetricPipelines moved = std::move(pipelines);
+    CHECK(moved.Ssimulacra2() == held && held != nullptr);
+    CHECK(held->Compute(orig, dist).Score() == want);
+    CHECK(moved.GetMemoryUsage().spatial.has_value());
+
+    // The top-level background reaches the metric: with alpha it changes
+    // the score, and equals the pipeline's at that background.
+    vqm::Image ao = orig, ad = dist;
+    for (vqm::Image* img : {&ao, &ad}) {
+        img->format.alpha = vqm::AlphaType::Straight;
+        img->data.resize(vqm::RequiredBytes(img->format, img->size));
+        for (std::size_t i = 0; i < 32 * 32; ++i) img->data[3 * 32 * 32 + i] = static_cast<std::uint8_t>(i * 7);
+    }
+    vqm::MetricsConfig alpha = Srgb8Metrics({32, 32});
+    alpha.original_format = ao.format;
+    alpha.distorted_format = ad.format;
+    alpha.background = 0.2f;
+    vqm::Metrics am(alpha);
+    const double at02 = pipe.Compute(ao, ad, 0.2f).Score(), at08 = pipe.Compute(ao, ad, 0.8f).Score();
+    CHECK(at02 != at08);
+    CHECK(*am.ComputeImage(ao, ad).ssimulacra2 == at02);
 }
 
 static void DeviceContract() {
@@ -161,6 +360,7 @@ static void DeviceContract() {
     CHECK(yuv_pipe.Compute(ya, ya).Score() == 100.0);
     const double yuv_score = yuv_pipe.Compute(ya, yb).Score();
     CHECK(yuv_score > 50.0 && yuv_score < 100.0);
+    MetricsDeviceContract();
 }
 
 int main(int argc, char** argv) {

@@ -41,6 +41,11 @@ class ImageDescC(ctypes.Structure):
     _fields_ = [("width", u32), ("height", u32), ("format", ImageFormatC), ("bytes", u64)]
 
 
+class HostViewC(ctypes.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("format", ImageFormatC), ("planes", vp * 4),
+                ("strides", u64 * 4)]
+
+
 class DeviceImageC(ctypes.Structure):
     _fields_ = [("data", vp), ("width", u32), ("height", u32), ("format", ImageFormatC),
                 ("bytes", u64)]
@@ -102,6 +107,7 @@ SIGNATURES = {
     "vqm_required_bytes": (ctypes.c_int, [ctypes.POINTER(ImageFormatC), u32, u32,
                                           ctypes.POINTER(u64)]),
     "vqm_validate_image_format": (ctypes.c_int, [ctypes.POINTER(ImageFormatC), u32, u32]),
+    "vqm_copy_image": (ctypes.c_int, [ctypes.POINTER(HostViewC), vp, ctypes.POINTER(u64)]),
     "vqm_plane_scale_offset": (ctypes.c_int, [ctypes.POINTER(ImageFormatC), ctypes.c_int, ctypes.c_int,
                                               ctypes.POINTER(f32), ctypes.POINTER(f32)]),
     "vqm_ss2_plan_scales": (ctypes.c_int, [u32, u32, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
@@ -119,6 +125,8 @@ SIGNATURES = {
     "vqm_ss2_submit_host": (ctypes.c_int, [vp, ctypes.POINTER(ImageDescC), vp,
                                            ctypes.POINTER(ImageDescC), vp, f32,
                                            ctypes.POINTER(u64)]),
+    "vqm_ss2_submit_views": (ctypes.c_int, [vp, ctypes.POINTER(HostViewC), ctypes.POINTER(HostViewC), f32,
+                                            ctypes.POINTER(u64)]),
     "vqm_ss2_submit_resident": (ctypes.c_int, [vp, u64, u64, f32, ctypes.POINTER(u64)]),
     "vqm_ss2_submit_resident_host": (ctypes.c_int, [vp, u64, ctypes.POINTER(ImageDescC), vp, f32,
                                                     ctypes.POINTER(u64)]),

@@ -96,6 +96,19 @@ int vqm_required_bytes(const vqm_image_format* format, uint32_t width, uint32_t 
     });
 }
 
+int vqm_copy_image(const vqm_host_view* view, void* out, uint64_t* bytes) {
+    return Guard([&] {
+        Need(view, "view");
+        Need(bytes, "bytes");
+        const uint64_t need = vqm::ss2::CheckView(*view);
+        if (out != nullptr && *bytes != need)
+            vqm::ss2::Fail(VQM_INVALID_ARGUMENT, "image view: the output buffer must hold exactly " +
+                                                     std::to_string(need) + " bytes");
+        if (out != nullptr) vqm::ss2::PackView(*view, static_cast<uint8_t*>(out));
+        *bytes = need;
+    });
+}
+
 int vqm_validate_image_format(const vqm_image_format* format, uint32_t width, uint32_t height) {
     return Guard([&] {
         Need(format, "format");
@@ -237,6 +250,17 @@ int vqm_ss2_submit_host(vqm_ss2_pipeline* p, const vqm_image_desc* a, const void
         Need(b, "distorted");
         Need(token, "token");
         *token = p->impl.SubmitHost(*a, da, *b, db, background);
+    });
+}
+
+int vqm_ss2_submit_views(vqm_ss2_pipeline* p, const vqm_host_view* a, const vqm_host_view* b,
+                         float background, uint64_t* token) {
+    return Guard([&] {
+        Need(p, "pipeline");
+        Need(a, "original");
+        Need(b, "distorted");
+        Need(token, "token");
+        *token = p->impl.SubmitViews(*a, *b, background);
     });
 }
 

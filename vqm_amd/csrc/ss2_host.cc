@@ -6,6 +6,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
+#include <cstring>
 #include <limits>
 
 namespace vqm::ss2 {
@@ -62,6 +64,9 @@ uint64_t CheckedMul(uint64_t a, uint64_t b) {
 }
 
 uint64_t Align(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
+
+// Bytes per sample of a validated pixel type (Byte, Word, Half, Float).
+uint64_t SampleBytes(const vqm_image_format& f) { return f.type == 0 ? 1 : f.type == 3 ? 4 : 2; }
 
 }  // namespace
 
@@ -177,6 +182,44 @@ void CheckImage(const vqm_image_desc& image, uint32_t width, uint32_t height, co
     if (image.bytes != expected)
         Fail(VQM_INVALID_ARGUMENT, who + ": data size does not match the image format (expected " +
                                        std::to_string(expected) + " bytes)");
+}
+
+uint64_t CheckView(const vqm_host_view& view) {
+    if (view.width == 0 || view.height == 0) Fail(VQM_INVALID_ARGUMENT, "image view: size must be non-zero");
+    ValidateFormat(view.format, view.width, view.height, "image view");
+    const uint64_t sample = SampleBytes(view.format);
+    const auto planes = PlaneLayout(view.format, view.width, view.height);
+    constexpr uint64_t kMaxOffset = static_cast<uint64_t>(std::numeric_limits<std::ptrdiff_t>::max());
+    for (int p = 0; p < 4; ++p) {
+        if (!planes[p].present) continue;
+        const uint64_t row = CheckedMul(planes[p].width, sample);
+        const uint64_t stride = view.strides[p];
+        if (view.planes[p] == nullptr || stride < row)
+            Fail(VQM_INVALID_ARGUMENT, "image view: missing plane or short stride");
+        // The last row's end, (height - 1) * stride + row, must be a
+        // representable pointer difference before any row address is formed.
+        if (row > kMaxOffset || (planes[p].height > 1 && stride > (kMaxOffset - row) / (planes[p].height - 1)))
+            Fail(VQM_OVERFLOW, "image view: plane stride is too large");
+    }
+    return RequiredBytes(view.format, view.width, view.height);
+}
+
+void PackView(const vqm_host_view& view, uint8_t* dst) {
+    const uint64_t sample = SampleBytes(view.format);
+    const auto planes = PlaneLayout(view.format, view.width, view.height);
+    for (int p = 0; p < 4; ++p) {
+        if (!planes[p].present) continue;
+        const uint64_t row = planes[p].width * sample, rows = planes[p].height;
+        const auto* src = static_cast<const uint8_t*>(view.planes[p]);
+        uint8_t* out = dst + planes[p].offset;
+        if (view.strides[p] == row) {
+            std::memcpy(out, src, row * rows);
+        } else {
+            for (uint64_t y = 0; y < rows; ++y) std::memcpy(out + y * row, src + y * view.strides[p], row);
+        }
+        const uint64_t end = planes[p].offset + row * rows;
+        std::memset(dst + end, 0, Align(end, 4) - end);
+    }
 }
 
 GeometryPlan BuildPlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows) {

@@ -55,6 +55,18 @@ uint64_t RequiredBytes(const vqm_image_format& format, uint32_t width, uint32_t 
 void CheckImage(const vqm_image_desc& image, uint32_t width, uint32_t height,
                 const char* name);
 
+// image_view.cc:16-45 (DescribeImageView): a non-empty size, a format the
+// reference accepts, a pointer and a stride of at least one row for every
+// present plane, and a last row whose end is addressable. Returns the
+// packed byte length. Throws Error{VQM_INVALID_ARGUMENT} or
+// Error{VQM_OVERFLOW}.
+uint64_t CheckView(const vqm_host_view& view);
+
+// image_view.cc:47-77 (PackImageView): copies a checked view into the
+// packed layout at `dst` (CheckView's length), zeroing the at most three
+// bytes that pad each plane to a 4-byte boundary.
+void PackView(const vqm_host_view& view, uint8_t* dst);
+
 // Per-geometry plan: level sizes, padded plane geometry, plane offsets per
 // image, the scale work items and their per-(level, channel) ranges.
 struct GeometryPlan {

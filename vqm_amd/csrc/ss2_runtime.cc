@@ -637,6 +637,28 @@ uint64_t Pipeline::SubmitHost(const vqm_image_desc& a, const void* da, const vqm
     return Launch(base, base + stream_image_bytes_);
 }
 
+uint64_t Pipeline::SubmitViews(const vqm_host_view& a, const vqm_host_view& b, float background) {
+    RequireIdle();
+    if (!config_.streaming_inputs)
+        Fail(VQM_INVALID_ARGUMENT, "streaming_inputs is false, so the pixel overloads are unavailable");
+    if (a.width != config_.width || a.height != config_.height || b.width != config_.width ||
+        b.height != config_.height)
+        Fail(VQM_INVALID_ARGUMENT, "image view geometry does not match the pipeline");
+    // Both sides are validated before either is written to the staging.
+    const uint64_t bytes_a = CheckView(a), bytes_b = CheckView(b);
+    EnsureStreaming(std::max(bytes_a, bytes_b));
+    auto* staging = staging_.as<uint8_t>();
+    PackView(a, staging);
+    PackView(b, staging + stream_image_bytes_);
+    Check(hipMemcpyAsync(stream_inputs_.get(), staging, 2 * stream_image_bytes_, hipMemcpyHostToDevice,
+                         stream_.get()),
+          "upload pair");
+    const auto* base = stream_inputs_.as<uint8_t>();
+    SetSources(a.format, b.format, background);
+    cached_original_ = 0;
+    return Launch(base, base + stream_image_bytes_);
+}
+
 uint64_t Pipeline::SubmitResident(uint64_t a, uint64_t b, float background) {
     RequireIdle();
     const Slot& sa = RequireSlot(a, "original");

@@ -274,6 +274,9 @@ public:
     uint64_t SubmitHost(const vqm_image_desc& a, const void* da, const vqm_image_desc& b,
                         const void* db, float background);
     uint64_t SubmitResident(uint64_t a, uint64_t b, float background);
+    // vulkan_runtime_inputs.cc:156-183: both views are checked, then packed
+    // straight into the pinned staging.
+    uint64_t SubmitViews(const vqm_host_view& a, const vqm_host_view& b, float background);
     uint64_t SubmitResidentHost(uint64_t a, const vqm_image_desc& b, const void* db, float background);
     uint64_t SubmitDevice(const vqm_device_image& a, const vqm_device_image& b,
                           hipStream_t wait_stream, float background);

@@ -290,6 +290,52 @@ class DeviceImage:
         return d
 
 
+@dataclass
+class ImageView:
+    """vqm_views.h:18-30: a non-owning planar host image. `planes` holds the
+    present planes in Image order (Y/R, U/G, V/B, alpha) as 2-D numpy arrays
+    of rows, each with its own row stride (a slice of a wider array is
+    fine); samples within a row must be adjacent. Absent planes may be None
+    or left out."""
+    size: Size
+    format: ImageFormat = field(default_factory=ImageFormat)
+    planes: Sequence[Optional[np.ndarray]] = ()
+
+    def to_c(self) -> L.HostViewC:
+        if len(self.planes) > 4:
+            raise InvalidArgument("image view: at most four planes")
+        v = L.HostViewC()
+        v.width, v.height = self.size.width, self.size.height
+        v.format = self.format.to_c()
+        for i, plane in enumerate(self.planes):
+            if plane is None:
+                continue
+            if plane.ndim != 2 or plane.strides[1] != plane.itemsize or plane.strides[0] < 0:
+                raise InvalidArgument("image view: a plane must be a 2-D array of rows with adjacent samples")
+            v.planes[i] = plane.ctypes.data
+            v.strides[i] = plane.strides[0]
+        return v
+
+
+def view_bytes(view: ImageView) -> int:
+    """DescribeImageView (image_view.cc:16-45): validates a view and returns
+    its packed length. InvalidArgument for an invalid image or plane,
+    OverflowError when a plane's geometry or stride is not addressable."""
+    n = L.u64(0)
+    _check(L.lib().vqm_copy_image(ctypes.byref(view.to_c()), None, ctypes.byref(n)))
+    return n.value
+
+
+def CopyImage(view: ImageView) -> Image:
+    """vqm_views.h:32-36: packs a host view into an owning Image (a CPU
+    copy); raises as view_bytes."""
+    v = view.to_c()
+    n = L.u64(view_bytes(view))
+    out = np.empty(n.value, np.uint8)
+    _check(L.lib().vqm_copy_image(ctypes.byref(v), out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(n)))
+    return Image(view.size, view.format, out)
+
+
 # ---- results (ssimulacra2.h:17-33) -----------------------------------------
 
 @dataclass
@@ -502,6 +548,11 @@ class Ssimulacra2Pipeline:
             _check(lib.vqm_ss2_submit_host(self._h, ctypes.byref(original.desc()), pa,
                                            ctypes.byref(distorted.desc()), pb, background,
                                            ctypes.byref(token)))
+        elif isinstance(original, ImageView) and isinstance(distorted, ImageView):
+            va, vb = original.to_c(), distorted.to_c()
+            self._keep = (original, distorted)
+            _check(lib.vqm_ss2_submit_views(self._h, ctypes.byref(va), ctypes.byref(vb), background,
+                                            ctypes.byref(token)))
         elif isinstance(original, UploadedImage) and isinstance(distorted, UploadedImage):
             _check(lib.vqm_ss2_submit_resident(self._h, original.slot, distorted.slot, background,
                                                ctypes.byref(token)))
@@ -550,6 +601,15 @@ class Ssimulacra2Pipeline:
         ms = (L.f32 * 3)()
         _check(L.lib().vqm_ss2_stage_times(self._h, ms))
         return list(ms)
+
+
+def Score(pipeline: Ssimulacra2Pipeline, original: ImageView, distorted: ImageView,
+          background: float = 0.5) -> float:
+    """vqm_views.h:38-46: one pair of views at one background (the
+    two-background alpha policy is not applied)."""
+    if original.size != distorted.size:
+        raise InvalidArgument("image views must have the same size")
+    return pipeline.Compute(original, distorted, background).Score()
 
 
 def ComputeWorstBackground(pipeline: Ssimulacra2Pipeline, original, distorted) -> Ssimulacra2Result:
