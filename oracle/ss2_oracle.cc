@@ -118,25 +118,31 @@ float round_unorm16(float v) {
 
 template <typename T> T fmaT(T a, T b, T c) { return std::fma(a, b, c); }
 
-// Device cube root: a bit-trick seed for x^(-1/3), two division-free Newton
-// steps r <- r + r(1 - x r^3)/3, then y = x r^2 and one Newton correction of
-// y. Faithfully rounded (<= 0.75 ulp) on the XYB domain; every operation is
-// an IEEE add/mul/fma, so CPU and GPU agree bit for bit.
+// Smallest normal float: the device clamps the XYB mixes to it instead of
+// 0; its root 2^-42 is below half an ulp of the cube-root bias, so
+// cbrt - bias rounds to -bias exactly as for a zero mix.
+constexpr float kMinNormal = 0x1p-126f;
+
+// Device cube root of max(x, 2^-126): a bit-trick seed for x^(-1/3) whose
+// division of the bits by 3 is one float multiply (uint -> float rounds to
+// nearest, float -> uint truncates), two division-free Newton steps
+// r <- r + r (1/3 - (x/3) r^3), then y = x r^2 and one Newton correction
+// of y. Faithfully rounded (<= 0.75 ulp) on the XYB domain; every operation
+// is an IEEE conversion, add, mul or fma, so CPU and GPU agree bit for bit.
 float cbrt_dev(float x) {
-    if (!(x > 0.0f)) return 0.0f;
-    const uint32_t i = float_to_bits(x);
-    float r = bits_to_float(0x54a21d2au - i / 3u);
+    x = std::max(x, kMinNormal);
     const float third = 0.333333343f;
+    const uint32_t third_bits = static_cast<uint32_t>(static_cast<float>(float_to_bits(x)) * third);
+    float r = bits_to_float(0x54a21d2au - third_bits);
+    const float x3 = x * third;
     for (int k = 0; k < 2; ++k) {
         const float r3 = (r * r) * r;
-        const float e = std::fma(-x, r3, 1.0f);
-        r = std::fma(r * e, third, r);
+        r = std::fma(r, std::fma(-x3, r3, third), r);
     }
-    float y = (x * r) * r;
-    const float y2 = y * y;
-    const float e = std::fma(-y2, y, x);
-    y = std::fma(e * (r * r), third, y);
-    return y;
+    const float rr = r * r;
+    const float y = x * rr;
+    const float e = std::fma(-(y * y), y, x);
+    return std::fma(e, rr * third, y);
 }
 
 float cbrt_exact(float x) {
@@ -170,6 +176,8 @@ void xyb_f32(float r, float g, float b, int cbrt_kind, float out[3]) {
     float mix0 = std::fma(m00, r, std::fma(m01, g, std::fma(m02, b, bias)));
     float mix1 = std::fma(m10, r, std::fma(m11, g, std::fma(m12, b, bias)));
     float mix2 = std::fma(m20, r, std::fma(m21, g, std::fma(m22, b, bias)));
+    // The device clamps to the smallest normal (cbrt_dev), the reference's
+    // order to 0; both give c = -bias for a mix at or below zero.
     mix0 = std::max(mix0, 0.0f);
     mix1 = std::max(mix1, 0.0f);
     mix2 = std::max(mix2, 0.0f);
@@ -350,20 +358,24 @@ void map_sums(const T* a, const T* b, uint32_t w, uint32_t h, int order,
             const T m1 = mu1[i], m2 = mu2[i];
             const T t = m1 - m2;
             const T num_s = fmaT(two, fmaT(-m1, m2, s12[i]), c2);
-            T d;
+            const T det_dist = one + std::fabs(b[i] - m2);
+            const T det_orig = one + std::fabs(a[i] - m1);
+            T d, d1;
             if (sum_variances) {
+                // The device form also shares one division between the two
+                // quotients: q = 1 / (den det_orig).
                 const T v = fmaT(-t, t, s11[i]);
                 const T den = num_s + v;
-                d = std::max(fmaT(t * t, num_s, v) / den, zero);
+                const T q = one / (den * det_orig);
+                d = std::max((fmaT(t * t, num_s, v) * det_orig) * q, zero);
+                d1 = ((det_dist - det_orig) * den) * q;
             } else {
                 const T num_m = fmaT(-t, t, one);
                 const T den = (fmaT(-m1, m1, s11[i]) + fmaT(-m2, m2, s22[i])) + c2;
                 d = std::max(fmaT(-num_m, num_s, den) / den, zero);
+                d1 = (det_dist - det_orig) / det_orig;
             }
             const T dd = d * d;
-            const T det_dist = one + std::fabs(b[i] - m2);
-            const T det_orig = one + std::fabs(a[i] - m1);
-            const T d1 = (det_dist - det_orig) / det_orig;
             const T ring = std::max(d1, zero);
             const T blurv = std::max(-d1, zero);
             const T rr = ring * ring, bb = blurv * blurv;

@@ -183,8 +183,13 @@ def test_xyb_closed_forms(oracle_lib):
 
 def test_device_cube_root(oracle_lib):
     """The kernels' Newton cube root is faithful (<= 0.75 ulp) on the XYB
-    domain and exact at 0 and at perfect cubes of powers of two."""
-    assert oracle_lib.cbrt_dev(0.0) == 0.0
+    domain and exact at perfect cubes of powers of two; a zero or negative
+    mix is clamped to the smallest normal, whose root 2^-42 vanishes against
+    the bias (c = -bias exactly, as for a zero mix)."""
+    assert oracle_lib.cbrt_dev(0.0) == 2.0 ** -42
+    assert oracle_lib.cbrt_dev(-3.0) == 2.0 ** -42
+    cbias = np.float32(0.15595419704914093)  # prepare.comp:49
+    assert np.float32(2.0 ** -42) - cbias == -cbias
     assert oracle_lib.cbrt_dev(0.125) == 0.5
     assert oracle_lib.cbrt_dev(1.0) == 1.0
     rng = np.random.default_rng(2)

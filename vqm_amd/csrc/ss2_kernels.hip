@@ -70,42 +70,51 @@ __device__ __forceinline__ float fma_(float a, float b, float c) {
     return __builtin_fmaf(a, b, c);
 }
 
-// Cube root: bit-trick seed of x^(-1/3), two division-free Newton steps, then
-// y = x r^2 with one Newton correction. <= 0.75 ulp on the XYB domain.
+// Smallest normal float: the XYB mixes are clamped to it instead of 0, so the
+// cube root never sees zero or a denormal. Its root, 2^-42, is below half an
+// ulp of kCbrtBias, so cbrt - kCbrtBias rounds to -kCbrtBias exactly as for 0.
+constexpr float kMinNormal = 0x1p-126f;
+
+// Cube root of a normal float: bit-trick seed of x^(-1/3) (the bits divided
+// by 3 through one float multiply), two division-free Newton steps
+// r <- r + r (1/3 - (x/3) r^3), then y = x r^2 with one Newton correction.
+// <= 0.75 ulp on the XYB domain; every step is an IEEE conversion, add, mul
+// or fma, so the oracle reproduces it bit for bit.
 __device__ __forceinline__ float cbrt_dev(float x) {
-    const uint32_t i = __float_as_uint(x);
-    float r = __uint_as_float(0x54a21d2au - i / 3u);
-    float r3 = (r * r) * r;
-    float e = fma_(-x, r3, 1.0f);
-    r = fma_(r * e, kThird, r);
-    r3 = (r * r) * r;
-    e = fma_(-x, r3, 1.0f);
-    r = fma_(r * e, kThird, r);
-    float y = (x * r) * r;
-    const float y2 = y * y;
-    e = fma_(-y2, y, x);
-    y = fma_(e * (r * r), kThird, y);
-    return x > 0.0f ? y : 0.0f;
+    const uint32_t third_bits = static_cast<uint32_t>(static_cast<float>(__float_as_uint(x)) * kThird);
+    float r = __uint_as_float(0x54a21d2au - third_bits);
+    const float x3 = x * kThird;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const float r3 = (r * r) * r;
+        r = fma_(r, fma_(-x3, r3, kThird), r);
+    }
+    const float rr = r * r;
+    const float y = x * rr;
+    const float e = fma_(-(y * y), y, x);
+    return fma_(e, rr * kThird, y);
 }
 
 // XYB of one linear sample, only the channels in `mask` (bit 0 X, 1 Y, 2 B).
-// prepare.comp:81-98 with its multiply-adds fused.
+// prepare.comp:81-98 with its multiply-adds fused; the halvings of
+// c0 - c1 and c0 + c1 are exact, so X = 7 (c0 - c1) + 0.42,
+// Y = s/2 + 0.01 and B = (c2 - s/2) + 0.55 round as the unfused order.
 __device__ __forceinline__ void xyb(float r, float g, float b, uint32_t mask, float out[3]) {
     float m0 = fma_(kM00, r, fma_(kM01, g, fma_(kM02, b, kOpsinBias)));
     float m1 = fma_(kM10, r, fma_(kM11, g, fma_(kM12, b, kOpsinBias)));
-    m0 = fmaxf(m0, 0.0f);
-    m1 = fmaxf(m1, 0.0f);
+    m0 = fmaxf(m0, kMinNormal);
+    m1 = fmaxf(m1, kMinNormal);
     const float c0 = cbrt_dev(m0) - kCbrtBias;
     const float c1 = cbrt_dev(m1) - kCbrtBias;
-    const float half_sum = 0.5f * (c0 + c1);
-    out[0] = fma_(0.5f * (c0 - c1), 14.0f, 0.42f);
-    out[1] = half_sum + 0.01f;
+    const float sum = c0 + c1;
+    out[0] = fma_(c0 - c1, 7.0f, 0.42f);
+    out[1] = fma_(0.5f, sum, 0.01f);
     out[2] = 0.0f;
     if (mask & 4u) {
         float m2 = fma_(kM20, r, fma_(kM21, g, fma_(kM22, b, kOpsinBias)));
-        m2 = fmaxf(m2, 0.0f);
+        m2 = fmaxf(m2, kMinNormal);
         const float c2 = cbrt_dev(m2) - kCbrtBias;
-        out[2] = (c2 - half_sum) + 0.55f;
+        out[2] = fma_(-0.5f, sum, c2) + 0.55f;
     }
 }
 
@@ -797,17 +806,20 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
                         // v + t^2 num_s. The small v replaces the difference
                         // of two large second moments (closer to FP64 than
                         // the direct form), and identical images give
-                        // v = t = 0, d = 0 exactly.
+                        // v = t = 0, d = 0 exactly. One IEEE division
+                        // serves both quotients: q = 1 / (den det_orig),
+                        // with den >= c2 - rounding > 0 and det_orig >= 1.
                         const float mu1 = bl[0][c], mu2 = bl[1][c];
                         const float t = mu1 - mu2;
                         const float num_s = fma_(2.0f, fma_(-mu1, mu2, bl[3][c]), kC2);
                         const float v = fma_(-t, t, bl[2][c]);
                         const float den = num_s + v;
-                        const float d = fmaxf(fma_(t * t, num_s, v) / den, 0.0f);
-                        const float dd = d * d;
                         const float det_dist = 1.0f + fabsf(bv[c] - mu2);
                         const float det_orig = 1.0f + fabsf(av[c] - mu1);
-                        const float d1 = (det_dist - det_orig) / det_orig;
+                        const float q = 1.0f / (den * det_orig);
+                        const float d = fmaxf((fma_(t * t, num_s, v) * det_orig) * q, 0.0f);
+                        const float dd = d * d;
+                        const float d1 = ((det_dist - det_orig) * den) * q;
                         const float ring = fmaxf(d1, 0.0f);
                         const float blur = fmaxf(-d1, 0.0f);
                         const float rr = ring * ring, bb = blur * blur;
