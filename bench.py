@@ -42,6 +42,29 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(kernel: str, planes: str, w: int, h: int, pairs: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by the
+    plain copy kernel's reading of a known 1 GiB), or None when no summary
+    covers it."""
+    if (w, h, pairs) != (1920, 1080, 300):
+        return None
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"ss2_kernels_1080p_batch300_{planes}.md")
+    if not os.path.exists(path):
+        return None
+    row = f"| {kernel}_kernel<{planes.upper()}> |"
+    derived = False
+    for line in open(path):
+        if line.startswith("| kernel | HBM read GB"):
+            derived = True
+        elif derived and line.startswith(row):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            return {"bytes": round((float(cells[1]) + float(cells[2])) * 1e9),
+                    "source": os.path.relpath(path, os.path.dirname(os.path.abspath(__file__)))}
+    return None
+
+
 def plane_bytes(w: int, h: int, fmt: int) -> int:
     """Stored plane bytes of one image (pruned planes excluded, no padding):
     the library's algorithmic pair bytes are 6N input + 4 x these."""
@@ -360,9 +383,11 @@ def main() -> int:
     dominant = max(("prepare", "scale"), key=lambda k: avg[k])
     achieved = launch_bytes[dominant] / (avg[dominant] * 1e-3) / 1e9
     pair_bytes = 2 * 3 * n_px + 4 * planes_bytes
+    traffic = profiled_traffic(dominant, args.planes, w, h, len(refs))
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                "traffic": None,
+                "traffic": traffic["bytes"] if traffic else None,
+                "traffic_source": traffic["source"] if traffic else None,
                 "bytes_per_launch": launch_bytes[dominant],
                 "stage_ms": {k: round(v, 4) for k, v in avg.items()},
                 "pair_algorithmic_bytes": pair_bytes,
