@@ -5,7 +5,9 @@ ABI; everything else runs on the CPU (oracle pins, host logic, ABI exports).
 """
 from __future__ import annotations
 
+import json
 import os
+import subprocess
 import sys
 
 import numpy as np
@@ -92,3 +94,33 @@ def oracle_lib():
     import oracle
     oracle.build()
     return oracle
+
+
+@pytest.fixture(scope="session")
+def image_loader(tmp_path_factory):
+    """The scoring command's file loaders (vqm_amd/cli/image_io.cc and
+    jpeg.cc) built into tests/cpp/image_io_dump.cc: load(data) writes the
+    bytes to a file and returns the decoded image as a dict (fields of the
+    format, "bytes" the packed planes) or {"error": ...}."""
+    d = tmp_path_factory.mktemp("io")
+    exe = str(d / "image_io_dump")
+    pkg = os.path.join(ROOT, "vqm_amd")
+    subprocess.run(["g++", "-std=c++20", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"),
+                    "-I", os.path.join(pkg, "cli"), os.path.join(ROOT, "tests", "cpp", "image_io_dump.cc"),
+                    os.path.join(pkg, "cli", "image_io.cc"), os.path.join(pkg, "cli", "jpeg.cc"), "-L", pkg,
+                    "-lvqm_hip", "-lz", f"-Wl,-rpath,{pkg}", "-o", exe], check=True)
+    counter = [0]
+
+    def run(data: bytes, sequence: bool = False):
+        counter[0] += 1
+        path = str(d / f"f{counter[0]}")
+        with open(path, "wb") as f:
+            f.write(data)
+        out = subprocess.run([exe] + (["--sequence"] if sequence else []) + [path], capture_output=True,
+                             text=True, check=True, timeout=60).stdout
+        objs = [json.loads(line) for line in out.splitlines()]
+        for o in objs:
+            if "data" in o:
+                o["bytes"] = bytes.fromhex(o["data"])
+        return objs if sequence else objs[0]
+    return run

@@ -174,3 +174,41 @@ def test_y4m_sequence_scores_the_mean_of_its_frames(tmp_path):
     write(pb, frames_b, rate=b"F30:1")
     r = subprocess.run([CLI, str(pa), str(pb)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and "Frame rate mismatch" in r.stderr
+
+
+def test_jpeg_pair_scores_like_its_decoded_samples(tmp_path):
+    """JPEG inputs (io.cc:277-355): the score of two JPEG files is the
+    library's score of the samples libjpeg-turbo decodes from them (read
+    through Pillow), 4:2:0 colour and progressive grey."""
+    PIL = pytest.importorskip("PIL.Image")
+    import io as _io
+    rng = np.random.default_rng(4)
+    y, x = np.mgrid[0:72, 0:96]
+    base = np.stack([x * 2, y * 3, (x + y) % 256], -1).astype(np.float64)
+    orig = np.clip(base + rng.normal(0, 4, base.shape), 0, 255).astype(np.uint8)
+    for mode, kw in (("RGB", dict(quality=92, subsampling=2)), ("L", dict(quality=90, progressive=True))):
+        src = orig if mode == "RGB" else orig[..., 0]
+        paths, samples = [], []
+        for i, q in enumerate((kw["quality"], 35)):
+            b = _io.BytesIO()
+            PIL.fromarray(src).save(b, "JPEG", **{**kw, "quality": q})
+            path = tmp_path / f"{mode}{i}.jpg"
+            path.write_bytes(b.getvalue())
+            paths.append(path)
+            samples.append(np.asarray(PIL.open(_io.BytesIO(b.getvalue()))))
+        fmt = ImageFormat.srgb8()
+        if mode == "L":
+            fmt = ImageFormat(type=PixelType.Byte, family=ColorFamily.Grey, matrix=MC.Bt709)
+        def planes(arr):
+            arr = arr[..., None] if arr.ndim == 2 else arr
+            n = 96 * 72
+            out = np.zeros(arr.shape[-1] * ((n + 3) // 4 * 4), np.uint8)
+            for c in range(arr.shape[-1]):
+                out[c * ((n + 3) // 4 * 4):c * ((n + 3) // 4 * 4) + n] = arr[..., c].ravel()
+            return out
+        with Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(96, 72))) as p:
+            want = p.Compute(Image(Size(96, 72), fmt, planes(samples[0])),
+                             Image(Size(96, 72), fmt, planes(samples[1]))).Score()
+        got = float(run(paths[0], paths[1])[0])
+        assert got == pytest.approx(want, abs=1e-8), mode
+        assert got < 100.0

@@ -22,29 +22,8 @@ ADAM7 = [(0, 0, 8, 8), (4, 0, 8, 8), (0, 4, 4, 8), (2, 0, 4, 4), (0, 2, 2, 4), (
 
 
 @pytest.fixture(scope="module")
-def load(tmp_path_factory):
-    d = tmp_path_factory.mktemp("io")
-    exe = str(d / "image_io_dump")
-    pkg = os.path.join(ROOT, "vqm_amd")
-    subprocess.run(["g++", "-std=c++20", "-O1", "-Wall", "-I", os.path.join(ROOT, "include"),
-                    "-I", os.path.join(pkg, "cli"), os.path.join(ROOT, "tests", "cpp", "image_io_dump.cc"),
-                    os.path.join(pkg, "cli", "image_io.cc"), "-L", pkg, "-lvqm_hip", "-lz",
-                    f"-Wl,-rpath,{pkg}", "-o", exe], check=True)
-    counter = [0]
-
-    def run(data: bytes, sequence: bool = False):
-        counter[0] += 1
-        path = str(d / f"f{counter[0]}")
-        with open(path, "wb") as f:
-            f.write(data)
-        out = subprocess.run([exe] + (["--sequence"] if sequence else []) + [path], capture_output=True,
-                             text=True, check=True, timeout=60).stdout
-        objs = [json.loads(line) for line in out.splitlines()]
-        for o in objs:
-            if "data" in o:
-                o["bytes"] = bytes.fromhex(o["data"])
-        return objs if sequence else objs[0]
-    return run
+def load(image_loader):
+    return image_loader
 
 
 # ---- a PNG writer from the specification ----------------------------------------------

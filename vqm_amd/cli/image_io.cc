@@ -1,4 +1,4 @@
-// PNG and YUV4MPEG2 loading for the scoring command. PNG follows the W3C
+// PNG and YUV4MPEG2 loading for the scoring command (JPEG is in jpeg.cc). PNG follows the W3C
 // PNG specification (chunks, zlib stream, the five row filters, Adam7) with
 // zlib's inflate; the output matches what the reference's libpng set-up
 // (tools/image_io/io.cc:156-238: expand, palette to RGB, tRNS to alpha,
@@ -583,14 +583,11 @@ std::optional<Image> LoadImage(std::string_view path, std::string& error) {
             return std::nullopt;
         }
     }
-    if (got >= 3 && head[0] == 0xff && head[1] == 0xd8 && head[2] == 0xff) {
+    // io.cc:76-87: JPEG by its first two bytes, PNG by its signature.
+    const bool jpeg = got >= 2 && head[0] == 0xff && head[1] == 0xd8;
+    if (!jpeg && (got < 8 || std::memcmp(head, kPngSignature, 8) != 0)) {
         std::fclose(f);
-        error = "JPEG input is not supported (no libjpeg-compatible decoder)";
-        return std::nullopt;
-    }
-    if (got < 8 || std::memcmp(head, kPngSignature, 8) != 0) {
-        std::fclose(f);
-        error = "unsupported image format (PNG and YUV4MPEG2 are supported)";
+        error = "unsupported image format (PNG, JPEG and YUV4MPEG2 are supported)";
         return std::nullopt;
     }
     std::vector<std::uint8_t> bytes;
@@ -609,7 +606,7 @@ std::optional<Image> LoadImage(std::string_view path, std::string& error) {
         error = "could not read file";
         return std::nullopt;
     }
-    return DecodePng(bytes.data(), bytes.size(), error);
+    return jpeg ? DecodeJpeg(bytes.data(), bytes.size(), error) : DecodePng(bytes.data(), bytes.size(), error);
 }
 
 // ---- --input-* option values (io.cc:690-780) -----------------------------------------

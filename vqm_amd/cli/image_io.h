@@ -1,4 +1,4 @@
-// On-disk inputs of the scoring command: PNG and YUV4MPEG2 decoded to the
+// On-disk inputs of the scoring command: PNG, JPEG and YUV4MPEG2 decoded to the
 // packed-plane Image of include/ssimulacra2.h, with the colour declarations
 // the reference's loader gives them (tools/image_io/io.h:17-60, io.cc).
 //
@@ -9,8 +9,10 @@
 //   420paldv, 422, 444, 411 and 444alpha, with p9/p10/p12/p14/p16 depth
 //   suffixes (little-endian words); limited range, BT.470BG matrix, BT.709
 //   transfer and primaries, all overridable by the --input-* options.
-// * JPEG is recognised and refused: this image has no libjpeg, and a decoder
-//   whose output differs from libjpeg-turbo's would change scores.
+// * JPEG (baseline, extended and progressive Huffman; 1 or 3 components):
+//   decoded to the samples libjpeg's default set-up yields (islow IDCT,
+//   fancy upsampling, JFIF YCbCr to RGB), as 8-bit RGB or grey with sRGB
+//   transfer and BT.709 primaries (io.cc:277-355).
 #pragma once
 
 #include <cstdint>
@@ -24,11 +26,14 @@
 
 namespace vqm::image_io {
 
-// The first frame of a PNG or Y4M file; nullopt with `error` set on failure.
+// The first frame of a PNG, JPEG or Y4M file; nullopt with `error` set on failure.
 std::optional<Image> LoadImage(std::string_view path, std::string& error);
 
 // PNG from memory (the decoder LoadImage uses for PNG files).
 std::optional<Image> DecodePng(const std::uint8_t* data, std::size_t size, std::string& error);
+
+// JPEG from memory (the decoder LoadImage uses for JPEG files).
+std::optional<Image> DecodeJpeg(const std::uint8_t* data, std::size_t size, std::string& error);
 
 // Rational frame rate of a Y4M stream (F tag); 0/0 when absent or invalid.
 struct FrameRate {

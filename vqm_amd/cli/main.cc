@@ -38,7 +38,7 @@ const char* kUsage =
     "          [--input-matrix M] [--input-transfer T] [--input-primaries P] [--input-range R]\n"
     "          original distorted\n"
     "\n"
-    "Inputs may be PNG (8/16-bit) or progressive YUV4MPEG2 (all frames are scored\n"
+    "Inputs may be PNG (8/16-bit), JPEG or progressive YUV4MPEG2 (all frames are scored\n"
     "and averaged; both inputs need the same frame count and frame rate).\n"
     "Y4M chroma tags: Cmono, C420, C420jpeg, C420mpeg2, C420paldv, C422, C444, C411,\n"
     "C444alpha, with p9, p10, p12, p14 or p16 for high bit depths. PNG is sRGB with\n"
