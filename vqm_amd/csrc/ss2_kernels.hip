@@ -494,16 +494,17 @@ __device__ __forceinline__ void prepare_level(ConstAS<Plan>* plan, GlobalAS<uint
     for (uint32_t c = 0; c < 3; ++c) next[c][jy * side_x + jx] = v[c];
 }
 
-// One prepare work item: tile `b` of the launch's work (a 64 x 32 base-pixel
-// tile of one image of one pair). Every thread of the block calls it with the
-// same `b`; the LUT's LDS stores are issued before the first call.
-template <typename Lay, bool General>
-__device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restrict__ pairs,
-                                             const WorkRef* __restrict__ work, uint32_t blocks,
-                                             uint32_t tiles_per_image, const float* lut,
-                                             float (*ladder)[3][kLadder], const uint8_t* src0,
-                                             const uint8_t* src1) {
-    const uint32_t tid = threadIdx.x;
+// Work item b's image, pair and tile origin (the launch's work list, or the
+// arithmetic order of a uniform batch).
+struct TileRef {
+    ConstAS<PairDesc>* pdp;
+    ConstAS<Plan>* plan;
+    uint32_t image, X0, Y0;
+};
+
+__device__ __forceinline__ TileRef resolve_tile(uint32_t b, const PairDesc* __restrict__ pairs,
+                                                const WorkRef* __restrict__ work, uint32_t blocks,
+                                                uint32_t tiles_per_image) {
     uint32_t pair_idx, image, tile;
     if (work != nullptr) {
         ConstAS<WorkRef>* w = as_const(work) + b;
@@ -516,12 +517,63 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
         image = (l / tiles_per_image) & 1u;
         pair_idx = l / (2 * tiles_per_image);
     }
-    ConstAS<PairDesc>* pdp = as_const(pairs) + pair_idx;
-    ConstAS<Plan>* plan = as_const(pdp->plan);
+    TileRef t;
+    t.pdp = as_const(pairs) + pair_idx;
+    t.plan = as_const(t.pdp->plan);
+    t.image = image;
+    const uint32_t tiles_x = t.plan->prep_tiles_x;
+    t.X0 = (tile % tiles_x) * kPrepTileW;
+    t.Y0 = (tile / tiles_x) * kPrepTileH;
+    return t;
+}
+
+// The 8-bit table path's source bytes of this thread's 4 x 2 pixels: one
+// 32-bit word per row and channel, when the row segment is whole and
+// 4-aligned (valid); loaded one tile ahead so their latency overlaps the
+// current tile's arithmetic.
+struct TileWords {
+    uint32_t w[2][3];
+    bool valid;
+};
+
+__device__ __forceinline__ TileWords load_tile_words(const TileRef& t, const uint8_t* src0, const uint8_t* src1) {
+    TileWords out;
+    out.valid = false;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t W = t.plan->width, H = t.plan->height;
+    const uint32_t bx = t.X0 + 4u * (tid % 16u);
+    if (W % 4u != 0u || bx + 3u >= W) return out;
+    const uint8_t* src_ptr = t.image ? (src1 != nullptr ? src1 : t.pdp->src[1])
+                                     : (src0 != nullptr ? src0 : t.pdp->src[0]);
+    GlobalAS<const uint8_t>* src = as_global(src_ptr);
+    const uint64_t plane_stride = t.plan->plane_stride;
+#pragma unroll
+    for (uint32_t r = 0; r < 2; ++r) {
+        const uint32_t y = min(t.Y0 + 2u * (tid / 16u) + r, H - 1u);
+#pragma unroll
+        for (uint32_t c = 0; c < 3; ++c)
+            out.w[r][c] = *reinterpret_cast<GlobalAS<const uint32_t>*>(src + c * plane_stride +
+                                                                       static_cast<uint64_t>(y) * W + bx);
+    }
+    out.valid = true;
+    return out;
+}
+
+// One prepare work item: tile `b` of the launch's work (a 64 x 32 base-pixel
+// tile of one image of one pair). Every thread of the block calls it with the
+// same `b`; the LUT's LDS stores are issued before the first call.
+template <typename Lay, bool General>
+__device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restrict__ pairs,
+                                             const WorkRef* __restrict__ work, uint32_t blocks,
+                                             uint32_t tiles_per_image, const float* lut,
+                                             float (*ladder)[3][kLadder], const uint8_t* src0,
+                                             const uint8_t* src1, TileWords pre) {
+    const uint32_t tid = threadIdx.x;
+    const TileRef tr = resolve_tile(b, pairs, work, blocks, tiles_per_image);
+    ConstAS<PairDesc>* pdp = tr.pdp;
+    ConstAS<Plan>* plan = tr.plan;
+    const uint32_t image = tr.image, X0 = tr.X0, Y0 = tr.Y0;
     const uint32_t W = plan->width, H = plan->height, levels = plan->levels;
-    const uint32_t tiles_x = plan->prep_tiles_x;
-    const uint32_t tile_x = tile % tiles_x, tile_y = tile / tiles_x;
-    const uint32_t X0 = tile_x * kPrepTileW, Y0 = tile_y * kPrepTileH;
     const uint32_t tx = tid % 16u, ty = tid / 16u;
     GlobalAS<uint8_t>* planes = as_global(static_cast<uint8_t*>(pdp->planes)) + (image ? plan->image_bytes : 0);
     // Single-pair launches name their inputs directly; batches use pd.src.
@@ -565,7 +617,7 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
             for (uint32_t c = 0; c < 3; ++c) {
                 GlobalAS<const uint8_t>* row = src + c * plane_stride + static_cast<uint64_t>(y) * W;
                 if (vec) {
-                    const uint32_t v = *reinterpret_cast<GlobalAS<const uint32_t>*>(row + bx);
+                    const uint32_t v = pre.valid ? pre.w[r][c] : *reinterpret_cast<GlobalAS<const uint32_t>*>(row + bx);
 #pragma unroll
                     for (uint32_t j = 0; j < 4; ++j) lin[r][j][c] = lut[(v >> (8u * j)) & 255u];
                 } else {
@@ -665,9 +717,29 @@ __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict
     // Two ladders alternate between levels: level l publishes into l & 1.
     __shared__ __align__(16) float ladder[2][3][kLadder];
     lut[threadIdx.x] = as_global(lut_g)[threadIdx.x];  // published inside prepare_tile
-    for (uint32_t b = blockIdx.x; b < blocks; b += gridDim.x) {
-        prepare_tile<Lay, General>(b, pairs, work, blocks, tiles_per_image, lut, ladder, src0, src1);
-        __syncthreads();  // the next tile reuses the ladder
+    if constexpr (General) {
+        TileWords none;
+        none.valid = false;
+        for (uint32_t b = blockIdx.x; b < blocks; b += gridDim.x) {
+            prepare_tile<Lay, General>(b, pairs, work, blocks, tiles_per_image, lut, ladder, src0, src1, none);
+            __syncthreads();  // the next tile reuses the ladder
+        }
+    } else {
+        // The 8-bit table path loads each next tile's source bytes before
+        // working on the current one.
+        uint32_t b = blockIdx.x;
+        if (b >= blocks) return;
+        TileWords cur = load_tile_words(resolve_tile(b, pairs, work, blocks, tiles_per_image), src0, src1);
+        for (; b < blocks; b += gridDim.x) {
+            const uint32_t next = b + gridDim.x;
+            TileWords ahead;
+            ahead.valid = false;
+            if (next < blocks)
+                ahead = load_tile_words(resolve_tile(next, pairs, work, blocks, tiles_per_image), src0, src1);
+            prepare_tile<Lay, General>(b, pairs, work, blocks, tiles_per_image, lut, ladder, src0, src1, cur);
+            __syncthreads();  // the next tile reuses the ladder
+            cur = ahead;
+        }
     }
 }
 
