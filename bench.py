@@ -61,6 +61,7 @@ def profiled_traffic(kernel: str, planes: str, w: int, h: int, pairs: int):
         elif derived and line.startswith(row):
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
             return {"bytes": round((float(cells[1]) + float(cells[2])) * 1e9),
+                    "valu_per_cu_cycle": float(cells[5]),
                     "source": os.path.relpath(path, os.path.dirname(os.path.abspath(__file__)))}
     return None
 
@@ -388,6 +389,12 @@ def main() -> int:
                 "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                 "traffic": traffic["bytes"] if traffic else None,
                 "traffic_source": traffic["source"] if traffic else None,
+                # The roof that binds: wave64 VALU instructions issued per CU
+                # per cycle (non-packed FP32 issues at most one), from the
+                # same counter summary.
+                "valu_issue": ({"achieved": traffic["valu_per_cu_cycle"], "peak": 1.0,
+                                "unit": "wave64 VALU instructions per CU-cycle",
+                                "frac": traffic["valu_per_cu_cycle"]} if traffic else None),
                 "bytes_per_launch": launch_bytes[dominant],
                 "stage_ms": {k: round(v, 4) for k, v in avg.items()},
                 "pair_algorithmic_bytes": pair_bytes,
