@@ -181,11 +181,22 @@ void xyb_f32(float r, float g, float b, int cbrt_kind, float out[3]) {
     mix0 = std::max(mix0, 0.0f);
     mix1 = std::max(mix1, 0.0f);
     mix2 = std::max(mix2, 0.0f);
-    float (*cb)(float) = cbrt_kind == 1 ? cbrt_dev : cbrt_exact;
     const float cbias = static_cast<float>(kCbrtBias);
-    const float c0 = cb(mix0) - cbias;
-    const float c1 = cb(mix1) - cbias;
-    const float c2 = cb(mix2) - cbias;
+    if (cbrt_kind == 1) {
+        // The device form cancels the bias where it cancels exactly:
+        // X = 7 (cbrt0 - cbrt1) + 0.42, Y = (cbrt0 + cbrt1)/2 + (0.01 - bias),
+        // B = cbrt2 - (cbrt0 + cbrt1)/2 + 0.55 (fused as the kernels do).
+        const float r0 = cbrt_dev(mix0), r1 = cbrt_dev(mix1), r2 = cbrt_dev(mix2);
+        const float sum = r0 + r1;
+        const float y_offset = 0.01f - cbias;
+        out[0] = std::fma(r0 - r1, 7.0f, 0.42f);
+        out[1] = std::fma(0.5f, sum, y_offset);
+        out[2] = std::fma(-0.5f, sum, r2) + 0.55f;
+        return;
+    }
+    const float c0 = cbrt_exact(mix0) - cbias;
+    const float c1 = cbrt_exact(mix1) - cbias;
+    const float c2 = cbrt_exact(mix2) - cbias;
     const float half_sum = 0.5f * (c0 + c1);
     out[0] = std::fma(0.5f * (c0 - c1), 14.0f, 0.42f);
     out[1] = half_sum + 0.01f;
@@ -352,6 +363,7 @@ void map_sums(const T* a, const T* b, uint32_t w, uint32_t h, int order,
     for (int k = 0; k < 6; ++k) out[k] = 0.0;
     for (uint32_t y = 0; y < h; ++y) {
         T quad[6] = {0, 0, 0, 0, 0, 0};
+        T pair[6] = {0, 0, 0, 0, 0, 0};
         int in_quad = 0;
         for (uint32_t x = 0; x < w; ++x) {
             const size_t i = static_cast<size_t>(y) * w + x;
@@ -389,6 +401,12 @@ void map_sums(const T* a, const T* b, uint32_t w, uint32_t h, int order,
                     }
                     in_quad = 0;
                 }
+            } else if (sum_variances) {
+                // The device form adds columns 2j and 2j+1 (one lane's two
+                // columns) in FP32, then accumulates in FP64.
+                for (int k = 0; k < 6; ++k) pair[k] = (x & 1u) == 0 ? v[k] : pair[k] + v[k];
+                if ((x & 1u) == 1 || x + 1 == w)
+                    for (int k = 0; k < 6; ++k) out[k] += static_cast<double>(pair[k]);
             } else {
                 for (int k = 0; k < 6; ++k) out[k] += static_cast<double>(v[k]);
             }

@@ -172,6 +172,11 @@ def test_xyb_closed_forms(oracle_lib):
         g = oracle_lib.xyb_f64(v, v, v)
         assert abs(g[0] - 0.42) < 1e-15
         assert g[1] == pytest.approx(np.cbrt(v + bias) - cbias + 0.01, abs=1e-15)
+    # The device form (bias cancelled, fused): black and greys keep X and B
+    # exactly at their offsets, as the reference order does.
+    for v in (0.0, 0.05, 0.5, 1.0):
+        dev = oracle_lib.xyb_f32(v, v, v, 1)
+        assert dev[0] == np.float32(0.42) and dev[2] == np.float32(0.55)
     rng = np.random.default_rng(1)
     for _ in range(200):
         r, g, b = rng.random(3).astype(np.float32)

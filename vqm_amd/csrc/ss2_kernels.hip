@@ -71,8 +71,8 @@ __device__ __forceinline__ float fma_(float a, float b, float c) {
 }
 
 // Smallest normal float: the XYB mixes are clamped to it instead of 0, so the
-// cube root never sees zero or a denormal. Its root, 2^-42, is below half an
-// ulp of kCbrtBias, so cbrt - kCbrtBias rounds to -kCbrtBias exactly as for 0.
+// cube root never sees zero or a denormal. Its root, 2^-42, vanishes in every
+// sum it enters (all are at least ~1e-2 away from it), as a zero root would.
 constexpr float kMinNormal = 0x1p-126f;
 
 // Cube root of a normal float: bit-trick seed of x^(-1/3) (the bits divided
@@ -96,25 +96,28 @@ __device__ __forceinline__ float cbrt_dev(float x) {
 }
 
 // XYB of one linear sample, only the channels in `mask` (bit 0 X, 1 Y, 2 B).
-// prepare.comp:81-98 with its multiply-adds fused; the halvings of
-// c0 - c1 and c0 + c1 are exact, so X = 7 (c0 - c1) + 0.42,
-// Y = s/2 + 0.01 and B = (c2 - s/2) + 0.55 round as the unfused order.
+// prepare.comp:81-98 with its multiply-adds fused and the cube-root bias
+// cancelled where it cancels exactly: with c_i = cbrt(m_i) - bias,
+// X = 7 (c0 - c1) + 0.42 = 7 (cbrt0 - cbrt1) + 0.42,
+// Y = (c0 + c1)/2 + 0.01 = (cbrt0 + cbrt1)/2 + (0.01 - bias) and
+// B = c2 - (c0 + c1)/2 + 0.55 = cbrt2 - (cbrt0 + cbrt1)/2 + 0.55, each
+// from the roots directly (three subtractions fewer, no rounding of c_i).
+constexpr float kYOffset = 0.01f - kCbrtBias;  // rounded once
+
 __device__ __forceinline__ void xyb(float r, float g, float b, uint32_t mask, float out[3]) {
     float m0 = fma_(kM00, r, fma_(kM01, g, fma_(kM02, b, kOpsinBias)));
     float m1 = fma_(kM10, r, fma_(kM11, g, fma_(kM12, b, kOpsinBias)));
     m0 = fmaxf(m0, kMinNormal);
     m1 = fmaxf(m1, kMinNormal);
-    const float c0 = cbrt_dev(m0) - kCbrtBias;
-    const float c1 = cbrt_dev(m1) - kCbrtBias;
-    const float sum = c0 + c1;
-    out[0] = fma_(c0 - c1, 7.0f, 0.42f);
-    out[1] = fma_(0.5f, sum, 0.01f);
+    const float r0 = cbrt_dev(m0), r1 = cbrt_dev(m1);
+    const float sum = r0 + r1;
+    out[0] = fma_(r0 - r1, 7.0f, 0.42f);
+    out[1] = fma_(0.5f, sum, kYOffset);
     out[2] = 0.0f;
     if (mask & 4u) {
         float m2 = fma_(kM20, r, fma_(kM21, g, fma_(kM22, b, kOpsinBias)));
         m2 = fmaxf(m2, kMinNormal);
-        const float c2 = cbrt_dev(m2) - kCbrtBias;
-        out[2] = fma_(-0.5f, sum, c2) + 0.55f;
+        out[2] = fma_(-0.5f, sum, cbrt_dev(m2)) + 0.55f;
     }
 }
 
@@ -869,6 +872,9 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
                 const float2 b2 = *reinterpret_cast<const float2*>(&ring_b[raw_slot][jm + kRadius]);
                 const float av[2] = {a2.x, a2.y}, bv[2] = {b2.x, b2.y};
                 if (y < H) {
+                    // The lane's two columns' map values are added in FP32,
+                    // then accumulated in FP64.
+                    float pair[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
                     for (int c = 0; c < 2; ++c) {
                         if (col + static_cast<uint32_t>(c) >= W) continue;
                         // scale.comp:305-333 rearranged: with e the blurred
@@ -895,13 +901,10 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
                         const float ring = fmaxf(d1, 0.0f);
                         const float blur = fmaxf(-d1, 0.0f);
                         const float rr = ring * ring, bb = blur * blur;
-                        sums[0] += static_cast<double>(d);
-                        sums[1] += static_cast<double>(dd * dd);
-                        sums[2] += static_cast<double>(ring);
-                        sums[3] += static_cast<double>(rr * rr);
-                        sums[4] += static_cast<double>(blur);
-                        sums[5] += static_cast<double>(bb * bb);
+                        const float v6[6] = {d, dd * dd, ring, rr * rr, blur, bb * bb};
+                        for (int k = 0; k < 6; ++k) pair[k] = c == 0 ? v6[k] : pair[k] + v6[k];
                     }
+                    for (int k = 0; k < 6; ++k) sums[k] += static_cast<double>(pair[k]);
                 }
             }
         });
