@@ -710,7 +710,7 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
 // when the grid covers the work, several per block for a persistent grid
 // sized to the resident occupancy. The grid is a multiple of 8, so item b
 // stays on XCD b % 8 and xcd_remap keeps neighbouring tiles on one L2.
-template <typename Lay, bool General>
+template <typename Lay, bool General, bool Early>
 __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict__ pairs,
                                                       const WorkRef* __restrict__ work,
                                                       uint32_t blocks, uint32_t tiles_per_image,
@@ -720,7 +720,7 @@ __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict
     // Two ladders alternate between levels: level l publishes into l & 1.
     __shared__ __align__(16) float ladder[2][3][kLadder];
     lut[threadIdx.x] = as_global(lut_g)[threadIdx.x];  // published inside prepare_tile
-    if constexpr (General) {
+    if constexpr (General || !Early) {
         TileWords none;
         none.valid = false;
         for (uint32_t b = blockIdx.x; b < blocks; b += gridDim.x) {
@@ -728,8 +728,10 @@ __global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict
             __syncthreads();  // the next tile reuses the ladder
         }
     } else {
-        // The 8-bit table path loads each next tile's source bytes before
-        // working on the current one.
+        // Uniform batches' 8-bit table path loads each tile's source bytes at
+        // kernel entry, and each next tile's before working on the current
+        // one. (Launches with a work list keep the loads after the barrier:
+        // measured faster for mixed-size FP32 batches.)
         uint32_t b = blockIdx.x;
         if (b >= blocks) return;
         TileWords cur = load_tile_words(resolve_tile(b, pairs, work, blocks, tiles_per_image), src0, src1);
@@ -1009,9 +1011,15 @@ template <typename Lay, bool General>
 hipError_t launch_prepare_g(const PairDesc* pairs, const WorkRef* work, uint32_t blocks, uint32_t tiles,
                             const float* lut, const uint8_t* src0, const uint8_t* src1, bool persistent,
                             hipStream_t stream) {
-    const uint32_t grid = GridOf(&prepare_kernel<Lay, General>, 256, blocks, persistent);
-    hipLaunchKernelGGL((prepare_kernel<Lay, General>), dim3(grid), dim3(256), 0, stream, pairs, work, blocks,
-                       tiles, lut, src0, src1);
+    if (work == nullptr) {
+        const uint32_t grid = GridOf(&prepare_kernel<Lay, General, true>, 256, blocks, persistent);
+        hipLaunchKernelGGL((prepare_kernel<Lay, General, true>), dim3(grid), dim3(256), 0, stream, pairs, work,
+                           blocks, tiles, lut, src0, src1);
+    } else {
+        const uint32_t grid = GridOf(&prepare_kernel<Lay, General, false>, 256, blocks, persistent);
+        hipLaunchKernelGGL((prepare_kernel<Lay, General, false>), dim3(grid), dim3(256), 0, stream, pairs, work,
+                           blocks, tiles, lut, src0, src1);
+    }
     return hipGetLastError();
 }
 
