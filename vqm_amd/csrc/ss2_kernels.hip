@@ -952,28 +952,44 @@ __global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ 
 
 // ---- reduce -----------------------------------------------------------------
 
-__global__ __launch_bounds__(128) void reduce_kernel(const PairDesc* __restrict__ pairs) {
+// One block per pair. Wave w takes the (level, channel) ranges w, w+4, ...
+// (18 at most); its lanes stride over the range's strips, each adding the
+// six partial sums of its strips in FP64, and the wave then sums its lanes.
+// Norm level*18 + k: k < 6 is channel k/2's SSIM' slot k%2; k >= 6 is
+// channel (k-6)/4's edge slot 2 + (k-6)%4 (ring 1-, 4-norm, blur 1-, 4-norm).
+constexpr uint32_t kReduceWaves = 4;
+
+__global__ __launch_bounds__(64 * kReduceWaves) void reduce_kernel(const PairDesc* __restrict__ pairs) {
     ConstAS<PairDesc>* pdp = as_const(pairs) + blockIdx.x;
     ConstAS<Plan>* plan = as_const(pdp->plan);
-    const uint32_t j = threadIdx.x;
-    if (j >= static_cast<uint32_t>(kNorms)) return;
-    const uint32_t level = j / 18u, k = j % 18u;
-    double value = 0.0;
-    if (level < plan->levels) {
-        const uint32_t c = k < 6u ? k / 2u : (k - 6u) / 4u;
-        const uint32_t slot = k < 6u ? k % 2u : 2u + (k - 6u) % 4u;
-        if (plan->mask[level] & (1u << c)) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    GlobalAS<const double>* partials = as_global(static_cast<const double*>(pdp->partials));
+    GlobalAS<double>* norms = as_global(pdp->norms);
+    for (uint32_t combo = wave; combo < static_cast<uint32_t>(kMaxScales * kChannels); combo += kReduceWaves) {
+        const uint32_t level = combo / kChannels, c = combo % kChannels;
+        const bool active = level < plan->levels && (plan->mask[level] & (1u << c)) != 0u;
+        double sum[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (active) {
             const uint32_t begin = plan->item_begin[level][c], count = plan->item_count[level][c];
-            GlobalAS<const double>* partials = as_global(static_cast<const double*>(pdp->partials));
-            double sum = 0.0;
-            for (uint32_t it = 0; it < count; ++it) sum += partials[(begin + it) * 6u + slot];
-            const double one_per_pixel =
-                1.0 / (static_cast<double>(plan->w[level]) * static_cast<double>(plan->h[level]));
-            value = one_per_pixel * sum;
+            for (uint32_t it = lane; it < count; it += 64u) {
+                GlobalAS<const double>* p = partials + static_cast<uint64_t>(begin + it) * 6u;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) sum[k] += p[k];
+            }
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sum[k] = wave_sum(sum[k]);
+        }
+        if (lane != 0) continue;
+        const double one_per_pixel =
+            active ? 1.0 / (static_cast<double>(plan->w[level]) * static_cast<double>(plan->h[level])) : 0.0;
+#pragma unroll
+        for (uint32_t slot = 0; slot < 6; ++slot) {
+            double value = one_per_pixel * sum[slot];
             if (slot & 1u) value = sqrt(sqrt(value));
+            const uint32_t k = slot < 2u ? 2u * c + slot : 6u + 4u * c + (slot - 2u);
+            norms[level * 18u + k] = value;
         }
     }
-    as_global(pdp->norms)[j] = value;
 }
 
 // A plain streaming copy (16-byte loads and stores): the reference point that
@@ -1091,7 +1107,7 @@ hipError_t LaunchScale(PlaneFormat format, uint32_t blur, const PairDesc* pairs,
 
 hipError_t LaunchReduce(const PairDesc* pairs, uint32_t pairs_count, hipStream_t stream) {
     if (pairs_count == 0) return hipSuccess;
-    hipLaunchKernelGGL(reduce_kernel, dim3(pairs_count), dim3(128), 0, stream, pairs);
+    hipLaunchKernelGGL(reduce_kernel, dim3(pairs_count), dim3(64 * kReduceWaves), 0, stream, pairs);
     return hipGetLastError();
 }
 
