@@ -73,6 +73,7 @@ def lib() -> ctypes.CDLL:
             "ss2o_srgb_lut_f32": (None, [f32p]),
             "ss2o_srgb_lut_f64": (None, [f64p]),
             "ss2o_cbrt_dev": (ctypes.c_float, [ctypes.c_float]),
+            "ss2o_rcp_dev": (ctypes.c_float, [ctypes.c_float]),
             "ss2o_xyb_f32": (None, [ctypes.c_float] * 3 + [ctypes.c_int32, f32p]),
             "ss2o_xyb_f64": (None, [ctypes.c_double] * 3 + [f64p]),
             "ss2o_blur_f32": (None, [f32p, f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32]),
@@ -188,6 +189,10 @@ def srgb_lut_f64() -> np.ndarray:
 
 def cbrt_dev(x: float) -> float:
     return float(lib().ss2o_cbrt_dev(x))
+
+
+def rcp_dev(x: float) -> float:
+    return float(lib().ss2o_rcp_dev(x))
 
 
 def xyb_f32(r: float, g: float, b: float, cbrt_kind: int = 1) -> np.ndarray:

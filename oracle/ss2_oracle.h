@@ -93,6 +93,10 @@ void ss2o_srgb_lut_f64(double lut[256]);
 /* The HIP kernels' cube root (bit-exact restatement). */
 float ss2o_cbrt_dev(float x);
 
+/* The kernels' reciprocal of a positive normal float (bit-trick seed and
+ * three Newton steps, <= 0.501 ulp). */
+float ss2o_rcp_dev(float x);
+
 /* linear RGB -> XYB in FP32 (fused multiply-adds as the kernels issue them)
  * and FP64. prepare.comp:81-98. */
 void ss2o_xyb_f32(float r, float g, float b, int32_t cbrt_kind, float out[3]);

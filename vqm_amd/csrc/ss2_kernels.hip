@@ -14,8 +14,8 @@
 //            4-norm fourth roots (reduce.comp:21-63, vulkan.cc:545-583).
 //
 // Arithmetic contract. Every FP32 operation below is an IEEE add, multiply,
-// fused multiply-add (written explicitly), correctly rounded divide, min/max
-// or abs, with contraction switched off, so the CPU oracle's FP32 "dev" form
+// fused multiply-add (written explicitly), min/max, abs or an integer/float
+// conversion (no division: see rcp_dev), with contraction switched off, so the CPU oracle's FP32 "dev" form
 // reproduces each pixel's values bit for bit. The blur runs horizontal pass
 // first with taps in ascending order, as scale.comp:149-301 does.
 //
@@ -93,6 +93,17 @@ __device__ __forceinline__ float cbrt_dev(float x) {
     const float y = x * rr;
     const float e = fma_(-(y * y), y, x);
     return fma_(e, rr * kThird, y);
+}
+
+// Reciprocal of a positive normal float: a bit-trick seed (within 5%) and
+// three Newton steps r += r (1 - x r). <= 0.501 ulp; fma only, so the oracle
+// reproduces it (the correctly rounded division takes ten instructions and a
+// quarter-rate reciprocal).
+__device__ __forceinline__ float rcp_dev(float x) {
+    float r = __uint_as_float(0x7ef311c7u - __float_as_uint(x));
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r = fma_(r, fma_(-x, r, 1.0f), r);
+    return r;
 }
 
 // XYB of one linear sample, only the channels in `mask` (bit 0 X, 1 Y, 2 B).
@@ -886,9 +897,9 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
                         // v + t^2 num_s. The small v replaces the difference
                         // of two large second moments (closer to FP64 than
                         // the direct form), and identical images give
-                        // v = t = 0, d = 0 exactly. One IEEE division
-                        // serves both quotients: q = 1 / (den det_orig),
-                        // with den >= c2 - rounding > 0 and det_orig >= 1.
+                        // v = t = 0, d = 0 exactly. One reciprocal serves
+                        // both quotients: q = 1 / (den det_orig), with
+                        // den >= c2 - rounding > 0 and det_orig >= 1.
                         const float mu1 = bl[0][c], mu2 = bl[1][c];
                         const float t = mu1 - mu2;
                         const float num_s = fma_(2.0f, fma_(-mu1, mu2, bl[3][c]), kC2);
@@ -896,7 +907,7 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
                         const float den = num_s + v;
                         const float det_dist = 1.0f + fabsf(bv[c] - mu2);
                         const float det_orig = 1.0f + fabsf(av[c] - mu1);
-                        const float q = 1.0f / (den * det_orig);
+                        const float q = rcp_dev(den * det_orig);
                         const float d = fmaxf((fma_(t * t, num_s, v) * det_orig) * q, 0.0f);
                         const float dd = d * d;
                         const float d1 = ((det_dist - det_orig) * den) * q;
