@@ -123,22 +123,21 @@ template <typename T> T fmaT(T a, T b, T c) { return std::fma(a, b, c); }
 // cbrt - bias rounds to -bias exactly as for a zero mix.
 constexpr float kMinNormal = 0x1p-126f;
 
-// Device cube root of max(x, 2^-126): a bit-trick seed for x^(-1/3) whose
-// division of the bits by 3 is one float multiply (uint -> float rounds to
-// nearest, float -> uint truncates), two division-free Newton steps
-// r <- r + r (1/3 - (x/3) r^3), then y = x r^2 and one Newton correction
-// of y. Faithfully rounded (<= 0.75 ulp) on the XYB domain; every operation
-// is an IEEE conversion, add, mul or fma, so CPU and GPU agree bit for bit.
+// Device cube root of max(x, 2^-126): a bit-trick seed for x^(-1/3),
+// magic - bits/3 formed in float (uint -> float rounds to nearest, one fma
+// with the magic 0x54a21d2a rounded to FP32, float -> uint truncates); one
+// third-order step r <- r (1 + h (1/3 + h (2/9 + h 14/81))) with
+// h = 1 - x r^3, the series of (1 - h)^(-1/3) to h^3; then y = x r^2 and one
+// Newton correction of y. <= 0.84 ulp on [2^-126, 64); every operation is
+// an IEEE conversion, add, mul or fma, so CPU and GPU agree bit for bit.
 float cbrt_dev(float x) {
     x = std::max(x, kMinNormal);
-    const float third = 0.333333343f;
-    const uint32_t third_bits = static_cast<uint32_t>(static_cast<float>(float_to_bits(x)) * third);
-    float r = bits_to_float(0x54a21d2au - third_bits);
-    const float x3 = x * third;
-    for (int k = 0; k < 2; ++k) {
-        const float r3 = (r * r) * r;
-        r = std::fma(r, std::fma(-x3, r3, third), r);
-    }
+    const float third = 0x1.555556p-2f, c2 = 0x1.c71c72p-3f, c3 = 0x1.61f9aep-3f;
+    const float seed = std::fma(static_cast<float>(float_to_bits(x)), -third, 1419910400.0f);
+    float r = bits_to_float(static_cast<uint32_t>(seed));
+    const float h = std::fma(-x, (r * r) * r, 1.0f);
+    const float p = std::fma(h, std::fma(h, c3, c2), third);
+    r = std::fma(r * h, p, r);
     const float rr = r * r;
     const float y = x * rr;
     const float e = std::fma(-(y * y), y, x);

@@ -187,7 +187,7 @@ def test_xyb_closed_forms(oracle_lib):
 
 
 def test_device_cube_root(oracle_lib):
-    """The kernels' Newton cube root is faithful (<= 0.75 ulp) on the XYB
+    """The kernels' cube root is faithful (<= 0.75 ulp) on the XYB
     domain and exact at perfect cubes of powers of two; a zero or negative
     mix is clamped to the smallest normal, whose root 2^-42 vanishes against
     the bias (c = -bias exactly, as for a zero mix)."""

@@ -75,20 +75,23 @@ __device__ __forceinline__ float fma_(float a, float b, float c) {
 // sum it enters (all are at least ~1e-2 away from it), as a zero root would.
 constexpr float kMinNormal = 0x1p-126f;
 
-// Cube root of a normal float: bit-trick seed of x^(-1/3) (the bits divided
-// by 3 through one float multiply), two division-free Newton steps
-// r <- r + r (1/3 - (x/3) r^3), then y = x r^2 with one Newton correction.
-// <= 0.75 ulp on the XYB domain; every step is an IEEE conversion, add, mul
-// or fma, so the oracle reproduces it bit for bit.
+// Cube root of a normal float: bit-trick seed of x^(-1/3) (magic - bits/3,
+// formed in float: one conversion each way around one fma), one third-order
+// step r <- r (1 + h/3 + 2h^2/9 + 14h^3/81) with h = 1 - x r^3 (the series
+// of (1 - h)^(-1/3); |h| <= 0.105 after the seed leaves r within 2.3e-5),
+// then y = x r^2 with one Newton correction. <= 0.84 ulp for x in
+// [2^-126, 64); every step is an IEEE conversion, add, mul or fma, so the
+// oracle reproduces it bit for bit. 16 operations.
+constexpr float kCbrtMagic = 1419910400.0f;  // 0x54a21d2a rounded to FP32
+constexpr float kCbrtC2 = 0x1.c71c72p-3f;     // 2/9
+constexpr float kCbrtC3 = 0x1.61f9aep-3f;     // 14/81
+
 __device__ __forceinline__ float cbrt_dev(float x) {
-    const uint32_t third_bits = static_cast<uint32_t>(static_cast<float>(__float_as_uint(x)) * kThird);
-    float r = __uint_as_float(0x54a21d2au - third_bits);
-    const float x3 = x * kThird;
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const float r3 = (r * r) * r;
-        r = fma_(r, fma_(-x3, r3, kThird), r);
-    }
+    const float seed = fma_(static_cast<float>(__float_as_uint(x)), -kThird, kCbrtMagic);
+    float r = __uint_as_float(static_cast<uint32_t>(seed));
+    const float h = fma_(-x, (r * r) * r, 1.0f);
+    const float p = fma_(h, fma_(h, kCbrtC3, kCbrtC2), kThird);
+    r = fma_(r * h, p, r);
     const float rr = r * r;
     const float y = x * rr;
     const float e = fma_(-(y * y), y, x);
@@ -721,8 +724,10 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
 // when the grid covers the work, several per block for a persistent grid
 // sized to the resident occupancy. The grid is a multiple of 8, so item b
 // stays on XCD b % 8 and xcd_remap keeps neighbouring tiles on one L2.
+// The table-decode instantiations are held to 80 VGPRs (six waves per SIMD)
+// without spilling; the per-pixel decode of general formats needs ~130.
 template <typename Lay, bool General, bool Early>
-__global__ __launch_bounds__(256) void prepare_kernel(const PairDesc* __restrict__ pairs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(General ? 1 : 6))) void prepare_kernel(const PairDesc* __restrict__ pairs,
                                                       const WorkRef* __restrict__ work,
                                                       uint32_t blocks, uint32_t tiles_per_image,
                                                       const float* __restrict__ lut_g,
