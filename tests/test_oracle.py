@@ -207,6 +207,19 @@ def test_device_cube_root(oracle_lib):
         ulp = np.spacing(np.float32(exact))
         worst = max(worst, abs(y - exact) / ulp)
     assert worst <= 0.75
+    # Every 4099th float of [2^-126, 64) (HDR mixes reach past 1): the seed's
+    # error, largest where the bits sit mid-binade, is left within 0.84 ulp,
+    # and the power-of-two cubes 2^(3k) have roots exactly 2^k.
+    lo = int(np.float32(2.0 ** -126).view(np.uint32))
+    hi = int(np.float32(64.0).view(np.uint32))
+    worst = 0.0
+    for bits in range(lo, hi, 4099):
+        x = float(np.uint32(bits).view(np.float32))
+        exact = np.cbrt(np.float64(x))
+        worst = max(worst, abs(oracle_lib.cbrt_dev(x) - exact) / np.spacing(np.float32(exact)))
+    assert worst <= 0.84
+    for k in range(-40, 3):
+        assert oracle_lib.cbrt_dev(2.0 ** (3 * k)) == 2.0 ** k
 
 
 def test_device_reciprocal(oracle_lib):
