@@ -512,7 +512,7 @@ __device__ __forceinline__ void prepare_level(ConstAS<Plan>* plan, GlobalAS<uint
 }
 
 // Work item b's image, pair and tile origin (the launch's work list, or the
-// arithmetic order of a uniform batch).
+// arithmetic order of a uniform batch), either one read in xcd_remap order.
 struct TileRef {
     ConstAS<PairDesc>* pdp;
     ConstAS<Plan>* plan;
@@ -524,7 +524,7 @@ __device__ __forceinline__ TileRef resolve_tile(uint32_t b, const PairDesc* __re
                                                 uint32_t tiles_per_image) {
     uint32_t pair_idx, image, tile;
     if (work != nullptr) {
-        ConstAS<WorkRef>* w = as_const(work) + b;
+        ConstAS<WorkRef>* w = as_const(work) + xcd_remap(b, blocks);
         pair_idx = w->pair;
         image = w->index >> 31;
         tile = w->index & 0x7FFFFFFFu;
@@ -726,7 +726,7 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
 // stays on XCD b % 8 and xcd_remap keeps neighbouring tiles on one L2.
 // The table-decode instantiations are held to 80 VGPRs (six waves per SIMD)
 // without spilling; the per-pixel decode of general formats needs ~130.
-template <typename Lay, bool General, bool Early>
+template <typename Lay, bool General>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(General ? 1 : 6))) void prepare_kernel(const PairDesc* __restrict__ pairs,
                                                       const WorkRef* __restrict__ work,
                                                       uint32_t blocks, uint32_t tiles_per_image,
@@ -736,7 +736,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(General ? 1
     // Two ladders alternate between levels: level l publishes into l & 1.
     __shared__ __align__(16) float ladder[2][3][kLadder];
     lut[threadIdx.x] = as_global(lut_g)[threadIdx.x];  // published inside prepare_tile
-    if constexpr (General || !Early) {
+    if constexpr (General) {
         TileWords none;
         none.valid = false;
         for (uint32_t b = blockIdx.x; b < blocks; b += gridDim.x) {
@@ -744,10 +744,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(General ? 1
             __syncthreads();  // the next tile reuses the ladder
         }
     } else {
-        // Uniform batches' 8-bit table path loads each tile's source bytes at
-        // kernel entry, and each next tile's before working on the current
-        // one. (Launches with a work list keep the loads after the barrier:
-        // measured faster for mixed-size FP32 batches.)
+        // The 8-bit table path loads each tile's source bytes at kernel
+        // entry, and each next tile's before working on the current one.
         uint32_t b = blockIdx.x;
         if (b >= blocks) return;
         TileWords cur = load_tile_words(resolve_tile(b, pairs, work, blocks, tiles_per_image), src0, src1);
@@ -948,7 +946,7 @@ __global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ 
     for (uint32_t b = blockIdx.x; b < blocks; b += gridDim.x) {
         uint32_t pair_idx, item_idx;
         if (work != nullptr) {
-            ConstAS<WorkRef>* w = as_const(work) + b;
+            ConstAS<WorkRef>* w = as_const(work) + xcd_remap(b, blocks);
             pair_idx = w->pair;
             item_idx = w->index;
         } else {
@@ -1043,15 +1041,9 @@ template <typename Lay, bool General>
 hipError_t launch_prepare_g(const PairDesc* pairs, const WorkRef* work, uint32_t blocks, uint32_t tiles,
                             const float* lut, const uint8_t* src0, const uint8_t* src1, bool persistent,
                             hipStream_t stream) {
-    if (work == nullptr) {
-        const uint32_t grid = GridOf(&prepare_kernel<Lay, General, true>, 256, blocks, persistent);
-        hipLaunchKernelGGL((prepare_kernel<Lay, General, true>), dim3(grid), dim3(256), 0, stream, pairs, work,
-                           blocks, tiles, lut, src0, src1);
-    } else {
-        const uint32_t grid = GridOf(&prepare_kernel<Lay, General, false>, 256, blocks, persistent);
-        hipLaunchKernelGGL((prepare_kernel<Lay, General, false>), dim3(grid), dim3(256), 0, stream, pairs, work,
-                           blocks, tiles, lut, src0, src1);
-    }
+    const uint32_t grid = GridOf(&prepare_kernel<Lay, General>, 256, blocks, persistent);
+    hipLaunchKernelGGL((prepare_kernel<Lay, General>), dim3(grid), dim3(256), 0, stream, pairs, work, blocks,
+                       tiles, lut, src0, src1);
     return hipGetLastError();
 }
 
