@@ -326,7 +326,7 @@ typedef struct vqm_ss2_batch vqm_ss2_batch;
 
 typedef struct vqm_ss2_batch_config {
     uint32_t plane_format;   /* VQM_PLANES_* */
-    uint32_t strip_rows;     /* 0 = default */
+    uint32_t strip_rows;     /* 0 = default (128; pipelines and servers: 64) */
     uint32_t max_pairs;      /* pairs per submission */
     uint32_t chunks;         /* 0 or 1: prepare, then scale, on one stream; 2..16:
                                 runs of consecutive pairs whose scale launches

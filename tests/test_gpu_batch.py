@@ -38,8 +38,14 @@ def alone(w, h, a, b, fmt=0):
 def test_mixed_batch_equals_each_pair_alone(fmt):
     shapes = [(64, 64), (200, 130), (33, 17), (640, 360), (64, 64), (129, 97), (8, 8)]
     pairs, keep, host = upload_pairs(shapes, seed=11)
-    batch = Ssimulacra2Batch(max_pairs=16, plane_format=fmt)
+    # Bit for bit with a pipeline at the pipeline's strip height (64 rows);
+    # the batch default (128) groups the FP64 strip sums differently.
+    batch = Ssimulacra2Batch(max_pairs=16, plane_format=fmt, strip_rows=64)
     scales, norms = batch.Finish(batch.Submit(pairs), raw=True)
+    tall = Ssimulacra2Batch(max_pairs=16, plane_format=fmt)
+    _, tall_norms = tall.Finish(tall.Submit(pairs), raw=True)
+    tall.close()
+    np.testing.assert_allclose(tall_norms, norms, rtol=1e-12, atol=0)
     for i, ((w, h), (a, b)) in enumerate(zip(shapes, host)):
         assert np.array_equal(norms[i], alone(w, h, a, b, fmt)), f"pair {i} {w}x{h}"
         assert scales[i] == len(vqm_amd.plan_scales(Size(w, h)))
@@ -55,7 +61,7 @@ def test_mixed_batch_equals_each_pair_alone(fmt):
 def test_uniform_batch_equals_each_pair_alone():
     shapes = [(320, 180)] * 5
     pairs, keep, host = upload_pairs(shapes, seed=21)
-    batch = Ssimulacra2Batch(max_pairs=8)
+    batch = Ssimulacra2Batch(max_pairs=8, strip_rows=64)
     _, norms = batch.Finish(batch.Submit(pairs), raw=True)
     for i, (a, b) in enumerate(host):
         assert np.array_equal(norms[i], alone(320, 180, a, b))
@@ -65,10 +71,10 @@ def test_uniform_batch_equals_each_pair_alone():
 
 def test_batch_budget_and_misuse():
     pairs, keep, _ = upload_pairs([(256, 256)] * 3, seed=31)
-    batch = Ssimulacra2Batch(max_pairs=2, hbm_budget=1)
+    batch = Ssimulacra2Batch(max_pairs=2, hbm_budget=1, strip_rows=64)
     with pytest.raises(InvalidArgument):
         batch.Submit(pairs[:1])           # one workspace exceeds a 1-byte budget
-    big = Ssimulacra2Batch(max_pairs=2)
+    big = Ssimulacra2Batch(max_pairs=2, strip_rows=64)
     with pytest.raises(InvalidArgument):
         big.Submit(pairs)                 # more than max_pairs
     token = big.Submit(pairs[:2])
@@ -87,7 +93,7 @@ def test_chunked_batch_equals_each_pair_alone(chunks):
     stream: uniform and mixed batches, more chunks than pairs included."""
     for shapes in ([(160, 90)] * 7, [(64, 64), (200, 130), (33, 17), (640, 360), (129, 97)]):
         pairs, keep, host = upload_pairs(shapes, seed=51)
-        batch = Ssimulacra2Batch(max_pairs=8, chunks=chunks)
+        batch = Ssimulacra2Batch(max_pairs=8, chunks=chunks, strip_rows=64)
         _, norms = batch.Finish(batch.Submit(pairs), raw=True)
         for i, ((w, h), (a, b)) in enumerate(zip(shapes, host)):
             assert np.array_equal(norms[i], alone(w, h, a, b)), f"pair {i} {w}x{h}"

@@ -196,7 +196,7 @@ def test_recursive_blur_matches_oracle(oracle_lib, fmt):
     dev = [vqm_amd.DeviceBuffer(a.nbytes), vqm_amd.DeviceBuffer(b.nbytes)]
     dev[0].upload(a)
     dev[1].upload(b)
-    batch = vqm_amd.Ssimulacra2Batch(max_pairs=2, plane_format=fmt, blur=rec)
+    batch = vqm_amd.Ssimulacra2Batch(max_pairs=2, plane_format=fmt, blur=rec, strip_rows=64)
     scales, norms = batch.Finish(batch.Submit([(dev[0].ptr, dev[1].ptr, 512, 512)] * 2), raw=True)
     batch.close()
     for k in range(2):

@@ -785,7 +785,10 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
                                            float (*ring_a)[kSrcCols], float (*ring_b)[kSrcCols],
                                            float (*prod)[kSrcCols]) {
     const uint32_t W = plan->w[level], H = plan->h[level], pitch = plan->pitch[level];
-    const int total = static_cast<int>(plan->strip_h) + 2 * kRadius;
+    // The strip's output rows stop at the level's last row (y0 < H for every
+    // item): rows past it would be blurred and then dropped, up to half of
+    // a strip on the small levels.
+    const int total = static_cast<int>(min(plan->strip_h, H - y0)) + 2 * kRadius;
 
     using E = typename Fmt::E;
     // Source row r (0 .. total-1) is image row y0 - 6 + r, padded row y0 + r;

@@ -795,10 +795,16 @@ const vqm_ss2_batch_config& ValidatedBatch(const vqm_ss2_batch_config& config) {
 }
 }  // namespace
 
+// Batches default to taller strips than single pairs: with many pairs in a
+// launch there are blocks enough, and each strip's 12 halo rows are blurred
+// over 128 output rows instead of 64.
+constexpr uint32_t kBatchStripRows = 128;
+
 Batch::Batch(const vqm_ss2_batch_config& config)
     : config_(ValidatedBatch(config)),
       ctx_(DeviceContext::Get()),
-      pools_(static_cast<PlaneFormat>(config.plane_format), config.strip_rows, config.hbm_budget),
+      pools_(static_cast<PlaneFormat>(config.plane_format),
+             config.strip_rows != 0 ? config.strip_rows : kBatchStripRows, config.hbm_budget),
       lane_(config.max_pairs) {}
 
 Batch::~Batch() = default;
