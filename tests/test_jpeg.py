@@ -195,3 +195,29 @@ def test_errors(image_loader):
     PIL.fromarray(arr).convert("CMYK").save(b, "JPEG")
     assert "number of components" in image_loader(b.getvalue())["error"]
     assert "error" in image_loader(b"\xff\xd8\xff\xd9")
+
+
+def dht_segment(tc_th: int, counts, values) -> bytes:
+    body = bytes([tc_th]) + bytes(counts) + bytes(values)
+    return b"\xff\xc4" + (len(body) + 2).to_bytes(2, "big") + body
+
+
+@pytest.mark.parametrize("counts", [
+    [200] + [0] * 15,                 # 200 one-bit codes: far more than 2 fit
+    [2] + [0] * 15,                   # both one-bit codes: the second is all ones
+    [0, 0, 8] + [0] * 13,             # eight three-bit codes: 111 is all ones
+    [0, 0, 0, 20] + [0] * 12,         # twenty four-bit codes where sixteen exist
+    [1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2],  # the last sixteen-bit code is all ones
+])
+def test_malformed_huffman_tables_fail(image_loader, counts):
+    """Per-length code counts that do not fit their lengths are rejected
+    before any lookahead entry is written (a DHT segment alone, after SOI)."""
+    n = sum(counts)
+    data = b"\xff\xd8" + dht_segment(0x10, counts, [i & 0xFF for i in range(n)]) + b"\xff\xd9"
+    assert "Bogus Huffman table definition" in image_loader(data)["error"]
+
+
+def test_huffman_counts_past_256_fail(image_loader):
+    counts = [0] * 8 + [255, 2] + [0] * 6
+    data = b"\xff\xd8" + dht_segment(0x10, counts, [0] * 257) + b"\xff\xd9"
+    assert "Bogus Huffman table definition" in image_loader(data)["error"]

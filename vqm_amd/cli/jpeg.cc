@@ -65,7 +65,11 @@ struct Huffman {
 void BuildHuffman(Huffman& h, const std::uint8_t counts[16], const std::uint8_t* values, int n, bool dc) {
     h = Huffman{};
     h.defined = true;
+    if (n < 0 || n > 256) Fail("Bogus Huffman table definition");
     std::copy(values, values + n, h.values.begin());
+    // First pass: the canonical code ranges per length. Every length's codes
+    // must fit in `len` bits with room for one more (no code is all ones),
+    // checked before any code is used as an index.
     std::int32_t code = 0;
     int k = 0;
     for (int len = 1; len <= 16; ++len) {
@@ -73,18 +77,23 @@ void BuildHuffman(Huffman& h, const std::uint8_t counts[16], const std::uint8_t*
         h.valptr[len] = k;
         h.mincode[len] = code;
         h.maxcode[len] = c ? code + c - 1 : -1;
-        for (int i = 0; i < c; ++i, ++code, ++k) {
-            if (len <= 9) {
-                const int shift = 9 - len;
-                for (int j = 0; j < (1 << shift); ++j)
-                    h.fast[(code << shift) | j] = static_cast<std::uint16_t>((len << 8) | values[k]);
-            }
-        }
-        // The next code must still fit in `len` bits: no code is all ones.
+        code += c;
+        k += c;
         if (code >= (1 << len)) Fail("Bogus Huffman table definition");
         code <<= 1;
     }
+    if (k != n) Fail("Bogus Huffman table definition");
     h.maxcode[17] = 0x7FFFFFFF;  // sentinel
+    // Second pass: the 9-bit lookahead table, from ranges already checked
+    // (code << (9 - len) | j stays below 512 because code < 2^len).
+    for (int len = 1; len <= 9; ++len) {
+        const int shift = 9 - len;
+        for (std::int32_t cd = h.mincode[len]; cd <= h.maxcode[len]; ++cd) {
+            const int v = values[h.valptr[len] + (cd - h.mincode[len])];
+            for (int j = 0; j < (1 << shift); ++j)
+                h.fast[static_cast<std::size_t>((cd << shift) | j)] = static_cast<std::uint16_t>((len << 8) | v);
+        }
+    }
     if (dc)
         for (int i = 0; i < n; ++i)
             if (values[i] > 15) Fail("Bogus Huffman table definition");
