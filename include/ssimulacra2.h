@@ -60,6 +60,19 @@ enum class Blur : std::uint32_t { Sampled = VQM_BLUR_SAMPLED, Recursive = VQM_BL
 enum class PlaneFormat : std::uint32_t {
     F32 = VQM_PLANES_F32, F16 = VQM_PLANES_F16, BF16 = VQM_PLANES_BF16, U16 = VQM_PLANES_U16
 };
+// The arithmetic of the XYB transform and the SSIM' map (VQM_MAP_*; no
+// reference field): Reference is the reference's own operation order,
+// Difference the rearranged map that tracks FP64 more closely, Default the
+// reference order except with U16 planes.
+enum class MapForm : std::uint32_t {
+    Default = VQM_MAP_DEFAULT, Reference = VQM_MAP_REFERENCE, Difference = VQM_MAP_DIFFERENCE
+};
+// vqm_memory_types.h:14: the arena allocation path. Device memory in HBM is
+// the only path here: Automatic and Direct select it, Staged (a host-visible
+// staging arena in front of device memory) is unavailable, so a pipeline
+// configured with it throws std::runtime_error, as the reference does for a
+// strict policy whose path does not exist.
+enum class MemoryPolicy : std::uint8_t { Automatic, Direct, Staged };
 
 // vqm_image.h:100-120. The reference's default is Float sRGB RGB;
 // ImageFormat::Srgb8() is the 8-bit form the table-driven fast decode serves,
@@ -239,20 +252,28 @@ struct Ssimulacra2Result {
     }
 };
 
-// ssimulacra2.h:35-64 (memory_policy has no meaning here: one HBM pool).
+// ssimulacra2.h:35-64: the reference's fields in the reference's order, so
+// designated and positional initialisers written for it compile unchanged;
+// the fields without a reference counterpart follow them.
 struct Ssimulacra2Config {
     Size size;
     std::size_t gpu_resident_images = 2;
     bool half_planes = false;             // FP16 planes
+    ChromaFilter chroma_filter = ChromaFilter::Bilinear;
     std::size_t input_bytes = 0;          // caller-owned image pool
     bool streaming_inputs = true;
+    MemoryPolicy memory_policy = MemoryPolicy::Automatic;
+    // No reference fields below.
     PlaneFormat plane_format = PlaneFormat::F32;  // overridden to F16 by half_planes
     std::uint32_t strip_rows = 0;
-    bool graph = false;  // replay one captured HIP graph per pair (no reference field)
-    ChromaFilter chroma_filter = ChromaFilter::Bilinear;
-    Blur blur = Blur::Sampled;  // no reference field
+    bool graph = false;  // replay one captured HIP graph per pair
+    Blur blur = Blur::Sampled;
+    MapForm map_form = MapForm::Default;
 
     [[nodiscard]] vqm_ss2_config ToC() const {
+        if (memory_policy == MemoryPolicy::Staged)
+            throw std::runtime_error(
+                "MemoryPolicy::Staged is unavailable: pipelines hold their arena in device memory");
         vqm_ss2_config c{};
         vqm_ss2_config_init(&c, detail::Dim(size.width), detail::Dim(size.height));
         if (gpu_resident_images > 0xFFFFFFFFull)
@@ -265,6 +286,7 @@ struct Ssimulacra2Config {
         c.graph = graph ? 1u : 0u;
         c.chroma_filter = static_cast<std::uint32_t>(chroma_filter);
         c.blur = static_cast<std::uint32_t>(blur);
+        c.map_form = static_cast<std::uint32_t>(map_form);
         return c;
     }
 };

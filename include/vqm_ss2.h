@@ -76,6 +76,28 @@ enum { VQM_PLANES_F32 = 0, VQM_PLANES_F16 = 1, VQM_PLANES_BF16 = 2, VQM_PLANES_U
  *              counterpart. */
 enum { VQM_BLUR_SAMPLED = 0, VQM_BLUR_RECURSIVE = 1 };
 
+/* The arithmetic of the XYB transform and the SSIM' map (the map_form
+ * fields below):
+ *   REFERENCE   the reference's own operation order: prepare.comp:81-98
+ *               with correctly rounded cube roots, and scale.comp:130-142,
+ *               305-333: a, b, a*a, b*b and a*b blurred, the variances taken
+ *               as s11 - mu1^2 and s22 - mu2^2, IEEE divisions. Matches the
+ *               FP32 restatement of the reference bit for bit per pixel.
+ *   DIFFERENCE  the same map rearranged: a, b, (a - b)^2 and a*b blurred,
+ *               s11 + s22 - mu1^2 - mu2^2 formed as blur((a - b)^2) -
+ *               (mu1 - mu2)^2 + 2 (s12 - mu1 mu2), so no difference of two
+ *               large second moments is rounded; one reciprocal for both
+ *               quotients, a 0.84-ulp cube root. Closer to FP64 than the
+ *               reference's order (scores within ~1.5e-4 of FP64) and a
+ *               fifth less blur work, but on ill-conditioned pairs (the
+ *               reference's 32x32 gradient pair C) it moves away from the
+ *               reference's FP32 result by up to ~3e-3. No reference
+ *               counterpart.
+ *   DEFAULT     REFERENCE, except with VQM_PLANES_U16 planes (the mixed
+ *               form), whose 16-bit fixed-point rounding holds the 1e-3
+ *               score bound only with DIFFERENCE. */
+enum { VQM_MAP_DEFAULT = 0, VQM_MAP_REFERENCE = 1, VQM_MAP_DIFFERENCE = 2 };
+
 /* ImageFormat, include/vqm_image.h:100-120; numeric enum values are zimg's
  * as in the reference (vqm_image.h:25-96). Validation follows the
  * reference's ValidateImage (input.cc:152-222): an unknown or unsupported
@@ -151,6 +173,8 @@ typedef struct vqm_ss2_config {
                                    13-tap sampled Gaussian; VQM_BLUR_RECURSIVE:
                                    the kernel of Cloudinary ssimulacra2's
                                    recursive Gaussian. No reference field. */
+    uint32_t map_form;          /* VQM_MAP_*, default VQM_MAP_DEFAULT. No
+                                   reference field. */
 } vqm_ss2_config;
 
 /* PipelineMemoryUsage, include/vqm_memory_types.h:29-38: device bytes the
@@ -339,7 +363,7 @@ typedef struct vqm_ss2_batch_config {
                                 resident at once (occupancy x compute units), each
                                 looping over the batch's tiles and strips in a
                                 fixed stride; 0: one block per tile or strip */
-    uint32_t reserved;       /* 0 */
+    uint32_t map_form;       /* VQM_MAP_* */
 } vqm_ss2_batch_config;
 
 /* One pair of caller-owned device images (packed u8 sRGB RGB). */
@@ -386,8 +410,8 @@ typedef struct vqm_ss2_tune_result {
  * of two, at most 512) on the device and returns the fastest; the choice is
  * cached for the process. Shapes do not change results, only time. Holds
  * up to 16 GiB of temporary device memory while it runs. */
-int vqm_ss2_autotune(uint32_t width, uint32_t height, uint32_t plane_format, uint32_t pairs,
-                     vqm_ss2_tune_result* result);
+int vqm_ss2_autotune(uint32_t width, uint32_t height, uint32_t plane_format, uint32_t map_form,
+                     uint32_t pairs, vqm_ss2_tune_result* result);
 
 /* ---- serving (no counterpart in the reference: its closest caller is the
  * worker pool of Ssimulacra2Pool, vqm_pool.h:35-86, pool.cc:24-116, and
@@ -412,7 +436,7 @@ typedef struct vqm_ss2_server_config {
     uint64_t hbm_budget;     /* device bytes for workspaces and host-input
                                 staging; 0 = unlimited */
     uint32_t blur;           /* VQM_BLUR_SAMPLED (default) or VQM_BLUR_RECURSIVE */
-    uint32_t reserved;       /* 0 */
+    uint32_t map_form;       /* VQM_MAP_* */
 } vqm_ss2_server_config;
 
 /* The outcome of one request. Times are vqm_now_ns() readings. */

@@ -49,6 +49,27 @@ def f32_dev_for(plane_format: int) -> Mode:
     return mode
 
 
+# The library's map forms (include/vqm_ss2.h VQM_MAP_*).
+MAP_DEFAULT, MAP_REFERENCE, MAP_DIFFERENCE = 0, 1, 2
+
+
+def form_for(plane_format: int = 0, map_form: int = MAP_DEFAULT, blur: int = 0) -> Mode:
+    """The oracle form the HIP path reproduces bit for bit per pixel under a
+    library configuration: the reference form (the FP32 ref form, with the
+    stored planes rounded to `plane_format`) or the difference form (the
+    FP32 dev form); the default map form is the reference form except for
+    16-bit fixed-point planes (plane_format 3)."""
+    if map_form == MAP_DEFAULT:
+        map_form = MAP_DIFFERENCE if plane_format == 3 else MAP_REFERENCE
+    if map_form == MAP_DIFFERENCE:
+        mode = Mode.from_buffer_copy(f32_dev_for(plane_format))
+    else:
+        mode = Mode(1, 0, 0, plane_format, 0x3F if plane_format else 0, 0,
+                    1 if plane_format == 1 else 0, 0, 0)
+    mode.blur_kind = blur
+    return mode
+
+
 def build() -> None:
     subprocess.run(["make", "-C", HERE], check=True, stdout=subprocess.DEVNULL)
 
@@ -73,6 +94,7 @@ def lib() -> ctypes.CDLL:
             "ss2o_srgb_lut_f32": (None, [f32p]),
             "ss2o_srgb_lut_f64": (None, [f64p]),
             "ss2o_cbrt_dev": (ctypes.c_float, [ctypes.c_float]),
+            "ss2o_cbrt_rn": (ctypes.c_float, [ctypes.c_float]),
             "ss2o_rcp_dev": (ctypes.c_float, [ctypes.c_float]),
             "ss2o_xyb_f32": (None, [ctypes.c_float] * 3 + [ctypes.c_int32, f32p]),
             "ss2o_xyb_f64": (None, [ctypes.c_double] * 3 + [f64p]),
@@ -189,6 +211,11 @@ def srgb_lut_f64() -> np.ndarray:
 
 def cbrt_dev(x: float) -> float:
     return float(lib().ss2o_cbrt_dev(x))
+
+
+def cbrt_rn(x: float) -> float:
+    """The cube root correctly rounded to FP32 (the FP32 ref form's root)."""
+    return float(lib().ss2o_cbrt_rn(x))
 
 
 def rcp_dev(x: float) -> float:

@@ -153,9 +153,60 @@ float rcp_dev(float x) {
 }
 double rcp_dev(double x) { return 1.0 / x; }  // the FP64 forms divide
 
+// A finite positive float or a midpoint of two adjacent ones as M * 2^k
+// with M an integer of at most 26 bits.
+struct Dyadic {
+    int64_t m;
+    int k;
+};
+
+Dyadic dyadic_of(float f) {
+    int e = 0;
+    const double fr = std::frexp(static_cast<double>(f), &e);  // f = fr * 2^e, fr in [0.5, 1)
+    return {static_cast<int64_t>(std::ldexp(fr, 24)), e - 24};
+}
+
+Dyadic midpoint_of(float lo, float hi) {
+    const Dyadic a = dyadic_of(lo), b = dyadic_of(hi);
+    const int k = std::min(a.k, b.k);
+    return {(a.m << (a.k - k)) + (b.m << (b.k - k)), k - 1};
+}
+
+// Sign of m^3 - x, exactly, in 128-bit integers: m^3 = M^3 2^(3k) with
+// M^3 < 2^78, x = X 2^j with X < 2^24.
+int cube_vs(Dyadic m, float x) {
+    const Dyadic xd = dyadic_of(x);
+    const __int128 m3 = static_cast<__int128>(m.m) * m.m * m.m;
+    const int s = 3 * m.k - xd.k;  // m^3 - x = 2^j (M^3 2^s - X)
+    __int128 lhs = m3, rhs = xd.m;
+    if (s >= 0) {
+        if (s > 40) return 1;       // M^3 2^s >= 2^40 > X
+        lhs <<= s;
+    } else {
+        if (-s > 100) return -1;    // M^3 < 2^78 <= X 2^-s
+        rhs <<= -s;
+    }
+    return lhs > rhs ? 1 : (lhs < rhs ? -1 : 0);
+}
+
+// The cube root of x correctly rounded to FP32 (round to nearest): libm's
+// FP64 root rounded to FP32 as the candidate, then moved to a neighbour
+// while the true root lies beyond the midpoint between them, decided in
+// exact integer arithmetic. 0 for x <= 0 (the reference clamps the opsin
+// mixes at 0, prepare.comp:81-98, and pow(0, 1/3) = 0).
 float cbrt_exact(float x) {
     if (!(x > 0.0f)) return 0.0f;
-    return static_cast<float>(std::cbrt(static_cast<double>(x)));
+    float c = static_cast<float>(std::cbrt(static_cast<double>(x)));
+    for (;;) {
+        const float up = std::nextafter(c, INFINITY), down = std::nextafter(c, 0.0f);
+        if (cube_vs(midpoint_of(c, up), x) < 0) {
+            c = up;                 // the root lies above the upper midpoint
+        } else if (down > 0.0f && cube_vs(midpoint_of(down, c), x) > 0) {
+            c = down;               // below the lower midpoint
+        } else {
+            return c;
+        }
+    }
 }
 
 // sRGB decode of one u8 code: the decode's scale (input.cc:714-757, full
@@ -694,6 +745,7 @@ void ss2o_srgb_lut_f64(double lut[256]) {
 void ss2o_cosine_taps(double taps[13]) { cosine_taps_f64(taps); }
 
 float ss2o_cbrt_dev(float x) { return cbrt_dev(x); }
+float ss2o_cbrt_rn(float x) { return cbrt_exact(x); }
 float ss2o_rcp_dev(float x) { return rcp_dev(x); }
 
 void ss2o_xyb_f32(float r, float g, float b, int32_t cbrt_kind, float out[3]) {

@@ -90,8 +90,12 @@ double ss2o_score(int32_t scales, const double norms[108], int32_t* status);
 void ss2o_srgb_lut_f32(float lut[256]);
 void ss2o_srgb_lut_f64(double lut[256]);
 
-/* The HIP kernels' cube root (bit-exact restatement). */
+/* The HIP kernels' fast cube root (bit-exact restatement). */
 float ss2o_cbrt_dev(float x);
+
+/* The cube root correctly rounded to FP32 (round to nearest): the FP32 ref
+ * form's root. 0 for x <= 0. */
+float ss2o_cbrt_rn(float x);
 
 /* The kernels' reciprocal of a positive normal float (bit-trick seed and
  * three Newton steps, <= 0.501 ulp). */

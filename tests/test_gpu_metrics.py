@@ -37,7 +37,8 @@ def config(size: Size, fmt=None, **kw) -> MetricsConfig:
 
 def test_views_score_like_images(oracle_lib):
     """Submit(ImageView) packs into the staging: the same norms as the packed
-    Image, for 8-bit sRGB (checked against the oracle's device form) and a
+    Image, for 8-bit sRGB (checked against the oracle's FP32 ref form, the
+    default map form's) and a
     10-bit 4:2:0 YUV general format."""
     w, h = 97, 61
     a, b = random_pair(w, h, seed=5)
@@ -47,7 +48,7 @@ def test_views_score_like_images(oracle_lib):
         got = p.Compute(srgb8_view(a, size, 3), srgb8_view(b, size, 17))
         assert np.array_equal(got.norms(), want.norms())
         assert vqm_amd.Score(p, srgb8_view(a, size, 0), srgb8_view(b, size, 1)) == want.Score()
-    n, dev, _ = oracle_lib.compute(oracle_lib.F32_DEV, w, h, a, b)
+    n, dev, _ = oracle_lib.compute(oracle_lib.F32_REF, w, h, a, b)
     assert np.allclose(want.norms(), dev, rtol=1e-10, atol=0)
 
     fmt = yuv_format(1, 1, vqm_amd.PixelType.Word, depth=10)

@@ -70,10 +70,10 @@ def test_each_result_is_the_same_alone_and_in_the_mix(fmt):
     assert stats["completed"] == len(reqs) and stats["failed"] == 0
     assert stats["batched_requests"] == len(reqs)
     assert stats["max_batch"] > 1, "no request was batched with another"
-    # And the mix is the metric: one pair against the oracle's device form.
+    # And the mix is the metric: one pair against the plane format's oracle
+    # form (the default map form).
     r = reqs[3]
-    _, ref, _ = oracle.compute(oracle.f32_dev_planes(fmt) if fmt else oracle.F32_DEV, r["w"], r["h"],
-                               r["a"], r["b"])
+    _, ref, _ = oracle.compute(oracle.form_for(fmt), r["w"], r["h"], r["a"], r["b"])
     np.testing.assert_allclose(results[3].norms, ref, rtol=1e-10, atol=1e-300)
 
 

@@ -54,7 +54,8 @@ class DeviceImageC(ctypes.Structure):
 class ConfigC(ctypes.Structure):
     _fields_ = [("width", u32), ("height", u32), ("resident_images", u32),
                 ("plane_format", u32), ("streaming_inputs", u32), ("strip_rows", u32),
-                ("input_bytes", u64), ("graph", u32), ("chroma_filter", u32), ("blur", u32)]
+                ("input_bytes", u64), ("graph", u32), ("chroma_filter", u32), ("blur", u32),
+                ("map_form", u32)]
 
 
 class MemoryC(ctypes.Structure):
@@ -66,7 +67,7 @@ class MemoryC(ctypes.Structure):
 class BatchConfigC(ctypes.Structure):
     _fields_ = [("plane_format", u32), ("strip_rows", u32), ("max_pairs", u32),
                 ("chunks", u32), ("hbm_budget", u64), ("autotune", u32), ("blur", u32),
-                ("persistent", u32), ("reserved", u32)]
+                ("persistent", u32), ("map_form", u32)]
 
 
 class TuneResultC(ctypes.Structure):
@@ -80,7 +81,7 @@ class DevicePairC(ctypes.Structure):
 
 class ServerConfigC(ctypes.Structure):
     _fields_ = [("plane_format", u32), ("lanes", u32), ("max_batch", u32), ("strip_rows", u32),
-                ("hbm_budget", u64), ("blur", u32), ("reserved", u32)]
+                ("hbm_budget", u64), ("blur", u32), ("map_form", u32)]
 
 
 class RequestResultC(ctypes.Structure):
@@ -149,7 +150,7 @@ SIGNATURES = {
     "vqm_ss2_batch_finish": (ctypes.c_int, [vp, u64, ctypes.POINTER(u32), ctypes.POINTER(f64)]),
     "vqm_ss2_batch_stage_times": (ctypes.c_int, [vp, ctypes.POINTER(f32)]),
     "vqm_ss2_batch_pool_bytes": (ctypes.c_int, [vp, ctypes.POINTER(u64)]),
-    "vqm_ss2_autotune": (ctypes.c_int, [u32, u32, u32, u32, ctypes.POINTER(TuneResultC)]),
+    "vqm_ss2_autotune": (ctypes.c_int, [u32, u32, u32, u32, u32, ctypes.POINTER(TuneResultC)]),
     "vqm_ss2_server_config_init": (None, [ctypes.POINTER(ServerConfigC)]),
     "vqm_ss2_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfigC), ctypes.POINTER(vp)]),
     "vqm_ss2_server_destroy": (None, [vp]),

@@ -27,6 +27,7 @@ struct Options {
     bool half_planes = false;
     std::optional<PlaneFormat> planes;
     Blur blur = Blur::Sampled;
+    MapForm map_form = MapForm::Default;
     image_io::InputColorOptions input;
     std::string original, distorted;
 };
@@ -35,6 +36,7 @@ const char* kUsage =
     "vqm-ss2 - SSIMULACRA 2.1 on AMD Instinct (HIP)\n"
     "Usage: %s [--metric ssimulacra2] [--raw] [--half-planes] [--planes f32|f16|bf16|u16]\n"
     "          [--chroma-filter bilinear|bicubic|lanczos] [--blur sampled|recursive]\n"
+    "          [--map reference|difference]\n"
     "          [--input-matrix M] [--input-transfer T] [--input-primaries P] [--input-range R]\n"
     "          original distorted\n"
     "\n"
@@ -46,7 +48,10 @@ const char* kUsage =
     "primaries, and limited range. The --input-* flags override both images.\n"
     "--raw prints the 108 norms, channel-major, before the score.\n"
     "--half-planes stores intermediate planes as FP16 (--planes f16); --planes u16\n"
-    "is the mixed form (16-bit fixed point at the two finest scales, FP32 above).\n"
+    "is the mixed form (planes in 16-bit fixed point at every scale, FP32 arithmetic).\n"
+    "--map reference (the default, except with --planes u16) computes the SSIM' map\n"
+    "in the reference's operation order; --map difference rearranges it to track FP64\n"
+    "more closely (not in the reference command).\n"
     "--blur recursive uses the kernel of Cloudinary ssimulacra2's recursive Gaussian\n"
     "instead of the 13-tap sampled Gaussian (not in the reference command).\n"
     "--input-matrix     rgb|bt709|fcc|bt470bg|st170m|st240m|ycgco|bt2020ncl|\n"
@@ -100,6 +105,11 @@ std::optional<Options> Parse(int argc, char** argv) {
             if (v == "sampled") o.blur = Blur::Sampled;
             else if (v == "recursive") o.blur = Blur::Recursive;
             else throw ParseError{"invalid --blur '" + std::string(v) + "' (expected sampled|recursive)"};
+        } else if (a == "--map") {
+            const std::string_view v = value();
+            if (v == "reference") o.map_form = MapForm::Reference;
+            else if (v == "difference") o.map_form = MapForm::Difference;
+            else throw ParseError{"invalid --map '" + std::string(v) + "' (expected reference|difference)"};
         } else if (a == "--chroma-filter") {
             const std::string_view v = value();
             if (!image_io::ParseChromaFilter(v, o.input.chroma_filter, accepted)) throw bad(v);
@@ -144,6 +154,7 @@ Ssimulacra2Config MakeConfig(const Options& o, Size size) {
     if (o.planes) c.plane_format = *o.planes;
     c.chroma_filter = o.input.chroma_filter;
     c.blur = o.blur;
+    c.map_form = o.map_form;
     return c;
 }
 

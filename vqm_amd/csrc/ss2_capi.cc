@@ -408,12 +408,13 @@ int vqm_ss2_batch_pool_bytes(const vqm_ss2_batch* batch, uint64_t* bytes) {
     });
 }
 
-int vqm_ss2_autotune(uint32_t width, uint32_t height, uint32_t plane_format, uint32_t pairs,
-                     vqm_ss2_tune_result* result) {
+int vqm_ss2_autotune(uint32_t width, uint32_t height, uint32_t plane_format, uint32_t map_form,
+                     uint32_t pairs, vqm_ss2_tune_result* result) {
     return Guard([&] {
         Need(result, "result");
+        const vqm::ss2::PlaneFormat format = vqm::ss2::ToFormat(plane_format);
         const vqm::ss2::TuneChoice c = vqm::ss2::Tuner::Get().Choose(
-            width, height, vqm::ss2::ToFormat(plane_format), pairs);
+            width, height, format, vqm::ss2::ToMapForm(map_form, format), pairs);
         result->strip_rows = c.strip_rows;
         result->chunks = c.chunks;
         result->candidates = c.candidates;

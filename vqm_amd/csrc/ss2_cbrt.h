@@ -1,0 +1,79 @@
+// Cube roots of the XYB transform (prepare.comp:81-98 takes pow(m, 1/3) of
+// each opsin mix), as host-and-device functions so that the CPU test that
+// checks them runs this very code.
+//
+//   cbrt_fast  <= 0.84 ulp on [2^-126, 64); 16 FP32 operations.
+//   cbrt_rn    the cube root correctly rounded to FP32 (round to nearest),
+//              for every normal positive float: cbrt_fast, then two exact
+//              comparisons done in FP64.
+//
+// Every step is an IEEE conversion, add, multiply or fused multiply-add
+// (written out), with contraction off, so the result does not depend on the
+// compiler or on where it runs. tests/cpp/cbrt_exhaustive.hip checks
+// cbrt_rn against an exact integer comparison for every float in
+// [2^-126, 2^24).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#pragma clang fp contract(off)
+
+namespace vqm::ss2 {
+
+__host__ __device__ inline uint32_t cbrt_bits(float x) { return __builtin_bit_cast(uint32_t, x); }
+__host__ __device__ inline float cbrt_float(uint32_t u) { return __builtin_bit_cast(float, u); }
+
+// Bit-trick seed of x^(-1/3) (magic - bits/3, formed in float: one
+// conversion each way around one fma), one third-order step
+// r <- r (1 + h/3 + 2h^2/9 + 14h^3/81) with h = 1 - x r^3 (the series of
+// (1 - h)^(-1/3); |h| <= 0.105 after the seed leaves r within 2.3e-5), then
+// y = x r^2 with one Newton correction. x must be a positive normal float.
+__host__ __device__ inline float cbrt_fast(float x) {
+    constexpr float kThird = 0x1.555556p-2f;       // 1/3 rounded to FP32
+    constexpr float kMagic = 1419910400.0f;        // 0x54a21d2a rounded to FP32
+    constexpr float kC2 = 0x1.c71c72p-3f;          // 2/9
+    constexpr float kC3 = 0x1.61f9aep-3f;          // 14/81
+    const float seed = __builtin_fmaf(static_cast<float>(cbrt_bits(x)), -kThird, kMagic);
+    float r = cbrt_float(static_cast<uint32_t>(seed));
+    const float h = __builtin_fmaf(-x, (r * r) * r, 1.0f);
+    const float p = __builtin_fmaf(h, __builtin_fmaf(h, kC3, kC2), kThird);
+    r = __builtin_fmaf(r * h, p, r);
+    const float rr = r * r;
+    const float y = x * rr;
+    const float e = __builtin_fmaf(-(y * y), y, x);
+    return __builtin_fmaf(e, rr * kThird, y);
+}
+
+// Sign of x - m^3 for FP64 values x and m where m carries at most 26
+// significant bits and m^3 lies within a factor of two of x: m^2 is exact
+// in FP64, m^3 = p + e exactly with p = RN(m^2 m) and e = fma(m^2, m, -p)
+// (the product's rounding error), and x - p is exact (Sterbenz), so
+// x > m^3 exactly when x - p > e. Returns +1, 0 or -1.
+__host__ __device__ inline int cube_cmp(double x, double m) {
+    const double m2 = m * m;
+    const double p = m2 * m;
+    const double e = __builtin_fma(m2, m, -p);
+    const double r = x - p;
+    return r > e ? 1 : (r < e ? -1 : 0);
+}
+
+// The cube root of a positive normal float, correctly rounded. cbrt_fast is
+// within one ulp, so the rounded root is y or a neighbour of y: the sign of
+// x - y^3 names the side the true root t lies on, and one more comparison
+// against the cube of the midpoint between y and that neighbour decides
+// whether t lies beyond it. A midpoint has 25 significant bits and its cube
+// is never a float, so there are no ties. Both midpoints and y are exact in
+// FP64 and their cubes are within a factor of two of x.
+__host__ __device__ inline float cbrt_rn(float x) {
+    const float y = cbrt_fast(x);
+    const double xd = x, yd = y;
+    const int side = cube_cmp(xd, yd);
+    if (side == 0) return y;  // x is the cube of y
+    const float n = cbrt_float(cbrt_bits(y) + (side > 0 ? 1u : 0xFFFFFFFFu));
+    const double mid = (yd + static_cast<double>(n)) * 0.5;
+    return cube_cmp(xd, mid) == side ? n : y;
+}
+
+}  // namespace vqm::ss2

@@ -80,6 +80,17 @@ struct Plan {
 // computes (ssimulacra2.cc:82-106).
 enum BlurKind : uint32_t { kBlurSampled = 0, kBlurRecursive = 1 };
 
+// The arithmetic form of the XYB transform and the SSIM' map.
+//   kMapReference   prepare.comp:81-98 and scale.comp:130-142,305-333 in
+//                   their own operation order: correctly rounded cube roots,
+//                   five blurred quantities (a, b, a*a, b*b, a*b), the map's
+//                   variances as s11 - mu1^2 and s22 - mu2^2, IEEE divisions.
+//   kMapDifference  the same quantities rearranged: four blurred quantities
+//                   (a, b, (a - b)^2, a*b), the variance sum as one small
+//                   blurred difference, one reciprocal for both quotients, a
+//                   fast cube root. Closer to FP64, a fifth less blur work.
+enum MapForm : uint32_t { kMapReference = 0, kMapDifference = 1 };
+
 // The 13 taps of MakeGaussian(1.5) (pipeline.cc:274-291), exact FP32 values.
 struct SampledTaps {
     static constexpr float w[kTaps] = {

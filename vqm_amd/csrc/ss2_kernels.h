@@ -27,17 +27,19 @@ struct WorkRef {
 // other than 8-bit sRGB RGB); the table-only kernel ignores descriptors.
 // `persistent`: launch only as many blocks as stay resident at once, each
 // looping over the work items (the grid otherwise covers them, one each).
-hipError_t LaunchPrepare(PlaneFormat format, bool general, const PairDesc* pairs, const WorkRef* work,
-                         uint32_t blocks, uint32_t tiles_per_image, const float* lut,
+// `map` picks the arithmetic form of the XYB transform (MapForm).
+hipError_t LaunchPrepare(PlaneFormat format, MapForm map, bool general, const PairDesc* pairs,
+                         const WorkRef* work, uint32_t blocks, uint32_t tiles_per_image, const float* lut,
                          const uint8_t* src0, const uint8_t* src1, hipStream_t stream,
                          bool persistent = false);
 
 // One 64-lane wave per scale work item (level, channel, 128-column strip).
 // Uniform geometry: blocks = pairs * items_per_pair, work = nullptr. `blur`
 // picks the kernel: kBlurSampled (13-tap sampled Gaussian) or kBlurRecursive
-// (the recursive Gaussian's 9-tap truncated-cosine kernel).
-hipError_t LaunchScale(PlaneFormat format, uint32_t blur, const PairDesc* pairs, const WorkRef* work,
-                       uint32_t blocks, uint32_t items_per_pair, hipStream_t stream,
+// (the recursive Gaussian's 9-tap truncated-cosine kernel); `map` the
+// SSIM' map's form, which must be the one the planes were prepared with.
+hipError_t LaunchScale(PlaneFormat format, uint32_t blur, MapForm map, const PairDesc* pairs,
+                       const WorkRef* work, uint32_t blocks, uint32_t items_per_pair, hipStream_t stream,
                        bool persistent = false);
 
 // Sums each (level, channel) norm's partials in FP64 and writes the 108 norms.

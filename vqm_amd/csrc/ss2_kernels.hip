@@ -3,21 +3,27 @@
 //   prepare  u8 sRGB -> linear (256-entry table) -> XYB, plus the 2x2 box
 //            pyramid of the linear planes, writing only the (level, channel)
 //            planes whose maps survive pruning (prepare.comp:170-232).
-//   scale    one wave per 128-column strip of one (level, channel): the five
-//            13-tap blurs of a, b, a*a, b*b, a*b, the SSIM' and edge maps, and
-//            six FP64 map sums (scale.comp:335-437). Rows stream through the
-//            wave: each input row is staged in LDS, blurred horizontally
+//   scale    one wave per 128-column strip of one (level, channel): the
+//            13-tap blurs of the map's quantities, the SSIM' and edge maps,
+//            and six FP64 map sums (scale.comp:335-437). Rows stream through
+//            the wave: each input row is staged in LDS, blurred horizontally
 //            (each lane owns two output columns), and the horizontal results
 //            enter a 13-row register ring from which the vertical blur of the
 //            row six above is taken. No map ever reaches HBM.
 //   reduce   FP64 sum of each norm's per-strip partials, 1-norm means and
 //            4-norm fourth roots (reduce.comp:21-63, vulkan.cc:545-583).
 //
+// Two arithmetic forms (MapForm) share the kernels: the reference form
+// (prepare.comp and scale.comp in their own operation order, correctly
+// rounded cube roots, five blurred quantities) and the difference form (the
+// SSIM' map from four blurred quantities, rearranged to avoid a cancellation).
+//
 // Arithmetic contract. Every FP32 operation below is an IEEE add, multiply,
-// fused multiply-add (written explicitly), min/max, abs or an integer/float
-// conversion (no division: see rcp_dev), with contraction switched off, so the CPU oracle's FP32 "dev" form
-// reproduces each pixel's values bit for bit. The blur runs horizontal pass
-// first with taps in ascending order, as scale.comp:149-301 does.
+// fused multiply-add (written explicitly), division, min/max, abs or an
+// integer/float conversion, with contraction switched off, so the CPU
+// oracle's FP32 forms (ref for the reference form, dev for the difference
+// form) reproduce each pixel's values bit for bit. The blur runs horizontal
+// pass first with taps in ascending order, as scale.comp:149-301 does.
 //
 // Memory access. Descriptors (pairs, plans, work items) are read through
 // constant-address-space pointers so that their uniform fields become scalar
@@ -30,6 +36,7 @@
 #include <type_traits>
 #include <utility>
 
+#include "ss2_cbrt.h"
 #include "ss2_common.h"
 #include "ss2_kernels.h"
 
@@ -63,7 +70,6 @@ constexpr float kM10 = 0x1.d70a3ep-3f, kM11 = 0x1.624dd2p-1f, kM12 = 0x1.3f7ceep
 constexpr float kM20 = 0x1.f28798p-3f, kM21 = 0x1.a35d1ep-3f, kM22 = 0x1.1a86d2p-1f;
 constexpr float kOpsinBias = 0x1.f12a6cp-9f;
 constexpr float kCbrtBias = 0x1.3f64eap-3f;
-constexpr float kThird = 0x1.555556p-2f;  // 1/3 rounded to FP32
 constexpr float kC2 = 0.0009f;            // scale.comp:87
 
 __device__ __forceinline__ float fma_(float a, float b, float c) {
@@ -71,32 +77,10 @@ __device__ __forceinline__ float fma_(float a, float b, float c) {
 }
 
 // Smallest normal float: the XYB mixes are clamped to it instead of 0, so the
-// cube root never sees zero or a denormal. Its root, 2^-42, vanishes in every
-// sum it enters (all are at least ~1e-2 away from it), as a zero root would.
+// cube roots never see zero or a denormal. Its root, 2^-42, vanishes in every
+// sum it enters (all are at least ~1e-2 away from it), as a zero root would:
+// 2^-42 minus the cube-root bias rounds to minus the bias.
 constexpr float kMinNormal = 0x1p-126f;
-
-// Cube root of a normal float: bit-trick seed of x^(-1/3) (magic - bits/3,
-// formed in float: one conversion each way around one fma), one third-order
-// step r <- r (1 + h/3 + 2h^2/9 + 14h^3/81) with h = 1 - x r^3 (the series
-// of (1 - h)^(-1/3); |h| <= 0.105 after the seed leaves r within 2.3e-5),
-// then y = x r^2 with one Newton correction. <= 0.84 ulp for x in
-// [2^-126, 64); every step is an IEEE conversion, add, mul or fma, so the
-// oracle reproduces it bit for bit. 16 operations.
-constexpr float kCbrtMagic = 1419910400.0f;  // 0x54a21d2a rounded to FP32
-constexpr float kCbrtC2 = 0x1.c71c72p-3f;     // 2/9
-constexpr float kCbrtC3 = 0x1.61f9aep-3f;     // 14/81
-
-__device__ __forceinline__ float cbrt_dev(float x) {
-    const float seed = fma_(static_cast<float>(__float_as_uint(x)), -kThird, kCbrtMagic);
-    float r = __uint_as_float(static_cast<uint32_t>(seed));
-    const float h = fma_(-x, (r * r) * r, 1.0f);
-    const float p = fma_(h, fma_(h, kCbrtC3, kCbrtC2), kThird);
-    r = fma_(r * h, p, r);
-    const float rr = r * r;
-    const float y = x * rr;
-    const float e = fma_(-(y * y), y, x);
-    return fma_(e, rr * kThird, y);
-}
 
 // Reciprocal of a positive normal float: a bit-trick seed (within 5%) and
 // three Newton steps r += r (1 - x r). <= 0.501 ulp; fma only, so the oracle
@@ -109,29 +93,54 @@ __device__ __forceinline__ float rcp_dev(float x) {
     return r;
 }
 
+// The two arithmetic forms of the path (MapForm):
+//
+//   Reference form (Ref = true): prepare.comp:81-98 and scale.comp:130-142,
+//   305-333 in their own operation order. The cube roots are correctly
+//   rounded (cbrt_rn), the five quantities a, b, a*a, b*b and a*b are
+//   blurred, and the SSIM' map subtracts the squared means from the blurred
+//   second moments directly, with IEEE divisions.
+//
+//   Difference form (Ref = false): the cube-root bias cancelled where it
+//   cancels exactly, the fast cube root (cbrt_fast), and the SSIM' map from
+//   four blurred quantities a, b, (a - b)^2 and a*b (see scale_item).
+//
 // XYB of one linear sample, only the channels in `mask` (bit 0 X, 1 Y, 2 B).
-// prepare.comp:81-98 with its multiply-adds fused and the cube-root bias
-// cancelled where it cancels exactly: with c_i = cbrt(m_i) - bias,
-// X = 7 (c0 - c1) + 0.42 = 7 (cbrt0 - cbrt1) + 0.42,
-// Y = (c0 + c1)/2 + 0.01 = (cbrt0 + cbrt1)/2 + (0.01 - bias) and
-// B = c2 - (c0 + c1)/2 + 0.55 = cbrt2 - (cbrt0 + cbrt1)/2 + 0.55, each
-// from the roots directly (three subtractions fewer, no rounding of c_i).
+// With c_i = cbrt(m_i) - bias, the reference computes X = (0.5 (c0 - c1)) 14
+// + 0.42, Y = 0.5 (c0 + c1) + 0.01 and B = (c2 - 0.5 (c0 + c1)) + 0.55. The
+// difference form takes them from the roots directly:
+// X = 7 (cbrt0 - cbrt1) + 0.42, Y = (cbrt0 + cbrt1)/2 + (0.01 - bias) and
+// B = cbrt2 - (cbrt0 + cbrt1)/2 + 0.55 (three subtractions fewer, no
+// rounding of c_i).
 constexpr float kYOffset = 0.01f - kCbrtBias;  // rounded once
 
+template <bool Ref>
 __device__ __forceinline__ void xyb(float r, float g, float b, uint32_t mask, float out[3]) {
     float m0 = fma_(kM00, r, fma_(kM01, g, fma_(kM02, b, kOpsinBias)));
     float m1 = fma_(kM10, r, fma_(kM11, g, fma_(kM12, b, kOpsinBias)));
     m0 = fmaxf(m0, kMinNormal);
     m1 = fmaxf(m1, kMinNormal);
-    const float r0 = cbrt_dev(m0), r1 = cbrt_dev(m1);
-    const float sum = r0 + r1;
-    out[0] = fma_(r0 - r1, 7.0f, 0.42f);
-    out[1] = fma_(0.5f, sum, kYOffset);
     out[2] = 0.0f;
-    if (mask & 4u) {
-        float m2 = fma_(kM20, r, fma_(kM21, g, fma_(kM22, b, kOpsinBias)));
-        m2 = fmaxf(m2, kMinNormal);
-        out[2] = fma_(-0.5f, sum, cbrt_dev(m2)) + 0.55f;
+    if constexpr (Ref) {
+        const float c0 = cbrt_rn(m0) - kCbrtBias, c1 = cbrt_rn(m1) - kCbrtBias;
+        const float half_sum = 0.5f * (c0 + c1);
+        out[0] = fma_(0.5f * (c0 - c1), 14.0f, 0.42f);
+        out[1] = half_sum + 0.01f;
+        if (mask & 4u) {
+            float m2 = fma_(kM20, r, fma_(kM21, g, fma_(kM22, b, kOpsinBias)));
+            m2 = fmaxf(m2, kMinNormal);
+            out[2] = (cbrt_rn(m2) - kCbrtBias - half_sum) + 0.55f;
+        }
+    } else {
+        const float r0 = cbrt_fast(m0), r1 = cbrt_fast(m1);
+        const float sum = r0 + r1;
+        out[0] = fma_(r0 - r1, 7.0f, 0.42f);
+        out[1] = fma_(0.5f, sum, kYOffset);
+        if (mask & 4u) {
+            float m2 = fma_(kM20, r, fma_(kM21, g, fma_(kM22, b, kOpsinBias)));
+            m2 = fmaxf(m2, kMinNormal);
+            out[2] = fma_(-0.5f, sum, cbrt_fast(m2)) + 0.55f;
+        }
     }
 }
 
@@ -468,7 +477,7 @@ __device__ float3 pixel(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* src, ui
 // previous one. The ladder keeps a level's linear samples channel-planar.
 constexpr uint32_t kLadder = (kPrepTileW / 2) * (kPrepTileH / 2);
 
-template <typename Lay, uint32_t L>
+template <typename Lay, bool Ref, uint32_t L>
 __device__ __forceinline__ void prepare_level(ConstAS<Plan>* plan, GlobalAS<uint8_t>* planes,
                                               uint32_t X0, uint32_t Y0, uint32_t tid,
                                               float (*ladder)[3][kLadder]) {
@@ -500,7 +509,7 @@ __device__ __forceinline__ void prepare_level(ConstAS<Plan>* plan, GlobalAS<uint
         const uint32_t mask = plan->mask[L];
         if (mask != 0u) {
             float o[3];
-            xyb(v[0], v[1], v[2], mask, o);
+            xyb<Ref>(v[0], v[1], v[2], mask, o);
             const uint64_t at = static_cast<uint64_t>(gy + kPadTop) * plan->pitch[L] + gx + kPadLeft;
 #pragma unroll
             for (uint32_t c = 0; c < 3; ++c)
@@ -579,7 +588,7 @@ __device__ __forceinline__ TileWords load_tile_words(const TileRef& t, const uin
 // One prepare work item: tile `b` of the launch's work (a 64 x 32 base-pixel
 // tile of one image of one pair). Every thread of the block calls it with the
 // same `b`; the LUT's LDS stores are issued before the first call.
-template <typename Lay, bool General>
+template <typename Lay, bool General, bool Ref>
 __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restrict__ pairs,
                                              const WorkRef* __restrict__ work, uint32_t blocks,
                                              uint32_t tiles_per_image, const float* lut,
@@ -654,7 +663,7 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
             float v[4][3];
 #pragma unroll
             for (uint32_t j = 0; j < 4; ++j)
-                xyb(lin[r][j][0], lin[r][j][1], lin[r][j][2], mask0, v[j]);
+                xyb<Ref>(lin[r][j][0], lin[r][j][1], lin[r][j][2], mask0, v[j]);
             const uint64_t at = static_cast<uint64_t>(y + kPadTop) * pitch0 + bx + kPadLeft;
 #pragma unroll
             for (uint32_t c = 0; c < 3; ++c) {
@@ -696,8 +705,8 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
         if (mask1 != 0u && gy < H1) {
             const uint32_t gx = X0 / 2u + 2u * tx;
             float x0[3], x1[3];
-            xyb(v1[0][0], v1[0][1], v1[0][2], mask1, x0);
-            xyb(v1[1][0], v1[1][1], v1[1][2], mask1, x1);
+            xyb<Ref>(v1[0][0], v1[0][1], v1[0][2], mask1, x0);
+            xyb<Ref>(v1[1][0], v1[1][1], v1[1][2], mask1, x1);
             const uint64_t at = static_cast<uint64_t>(gy + kPadTop) * plan->pitch[1] + gx + kPadLeft;
 #pragma unroll
             for (uint32_t c = 0; c < 3; ++c) {
@@ -714,10 +723,10 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
     }
 
     // Levels 2 .. 5: each waits for the previous level's ladder.
-    if (levels > 2) { __syncthreads(); prepare_level<Lay, 2>(plan, planes, X0, Y0, tid, ladder); }
-    if (levels > 3) { __syncthreads(); prepare_level<Lay, 3>(plan, planes, X0, Y0, tid, ladder); }
-    if (levels > 4) { __syncthreads(); prepare_level<Lay, 4>(plan, planes, X0, Y0, tid, ladder); }
-    if (levels > 5) { __syncthreads(); prepare_level<Lay, 5>(plan, planes, X0, Y0, tid, ladder); }
+    if (levels > 2) { __syncthreads(); prepare_level<Lay, Ref, 2>(plan, planes, X0, Y0, tid, ladder); }
+    if (levels > 3) { __syncthreads(); prepare_level<Lay, Ref, 3>(plan, planes, X0, Y0, tid, ladder); }
+    if (levels > 4) { __syncthreads(); prepare_level<Lay, Ref, 4>(plan, planes, X0, Y0, tid, ladder); }
+    if (levels > 5) { __syncthreads(); prepare_level<Lay, Ref, 5>(plan, planes, X0, Y0, tid, ladder); }
 }
 
 // Work items blockIdx.x, blockIdx.x + gridDim.x, ... of `blocks`: one each
@@ -725,9 +734,11 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
 // sized to the resident occupancy. The grid is a multiple of 8, so item b
 // stays on XCD b % 8 and xcd_remap keeps neighbouring tiles on one L2.
 // The table-decode instantiations are held to 80 VGPRs (six waves per SIMD)
-// without spilling; the per-pixel decode of general formats needs ~130.
-template <typename Lay, bool General>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(General ? 1 : 6))) void prepare_kernel(const PairDesc* __restrict__ pairs,
+// in the difference form and 96 (five) in the reference form, whose
+// correctly rounded cube roots hold FP64 temporaries, both without spilling;
+// the per-pixel decode of general formats needs ~130.
+template <typename Lay, bool General, bool Ref>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(General ? 1 : (Ref ? 5 : 6)))) void prepare_kernel(const PairDesc* __restrict__ pairs,
                                                       const WorkRef* __restrict__ work,
                                                       uint32_t blocks, uint32_t tiles_per_image,
                                                       const float* __restrict__ lut_g,
@@ -740,7 +751,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(General ? 1
         TileWords none;
         none.valid = false;
         for (uint32_t b = blockIdx.x; b < blocks; b += gridDim.x) {
-            prepare_tile<Lay, General>(b, pairs, work, blocks, tiles_per_image, lut, ladder, src0, src1, none);
+            prepare_tile<Lay, General, Ref>(b, pairs, work, blocks, tiles_per_image, lut, ladder, src0, src1, none);
             __syncthreads();  // the next tile reuses the ladder
         }
     } else {
@@ -755,7 +766,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(General ? 1
             ahead.valid = false;
             if (next < blocks)
                 ahead = load_tile_words(resolve_tile(next, pairs, work, blocks, tiles_per_image), src0, src1);
-            prepare_tile<Lay, General>(b, pairs, work, blocks, tiles_per_image, lut, ladder, src0, src1, cur);
+            prepare_tile<Lay, General, Ref>(b, pairs, work, blocks, tiles_per_image, lut, ladder, src0, src1, cur);
             __syncthreads();  // the next tile reuses the ladder
             cur = ahead;
         }
@@ -765,7 +776,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(General ? 1
 // ---- scale ------------------------------------------------------------------
 
 constexpr int kRing = 8;  // raw rows kept in LDS: the map needs the row six back
-constexpr int kBlurred = 4;  // a, b, (a - b)^2, ab
+
+// Blurred quantities and the products staged beside the raw rows, by form:
+// the reference form blurs a, b, a*a, b*b and a*b (scale.comp:130-142); the
+// difference form a, b, (a - b)^2 and a*b.
+template <bool Ref> struct MapQuantities {
+    static constexpr int kBlurred = Ref ? 5 : 4;
+    static constexpr int kProducts = kBlurred - 2;
+};
 
 template <int... P, typename F>
 __device__ __forceinline__ void unroll(std::integer_sequence<int, P...>, F&& f) {
@@ -777,13 +795,78 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// The products of one lane's two samples of a and b that the form blurs.
+template <bool Ref>
+__device__ __forceinline__ void stage_products(float (*prod)[kSrcCols], int j, float2 a, float2 b) {
+    if constexpr (Ref) {
+        *reinterpret_cast<float2*>(&prod[0][j]) = make_float2(a.x * a.x, a.y * a.y);
+        *reinterpret_cast<float2*>(&prod[1][j]) = make_float2(b.x * b.x, b.y * b.y);
+        *reinterpret_cast<float2*>(&prod[2][j]) = make_float2(a.x * b.x, a.y * b.y);
+    } else {
+        const float dx = a.x - b.x, dy = a.y - b.y;
+        *reinterpret_cast<float2*>(&prod[0][j]) = make_float2(dx * dx, dy * dy);
+        *reinterpret_cast<float2*>(&prod[1][j]) = make_float2(a.x * b.x, a.y * b.y);
+    }
+}
+
+// The six map values of one pixel from its blurred quantities `bl` (in the
+// form's order) and its raw samples a and b: d, d^4, ring, ring^4, blur,
+// blur^4 (scale.comp:305-333).
+template <bool Ref>
+__device__ __forceinline__ void map_values(const float* bl, float a, float b, float v6[6]) {
+    const float mu1 = bl[0], mu2 = bl[1];
+    const float det_dist = 1.0f + fabsf(b - mu2);
+    const float det_orig = 1.0f + fabsf(a - mu1);
+    float d, d1;
+    if constexpr (Ref) {
+        // scale.comp:305-333 in its order (the multiply-adds fused):
+        // num_m = 1 - (mu1 - mu2)^2, num_s = 2 (s12 - mu1 mu2) + c2,
+        // den = (s11 - mu1^2) + (s22 - mu2^2) + c2,
+        // d = max((den - num_m num_s) / den, 0), d1 = (det_dist - det_orig) / det_orig,
+        // with IEEE divisions.
+        const float t = mu1 - mu2;
+        const float num_m = fma_(-t, t, 1.0f);
+        const float num_s = fma_(2.0f, fma_(-mu1, mu2, bl[4]), kC2);
+        const float den = (fma_(-mu1, mu1, bl[2]) + fma_(-mu2, mu2, bl[3])) + kC2;
+        d = fmaxf(fma_(-num_m, num_s, den) / den, 0.0f);
+        d1 = (det_dist - det_orig) / det_orig;
+    } else {
+        // scale.comp:305-333 rearranged: with e the blurred (a - b)^2 and
+        // t = mu1 - mu2, s11 + s22 - mu1^2 - mu2^2 = num_s - c2 + v where
+        // v = e - t^2, so den = num_s + v and den - num_m num_s =
+        // v + t^2 num_s. The small v replaces the difference of two large
+        // second moments (closer to FP64 than the direct form), and
+        // identical images give v = t = 0, d = 0 exactly. One reciprocal
+        // serves both quotients: q = 1 / (den det_orig), with
+        // den >= c2 - rounding > 0 and det_orig >= 1.
+        const float t = mu1 - mu2;
+        const float num_s = fma_(2.0f, fma_(-mu1, mu2, bl[3]), kC2);
+        const float v = fma_(-t, t, bl[2]);
+        const float den = num_s + v;
+        const float q = rcp_dev(den * det_orig);
+        d = fmaxf((fma_(t * t, num_s, v) * det_orig) * q, 0.0f);
+        d1 = ((det_dist - det_orig) * den) * q;
+    }
+    const float dd = d * d;
+    const float ring = fmaxf(d1, 0.0f);
+    const float blur = fmaxf(-d1, 0.0f);
+    const float rr = ring * ring, bb = blur * blur;
+    v6[0] = d;
+    v6[1] = dd * dd;
+    v6[2] = ring;
+    v6[3] = rr * rr;
+    v6[4] = blur;
+    v6[5] = bb * bb;
+}
+
 // One scale work item: the strip of `level`, `channel` starting at (x0, y0),
 // with the plane elements in format Fmt.
-template <typename Fmt, typename TapsT>
+template <typename Fmt, typename TapsT, bool Ref>
 __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>* plan, uint32_t item_idx,
                                            uint32_t level, uint32_t channel, uint32_t x0, uint32_t y0,
                                            float (*ring_a)[kSrcCols], float (*ring_b)[kSrcCols],
                                            float (*prod)[kSrcCols]) {
+    constexpr int kBlurred = MapQuantities<Ref>::kBlurred;
     const uint32_t W = plan->w[level], H = plan->h[level], pitch = plan->pitch[level];
     // The strip's output rows stop at the level's last row (y0 < H for every
     // item): rows past it would be blurred and then dropped, up to half of
@@ -815,10 +898,10 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
     double sums[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const uint32_t col = x0 + 2u * static_cast<uint32_t>(lane);
 
-    for (int base = 0; base < total; base += kTaps) {
+    for (int base_row = 0; base_row < total; base_row += kTaps) {
         unroll(std::make_integer_sequence<int, kTaps>{}, [&](auto pc) {
             constexpr int P = decltype(pc)::value;
-            const int i = base + P;
+            const int i = base_row + P;
             if (i >= total) return;
             // Stage input row i: raw samples into the ring, products alongside.
             float* ra = ring_a[i & (kRing - 1)];
@@ -827,17 +910,13 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
                 const float2 a = Fmt::unpack(cur_a), b = Fmt::unpack(cur_b);
                 *reinterpret_cast<float2*>(ra + jm) = a;
                 *reinterpret_cast<float2*>(rb + jm) = b;
-                const float dx = a.x - b.x, dy = a.y - b.y;
-                *reinterpret_cast<float2*>(&prod[0][jm]) = make_float2(dx * dx, dy * dy);
-                *reinterpret_cast<float2*>(&prod[1][jm]) = make_float2(a.x * b.x, a.y * b.y);
+                stage_products<Ref>(prod, jm, a, b);
             }
             if (halo) {
                 const float2 a = Fmt::unpack(cur_ha), b = Fmt::unpack(cur_hb);
                 *reinterpret_cast<float2*>(ra + jh) = a;
                 *reinterpret_cast<float2*>(rb + jh) = b;
-                const float dx = a.x - b.x, dy = a.y - b.y;
-                *reinterpret_cast<float2*>(&prod[0][jh]) = make_float2(dx * dx, dy * dy);
-                *reinterpret_cast<float2*>(&prod[1][jh]) = make_float2(a.x * b.x, a.y * b.y);
+                stage_products<Ref>(prod, jh, a, b);
             }
             // Prefetch input row i + 1.
             if (i + 1 < total) {
@@ -851,10 +930,8 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
             }
             __syncthreads();
 
-            // Horizontal 13-tap pass of the four quantities a, b, (a - b)^2
-            // and ab at this lane's two output columns (source columns
-            // jm + 6 and jm + 7). The SSIM' map needs s11 + s22 only, which
-            // is blur((a - b)^2) + 2 s12.
+            // Horizontal 13-tap pass of each blurred quantity at this lane's
+            // two output columns (source columns jm + 6 and jm + 7).
             for (int q = 0; q < kBlurred; ++q) {
                 const float* s = q == 0 ? ra : q == 1 ? rb : prod[q - 2];
                 float v[14];
@@ -876,13 +953,13 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
             // Vertical pass for output row i - 12 (image row y0 + i - 12),
             // oldest ring entry first: scale.comp:243-298's ascending taps.
             if (i >= 2 * kRadius) {
-                float bl[kBlurred][2];
+                float bl[2][kBlurred];
                 for (int q = 0; q < kBlurred; ++q) {
                     for (int c = 0; c < 2; ++c) {
                         float acc = TapsT::w[0] * win[(P + 1) % kTaps][q][c];
                         for (int k = 1; k < kTaps; ++k)
                             acc = fma_(TapsT::w[k], win[(P + 1 + k) % kTaps][q][c], acc);
-                        bl[q][c] = acc;
+                        bl[c][q] = acc;
                     }
                 }
                 const uint32_t y = y0 + static_cast<uint32_t>(i - 2 * kRadius);
@@ -891,39 +968,26 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
                 const float2 b2 = *reinterpret_cast<const float2*>(&ring_b[raw_slot][jm + kRadius]);
                 const float av[2] = {a2.x, a2.y}, bv[2] = {b2.x, b2.y};
                 if (y < H) {
-                    // The lane's two columns' map values are added in FP32,
-                    // then accumulated in FP64.
-                    float pair[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-                    for (int c = 0; c < 2; ++c) {
-                        if (col + static_cast<uint32_t>(c) >= W) continue;
-                        // scale.comp:305-333 rearranged: with e the blurred
-                        // (a - b)^2 and t = mu1 - mu2, s11 + s22 - mu1^2 -
-                        // mu2^2 = num_s - c2 + v where v = e - t^2, so
-                        // den = num_s + v and den - num_m num_s =
-                        // v + t^2 num_s. The small v replaces the difference
-                        // of two large second moments (closer to FP64 than
-                        // the direct form), and identical images give
-                        // v = t = 0, d = 0 exactly. One reciprocal serves
-                        // both quotients: q = 1 / (den det_orig), with
-                        // den >= c2 - rounding > 0 and det_orig >= 1.
-                        const float mu1 = bl[0][c], mu2 = bl[1][c];
-                        const float t = mu1 - mu2;
-                        const float num_s = fma_(2.0f, fma_(-mu1, mu2, bl[3][c]), kC2);
-                        const float v = fma_(-t, t, bl[2][c]);
-                        const float den = num_s + v;
-                        const float det_dist = 1.0f + fabsf(bv[c] - mu2);
-                        const float det_orig = 1.0f + fabsf(av[c] - mu1);
-                        const float q = rcp_dev(den * det_orig);
-                        const float d = fmaxf((fma_(t * t, num_s, v) * det_orig) * q, 0.0f);
-                        const float dd = d * d;
-                        const float d1 = ((det_dist - det_orig) * den) * q;
-                        const float ring = fmaxf(d1, 0.0f);
-                        const float blur = fmaxf(-d1, 0.0f);
-                        const float rr = ring * ring, bb = blur * blur;
-                        const float v6[6] = {d, dd * dd, ring, rr * rr, blur, bb * bb};
-                        for (int k = 0; k < 6; ++k) pair[k] = c == 0 ? v6[k] : pair[k] + v6[k];
+                    if constexpr (Ref) {
+                        // Each pixel's map values accumulate in FP64.
+                        for (int c = 0; c < 2; ++c) {
+                            if (col + static_cast<uint32_t>(c) >= W) continue;
+                            float v6[6];
+                            map_values<true>(bl[c], av[c], bv[c], v6);
+                            for (int k = 0; k < 6; ++k) sums[k] += static_cast<double>(v6[k]);
+                        }
+                    } else {
+                        // The lane's two columns' map values are added in
+                        // FP32, then accumulated in FP64.
+                        float pair[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                        for (int c = 0; c < 2; ++c) {
+                            if (col + static_cast<uint32_t>(c) >= W) continue;
+                            float v6[6];
+                            map_values<false>(bl[c], av[c], bv[c], v6);
+                            for (int k = 0; k < 6; ++k) pair[k] = c == 0 ? v6[k] : pair[k] + v6[k];
+                        }
+                        for (int k = 0; k < 6; ++k) sums[k] += static_cast<double>(pair[k]);
                     }
-                    for (int k = 0; k < 6; ++k) sums[k] += static_cast<double>(pair[k]);
                 }
             }
         });
@@ -938,13 +1002,13 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
 
 // Work items blockIdx.x, blockIdx.x + gridDim.x, ... of `blocks`, as in
 // prepare_kernel.
-template <typename Lay, typename TapsT>
+template <typename Lay, typename TapsT, bool Ref>
 __global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ pairs,
                                                    const WorkRef* __restrict__ work,
                                                    uint32_t blocks, uint32_t items_per_pair) {
     __shared__ __align__(16) float ring_a[kRing][kSrcCols];
     __shared__ __align__(16) float ring_b[kRing][kSrcCols];
-    __shared__ __align__(16) float prod[2][kSrcCols];
+    __shared__ __align__(16) float prod[MapQuantities<Ref>::kProducts][kSrcCols];
 
     for (uint32_t b = blockIdx.x; b < blocks; b += gridDim.x) {
         uint32_t pair_idx, item_idx;
@@ -962,7 +1026,8 @@ __global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ 
         ConstAS<ScaleItem>* item = as_const(pdp->items) + item_idx;
         const uint32_t level = item->level, channel = item->channel;
         const uint32_t x0 = item->x0, y0 = item->y0;
-        scale_item<typename Lay::Fmt, TapsT>(pdp, plan, item_idx, level, channel, x0, y0, ring_a, ring_b, prod);
+        scale_item<typename Lay::Fmt, TapsT, Ref>(pdp, plan, item_idx, level, channel, x0, y0, ring_a, ring_b,
+                                                  prod);
         __syncthreads();  // the next item reuses the row ring
     }
 }
@@ -1040,37 +1105,55 @@ uint32_t GridOf(K kernel, int block_threads, uint32_t blocks, bool persistent) {
     return resident == 0 ? blocks : std::min(blocks, resident);
 }
 
-template <typename Lay, bool General>
+template <typename Lay, bool General, bool Ref>
 hipError_t launch_prepare_g(const PairDesc* pairs, const WorkRef* work, uint32_t blocks, uint32_t tiles,
                             const float* lut, const uint8_t* src0, const uint8_t* src1, bool persistent,
                             hipStream_t stream) {
-    const uint32_t grid = GridOf(&prepare_kernel<Lay, General>, 256, blocks, persistent);
-    hipLaunchKernelGGL((prepare_kernel<Lay, General>), dim3(grid), dim3(256), 0, stream, pairs, work, blocks,
-                       tiles, lut, src0, src1);
+    const uint32_t grid = GridOf(&prepare_kernel<Lay, General, Ref>, 256, blocks, persistent);
+    hipLaunchKernelGGL((prepare_kernel<Lay, General, Ref>), dim3(grid), dim3(256), 0, stream, pairs, work,
+                       blocks, tiles, lut, src0, src1);
     return hipGetLastError();
 }
 
-template <typename Lay>
-hipError_t launch_prepare_t(bool general, const PairDesc* pairs, const WorkRef* work, uint32_t blocks,
-                            uint32_t tiles, const float* lut, const uint8_t* src0,
-                            const uint8_t* src1, bool persistent, hipStream_t stream) {
-    return general ? launch_prepare_g<Lay, true>(pairs, work, blocks, tiles, lut, src0, src1, persistent, stream)
-                   : launch_prepare_g<Lay, false>(pairs, work, blocks, tiles, lut, src0, src1, persistent, stream);
+template <typename Lay, bool Ref>
+hipError_t launch_prepare_m(bool general, const PairDesc* pairs, const WorkRef* work, uint32_t blocks,
+                            uint32_t tiles, const float* lut, const uint8_t* src0, const uint8_t* src1,
+                            bool persistent, hipStream_t stream) {
+    return general ? launch_prepare_g<Lay, true, Ref>(pairs, work, blocks, tiles, lut, src0, src1, persistent, stream)
+                   : launch_prepare_g<Lay, false, Ref>(pairs, work, blocks, tiles, lut, src0, src1, persistent, stream);
 }
 
-template <typename Lay, typename TapsT>
+template <typename Lay>
+hipError_t launch_prepare_t(MapForm map, bool general, const PairDesc* pairs, const WorkRef* work,
+                            uint32_t blocks, uint32_t tiles, const float* lut, const uint8_t* src0,
+                            const uint8_t* src1, bool persistent, hipStream_t stream) {
+    return map == kMapReference
+               ? launch_prepare_m<Lay, true>(general, pairs, work, blocks, tiles, lut, src0, src1, persistent, stream)
+               : launch_prepare_m<Lay, false>(general, pairs, work, blocks, tiles, lut, src0, src1, persistent, stream);
+}
+
+template <typename Lay, typename TapsT, bool Ref>
 hipError_t launch_scale_g(const PairDesc* pairs, const WorkRef* work, uint32_t blocks, uint32_t items,
                           bool persistent, hipStream_t stream) {
-    const uint32_t grid = GridOf(&scale_kernel<Lay, TapsT>, 64, blocks, persistent);
-    hipLaunchKernelGGL((scale_kernel<Lay, TapsT>), dim3(grid), dim3(64), 0, stream, pairs, work, blocks, items);
+    const uint32_t grid = GridOf(&scale_kernel<Lay, TapsT, Ref>, 64, blocks, persistent);
+    hipLaunchKernelGGL((scale_kernel<Lay, TapsT, Ref>), dim3(grid), dim3(64), 0, stream, pairs, work, blocks,
+                       items);
     return hipGetLastError();
 }
 
-template <typename Lay>
-hipError_t launch_scale_t(uint32_t blur, const PairDesc* pairs, const WorkRef* work, uint32_t blocks,
+template <typename Lay, bool Ref>
+hipError_t launch_scale_m(uint32_t blur, const PairDesc* pairs, const WorkRef* work, uint32_t blocks,
                           uint32_t items, bool persistent, hipStream_t stream) {
-    return blur == kBlurRecursive ? launch_scale_g<Lay, CosineTaps>(pairs, work, blocks, items, persistent, stream)
-                                  : launch_scale_g<Lay, SampledTaps>(pairs, work, blocks, items, persistent, stream);
+    return blur == kBlurRecursive
+               ? launch_scale_g<Lay, CosineTaps, Ref>(pairs, work, blocks, items, persistent, stream)
+               : launch_scale_g<Lay, SampledTaps, Ref>(pairs, work, blocks, items, persistent, stream);
+}
+
+template <typename Lay>
+hipError_t launch_scale_t(uint32_t blur, MapForm map, const PairDesc* pairs, const WorkRef* work,
+                          uint32_t blocks, uint32_t items, bool persistent, hipStream_t stream) {
+    return map == kMapReference ? launch_scale_m<Lay, true>(blur, pairs, work, blocks, items, persistent, stream)
+                                : launch_scale_m<Lay, false>(blur, pairs, work, blocks, items, persistent, stream);
 }
 
 }  // namespace
@@ -1084,34 +1167,40 @@ hipError_t LaunchCopy(void* dst, const void* src, uint64_t bytes, hipStream_t st
     return hipGetLastError();
 }
 
-hipError_t LaunchPrepare(PlaneFormat format, bool general, const PairDesc* pairs, const WorkRef* work,
-                         uint32_t blocks, uint32_t tiles_per_image, const float* lut,
+hipError_t LaunchPrepare(PlaneFormat format, MapForm map, bool general, const PairDesc* pairs,
+                         const WorkRef* work, uint32_t blocks, uint32_t tiles_per_image, const float* lut,
                          const uint8_t* src0, const uint8_t* src1, hipStream_t stream, bool persistent) {
     if (blocks == 0) return hipSuccess;
+    if (map > kMapDifference) return hipErrorInvalidValue;
     const uint32_t t = tiles_per_image;
     switch (format) {
         case kPlaneF32:
-            return launch_prepare_t<LayoutF32>(general, pairs, work, blocks, t, lut, src0, src1, persistent, stream);
+            return launch_prepare_t<LayoutF32>(map, general, pairs, work, blocks, t, lut, src0, src1, persistent,
+                                               stream);
         case kPlaneF16:
-            return launch_prepare_t<LayoutF16>(general, pairs, work, blocks, t, lut, src0, src1, persistent, stream);
+            return launch_prepare_t<LayoutF16>(map, general, pairs, work, blocks, t, lut, src0, src1, persistent,
+                                               stream);
         case kPlaneBF16:
-            return launch_prepare_t<LayoutBF16>(general, pairs, work, blocks, t, lut, src0, src1, persistent, stream);
+            return launch_prepare_t<LayoutBF16>(map, general, pairs, work, blocks, t, lut, src0, src1, persistent,
+                                                stream);
         case kPlaneU16:
-            return launch_prepare_t<LayoutU16>(general, pairs, work, blocks, t, lut, src0, src1, persistent, stream);
+            return launch_prepare_t<LayoutU16>(map, general, pairs, work, blocks, t, lut, src0, src1, persistent,
+                                               stream);
     }
     return hipErrorInvalidValue;
 }
 
-hipError_t LaunchScale(PlaneFormat format, uint32_t blur, const PairDesc* pairs, const WorkRef* work,
-                       uint32_t blocks, uint32_t items_per_pair, hipStream_t stream, bool persistent) {
+hipError_t LaunchScale(PlaneFormat format, uint32_t blur, MapForm map, const PairDesc* pairs,
+                       const WorkRef* work, uint32_t blocks, uint32_t items_per_pair, hipStream_t stream,
+                       bool persistent) {
     if (blocks == 0) return hipSuccess;
-    if (blur > kBlurRecursive) return hipErrorInvalidValue;
+    if (blur > kBlurRecursive || map > kMapDifference) return hipErrorInvalidValue;
     const uint32_t n = items_per_pair;
     switch (format) {
-        case kPlaneF32: return launch_scale_t<LayoutF32>(blur, pairs, work, blocks, n, persistent, stream);
-        case kPlaneF16: return launch_scale_t<LayoutF16>(blur, pairs, work, blocks, n, persistent, stream);
-        case kPlaneBF16: return launch_scale_t<LayoutBF16>(blur, pairs, work, blocks, n, persistent, stream);
-        case kPlaneU16: return launch_scale_t<LayoutU16>(blur, pairs, work, blocks, n, persistent, stream);
+        case kPlaneF32: return launch_scale_t<LayoutF32>(blur, map, pairs, work, blocks, n, persistent, stream);
+        case kPlaneF16: return launch_scale_t<LayoutF16>(blur, map, pairs, work, blocks, n, persistent, stream);
+        case kPlaneBF16: return launch_scale_t<LayoutBF16>(blur, map, pairs, work, blocks, n, persistent, stream);
+        case kPlaneU16: return launch_scale_t<LayoutU16>(blur, map, pairs, work, blocks, n, persistent, stream);
     }
     return hipErrorInvalidValue;
 }

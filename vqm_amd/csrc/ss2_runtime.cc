@@ -40,6 +40,15 @@ PlaneFormat ToFormat(uint32_t f) {
     return static_cast<PlaneFormat>(f);
 }
 
+MapForm ToMapForm(uint32_t map_form, PlaneFormat format) {
+    switch (map_form) {
+        case VQM_MAP_DEFAULT: return format == kPlaneU16 ? kMapDifference : kMapReference;
+        case VQM_MAP_REFERENCE: return kMapReference;
+        case VQM_MAP_DIFFERENCE: return kMapDifference;
+    }
+    Fail(VQM_INVALID_ARGUMENT, "map_form must be VQM_MAP_DEFAULT, _REFERENCE or _DIFFERENCE");
+}
+
 float Elapsed(const Event& a, const Event& b) {
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, a.get(), b.get()) != hipSuccess) {
@@ -269,8 +278,8 @@ Lane::~Lane() {
     if (pending_) (void)hipEventSynchronize(Ev(DoneEvent()).get());
 }
 
-void Lane::Launch(PlaneFormat fmt, const float* lut, const LaunchEntry* entries, uint32_t count,
-                  uint32_t chunks, uint32_t blur, bool persistent) {
+void Lane::Launch(PlaneFormat fmt, MapForm map, const float* lut, const LaunchEntry* entries,
+                  uint32_t count, uint32_t chunks, uint32_t blur, bool persistent) {
     if (pending_) Fail(VQM_LOGIC_ERROR, "lane: the previous launch has not finished");
     if (count == 0 || count > max_pairs_) Fail(VQM_INVALID_ARGUMENT, "lane: bad pair count");
     chunks = std::max(1u, std::min({chunks, count, kMaxChunks}));
@@ -349,7 +358,7 @@ void Lane::Launch(PlaneFormat fmt, const float* lut, const LaunchEntry* entries,
         const WorkRef* wp = uniform ? nullptr : wprep + prep_at[i0];
         const auto np = static_cast<uint32_t>(prep_at[i1] - prep_at[i0]);
         Check(hipEventRecord(Ev(4 * c).get(), s), "hipEventRecord");
-        Check(LaunchPrepare(fmt, general, base, wp, np, g0.prep_tiles, lut, nullptr, nullptr, s, persistent),
+        Check(LaunchPrepare(fmt, map, general, base, wp, np, g0.prep_tiles, lut, nullptr, nullptr, s, persistent),
               "launch prepare");
         Check(hipEventRecord(Ev(4 * c + 1).get(), s), "hipEventRecord");
     }
@@ -361,7 +370,7 @@ void Lane::Launch(PlaneFormat fmt, const float* lut, const LaunchEntry* entries,
         const auto ns = static_cast<uint32_t>(scale_at[i1] - scale_at[i0]);
         if (t != s) Check(hipStreamWaitEvent(t, Ev(4 * c + 1).get(), 0), "order scale after prepare");
         Check(hipEventRecord(Ev(4 * c + 2).get(), t), "hipEventRecord");
-        Check(LaunchScale(fmt, blur, base, ws, ns, g0.plan.items, t, persistent), "launch scale");
+        Check(LaunchScale(fmt, blur, map, base, ws, ns, g0.plan.items, t, persistent), "launch scale");
         Check(hipEventRecord(Ev(4 * c + 3).get(), t), "hipEventRecord");
     }
     const uint32_t r = 4 * chunks;
@@ -422,11 +431,13 @@ static const vqm_ss2_config& Validated(const vqm_ss2_config& config) {
     if (config.chroma_filter > 2)
         Fail(VQM_INVALID_ARGUMENT, "chroma_filter must be bilinear, bicubic or lanczos");
     if (config.blur > kBlurRecursive) Fail(VQM_INVALID_ARGUMENT, "blur must be sampled (0) or recursive (1)");
+    ToMapForm(config.map_form, static_cast<PlaneFormat>(config.plane_format));
     return config;
 }
 
 Pipeline::Pipeline(const vqm_ss2_config& config)
     : config_(Validated(config)),
+      map_(ToMapForm(config.map_form, static_cast<PlaneFormat>(config.plane_format))),
       ctx_(DeviceContext::Get()),
       plan_(config.width, config.height, static_cast<PlaneFormat>(config.plane_format),
             config.strip_rows),
@@ -513,8 +524,9 @@ void Pipeline::BuildGraph() {
     const GeometryPlan& g = plan_.host;
     Check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin graph capture");
     hipError_t e = hipMemcpyAsync(pair_.get(), h_pair_.get(), sizeof(PairDesc), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = LaunchPrepare(fmt, false, pd, nullptr, 2 * g.prep_tiles, g.prep_tiles, ctx_.lut, nullptr, nullptr, s);
-    if (e == hipSuccess) e = LaunchScale(fmt, config_.blur, pd, nullptr, g.plan.items, g.plan.items, s);
+    if (e == hipSuccess)
+        e = LaunchPrepare(fmt, map_, false, pd, nullptr, 2 * g.prep_tiles, g.prep_tiles, ctx_.lut, nullptr, nullptr, s);
+    if (e == hipSuccess) e = LaunchScale(fmt, config_.blur, map_, pd, nullptr, g.plan.items, g.plan.items, s);
     if (e == hipSuccess) e = LaunchReduce(pd, 1, s);
     if (e == hipSuccess)
         e = hipMemcpyAsync(host_norms_.get(), ws_.norms, kNorms * sizeof(double), hipMemcpyDeviceToHost, s);
@@ -595,14 +607,15 @@ uint64_t Pipeline::Launch(const uint8_t* src0, const uint8_t* src1, bool reuse_o
     }
     Check(hipEventRecord(ev_[0].get(), s), "hipEventRecord");
     if (reuse_original)
-        Check(LaunchPrepare(fmt, general_, pd, distorted_work_.as<WorkRef>(), g.prep_tiles, g.prep_tiles, ctx_.lut,
-                            src0, src1, s),
+        Check(LaunchPrepare(fmt, map_, general_, pd, distorted_work_.as<WorkRef>(), g.prep_tiles, g.prep_tiles,
+                            ctx_.lut, src0, src1, s),
               "launch prepare");
     else
-        Check(LaunchPrepare(fmt, general_, pd, nullptr, 2 * g.prep_tiles, g.prep_tiles, ctx_.lut, src0, src1, s),
+        Check(LaunchPrepare(fmt, map_, general_, pd, nullptr, 2 * g.prep_tiles, g.prep_tiles, ctx_.lut, src0, src1,
+                            s),
               "launch prepare");
     Check(hipEventRecord(ev_[1].get(), s), "hipEventRecord");
-    Check(LaunchScale(fmt, config_.blur, pd, nullptr, g.plan.items, g.plan.items, s), "launch scale");
+    Check(LaunchScale(fmt, config_.blur, map_, pd, nullptr, g.plan.items, g.plan.items, s), "launch scale");
     Check(hipEventRecord(ev_[2].get(), s), "hipEventRecord");
     Check(LaunchReduce(pd, 1, s), "launch reduce");
     Check(hipEventRecord(ev_[3].get(), s), "hipEventRecord");
@@ -791,6 +804,7 @@ const vqm_ss2_batch_config& ValidatedBatch(const vqm_ss2_batch_config& config) {
     if (config.chunks > Lane::kMaxChunks) Fail(VQM_INVALID_ARGUMENT, "chunks must be at most 16");
     if (config.blur > kBlurRecursive) Fail(VQM_INVALID_ARGUMENT, "blur must be sampled (0) or recursive (1)");
     if (config.persistent > 1) Fail(VQM_INVALID_ARGUMENT, "persistent must be 0 or 1");
+    ToMapForm(config.map_form, static_cast<PlaneFormat>(config.plane_format));
     return config;
 }
 }  // namespace
@@ -802,6 +816,7 @@ constexpr uint32_t kBatchStripRows = 128;
 
 Batch::Batch(const vqm_ss2_batch_config& config)
     : config_(ValidatedBatch(config)),
+      map_(ToMapForm(config.map_form, static_cast<PlaneFormat>(config.plane_format))),
       ctx_(DeviceContext::Get()),
       pools_(static_cast<PlaneFormat>(config.plane_format),
              config.strip_rows != 0 ? config.strip_rows : kBatchStripRows, config.hbm_budget),
@@ -829,7 +844,7 @@ uint64_t Batch::Submit(const vqm_ss2_device_pair* pairs, uint32_t count) {
         uint32_t most = 0;
         for (const auto& [geometry, n] : per_geometry) {
             const auto [w, h] = geometry;
-            const TuneChoice c = Tuner::Get().Choose(w, h, static_cast<PlaneFormat>(config_.plane_format), n);
+            const TuneChoice c = Tuner::Get().Choose(w, h, static_cast<PlaneFormat>(config_.plane_format), map_, n);
             if (!pools_.HasPlan(w, h)) pools_.SetStripRows(w, h, c.strip_rows);
             if (n > most) {
                 most = n;
@@ -851,7 +866,7 @@ uint64_t Batch::Submit(const vqm_ss2_device_pair* pairs, uint32_t count) {
             e.src[1] = static_cast<const uint8_t*>(pairs[i].distorted);
             taken.push_back(Taken{pairs[i].width, pairs[i].height, std::move(ws)});
         }
-        lane_.Launch(static_cast<PlaneFormat>(config_.plane_format), ctx_.lut, entries.data(), count,
+        lane_.Launch(static_cast<PlaneFormat>(config_.plane_format), map_, ctx_.lut, entries.data(), count,
                      chunks, config_.blur, config_.persistent != 0);
     } catch (...) {
         for (Taken& t : taken) pools_.Return(t.width, t.height, false, std::move(t.ws));

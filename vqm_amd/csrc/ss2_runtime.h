@@ -97,6 +97,11 @@ vqm_image_format Srgb8Format();
 // VQM_PLANES_* to PlaneFormat; INVALID_ARGUMENT otherwise.
 PlaneFormat ToFormat(uint32_t plane_format);
 
+// VQM_MAP_* to the kernels' MapForm for planes of `format`: the default is
+// the reference form, except for the fixed-point (U16) planes, which hold
+// the score bound only in the difference form. INVALID_ARGUMENT otherwise.
+MapForm ToMapForm(uint32_t map_form, PlaneFormat format);
+
 // Per-process device state: the current device must be a gfx950; the sRGB
 // decode table lives in its memory.
 struct DeviceContext {
@@ -230,10 +235,12 @@ public:
     // one another on the lane's first stream, and each run's scale launch
     // follows on a second stream as soon as its prepare is done, so the
     // scale of one run overlaps the prepare of the next.
-    // `blur` is the scale kernel's BlurKind; `persistent` sizes both kernels'
-    // grids to the resident occupancy (LaunchPrepare, LaunchScale).
-    void Launch(PlaneFormat format, const float* lut, const LaunchEntry* entries, uint32_t count,
-                uint32_t chunks = 1, uint32_t blur = kBlurSampled, bool persistent = false);
+    // `blur` is the scale kernel's BlurKind, `map` both kernels' MapForm;
+    // `persistent` sizes both kernels' grids to the resident occupancy
+    // (LaunchPrepare, LaunchScale).
+    void Launch(PlaneFormat format, MapForm map, const float* lut, const LaunchEntry* entries,
+                uint32_t count, uint32_t chunks = 1, uint32_t blur = kBlurSampled,
+                bool persistent = false);
     bool Done() const;
     void Wait();
     // Forgets a launch that failed part-way (its streams already synchronized).
@@ -314,6 +321,7 @@ private:
     void BuildGraph();
 
     vqm_ss2_config config_;
+    MapForm map_;
     DeviceContext& ctx_;
     DevicePlan plan_;
     Workspace ws_;
@@ -365,6 +373,7 @@ private:
         std::unique_ptr<Workspace> ws;
     };
     vqm_ss2_batch_config config_;
+    MapForm map_;
     DeviceContext& ctx_;
     WorkspacePools pools_;
     Lane lane_;

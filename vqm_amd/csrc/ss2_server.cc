@@ -24,6 +24,7 @@ const vqm_ss2_server_config& ValidatedServer(const vqm_ss2_server_config& c) {
     if (c.lanes > 16) Fail(VQM_INVALID_ARGUMENT, "lanes must be between 1 and 16");
     if (c.max_batch > 4096) Fail(VQM_INVALID_ARGUMENT, "max_batch must be between 1 and 4096");
     if (c.blur > kBlurRecursive) Fail(VQM_INVALID_ARGUMENT, "blur must be sampled (0) or recursive (1)");
+    ToMapForm(c.map_form, static_cast<PlaneFormat>(c.plane_format));
     BuildPlan(8, 8, static_cast<PlaneFormat>(c.plane_format), c.strip_rows);  // validates strip_rows
     return c;
 }
@@ -40,6 +41,7 @@ Server::Server(const vqm_ss2_server_config& config)
     : config_(WithDefaults(ValidatedServer(config))),
       ctx_(DeviceContext::Get()),
       format_(static_cast<PlaneFormat>(config.plane_format)),
+      map_(ToMapForm(config.map_form, static_cast<PlaneFormat>(config.plane_format))),
       pools_(static_cast<PlaneFormat>(config.plane_format), config.strip_rows, config.hbm_budget) {
     for (uint32_t i = 0; i < config_.lanes; ++i) {
         lanes_.push_back(std::make_unique<Lane>(config_.max_batch));
@@ -193,7 +195,7 @@ void Server::DispatchLoop() {
                 }
                 e.image_bytes = r.image_bytes;
             }
-            lanes_[lane]->Launch(format_, ctx_.lut, entries.data(), n, 1, config_.blur);
+            lanes_[lane]->Launch(format_, map_, ctx_.lut, entries.data(), n, 1, config_.blur);
         } catch (const Error& e) {
             status = e.status;
             message = e.message;

@@ -72,6 +72,7 @@ private:
     vqm_ss2_server_config config_;
     DeviceContext& ctx_;
     PlaneFormat format_;
+    MapForm map_;
 
     std::mutex mu_;                 // queue, requests, lanes, stats
     std::condition_variable cv_;    // any state change

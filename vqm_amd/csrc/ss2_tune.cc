@@ -29,11 +29,11 @@ uint32_t Tuner::SizeClass(uint32_t pairs) {
     return c;
 }
 
-TuneChoice Tuner::Choose(uint32_t width, uint32_t height, PlaneFormat format, uint32_t pairs) {
+TuneChoice Tuner::Choose(uint32_t width, uint32_t height, PlaneFormat format, MapForm map, uint32_t pairs) {
     PlanScales(width, height);
     ToFormat(format);
     const uint32_t cls = SizeClass(std::max(pairs, 1u));
-    const auto key = std::make_tuple(width, height, static_cast<uint32_t>(format), cls);
+    const auto key = std::make_tuple(width, height, static_cast<uint32_t>(format), static_cast<uint32_t>(map), cls);
     std::lock_guard<std::mutex> lock(mu_);
     if (auto it = cache_.find(key); it != cache_.end()) {
         TuneChoice c = it->second;
@@ -68,12 +68,12 @@ TuneChoice Tuner::Choose(uint32_t width, uint32_t height, PlaneFormat format, ui
         }
         for (uint32_t chunks : kChunks) {
             if (chunks > n) continue;
-            lane.Launch(format, ctx.lut, entries.data(), n, chunks);  // warm-up
+            lane.Launch(format, map, ctx.lut, entries.data(), n, chunks);  // warm-up
             lane.Wait();
             float ms = std::numeric_limits<float>::infinity();
             for (int r = 0; r < kRepeats; ++r) {
                 const auto t0 = std::chrono::steady_clock::now();
-                lane.Launch(format, ctx.lut, entries.data(), n, chunks);
+                lane.Launch(format, map, ctx.lut, entries.data(), n, chunks);
                 lane.Wait();
                 const auto t1 = std::chrono::steady_clock::now();
                 ms = std::min(ms, std::chrono::duration<float, std::milli>(t1 - t0).count());

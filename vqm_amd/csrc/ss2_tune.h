@@ -3,7 +3,8 @@
 // candidate on the device. Neither parameter changes a single result bit
 // (each pixel's arithmetic and each pair's reduction order are fixed by the
 // geometry), so the choice is free to follow the measurement. Choices are
-// cached per (width, height, format, batch-size class) for the process.
+// cached per (width, height, format, map form, batch-size class) for the
+// process.
 #pragma once
 
 #include <cstdint>
@@ -34,11 +35,11 @@ public:
     // The batch-size class a pair count falls in: the next power of two,
     // capped at 512.
     static uint32_t SizeClass(uint32_t pairs);
-    TuneChoice Choose(uint32_t width, uint32_t height, PlaneFormat format, uint32_t pairs);
+    TuneChoice Choose(uint32_t width, uint32_t height, PlaneFormat format, MapForm map, uint32_t pairs);
 
 private:
     std::mutex mu_;
-    std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>, TuneChoice> cache_;
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>, TuneChoice> cache_;
 };
 
 }  // namespace vqm::ss2

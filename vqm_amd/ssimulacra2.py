@@ -193,6 +193,19 @@ class Blur(enum.IntEnum):
     Recursive = 1
 
 
+class MapForm(enum.IntEnum):
+    """The arithmetic of the XYB transform and the SSIM' map
+    (include/vqm_ss2.h VQM_MAP_*): Reference is the reference's own
+    operation order (prepare.comp:81-98, scale.comp:130-142,305-333) with
+    correctly rounded cube roots; Difference blurs (a - b)^2 in place of a^2
+    and b^2 (closer to FP64, a fifth less blur work, no reference
+    counterpart); Default is Reference except with 16-bit fixed-point (U16)
+    planes."""
+    Default = 0
+    Reference = 1
+    Difference = 2
+
+
 def blur_taps(blur: int = Blur.Sampled) -> np.ndarray:
     """The 13 FP32 taps the scale kernel applies for `blur`."""
     t = (L.f32 * 13)()
@@ -414,6 +427,7 @@ class Ssimulacra2Config:
     graph: bool = False  # replay one captured HIP graph per pair (no reference field)
     chroma_filter: int = 0  # ChromaFilter: 0 bilinear, 1 bicubic, 2 lanczos
     blur: int = 0  # Blur: 0 sampled (the reference), 1 recursive (no reference field)
+    map_form: int = 0  # MapForm: 0 default, 1 reference order, 2 difference (no reference field)
 
     def to_c(self) -> L.ConfigC:
         c = L.ConfigC()
@@ -421,6 +435,7 @@ class Ssimulacra2Config:
         c.graph = 1 if self.graph else 0
         c.chroma_filter = int(self.chroma_filter)
         c.blur = int(self.blur)
+        c.map_form = int(self.map_form)
         c.width, c.height = self.size.width, self.size.height
         c.resident_images = self.gpu_resident_images
         fmt = self.plane_format
@@ -737,9 +752,10 @@ class Ssimulacra2Batch:
 
     def __init__(self, max_pairs: int, plane_format: int = L.PLANES_F32, hbm_budget: int = 0,
                  strip_rows: int = 0, chunks: int = 1, autotune: bool = False, blur: int = 0,
-                 persistent: bool = False):
+                 persistent: bool = False, map_form: int = 0):
         cfg = L.BatchConfigC()
         cfg.blur = int(blur)
+        cfg.map_form = int(map_form)
         cfg.persistent = 1 if persistent else 0
         cfg.plane_format = plane_format
         cfg.strip_rows = strip_rows
@@ -822,11 +838,13 @@ class PinnedBuffer:
             pass
 
 
-def autotune(size: Size, plane_format: int = L.PLANES_F32, pairs: int = 1) -> dict:
+def autotune(size: Size, plane_format: int = L.PLANES_F32, pairs: int = 1, map_form: int = 0) -> dict:
     """The launch shape (strip height, chunk count) measured fastest for a
-    batch of `pairs` pairs of this size and plane format (vqm_ss2_autotune)."""
+    batch of `pairs` pairs of this size, plane format and map form
+    (vqm_ss2_autotune)."""
     r = L.TuneResultC()
-    _check(L.lib().vqm_ss2_autotune(size.width, size.height, plane_format, pairs, ctypes.byref(r)))
+    _check(L.lib().vqm_ss2_autotune(size.width, size.height, plane_format, int(map_form), pairs,
+                                    ctypes.byref(r)))
     return {name: getattr(r, name) for name, _ in L.TuneResultC._fields_}
 
 
@@ -871,10 +889,11 @@ class Ssimulacra2Server:
     against `hbm_budget`, and overlap on the device. Thread-safe."""
 
     def __init__(self, plane_format: int = L.PLANES_F32, lanes: int = 3, max_batch: int = 64,
-                 hbm_budget: int = 0, strip_rows: int = 0, blur: int = 0):
+                 hbm_budget: int = 0, strip_rows: int = 0, blur: int = 0, map_form: int = 0):
         cfg = L.ServerConfigC()
         L.lib().vqm_ss2_server_config_init(ctypes.byref(cfg))
         cfg.blur = int(blur)
+        cfg.map_form = int(map_form)
         cfg.plane_format = plane_format
         cfg.lanes = lanes
         cfg.max_batch = max_batch
