@@ -2,21 +2,26 @@
 """SSIMULACRA 2.1 throughput on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1080p|4k|512]
-                    [--planes f32|u16|f16|bf16] [--cpu-sample P | --no-cpu]
+                    [--planes f32|u16|f16|bf16] [--map default|reference|difference]
+                    [--no-cpu]
 
 A step scores one batch of synthetic pairs already resident in HBM: the 1080p
 workload is 3 distortion strengths x 100 pairs (block step 4/12/24, noise
 sigma 1/3/8), generated deterministically (vqm_amd/synth.py). Every pair of a
 step goes through one prepare, one scale and one reduce launch of the batch
 engine; the norms of all pairs come back to the host before the step ends.
+The map form is the library default (the reference's operation order for
+FP32 planes) unless --map names another.
 
 Printed: one JSON line with pairs/s (`value`), the roofline of the dominant
 kernel (algorithmic bytes per launch over its HIP-event duration on the batch
-stream, against 8 TB/s), the CPU baseline (the oracle's FP32 device form on
-the host's cores, one pair per thread), and extras: the other plane form and
-the 4K batch (`plane_forms`, `workload_4k`), single-pair and streaming
-latency (`modes`), the uneven-size batch (`ladder_batch`) and the serving
-loop (`serving`).
+stream, against 8 TB/s, with the VALU issue rate that binds it), the CPU
+baseline (the oracle's FP32 ref form on the host's cores, one pair per
+thread, median of rounds), and extras: the other map form and plane form at
+the same workload (`map_forms`, `plane_forms`), the 4K batch in FP32 and the
+mixed form (`workload_4k`), streaming through Ssimulacra2Pool with 1 and 3
+workers (`streaming_pool`), single-pair latency (`modes`), the uneven-size
+batch (`ladder_batch`) and the serving loop (`serving`).
 """
 from __future__ import annotations
 
@@ -36,13 +41,23 @@ HBM_PEAK_GBPS = 8000.0
 RTX5070_1080P_RESIDENT = 3836.7  # 1080p resident, 3 workers, RTX 5070
 WORKLOADS = {"1080p": (1920, 1080, 1), "4k": (3840, 2160, 2), "512": (512, 512, 0)}
 PLANES = {"f32": 0, "f16": 1, "bf16": 2, "u16": 3}  # u16: the mixed form
+MAPS = {"default": 0, "reference": 1, "difference": 2}
+# vqm's published streaming rates with 3 workers (RTX 5070).
+RTX5070_STREAMING = {"1920x1080": 1816.4, "3840x2160": 455.2}
+
+
+def map_name(fmt: int, map_form: int) -> str:
+    """The map form a configuration runs (VQM_MAP_DEFAULT resolved)."""
+    if map_form == 0:
+        return "difference" if fmt == 3 else "reference"
+    return "reference" if map_form == 1 else "difference"
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def profiled_traffic(kernel: str, planes: str, w: int, h: int, pairs: int):
+def profiled_traffic(kernel: str, planes: str, form: str, w: int, h: int, pairs: int):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by the
     plain copy kernel's reading of a known 1 GiB), or None when no summary
@@ -50,10 +65,10 @@ def profiled_traffic(kernel: str, planes: str, w: int, h: int, pairs: int):
     if (w, h, pairs) != (1920, 1080, 300):
         return None
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        f"ss2_kernels_1080p_batch300_{planes}.md")
+                        f"ss2_kernels_1080p_batch300_{planes}_{form}.md")
     if not os.path.exists(path):
         return None
-    row = f"| {kernel}_kernel<{planes.upper()}> |"
+    row = f"| {kernel}_kernel<{planes.upper()}, {form}> |"
     derived = False
     for line in open(path):
         if line.startswith("| kernel | HBM read GB"):
@@ -80,6 +95,25 @@ def make_pairs(w, h, cfg, per_strength):
         for i in range(per_strength):
             pairs.append(synth.pair(cfg, k * per_strength + i, w, h, strength))
     return pairs
+
+
+def make_pairs_cycled(w, h, cfg, count):
+    """`count` pairs whose strengths cycle through the three."""
+    from vqm_amd import synth
+    return [synth.pair(cfg, i, w, h, synth.STRENGTHS[i % 3]) for i in range(count)]
+
+
+def upload_pairs(pairs, w, h):
+    """One device buffer holding every pair; returns it and the batch refs."""
+    from vqm_amd import DeviceBuffer
+    slot = (pairs[0][0].nbytes + 255) // 256 * 256
+    dev = DeviceBuffer(2 * slot * len(pairs))
+    refs = []
+    for i, (a, b) in enumerate(pairs):
+        dev.upload(a, 2 * i * slot)
+        dev.upload(b, (2 * i + 1) * slot)
+        refs.append((dev.ptr + 2 * i * slot, dev.ptr + (2 * i + 1) * slot, w, h))
+    return dev, refs
 
 
 class LadderPool:
@@ -200,7 +234,7 @@ def ladder_batch(args, pool):
     """256 pairs drawn uniformly from the ladder in one batched launch per
     kernel, against the same pairs scored one at a time through one
     single-pair pipeline per size."""
-    from vqm_amd import DeviceImage, Size, Ssimulacra2Batch, Ssimulacra2Config, Ssimulacra2Pipeline
+    from vqm_amd import DeviceImage, ImageFormat, Size, Ssimulacra2Batch, Ssimulacra2Config, Ssimulacra2Pipeline
     n = 256
     picks = [pool.pick(3, i, None) for i in range(n)]
     refs = [(e["dev"][0], e["dev"][1], w, h) for w, h, e in picks]
@@ -224,10 +258,10 @@ def ladder_batch(args, pool):
             if (w, h) not in pipes:
                 pipes[(w, h)] = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(w, h), plane_format=PLANES[planes]))
         for w, h, e in picks[:10]:
-            pipes[(w, h)].Compute(DeviceImage(e["dev"][0], Size(w, h)), DeviceImage(e["dev"][1], Size(w, h)))
+            pipes[(w, h)].Compute(DeviceImage(e["dev"][0], Size(w, h), ImageFormat.srgb8()), DeviceImage(e["dev"][1], Size(w, h), ImageFormat.srgb8()))
         t0 = time.perf_counter()
         for w, h, e in picks:
-            pipes[(w, h)].Compute(DeviceImage(e["dev"][0], Size(w, h)), DeviceImage(e["dev"][1], Size(w, h)))
+            pipes[(w, h)].Compute(DeviceImage(e["dev"][0], Size(w, h), ImageFormat.srgb8()), DeviceImage(e["dev"][1], Size(w, h), ImageFormat.srgb8()))
         dt = time.perf_counter() - t0
         out[f"single_pair_loop_{planes}"] = {"pairs_per_s": round(n / dt, 1)}
         for p in pipes.values():
@@ -236,12 +270,12 @@ def ladder_batch(args, pool):
     return out
 
 
-def resident_batch(refs, w, h, fmt, steps, warmup, chunks=1, strip_rows=0, persistent=False):
+def resident_batch(refs, w, h, fmt, steps, warmup, chunks=1, strip_rows=0, persistent=False, map_form=0):
     """Pairs/s of the batch engine over resident pairs, with the per-stage
     HIP-event times and the scale kernel's HBM fraction."""
     from vqm_amd import Ssimulacra2Batch
     batch = Ssimulacra2Batch(max_pairs=len(refs), plane_format=fmt, strip_rows=strip_rows, chunks=chunks,
-                             persistent=persistent)
+                             persistent=persistent, map_form=map_form)
     for _ in range(warmup):
         batch.Finish(batch.Submit(refs), raw=True)
     stages = []
@@ -258,6 +292,7 @@ def resident_batch(refs, w, h, fmt, steps, warmup, chunks=1, strip_rows=0, persi
     prep_gbps = len(refs) * (6 * w * h + 2 * planes) / (stage[0] * 1e-3) / 1e9
     scale_gbps = len(refs) * 2 * planes / (stage[1] * 1e-3) / 1e9
     return {"pairs_per_s": round(len(refs) * steps / total, 1),
+            "map_form": map_name(fmt, map_form),
             "ms_per_step": round(total / steps * 1e3, 3),
             "pairs_per_step": len(refs),
             "stage_ms": {"prepare": round(stage[0], 4), "scale": round(stage[1], 4),
@@ -266,17 +301,92 @@ def resident_batch(refs, w, h, fmt, steps, warmup, chunks=1, strip_rows=0, persi
                          "scale": round(scale_gbps / HBM_PEAK_GBPS, 4)}}
 
 
-def cpu_baseline(w, h, pairs, sample):
+def host_cores() -> int:
+    """The cores this process may run on: its CPU affinity, capped by the
+    job's CPU share when the environment states one (OMP_NUM_THREADS)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, h, pairs, rounds=5, warmups=3):
+    """The oracle's FP32 ref form (the reference's operation order) on the
+    host: one pair per thread on every available core, 3 warm-up rounds,
+    then the median of `rounds` rounds of two pairs per thread; the oracle's
+    device form beside it, and single-thread latency per pair."""
     import oracle
-    threads = max(1, min(16, os.cpu_count() or 1))
-    chosen = [pairs[(i * 37) % len(pairs)] for i in range(sample)]
-    oracle.compute_many(oracle.F32_DEV, w, h, chosen[:threads], threads)  # warm
-    t0 = time.perf_counter()
-    oracle.compute_many(oracle.F32_DEV, w, h, chosen, threads)
-    dt = time.perf_counter() - t0
-    return {"value": round(sample / dt, 3), "unit": "pairs/s", "cores": threads, "kind": "port",
-            "sample": f"{sample} pairs of the {w}x{h} workload, oracle FP32 device form, one pair "
-                      f"per host thread ({dt:.1f} s wall)"}
+    threads = host_cores()
+    per_round = 2 * threads
+    chosen = [pairs[(i * 37) % len(pairs)] for i in range(per_round)]
+
+    def rate(mode, n_rounds, n_warm):
+        for _ in range(n_warm):
+            oracle.compute_many(mode, w, h, chosen[:threads], threads)
+        times = []
+        for _ in range(n_rounds):
+            t0 = time.perf_counter()
+            oracle.compute_many(mode, w, h, chosen, threads)
+            times.append(time.perf_counter() - t0)
+        return per_round / statistics.median(times), times
+
+    value, times = rate(oracle.F32_REF, rounds, warmups)
+    dev_value, _ = rate(oracle.F32_DEV, 3, 1)
+    lat = []
+    for a, b in chosen[:3]:
+        t0 = time.perf_counter()
+        oracle.compute(oracle.F32_REF, w, h, a, b)
+        lat.append((time.perf_counter() - t0) * 1e3)
+    return {"value": round(value, 3), "unit": "pairs/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
+            "single_thread_ms_per_pair": round(statistics.median(lat), 1),
+            "device_form_pairs_per_s": round(dev_value, 3),
+            "round_s": [round(t, 3) for t in times],
+            "sample": f"{per_round} pairs of the {w}x{h} workload per round ({threads} threads, one pair "
+                      f"per thread at a time), oracle FP32 ref form; {warmups} warm-up rounds, median "
+                      f"of {rounds}; latency: median of 3 pairs on one thread"}
+
+
+def streaming_pool(w, h, cfg, map_form, workers_list=(1, 3), count=60, rounds=3):
+    """Host images scored through Ssimulacra2Pool (Submit(Image, Image):
+    each call copies both images to the device), round-robin over 1 and 3
+    pipelines as vqm's bench workers do (bench/driver/ssimulacra2.cc:175-285),
+    median pairs/s of `rounds` passes over `count` distinct pairs."""
+    from vqm_amd import Image, ImageFormat, Size, Ssimulacra2Config, Ssimulacra2Pool
+    pairs = make_pairs_cycled(w, h, cfg, count)
+    size = Size(w, h)
+    imgs = [(Image(size, ImageFormat.srgb8(), a), Image(size, ImageFormat.srgb8(), b)) for a, b in pairs]
+    out = {"pairs": count, "map_form": map_name(0, map_form)}
+    for workers in workers_list:
+        pool = Ssimulacra2Pool(Ssimulacra2Config(size=size, map_form=map_form), workers)
+        pool.ComputeMany(imgs[:max(workers, 4)])  # warm
+        rates = []
+        for _ in range(rounds):
+            t0 = time.perf_counter()
+            res = pool.ComputeMany(imgs)
+            rates.append(count / (time.perf_counter() - t0))
+        if not all(np.isfinite(r.Score()) for r in res):
+            raise RuntimeError("non-finite streaming score")
+        pool.close()
+        out[f"workers_{workers}"] = {"pairs_per_s": round(statistics.median(rates), 1)}
+    key = f"{w}x{h}"
+    if key in RTX5070_STREAMING:
+        out["vqm_rtx5070_workers_3"] = RTX5070_STREAMING[key]
+    return out
 
 
 def main() -> int:
@@ -287,6 +397,8 @@ def main() -> int:
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="1080p")
     ap.add_argument("--pairs-per-strength", type=int, default=100)
     ap.add_argument("--planes", choices=sorted(PLANES), default="f32")
+    ap.add_argument("--map", choices=sorted(MAPS), default="default",
+                    help="SSIM' map form (default: the library's, the reference order for FP32 planes)")
     ap.add_argument("--strip-rows", type=int, default=0,
                     help="output rows per scale work item of the batch (0 = the library default)")
     ap.add_argument("--chunks", type=int, default=1,
@@ -303,8 +415,9 @@ def main() -> int:
     ap.add_argument("--serve-budget-gib", type=int, default=8)
     ap.add_argument("--ladder-variants", type=int, default=2,
                     help="distinct pairs per ladder size and strength")
-    ap.add_argument("--cpu-sample", type=int, default=160)
+    ap.add_argument("--cpu-rounds", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-streaming", action="store_true", help="skip the Ssimulacra2Pool streaming legs")
     ap.add_argument("--no-modes", action="store_true")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the 4K batch and the other plane forms at the main workload")
@@ -322,21 +435,16 @@ def main() -> int:
 
     w, h, cfg = WORKLOADS[args.workload]
     fmt = PLANES[args.planes]
+    map_form = MAPS[args.map]
+    form = map_name(fmt, map_form)
     info = device_info()
     log(f"device {info['name']} {info['arch']} cus={info['cus']}")
 
     t0 = time.perf_counter()
     pairs = make_pairs(w, h, cfg, args.pairs_per_strength)
     log(f"generated {len(pairs)} pairs of {w}x{h} in {time.perf_counter() - t0:.1f} s")
-    img_bytes = pairs[0][0].nbytes
-    slot = (img_bytes + 255) // 256 * 256
-    dev = DeviceBuffer(2 * slot * len(pairs))
-    refs = []
-    for i, (a, b) in enumerate(pairs):
-        dev.upload(a, 2 * i * slot)
-        dev.upload(b, (2 * i + 1) * slot)
-        refs.append((dev.ptr + 2 * i * slot, dev.ptr + (2 * i + 1) * slot, w, h))
-    log(f"uploaded {2 * len(pairs)} images ({2 * slot * len(pairs) / 1e9:.2f} GB)")
+    dev, refs = upload_pairs(pairs, w, h)
+    log(f"uploaded {2 * len(pairs)} images ({dev.nbytes / 1e9:.2f} GB)")
 
     if args.copy_calibration:
         import ctypes
@@ -353,7 +461,7 @@ def main() -> int:
         dst_buf.free()
 
     batch = Ssimulacra2Batch(max_pairs=len(refs), plane_format=fmt, strip_rows=args.strip_rows,
-                             chunks=args.chunks, persistent=args.persistent)
+                             chunks=args.chunks, persistent=args.persistent, map_form=map_form)
     for _ in range(args.warmup):
         scales, norms = batch.Finish(batch.Submit(refs), raw=True)
     step_ms, stages = [], []
@@ -368,7 +476,7 @@ def main() -> int:
         log("non-finite norms")
         return 1
     value = len(refs) * args.steps / total
-    log(f"batch: {value:.1f} pairs/s, {total / args.steps * 1e3:.2f} ms/step")
+    log(f"batch ({form} map form): {value:.1f} pairs/s, {total / args.steps * 1e3:.2f} ms/step")
 
     # Roofline of the dominant kernel: algorithmic bytes per launch over the
     # average HIP-event duration of that launch on the batch stream.
@@ -384,7 +492,7 @@ def main() -> int:
     dominant = max(("prepare", "scale"), key=lambda k: avg[k])
     achieved = launch_bytes[dominant] / (avg[dominant] * 1e-3) / 1e9
     pair_bytes = 2 * 3 * n_px + 4 * planes_bytes
-    traffic = profiled_traffic(dominant, args.planes, w, h, len(refs))
+    traffic = profiled_traffic(dominant, args.planes, form, w, h, len(refs))
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                 "traffic": traffic["bytes"] if traffic else None,
@@ -392,6 +500,7 @@ def main() -> int:
                 # The roof that binds: wave64 VALU instructions issued per CU
                 # per cycle (non-packed FP32 issues at most one), from the
                 # same counter summary.
+                "binding": "valu_issue" if traffic and traffic["valu_per_cu_cycle"] > 0.8 else "hbm",
                 "valu_issue": ({"achieved": traffic["valu_per_cu_cycle"], "peak": 1.0,
                                 "unit": "wave64 VALU instructions per CU-cycle",
                                 "frac": traffic["valu_per_cu_cycle"]} if traffic else None),
@@ -416,38 +525,44 @@ def main() -> int:
         "data": "synthetic (vqm_amd/synth.py: sinusoid field + block quantisation + Gaussian noise)",
         "config": {"workload": f"{w}x{h} 8-bit sRGB RGB, 6 scales, {len(refs)} pairs per step "
                                f"(3 strengths x {args.pairs_per_strength}), resident in HBM, "
-                               f"{args.planes} planes, one batched launch per kernel",
-                   "pairs_per_step": len(refs), "width": w, "height": h, "parallelism": "single GPU"},
+                               f"{args.planes} planes, {form} map form, one batched launch per kernel",
+                   "pairs_per_step": len(refs), "width": w, "height": h, "map_form": form,
+                   "parallelism": "single GPU"},
         "roofline": roofline,
         "vs_baseline_note": "value / 3836.7 pairs/s (vqm 1080p resident, 3 workers, RTX 5070, BASELINE.md)",
     }
 
     if not args.no_secondary:
-        # The other plane forms on the same resident pairs, then the 4K batch.
+        sec_steps = max(3, args.steps // 4)
+        # The other map form and the other plane form on the same resident
+        # pairs, then the 4K batch in FP32 and the mixed form.
+        other = 2 if form == "reference" else 1
+        out["map_forms"] = {map_name(fmt, other): resident_batch(refs, w, h, fmt, sec_steps, 1, args.chunks,
+                                                                 persistent=args.persistent, map_form=other)}
+        log(f"map forms: {out['map_forms']}")
         forms = {}
         for name, f in sorted(PLANES.items()):
             if f != fmt and name in ("f32", "u16"):
-                forms[name] = resident_batch(refs, w, h, f, max(3, args.steps // 4), 1, args.chunks,
-                                             persistent=args.persistent)
+                forms[name] = resident_batch(refs, w, h, f, sec_steps, 1, args.chunks, persistent=args.persistent)
         out["plane_forms"] = forms
         log(f"plane forms: {forms}")
         if args.workload == "1080p":
             w4, h4, cfg4 = WORKLOADS["4k"]
-            pairs4 = make_pairs(w4, h4, cfg4, 10)
-            slot4 = (pairs4[0][0].nbytes + 255) // 256 * 256
-            dev4 = DeviceBuffer(2 * slot4 * len(pairs4))
-            refs4 = []
-            for i, (a, b) in enumerate(pairs4):
-                dev4.upload(a, 2 * i * slot4)
-                dev4.upload(b, (2 * i + 1) * slot4)
-                refs4.append((dev4.ptr + 2 * i * slot4, dev4.ptr + (2 * i + 1) * slot4, w4, h4))
+            pairs4 = make_pairs_cycled(w4, h4, cfg4, 50)
+            dev4, refs4 = upload_pairs(pairs4, w4, h4)
             out["workload_4k"] = {"workload": f"{w4}x{h4} 8-bit sRGB RGB, {len(refs4)} resident pairs per step "
-                                              f"(3 strengths x 10), {args.planes} planes",
-                                  **resident_batch(refs4, w4, h4, fmt, max(3, args.steps // 4), 1, args.chunks,
-                                                   persistent=args.persistent)}
+                                              f"(strengths cycled)"}
+            for name in ("f32", "u16"):
+                out["workload_4k"][name] = resident_batch(refs4, w4, h4, PLANES[name], sec_steps, 1, args.chunks,
+                                                          persistent=args.persistent)
             dev4.free()
             del pairs4
             log(f"4k: {out['workload_4k']}")
+
+    if not args.no_streaming:
+        out["streaming_pool"] = {"1920x1080": streaming_pool(1920, 1080, 1, map_form, count=90),
+                                 "3840x2160": streaming_pool(3840, 2160, 2, map_form, count=30)}
+        log(f"streaming pool: {out['streaming_pool']}")
 
     if not args.no_modes:
         modes = {}
@@ -455,13 +570,13 @@ def main() -> int:
         # Single pair at a time, resident inputs: per-pair latency, with the
         # pair's launches replayed from one HIP graph and launched eagerly.
         for key, graph in (("resident_single_pair_graph", True), ("resident_single_pair", False)):
-            pipe = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(w, h), plane_format=fmt,
+            pipe = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(w, h), plane_format=fmt, map_form=map_form,
                                                          strip_rows=args.single_strip_rows, graph=graph))
             lat = []
             for i in range(min(60, len(refs))):
                 a, b = refs[i][0], refs[i][1]
                 t = time.perf_counter()
-                pipe.Compute(DeviceImage(a, Size(w, h)), DeviceImage(b, Size(w, h)))
+                pipe.Compute(DeviceImage(a, Size(w, h), ImageFormat.srgb8()), DeviceImage(b, Size(w, h), ImageFormat.srgb8()))
                 lat.append((time.perf_counter() - t) * 1e3)
             lat = lat[5:]
             modes[key] = {"pairs_per_s": round(1e3 / statistics.mean(lat), 1),
@@ -494,7 +609,7 @@ def main() -> int:
         del pool
 
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, h, pairs, args.cpu_sample)
+        out["cpu_baseline"] = cpu_baseline(w, h, pairs, rounds=args.cpu_rounds)
         log(f"cpu baseline: {out['cpu_baseline']}")
 
     print(json.dumps(out), flush=True)

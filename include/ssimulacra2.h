@@ -17,11 +17,13 @@
 #include <array>
 #include <cstddef>
 #include <cstdint>
+#include <span>
 #include <stdexcept>
 #include <string>
 #include <utility>
 #include <vector>
 
+#include "vqm_memory_types.h"
 #include "vqm_ss2.h"
 
 namespace vqm {
@@ -67,12 +69,7 @@ enum class PlaneFormat : std::uint32_t {
 enum class MapForm : std::uint32_t {
     Default = VQM_MAP_DEFAULT, Reference = VQM_MAP_REFERENCE, Difference = VQM_MAP_DIFFERENCE
 };
-// vqm_memory_types.h:14: the arena allocation path. Device memory in HBM is
-// the only path here: Automatic and Direct select it, Staged (a host-visible
-// staging arena in front of device memory) is unavailable, so a pipeline
-// configured with it throws std::runtime_error, as the reference does for a
-// strict policy whose path does not exist.
-enum class MemoryPolicy : std::uint8_t { Automatic, Direct, Staged };
+
 
 // vqm_image.h:100-120. The reference's default is Float sRGB RGB;
 // ImageFormat::Srgb8() is the 8-bit form the table-driven fast decode serves,
@@ -116,9 +113,10 @@ struct ImageFormat {
 };
 
 // vqm_image.h:134-141: packed planes, each starting on a 4-byte boundary.
+// The format defaults to ImageFormat{} (Float sRGB RGB), as in the reference.
 struct Image {
     Size size;
-    ImageFormat format = ImageFormat::Srgb8();
+    ImageFormat format{};
     std::vector<std::uint8_t> data;
     [[nodiscard]] bool HasAlpha() const noexcept { return format.alpha != AlphaType::None; }
 };
@@ -148,11 +146,12 @@ struct ImageView {
 };
 
 // Caller-owned device image (replaces VulkanBufferView, vqm_vulkan.h:74-101):
-// must stay alive and unmodified until Finish returns.
+// must stay alive and unmodified until Finish returns. Like the reference's
+// view, its format defaults to ImageFormat{}.
 struct DeviceImageView {
     const void* data = nullptr;
     Size size;
-    ImageFormat format = ImageFormat::Srgb8();
+    ImageFormat format{};
 };
 
 namespace detail {
@@ -291,38 +290,20 @@ struct Ssimulacra2Config {
     }
 };
 
-// vqm_memory_types.h:29-38, by region.
-struct PipelineMemoryUsage {
-    std::uint64_t workspace_bytes = 0;
-    std::uint64_t resident_bytes = 0;
-    std::uint64_t streaming_bytes = 0;
-    std::uint64_t input_pool_bytes = 0;
-    std::uint64_t pinned_host_bytes = 0;
-    std::uint64_t total_device_bytes = 0;
-};
+namespace detail {
 
-// GetMemoryRequirements (vqm_memory.h:63-69): the device and pinned host
-// bytes a pipeline with `config` allocates, planned before construction
-// without touching the device. Throws std::invalid_argument for a geometry
-// or configuration the pipeline would refuse.
-[[nodiscard]] inline PipelineMemoryUsage GetMemoryRequirements(const Ssimulacra2Config& config) {
-    const vqm_ss2_config c = config.ToC();
-    vqm_ss2_memory m{};
-    detail::Check(vqm_ss2_memory_plan(&c, &m));
-    return PipelineMemoryUsage{m.workspace_bytes, m.resident_bytes, m.streaming_bytes,
-                               m.input_pool_bytes, m.pinned_host_bytes, m.total_device_bytes};
+// A pipeline's vqm_ss2_memory as the reference's PipelineMemoryUsage: the
+// device memory is the arena (heap 0), the pinned staging and read-back
+// buffers the staging (heap 1).
+inline PipelineMemoryUsage Usage(const vqm_ss2_memory& m) {
+    PipelineMemoryUsage u;
+    u.arena = MemoryAllocation{m.total_device_bytes, m.total_device_bytes, kDeviceHeap, kDeviceHeap};
+    u.staging = MemoryAllocation{m.pinned_host_bytes, m.pinned_host_bytes, kPinnedHostHeap, kPinnedHostHeap};
+    u.uses_staging = m.streaming_bytes > 0;
+    return u;
 }
 
-// MaxWorkersForBudget (vqm_memory.h:110-124) for HBM, the one device heap
-// here: how many independent pipelines of `per_worker` fit `device_budget`
-// bytes. Arithmetic, not an allocation guarantee. Throws
-// std::invalid_argument for an empty requirement.
-[[nodiscard]] inline std::size_t MaxWorkersForBudget(const PipelineMemoryUsage& per_worker,
-                                                     std::uint64_t device_budget) {
-    if (per_worker.total_device_bytes == 0)
-        throw std::invalid_argument("MaxWorkersForBudget: the requirement holds no device bytes");
-    return static_cast<std::size_t>(device_budget / per_worker.total_device_bytes);
-}
+}  // namespace detail
 
 class Metrics;
 
@@ -442,11 +423,18 @@ public:
     }
     void Release(ImageBuffer buffer) { detail::Check(vqm_ss2_release_image(p_, buffer.slot)); }
 
+    // ssimulacra2.h:204-205: the allocations the pipeline holds.
     [[nodiscard]] PipelineMemoryUsage GetMemoryUsage() const {
         vqm_ss2_memory m{};
         detail::Check(vqm_ss2_memory_usage(p_, &m));
-        return PipelineMemoryUsage{m.workspace_bytes, m.resident_bytes, m.streaming_bytes,
-                                   m.input_pool_bytes, m.pinned_host_bytes, m.total_device_bytes};
+        return detail::Usage(m);
+    }
+    // The same by region (workspace, resident slots, streaming buffers,
+    // input pool, pinned host); no reference counterpart.
+    [[nodiscard]] vqm_ss2_memory GetMemoryRegions() const {
+        vqm_ss2_memory m{};
+        detail::Check(vqm_ss2_memory_usage(p_, &m));
+        return m;
     }
 
 private:
@@ -485,24 +473,71 @@ template <typename Frame>
     return pipeline.Compute(original, distorted, background).Score();
 }
 
-struct ImagePair {  // vqm_pool.h:15-18
+// vqm_pool.h:20-33: non-owning input pairs; both images must stay alive
+// and unchanged until ComputeMany/ScoreMany returns.
+struct ImagePair {
     const Image& original;
     const Image& distorted;
 };
+struct ImageViewPair {
+    const ImageView& original;
+    const ImageView& distorted;
+};
+// Replaces VulkanBufferPair: caller-owned device images.
+struct DeviceImagePair {
+    const DeviceImageView& original;
+    const DeviceImageView& distorted;
+};
 
-// vqm_pool.h:51-86 / pool.cc:24-116: N pipelines round-robin, one pair in
-// flight each; results in input order. After a failure the pending pairs
-// are drained and the error rethrown.
+// vqm_pool.h:35-86 / pool.cc:24-116: `workers` pipelines of one geometry
+// driven round-robin from the calling thread, one submission in flight
+// each, results in input order. Not thread-safe. If a pair fails, the
+// submissions already queued are drained before the error is rethrown; if
+// the drain itself fails, the pipelines are destroyed and every later call
+// throws std::logic_error (the pool must then be destroyed).
 class Ssimulacra2Pool {
 public:
-    Ssimulacra2Pool(Ssimulacra2Config config, std::size_t workers) {
-        if (workers == 0) throw std::invalid_argument("a pool needs at least one worker");
+    Ssimulacra2Pool(Ssimulacra2Config config, std::size_t workers) : workers_(workers) {
+        if (workers == 0) throw std::invalid_argument("pool needs at least one worker");
+        pipes_.reserve(workers);
         for (std::size_t i = 0; i < workers; ++i) pipes_.emplace_back(config);
     }
-    [[nodiscard]] std::size_t Workers() const noexcept { return pipes_.size(); }
+    [[nodiscard]] std::size_t Workers() const noexcept { return workers_; }
 
-    [[nodiscard]] std::vector<Ssimulacra2Result> ComputeMany(const std::vector<ImagePair>& pairs,
+    [[nodiscard]] std::vector<Ssimulacra2Result> ComputeMany(std::span<const ImagePair> pairs,
                                                              float background = 0.5f) {
+        return Run(pairs, background);
+    }
+    [[nodiscard]] std::vector<Ssimulacra2Result> ComputeMany(std::span<const ImageViewPair> pairs,
+                                                             float background = 0.5f) {
+        return Run(pairs, background);
+    }
+    [[nodiscard]] std::vector<Ssimulacra2Result> ComputeMany(std::span<const DeviceImagePair> pairs,
+                                                             float background = 0.5f) {
+        return Run(pairs, background);
+    }
+    [[nodiscard]] std::vector<double> ScoreMany(std::span<const ImagePair> pairs, float background = 0.5f) {
+        return Scores(ComputeMany(pairs, background));
+    }
+    [[nodiscard]] std::vector<double> ScoreMany(std::span<const ImageViewPair> pairs, float background = 0.5f) {
+        return Scores(ComputeMany(pairs, background));
+    }
+    [[nodiscard]] std::vector<double> ScoreMany(std::span<const DeviceImagePair> pairs,
+                                                float background = 0.5f) {
+        return Scores(ComputeMany(pairs, background));
+    }
+
+private:
+    static std::vector<double> Scores(const std::vector<Ssimulacra2Result>& results) {
+        std::vector<double> scores;
+        scores.reserve(results.size());
+        for (const auto& r : results) scores.push_back(r.Score());
+        return scores;
+    }
+
+    template <typename Pair>
+    std::vector<Ssimulacra2Result> Run(std::span<const Pair> pairs, float background) {
+        if (poisoned_) throw std::logic_error("pool cannot be reused after a failed drain");
         const std::size_t n = pipes_.size();
         std::vector<std::uint64_t> tokens(n, 0);
         std::vector<std::size_t> owner(n, 0);
@@ -510,35 +545,41 @@ public:
         try {
             for (std::size_t i = 0; i < pairs.size(); ++i) {
                 const std::size_t slot = i % n;
-                if (tokens[slot] != 0) {
-                    out[owner[slot]] = pipes_[slot].Finish(std::exchange(tokens[slot], 0));
-                }
+                if (tokens[slot] != 0) out[owner[slot]] = pipes_[slot].Finish(std::exchange(tokens[slot], 0));
                 tokens[slot] = pipes_[slot].Submit(pairs[i].original, pairs[i].distorted, background);
                 owner[slot] = i;
             }
             for (std::size_t slot = 0; slot < n; ++slot)
                 if (tokens[slot] != 0) out[owner[slot]] = pipes_[slot].Finish(std::exchange(tokens[slot], 0));
         } catch (...) {
-            for (std::size_t slot = 0; slot < n; ++slot) {
-                if (tokens[slot] == 0) continue;
-                try {
-                    (void)pipes_[slot].Finish(std::exchange(tokens[slot], 0));
-                } catch (...) {
-                }
-            }
+            DrainAfterFailure(tokens);
             throw;
         }
         return out;
     }
-    [[nodiscard]] std::vector<double> ScoreMany(const std::vector<ImagePair>& pairs,
-                                                float background = 0.5f) {
-        std::vector<double> scores;
-        for (const auto& r : ComputeMany(pairs, background)) scores.push_back(r.Score());
-        return scores;
+
+    // Finishes every queued submission; when one of those waits fails, the
+    // pipelines (whose destructors wait for their device work) are dropped
+    // and the pool is poisoned (pool.cc:94-112).
+    void DrainAfterFailure(std::vector<std::uint64_t>& tokens) noexcept {
+        bool failed = false;
+        for (std::size_t slot = 0; slot < tokens.size(); ++slot) {
+            if (tokens[slot] == 0) continue;
+            try {
+                (void)pipes_[slot].Finish(std::exchange(tokens[slot], 0));
+            } catch (...) {
+                failed = true;
+            }
+        }
+        if (failed) {
+            pipes_.clear();
+            poisoned_ = true;
+        }
     }
 
-private:
+    std::size_t workers_ = 0;
     std::vector<Ssimulacra2Pipeline> pipes_;
+    bool poisoned_ = false;
 };
 
 }  // namespace vqm

@@ -27,7 +27,7 @@
 // metrics. With one metric there is nothing to share, so `scratch` is always
 // empty and `spatial` holds the pipeline's own allocations, the same figure
 // GetMemoryRequirements plans. Budgets have two heaps here: heap 0 is device
-// memory (HBM), heap 1 pinned host memory.
+// memory (HBM), heap 1 pinned host memory (vqm_memory_types.h).
 #pragma once
 
 #include <algorithm>
@@ -44,6 +44,7 @@
 #include <vector>
 
 #include "ssimulacra2.h"
+#include "vqm_memory.h"
 
 namespace vqm {
 
@@ -55,6 +56,7 @@ struct MetricsConfig {
     ImageFormat original_format{};
     ImageFormat distorted_format{};
     bool streaming_inputs = true;
+    MemoryPolicy memory_policy = MemoryPolicy::Automatic;
     float background = 0.5f;
     std::optional<Ssimulacra2Config> ssimulacra2 = std::nullopt;
 };
@@ -73,8 +75,8 @@ struct MetricsMemoryUsage {
 
 // vqm_metrics.h:80-87.
 struct MetricsMemoryRequirements {
-    std::optional<PipelineMemoryUsage> spatial;
-    std::optional<PipelineMemoryUsage> scratch;
+    std::optional<MetricMemoryRequirements> spatial;
+    std::optional<MetricMemoryRequirements> scratch;
 };
 
 // vqm_metrics.h:147-152 (no CVVDP clip score here).
@@ -108,16 +110,10 @@ inline Ssimulacra2Config SelectSsimulacra2(const MetricsConfig& config) {
     Ssimulacra2Config metric = *config.ssimulacra2;
     metric.size = config.size;
     metric.streaming_inputs = config.streaming_inputs;
+    metric.memory_policy = config.memory_policy;
     metric.gpu_resident_images = 0;
     metric.input_bytes = 0;
     return metric;
-}
-
-// AddBytes (vulkan_memory_query.cc:233-238).
-inline void AddHeapBytes(std::uint64_t& total, std::uint64_t value) {
-    if (value > std::numeric_limits<std::uint64_t>::max() - total)
-        throw std::overflow_error("memory: heap allocation size overflow");
-    total += value;
 }
 
 }  // namespace detail
@@ -130,28 +126,17 @@ inline void AddHeapBytes(std::uint64_t& total, std::uint64_t value) {
     return r;
 }
 
-// MaxWorkersForBudgetWorstCase (metric_pipelines.cc:197-212,
-// vulkan_memory_query.cc:259-269,395-424): how many workers of these
-// requirements fit the per-heap budgets (heap 0 device memory, heap 1
-// pinned host memory). Arithmetic, not an allocation guarantee.
+// MaxWorkersForBudgetWorstCase (metric_pipelines.cc:197-212): the
+// selected metrics' requirements (and the shared scratch, never present
+// here) charged together per worker against the per-heap budgets (heap 0
+// device memory, heap 1 pinned host memory). Arithmetic, not an allocation
+// guarantee.
 [[nodiscard]] inline std::size_t MaxWorkersForBudgetWorstCase(const MetricsMemoryRequirements& requirements,
                                                               std::span<const std::uint64_t> heap_budgets) {
-    std::vector<const PipelineMemoryUsage*> parts;
-    if (requirements.scratch) parts.push_back(&*requirements.scratch);
-    if (requirements.spatial) parts.push_back(&*requirements.spatial);
-    if (parts.empty()) throw std::invalid_argument("memory: no requirements");
-    std::uint64_t charge[2] = {0, 0};
-    for (const PipelineMemoryUsage* p : parts) {
-        detail::AddHeapBytes(charge[0], p->total_device_bytes);
-        detail::AddHeapBytes(charge[1], p->pinned_host_bytes);
-    }
-    std::uint64_t workers = std::numeric_limits<std::uint64_t>::max();
-    for (std::size_t heap = 0; heap < 2; ++heap) {
-        if (charge[heap] == 0) continue;
-        if (heap >= heap_budgets.size()) throw std::invalid_argument("memory: missing heap budget");
-        workers = std::min(workers, heap_budgets[heap] / charge[heap]);
-    }
-    return static_cast<std::size_t>(std::min<std::uint64_t>(workers, std::numeric_limits<std::size_t>::max()));
+    std::vector<MetricMemoryRequirements> parts;
+    if (requirements.scratch) parts.push_back(*requirements.scratch);
+    if (requirements.spatial) parts.push_back(*requirements.spatial);
+    return MaxWorkersForBudgetWorstCase(std::span<const MetricMemoryRequirements>(parts), heap_budgets);
 }
 
 // vqm_metrics.h:97-119: owns the selected SSIMULACRA 2 pipeline, configured

@@ -6,7 +6,7 @@
  *
  * This is the operator boundary of the reference's backend seam:
  * internal::Ssimulacra2GpuPipeline (src/metrics/ssimulacra2/
- * ssimulacra2_internal.h:267-317), created by CreateVulkanSsimulacra2Pipeline
+ * ssimulacra2_internal.h:102-152), created by CreateVulkanSsimulacra2Pipeline
  * (src/metrics/ssimulacra2/vulkan.h:19-30, vulkan.cc:751-774). The C++ facade
  * in include/ssimulacra2.h rebuilds the reference's public API
  * (include/ssimulacra2.h:17-215) on top of it, mapping status codes to the
@@ -248,6 +248,20 @@ void vqm_ss2_srgb_table(float table[256]);
 
 /* Device bytes a pipeline with `config` allocates (layout.cc:106-325). */
 int vqm_ss2_memory_plan(const vqm_ss2_config* config, vqm_ss2_memory* plan);
+
+/* MemoryHeapInfo (include/vqm_memory.h:33-42) of the two heaps allocations
+ * come from: heap 0 device memory (HBM, device-local; budget = total, usage
+ * = total - free from hipMemGetInfo), heap 1 host memory for pinned buffers
+ * (budget = total, usage = total - available RAM). Without a usable device
+ * heap 0 reports zeros and no budget. Never fails. */
+typedef struct vqm_memory_heap {
+    uint64_t size_bytes;
+    uint32_t device_local;
+    uint32_t budget_available;
+    uint64_t budget_bytes;
+    uint64_t usage_bytes;
+} vqm_memory_heap;
+int vqm_memory_heaps(vqm_memory_heap heaps[2]);
 
 /* Algorithmic HBM bytes of one resident pair of this geometry: both u8
  * inputs read, every stored plane element written and read once. */

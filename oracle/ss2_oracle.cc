@@ -196,7 +196,14 @@ int cube_vs(Dyadic m, float x) {
 // mixes at 0, prepare.comp:81-98, and pow(0, 1/3) = 0).
 float cbrt_exact(float x) {
     if (!(x > 0.0f)) return 0.0f;
-    float c = static_cast<float>(std::cbrt(static_cast<double>(x)));
+    const double d = std::cbrt(static_cast<double>(x));
+    float c = static_cast<float>(d);
+    // Fast path: when libm's root (within an FP64 ulp of the true root) lies
+    // well away from the midpoint between c and its neighbour on d's side,
+    // the true root rounds to c as well.
+    const float side = d > static_cast<double>(c) ? std::nextafter(c, INFINITY) : std::nextafter(c, 0.0f);
+    const double half = 0.5 * std::fabs(static_cast<double>(side) - static_cast<double>(c));
+    if (std::fabs(std::fabs(d - static_cast<double>(c)) - half) > 0x1p-40 * d) return c;
     for (;;) {
         const float up = std::nextafter(c, INFINITY), down = std::nextafter(c, 0.0f);
         if (cube_vs(midpoint_of(c, up), x) < 0) {

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "ssimulacra2.h"
+#include "vqm_memory.h"
 #include "vqm_metrics.h"
 
 static int failures = 0;
@@ -168,14 +169,15 @@ static void ViewAndMetricsHostContract() {
     const vqm::MetricsMemoryRequirements req = vqm::GetMemoryRequirements(config);
     vqm::Ssimulacra2Config single{{64, 48}};
     single.gpu_resident_images = 0;
-    const vqm::PipelineMemoryUsage expected = vqm::GetMemoryRequirements(single);
+    const vqm::MetricMemoryRequirements expected = vqm::GetMemoryRequirements(single);
     CHECK(req.spatial.has_value() && !req.scratch.has_value());
-    CHECK(req.spatial->total_device_bytes == expected.total_device_bytes);
-    CHECK(req.spatial->resident_bytes == 0 && req.spatial->input_pool_bytes == 0);
-    CHECK(req.spatial->pinned_host_bytes == expected.pinned_host_bytes);
+    CHECK(req.spatial->direct.supported && !req.spatial->staged.supported);
+    CHECK(req.spatial->direct.arena.allocation_bytes == expected.direct.arena.allocation_bytes);
+    CHECK(req.spatial->direct.staging.allocation_bytes == expected.direct.staging.allocation_bytes);
 
     // metrics_test.cc:187-240: per-heap worker counts.
-    const std::uint64_t dev = req.spatial->total_device_bytes, host = req.spatial->pinned_host_bytes;
+    const std::uint64_t dev = req.spatial->direct.arena.allocation_bytes;
+    const std::uint64_t host = req.spatial->direct.staging.allocation_bytes;
     const std::array<std::uint64_t, 2> roomy{5 * dev + 1, 9 * host};
     CHECK(vqm::MaxWorkersForBudgetWorstCase(req, roomy) == 5);
     const std::array<std::uint64_t, 2> host_bound{5 * dev, 2 * host + host / 2};
@@ -185,9 +187,58 @@ static void ViewAndMetricsHostContract() {
     CHECK_THROWS_AS(vqm::MaxWorkersForBudgetWorstCase(vqm::MetricsMemoryRequirements{}, roomy),
                     std::invalid_argument);
     vqm::MetricsMemoryRequirements huge_req = req;
-    huge_req.scratch = vqm::PipelineMemoryUsage{};
-    huge_req.scratch->total_device_bytes = std::numeric_limits<std::uint64_t>::max();
+    huge_req.scratch = vqm::MetricMemoryRequirements{};
+    huge_req.scratch->direct.supported = true;
+    huge_req.scratch->direct.arena = {1, std::numeric_limits<std::uint64_t>::max(), 0, 0};
     CHECK_THROWS_AS(vqm::MaxWorkersForBudgetWorstCase(huge_req, roomy), std::overflow_error);
+    vqm::MetricsConfig staged = Srgb8Metrics({64, 48});
+    staged.memory_policy = vqm::MemoryPolicy::Staged;  // reaches the metric
+    CHECK(!vqm::GetMemoryRequirements(staged).spatial->direct.supported);
+    CHECK_THROWS_AS(vqm::MaxWorkersForBudgetWorstCase(vqm::GetMemoryRequirements(staged), roomy),
+                    std::invalid_argument);
+}
+
+// Caller code written for the reference's headers, unchanged: the config's
+// designated and positional initialisers in the reference's field order
+// (ssimulacra2.h:35-64), the memory planning types (vqm_memory.h:26-135,
+// vqm_memory_types.h:14-42), the pool's span overloads (vqm_pool.h:65-80)
+// and the Image and view format defaults (vqm_image.h:134-141).
+static void ReferenceCallerCode() {
+    const vqm::Ssimulacra2Config designated{.size = {1920, 1080},
+                                            .gpu_resident_images = 2,
+                                            .half_planes = false,
+                                            .chroma_filter = vqm::ChromaFilter::Bilinear,
+                                            .input_bytes = 0,
+                                            .streaming_inputs = true,
+                                            .memory_policy = vqm::MemoryPolicy::Automatic};
+    const vqm::Ssimulacra2Config positional{{1920, 1080}, 2, false, vqm::ChromaFilter::Bicubic, 0, true,
+                                            vqm::MemoryPolicy::Direct};
+    const vqm::MetricMemoryRequirements plan = vqm::GetMemoryRequirements(designated);
+    CHECK(plan.direct.supported && !plan.staged.supported);
+    CHECK(plan.direct.arena.heap == vqm::kDeviceHeap && plan.direct.staging.heap == vqm::kPinnedHostHeap);
+    CHECK(plan.direct.arena.buffer_bytes == plan.direct.arena.allocation_bytes);
+    CHECK(vqm::GetMemoryRequirements(positional).direct.arena.allocation_bytes ==
+          plan.direct.arena.allocation_bytes);
+    CHECK(plan.heaps.size() == 2 && plan.heaps[0].device_local && !plan.heaps[1].device_local);
+    const std::vector<std::uint64_t> budgets{3 * plan.direct.arena.allocation_bytes + 7,
+                                             10 * plan.direct.staging.allocation_bytes};
+    CHECK(vqm::MaxWorkersForBudget(plan.direct, budgets) == 3);
+    const std::array<vqm::MemoryScenario, 2> two{plan.direct, plan.direct};
+    CHECK(vqm::MaxWorkersForBudget(std::span<const vqm::MemoryScenario>(two), budgets) == 1);
+    const std::array<vqm::MetricMemoryRequirements, 1> worst{plan};
+    CHECK(vqm::MaxWorkersForBudgetWorstCase(std::span<const vqm::MetricMemoryRequirements>(worst), budgets) == 3);
+    CHECK_THROWS_AS(vqm::MaxWorkersForBudget(plan.staged, budgets), std::invalid_argument);
+    const std::array<std::uint64_t, 1> one_heap{budgets[0]};
+    CHECK_THROWS_AS(vqm::MaxWorkersForBudget(plan.direct, one_heap), std::invalid_argument);
+    if (const auto available = vqm::AvailableHeapBudgets(plan)) CHECK(available->size() == 2);
+    vqm::Ssimulacra2Config strict = designated;
+    strict.memory_policy = vqm::MemoryPolicy::Staged;
+    CHECK(!vqm::GetMemoryRequirements(strict).direct.supported);
+    CHECK_THROWS_AS(vqm::Ssimulacra2Pipeline{strict}, std::runtime_error);
+    const vqm::Image image;
+    const vqm::DeviceImageView view;
+    CHECK(image.format == vqm::ImageFormat{} && image.format.type == vqm::PixelType::Float);
+    CHECK(view.format == vqm::ImageFormat{});
 }
 
 static void HostContract() {
@@ -218,13 +269,17 @@ static void HostContract() {
     CHECK_THROWS_AS(r.Score(), std::invalid_argument);
     CHECK(vqm::RequiredBytes(vqm::ImageFormat::Srgb8(), {13, 9}) == 360);
     CHECK(vqm::RequiredBytes(Yuv420(), {8, 4}) == 48);  // input_format_test.cc:122-177
-    // vqm_memory.h:63-124.
-    const vqm::PipelineMemoryUsage plan = vqm::GetMemoryRequirements(vqm::Ssimulacra2Config{{1920, 1080}});
-    CHECK(plan.resident_bytes == 2ull * 3 * 1920 * 1080);
-    CHECK(vqm::MaxWorkersForBudget(plan, 3 * plan.total_device_bytes) == 3);
-    CHECK_THROWS_AS(vqm::MaxWorkersForBudget(vqm::PipelineMemoryUsage{}, 1), std::invalid_argument);
+    // vqm_memory.h:62-124.
+    vqm::Ssimulacra2Config no_slots{{1920, 1080}};
+    no_slots.gpu_resident_images = 0;
+    const std::uint64_t with = vqm::GetMemoryRequirements(vqm::Ssimulacra2Config{{1920, 1080}}).direct.arena.allocation_bytes;
+    const std::uint64_t without = vqm::GetMemoryRequirements(no_slots).direct.arena.allocation_bytes;
+    CHECK(with - without == 2ull * 3 * 1920 * 1080);  // two resident 8-bit RGB slots
+    CHECK_THROWS_AS(vqm::MaxWorkersForBudget(vqm::MemoryScenario{}, std::vector<std::uint64_t>{1, 1}),
+                    std::invalid_argument);
     CHECK_THROWS_AS(vqm::GetMemoryRequirements(vqm::Ssimulacra2Config{{4, 8}}), std::invalid_argument);
     ViewAndMetricsHostContract();
+    ReferenceCallerCode();
 }
 
 // tests/core/metrics_test.cc:241-575, the SSIMULACRA 2 parts: every input
@@ -270,8 +325,8 @@ static void MetricsDeviceContract() {
     CHECK(again.frame_index == 0 && *again.ssimulacra2 == want);
     const auto usage = metrics.GetMemoryUsage();
     CHECK(usage.spatial && !usage.scratch);
-    CHECK(usage.spatial->total_device_bytes ==
-          vqm::GetMemoryRequirements(Srgb8Metrics({32, 32})).spatial->total_device_bytes);
+    CHECK(usage.spatial->arena.allocation_bytes ==
+          vqm::GetMemoryRequirements(Srgb8Metrics({32, 32})).spatial->direct.arena.allocation_bytes);
 
     // Device inputs, and streaming disabled: pixel overloads are refused
     // before any work, device views still score.
@@ -348,10 +403,20 @@ static void DeviceContract() {
     CHECK_THROWS_AS(pipe.Upload(orig), std::invalid_argument);  // both slots taken
     // Pool ordering (pool_test.cc:27-62).
     vqm::Ssimulacra2Pool pool(vqm::Ssimulacra2Config{{32, 32}}, 2);
-    const auto scores = pool.ScoreMany({{orig, dist_b}, {orig, dist_c}, {orig, orig}});
+    const std::vector<vqm::ImagePair> pairs{{orig, dist_b}, {orig, dist_c}, {orig, orig}};
+    const auto scores = pool.ScoreMany(pairs);
     CHECK(scores.size() == 3 && scores[0] == b && scores[1] == c && scores[2] == 100.0);
+    std::vector<std::vector<std::byte>> store;
+    const vqm::ImageView vo = PaddedView(orig, 3, store), vb = PaddedView(dist_b, 0, store);
+    const std::array<vqm::ImageViewPair, 2> view_pairs{{{vo, vb}, {vo, vo}}};
+    const auto view_scores = pool.ScoreMany(view_pairs);
+    CHECK(view_scores[0] == b && view_scores[1] == 100.0);
+    CHECK(pool.Workers() == 2);
+    // ssimulacra2.h:204-205 as reference callers read it.
     const auto usage = pipe.GetMemoryUsage();
-    CHECK(usage.resident_bytes == 2 * 3 * 1024);
+    CHECK(usage.arena.buffer_bytes > 0 && usage.arena.heap == vqm::kDeviceHeap && usage.uses_staging);
+    CHECK(usage.staging.allocation_bytes > 0 && usage.auxiliary.empty());
+    CHECK(pipe.GetMemoryRegions().resident_bytes == 2 * 3 * 1024);
     // A general format with a non-default chroma filter (input_format_chroma_test.cc).
     vqm::Ssimulacra2Config lanczos{{32, 32}};
     lanczos.chroma_filter = vqm::ChromaFilter::Lanczos;

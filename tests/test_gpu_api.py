@@ -43,7 +43,7 @@ def test_every_input_kind_agrees():
     da, db = DeviceBuffer(3 * 1024), DeviceBuffer(3 * 1024)
     da.upload(orig)
     db.upload(dist)
-    assert same(p.Compute(DeviceImage(da.ptr, S), DeviceImage(db.ptr, S)), streamed)
+    assert same(p.Compute(DeviceImage(da.ptr, S, ImageFormat.srgb8()), DeviceImage(db.ptr, S, ImageFormat.srgb8())), streamed)
     assert streamed.Score() == pytest.approx(93.17505696, abs=5e-3)
 
 
@@ -169,9 +169,10 @@ def test_memory_usage_matches_plan():
     cfg = Ssimulacra2Config(size=Size(640, 360))
     p = Ssimulacra2Pipeline(cfg)
     used = p.GetMemoryUsage()
-    plan = vqm_amd.memory_plan(cfg)
-    assert used.total_device_bytes == plan.total_device_bytes
-    assert used.resident_bytes == 2 * 3 * 640 * 360
+    plan = vqm_amd.GetMemoryRequirements(cfg)
+    assert used.arena == plan.direct.arena and used.staging == plan.direct.staging
+    assert used.uses_staging and used.auxiliary == []
+    assert p.GetMemoryRegions().resident_bytes == 2 * 3 * 640 * 360
 
 
 def test_pool_matches_single_pipeline():

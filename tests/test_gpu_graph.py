@@ -37,7 +37,7 @@ def test_graph_replays_equal_eager_launches(w, h, fmt):
         da.upload(a)
         db.upload(b)
         bufs += [da, db]
-        got = graph.Compute(DeviceImage(da.ptr, size), DeviceImage(db.ptr, size)).norms()
+        got = graph.Compute(DeviceImage(da.ptr, size, ImageFormat.srgb8()), DeviceImage(db.ptr, size, ImageFormat.srgb8())).norms()
         assert np.array_equal(got, want), f"device pair {i}"
         # Resident slots.
         ua, ub = graph.Upload(img(a)), graph.Upload(img(b))

@@ -109,7 +109,8 @@ def test_metrics_sequence_and_input_forms():
     assert still.ssimulacra2 == again.ssimulacra2 == want
     usage = m.GetMemoryUsage()
     assert usage.scratch is None
-    assert usage.spatial == vqm_amd.metrics.GetMemoryRequirements(config(size)).spatial
+    plan = vqm_amd.metrics.GetMemoryRequirements(config(size)).spatial
+    assert usage.spatial.arena == plan.direct.arena and usage.spatial.staging == plan.direct.staging
     m.close()
 
 
@@ -121,17 +122,17 @@ def test_metrics_device_inputs_without_streaming():
     with Ssimulacra2Pipeline(Ssimulacra2Config(size=size)) as p:
         want = p.Compute(Image(size, ImageFormat.srgb8(), orig), Image(size, ImageFormat.srgb8(), dist)).Score()
     m = vqm_amd.Metrics(config(size, streaming_inputs=False))
-    assert m.GetMemoryUsage().spatial.streaming_bytes == 0
+    assert not m.GetMemoryUsage().spatial.uses_staging
     with pytest.raises(InvalidArgument, match="streaming inputs are disabled"):
         m.Compute(orig, dist)
     bufs = [DeviceBuffer(orig.nbytes), DeviceBuffer(dist.nbytes)]
     bufs[0].upload(orig)
     bufs[1].upload(dist)
-    da, db = DeviceImage(bufs[0].ptr, size), DeviceImage(bufs[1].ptr, size)
+    da, db = DeviceImage(bufs[0].ptr, size, ImageFormat.srgb8()), DeviceImage(bufs[1].ptr, size, ImageFormat.srgb8())
     r = m.Compute(da, db)
     assert r.frame_index == 0 and r.ssimulacra2 == want
     with pytest.raises(InvalidArgument, match="invalid device input"):
-        m.Compute(da, DeviceImage(0, size))
+        m.Compute(da, DeviceImage(0, size, ImageFormat.srgb8()))
     assert m.Compute(da, db).frame_index == 1
     m.close()
     for buf in bufs:
@@ -156,5 +157,5 @@ def test_metrics_background_and_pipelines():
     m.close()
     pipes = vqm_amd.MetricPipelines(config(size, fmt, background=0.8))
     assert pipes.Ssimulacra2().Compute(ia, ib, 0.8).Score() == at08
-    assert pipes.GetMemoryUsage().spatial.resident_bytes == 0
+    assert pipes.Ssimulacra2().GetMemoryRegions().resident_bytes == 0
     pipes.close()

@@ -145,12 +145,30 @@ def test_memory_plan_and_algorithmic_bytes():
     assert m.total_device_bytes == m.workspace_bytes + m.resident_bytes + m.streaming_bytes
     half = vqm_amd.memory_plan(Ssimulacra2Config(size=Size(1920, 1080), half_planes=True))
     assert half.workspace_bytes < 0.6 * m.workspace_bytes
-    # vqm_memory.h:63-124 under the reference's names.
-    assert vqm_amd.GetMemoryRequirements(Ssimulacra2Config(size=Size(1920, 1080))) == m
-    assert vqm_amd.MaxWorkersForBudget(m, 10 * m.total_device_bytes + 1) == 10
-    assert vqm_amd.MaxWorkersForBudget(m, m.total_device_bytes - 1) == 0
-    with pytest.raises(vqm_amd.InvalidArgument):
-        vqm_amd.MaxWorkersForBudget(vqm_amd.PipelineMemoryUsage(), 1 << 30)
+    # vqm_memory.h:26-135 under the reference's names and shapes: the device
+    # bytes are the direct scenario's arena (heap 0), the pinned host bytes
+    # its staging (heap 1); there is no staged scenario.
+    req = vqm_amd.GetMemoryRequirements(Ssimulacra2Config(size=Size(1920, 1080)))
+    assert req.direct.supported and not req.staged.supported
+    assert req.direct.arena.allocation_bytes == m.total_device_bytes and req.direct.arena.heap == 0
+    assert req.direct.staging.allocation_bytes == m.pinned_host_bytes and req.direct.staging.heap == 1
+    assert len(req.heaps) == 2 and req.heaps[0].device_local and not req.heaps[1].device_local
+    dev, host = m.total_device_bytes, m.pinned_host_bytes
+    assert vqm_amd.MaxWorkersForBudget(req.direct, [10 * dev + 1, 100 * host]) == 10
+    assert vqm_amd.MaxWorkersForBudget(req.direct, [dev - 1, 100 * host]) == 0
+    assert vqm_amd.MaxWorkersForBudget([req.direct, req.direct], [10 * dev, 100 * host]) == 5
+    with pytest.raises(vqm_amd.InvalidArgument, match="missing heap budget"):
+        vqm_amd.MaxWorkersForBudget(req.direct, [10 * dev])
+    with pytest.raises(vqm_amd.InvalidArgument, match="unsupported"):
+        vqm_amd.MaxWorkersForBudget(req.staged, [1 << 40, 1 << 40])
+    assert vqm_amd.MaxWorkersForBudgetWorstCase([req], [3 * dev, 100 * host]) == 3
+    staged = vqm_amd.GetMemoryRequirements(Ssimulacra2Config(size=Size(1920, 1080),
+                                                             memory_policy=vqm_amd.MemoryPolicy.Staged))
+    assert not staged.direct.supported
+    with pytest.raises(vqm_amd.InvalidArgument, match="unsupported requirements"):
+        vqm_amd.MaxWorkersForBudgetWorstCase([staged], [1 << 40, 1 << 40])
+    with pytest.raises(vqm_amd.DeviceError):
+        vqm_amd.Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(64, 64), memory_policy=vqm_amd.MemoryPolicy.Staged))
     with pytest.raises(vqm_amd.InvalidArgument):
         vqm_amd.GetMemoryRequirements(Ssimulacra2Config(size=Size(4, 8)))
 
@@ -179,3 +197,50 @@ def test_server_config_validation_before_device_work():
     assert (cfg.plane_format, cfg.lanes, cfg.max_batch, cfg.strip_rows, cfg.hbm_budget) == (0, 3, 64, 0, 0)
     t0 = vqm_amd.now_ns()
     assert vqm_amd.now_ns() >= t0 > 0
+
+
+class _FakePipeline:
+    """Stands in for a pipeline: Submit fails for pair `bad`, Finish fails
+    when `finish_fails`."""
+
+    def __init__(self, bad=None, finish_fails=False):
+        self.bad, self.finish_fails, self.pending, self.closed = bad, finish_fails, None, False
+
+    def Submit(self, a, b, background=0.5):
+        if a == self.bad:
+            raise vqm_amd.InvalidArgument("pair refused")
+        self.pending = a
+        return a + 1
+
+    def Finish(self, token):
+        self.pending = None
+        if self.finish_fails:
+            raise vqm_amd.DeviceError("wait failed")
+        return token
+
+    def close(self):
+        self.closed = True
+
+
+def _fake_pool(pipes):
+    pool = vqm_amd.Ssimulacra2Pool.__new__(vqm_amd.Ssimulacra2Pool)
+    pool._pipes, pool._workers, pool._poisoned = pipes, len(pipes), False
+    return pool
+
+
+def test_pool_drains_and_poisons_like_the_reference():
+    """pool.cc:47-49,94-112: a failing pair drains the queued submissions
+    and rethrows; when a drain wait fails too, the pipelines are dropped and
+    every later call raises LogicError."""
+    ok = _fake_pool([_FakePipeline(bad=2), _FakePipeline()])
+    with pytest.raises(vqm_amd.InvalidArgument, match="pair refused"):
+        ok.ComputeMany([(0, 0), (1, 1), (2, 2), (3, 3)])
+    assert all(p.pending is None for p in ok._pipes)  # drained
+    assert ok.ComputeMany([(4, 4), (5, 5)]) == [5, 6]  # still usable
+    bad = _fake_pool([_FakePipeline(bad=2), _FakePipeline(finish_fails=True)])
+    pipes = list(bad._pipes)
+    with pytest.raises(vqm_amd.InvalidArgument, match="pair refused"):
+        bad.ComputeMany([(0, 0), (1, 1), (2, 2)])
+    assert all(p.closed for p in pipes)
+    with pytest.raises(vqm_amd.LogicError, match="failed drain"):
+        bad.ComputeMany([(0, 0)])

@@ -149,15 +149,20 @@ def test_metrics_memory_plan_overrides_the_metric():
     assert req.scratch is None
     want = vqm_amd.GetMemoryRequirements(Ssimulacra2Config(size=Size(64, 48), gpu_resident_images=0,
                                                            streaming_inputs=False))
-    assert req.spatial == want
-    assert want.resident_bytes == 0 and want.input_pool_bytes == 0 and want.streaming_bytes == 0
+    assert req.spatial.direct == want.direct
+    regions = vqm_amd.memory_plan(Ssimulacra2Config(size=Size(64, 48), gpu_resident_images=0,
+                                                    streaming_inputs=False))
+    assert regions.resident_bytes == 0 and regions.input_pool_bytes == 0 and regions.streaming_bytes == 0
+    # memory_policy is the top-level one (vqm_metrics.h:40).
+    staged = metrics_config(memory_policy=vqm_amd.MemoryPolicy.Staged)
+    assert not vqm_amd.metrics.GetMemoryRequirements(staged).spatial.direct.supported
 
 
 def test_worst_case_worker_budget():
     """vulkan_memory_query.cc:259-269,395-424: per-heap sums, the tightest
     heap decides, a charged heap needs a budget, sums must not overflow."""
     req = vqm_amd.metrics.GetMemoryRequirements(metrics_config())
-    dev, host = req.spatial.total_device_bytes, req.spatial.pinned_host_bytes
+    dev, host = req.spatial.direct.arena.allocation_bytes, req.spatial.direct.staging.allocation_bytes
     assert dev > 0 and host > 0
     W = vqm_amd.MaxWorkersForBudgetWorstCase
     assert W(req, [4 * dev, 100 * host]) == 4
@@ -169,7 +174,8 @@ def test_worst_case_worker_budget():
         W(MetricsMemoryRequirements(), [1, 1])
     both = MetricsMemoryRequirements(spatial=req.spatial, scratch=req.spatial)
     assert W(both, [4 * dev, 100 * host]) == 2
+    huge_scenario = vqm_amd.MemoryScenario(True, vqm_amd.MemoryAllocation(1, (1 << 64) - 1, 0, 0))
     huge = MetricsMemoryRequirements(spatial=req.spatial,
-                                     scratch=PipelineMemoryUsage(total_device_bytes=(1 << 64) - 1))
+                                     scratch=vqm_amd.MetricMemoryRequirements(direct=huge_scenario))
     with pytest.raises(OverflowError):
         W(huge, [1, 1])

@@ -64,6 +64,11 @@ class MemoryC(ctypes.Structure):
                 ("total_device_bytes", u64)]
 
 
+class MemoryHeapC(ctypes.Structure):
+    _fields_ = [("size_bytes", u64), ("device_local", u32), ("budget_available", u32),
+                ("budget_bytes", u64), ("usage_bytes", u64)]
+
+
 class BatchConfigC(ctypes.Structure):
     _fields_ = [("plane_format", u32), ("strip_rows", u32), ("max_pairs", u32),
                 ("chunks", u32), ("hbm_budget", u64), ("autotune", u32), ("blur", u32),
@@ -150,6 +155,7 @@ SIGNATURES = {
     "vqm_ss2_batch_finish": (ctypes.c_int, [vp, u64, ctypes.POINTER(u32), ctypes.POINTER(f64)]),
     "vqm_ss2_batch_stage_times": (ctypes.c_int, [vp, ctypes.POINTER(f32)]),
     "vqm_ss2_batch_pool_bytes": (ctypes.c_int, [vp, ctypes.POINTER(u64)]),
+    "vqm_memory_heaps": (ctypes.c_int, [ctypes.POINTER(MemoryHeapC)]),
     "vqm_ss2_autotune": (ctypes.c_int, [u32, u32, u32, u32, u32, ctypes.POINTER(TuneResultC)]),
     "vqm_ss2_server_config_init": (None, [ctypes.POINTER(ServerConfigC)]),
     "vqm_ss2_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfigC), ctypes.POINTER(vp)]),

@@ -1,6 +1,10 @@
 // C ABI of libvqm_hip.so (include/vqm_ss2.h): status codes, thread-local
 // error messages, and the opaque handles over the C++ runtime.
 
+#include <sys/sysinfo.h>
+
+#include <algorithm>
+
 #include <cstring>
 #include <exception>
 #include <new>
@@ -73,6 +77,36 @@ void vqm_ss2_config_init(vqm_ss2_config* config, uint32_t width, uint32_t height
     config->plane_format = VQM_PLANES_F32;
     config->streaming_inputs = 1;
     config->graph = 0;
+}
+
+int vqm_memory_heaps(vqm_memory_heap heaps[2]) {
+    if (heaps == nullptr) return SetError(VQM_INVALID_ARGUMENT, "heaps is null");
+    std::memset(heaps, 0, 2 * sizeof(vqm_memory_heap));
+    int count = 0;
+    size_t free_bytes = 0, total_bytes = 0;
+    if (hipGetDeviceCount(&count) == hipSuccess && count > 0 &&
+        hipMemGetInfo(&free_bytes, &total_bytes) == hipSuccess) {
+        heaps[0].size_bytes = total_bytes;
+        heaps[0].device_local = 1;
+        heaps[0].budget_available = 1;
+        heaps[0].budget_bytes = total_bytes;
+        heaps[0].usage_bytes = total_bytes - std::min(free_bytes, total_bytes);
+    } else {
+        (void)hipGetLastError();
+        heaps[0].device_local = 1;
+    }
+    struct sysinfo si {};
+    if (sysinfo(&si) == 0) {
+        const uint64_t unit = si.mem_unit ? si.mem_unit : 1;
+        const uint64_t total = static_cast<uint64_t>(si.totalram) * unit;
+        const uint64_t avail = static_cast<uint64_t>(si.freeram + si.bufferram) * unit;
+        heaps[1].size_bytes = total;
+        heaps[1].budget_available = 1;
+        heaps[1].budget_bytes = total;
+        heaps[1].usage_bytes = total - std::min(avail, total);
+    }
+    g_last_error.clear();
+    return VQM_OK;
 }
 
 void vqm_image_format_srgb8(vqm_image_format* f) {

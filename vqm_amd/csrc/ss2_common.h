@@ -145,6 +145,24 @@ struct SourceDesc {
     float cl[4];              // CL op constants nb, pb, nr, pr
 };
 
+// Scale work item i of a plan: the (level, channel) ranges follow one
+// another in level-major order, and within a range the items run row-major
+// over strip_h-row bands of 128-column strips. constexpr, so host code and
+// device code share it.
+constexpr ScaleItem ItemAt(const Plan& p, uint32_t i) {
+    for (uint32_t l = 0; l < p.levels; ++l) {
+        for (uint32_t c = 0; c < static_cast<uint32_t>(kChannels); ++c) {
+            const uint32_t begin = p.item_begin[l][c], count = p.item_count[l][c];
+            if (i >= begin && i - begin < count) {
+                const uint32_t k = i - begin;
+                const uint32_t strips = (p.w[l] + kStripCols - 1) / kStripCols;
+                return ScaleItem{l, c, (k % strips) * kStripCols, (k / strips) * p.strip_h};
+            }
+        }
+    }
+    return ScaleItem{0, 0, 0, 0};
+}
+
 // One pair as the kernels see it.
 struct PairDesc {
     const uint8_t* src[2];     // packed planar images, original then distorted

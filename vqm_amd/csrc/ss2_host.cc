@@ -222,21 +222,19 @@ void PackView(const vqm_host_view& view, uint8_t* dst) {
     }
 }
 
-GeometryPlan BuildPlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows) {
+Plan FillPlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows) {
     const ScalePlan scales = PlanScales(width, height);
     if (strip_rows == 0) strip_rows = 64;
     if (strip_rows % 4 != 0 || strip_rows > 1024)
         Fail(VQM_INVALID_ARGUMENT, "strip_rows must be a multiple of 4 up to 1024");
     if (width > (1u << 20) || height > (1u << 20))
         Fail(VQM_INVALID_ARGUMENT, "image dimensions above 1048576 are not supported");
-    GeometryPlan g;
-    Plan& p = g.plan;
+    Plan p{};
     p.width = width;
     p.height = height;
     p.levels = scales.count;
     p.strip_h = strip_rows;
     p.plane_stride = static_cast<uint32_t>(Align(static_cast<uint64_t>(width) * height, 4));
-    g.format = format;
     uint64_t cursor = 0;  // bytes
     for (uint32_t l = 0; l < static_cast<uint32_t>(kMaxScales); ++l) {
         for (int c = 0; c < kChannels; ++c) p.plane_off[l][c] = ~0ull;
@@ -256,23 +254,34 @@ GeometryPlan BuildPlan(uint32_t width, uint32_t height, PlaneFormat format, uint
         }
     }
     p.image_bytes = cursor;
-    g.plane_bytes_per_image = cursor;
     // Scale work: per level and active channel, bands of strip_rows rows by
-    // 128-column strips, row-major so that consecutive items are neighbours.
+    // 128-column strips, row-major so that consecutive items are neighbours
+    // (ItemAt enumerates them).
+    uint32_t items = 0;
     for (uint32_t l = 0; l < scales.count; ++l) {
         for (uint32_t c = 0; c < static_cast<uint32_t>(kChannels); ++c) {
-            p.item_begin[l][c] = static_cast<uint32_t>(g.items.size());
+            p.item_begin[l][c] = items;
             if (!(p.mask[l] & (1u << c))) continue;
-            for (uint32_t y0 = 0; y0 < p.h[l]; y0 += strip_rows)
-                for (uint32_t x0 = 0; x0 < p.w[l]; x0 += kStripCols)
-                    g.items.push_back(ScaleItem{l, c, x0, y0});
-            p.item_count[l][c] = static_cast<uint32_t>(g.items.size()) - p.item_begin[l][c];
+            const uint32_t strips = (p.w[l] + kStripCols - 1) / kStripCols;
+            const uint32_t bands = (p.h[l] + strip_rows - 1) / strip_rows;
+            p.item_count[l][c] = strips * bands;
+            items += strips * bands;
         }
     }
-    p.items = static_cast<uint32_t>(g.items.size());
+    p.items = items;
     p.prep_tiles_x = (width + kPrepTileW - 1) / kPrepTileW;
     p.prep_tiles_y = (height + kPrepTileH - 1) / kPrepTileH;
-    g.prep_tiles = p.prep_tiles_x * p.prep_tiles_y;
+    return p;
+}
+
+GeometryPlan BuildPlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows) {
+    GeometryPlan g;
+    g.plan = FillPlan(width, height, format, strip_rows);
+    g.format = format;
+    g.plane_bytes_per_image = g.plan.image_bytes;
+    g.prep_tiles = g.plan.prep_tiles_x * g.plan.prep_tiles_y;
+    g.items.resize(g.plan.items);
+    for (uint32_t i = 0; i < g.plan.items; ++i) g.items[i] = ItemAt(g.plan, i);
     return g;
 }
 

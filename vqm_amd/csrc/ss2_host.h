@@ -80,6 +80,10 @@ struct GeometryPlan {
 GeometryPlan BuildPlan(uint32_t width, uint32_t height, PlaneFormat format,
                        uint32_t strip_rows);
 
+// The plan alone, without the item table (ItemAt enumerates the items):
+// nothing is allocated.
+Plan FillPlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows);
+
 // Bytes of one pair's workspace: planes of both images, partials, norms,
 // each rounded to 256 bytes.
 struct WorkspaceLayout {

@@ -541,6 +541,10 @@ void Pipeline::BuildGraph() {
 }
 
 void Pipeline::RequireIdle() const {
+    if (broken_)
+        Fail(VQM_RUNTIME_ERROR,
+             "a submission of this pipeline failed on the device; its buffers are in an unknown state, "
+             "so the pipeline must be destroyed");
     if (in_flight_ != 0)
         Fail(VQM_LOGIC_ERROR, "a pair is in flight on this pipeline: call Finish first");
 }
@@ -774,8 +778,14 @@ uint64_t Pipeline::SubmitBuffers(uint64_t a, uint64_t b, float background) {
 uint32_t Pipeline::Finish(uint64_t token, double* norms) {
     if (token == 0 || token != in_flight_)
         Fail(VQM_INVALID_ARGUMENT, "unknown or already finished submission token");
+    const hipError_t waited = hipEventSynchronize(ev_[4].get());
     in_flight_ = 0;
-    Check(hipEventSynchronize(ev_[4].get()), "wait for pair");
+    if (waited != hipSuccess) {
+        // The pair's work may still touch the staging and the workspace:
+        // no later submission may reuse them.
+        broken_ = true;
+        Check(waited, "wait for pair");
+    }
     if (graph_launched_)
         stage_ms_ = {Elapsed(ev_[0], ev_[4]), 0.0f, 0.0f};
     else

@@ -1,6 +1,6 @@
 // HIP runtime of the SSIMULACRA 2.1 path: device context, device/pinned
 // buffers, the single-geometry pipeline (the backend behind
-// internal::Ssimulacra2GpuPipeline, ssimulacra2_internal.h:267-317) and the
+// internal::Ssimulacra2GpuPipeline, ssimulacra2_internal.h:102-152) and the
 // multi-geometry batch engine.
 #pragma once
 
@@ -344,6 +344,7 @@ private:
     Stream stream_;
     Event ev_[5];
     uint64_t in_flight_ = 0;
+    bool broken_ = false;  // a wait failed: every later call is refused
     std::array<float, 3> stage_ms_{};
     // Resident reference cache (vulkan.cc:238-241,716-735): the handle of
     // the resident original whose planes the workspace holds, or 0.

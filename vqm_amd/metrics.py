@@ -29,11 +29,11 @@ from typing import Optional, Sequence
 import numpy as np
 
 from .ssimulacra2 import (DeviceImage, Image, ImageFormat, ImageView, InvalidArgument, LogicError,
-                          PipelineMemoryUsage, Size, Ssimulacra2Config, Ssimulacra2Pipeline,
-                          GetMemoryRequirements as _PipelineRequirements, required_bytes,
+                          MemoryPolicy, MetricMemoryRequirements, PipelineMemoryUsage, Size,
+                          Ssimulacra2Config, Ssimulacra2Pipeline,
+                          GetMemoryRequirements as _PipelineRequirements,
+                          MaxWorkersForBudgetWorstCase as _WorstCase, required_bytes,
                           validate_image_format, view_bytes)
-
-_U64_MAX = (1 << 64) - 1
 
 
 @dataclass
@@ -43,6 +43,7 @@ class MetricsConfig:
     original_format: ImageFormat = field(default_factory=ImageFormat)
     distorted_format: ImageFormat = field(default_factory=ImageFormat)
     streaming_inputs: bool = True
+    memory_policy: int = MemoryPolicy.Automatic
     background: float = 0.5
     ssimulacra2: Optional[Ssimulacra2Config] = None
 
@@ -64,8 +65,8 @@ class MetricsMemoryUsage:
 @dataclass
 class MetricsMemoryRequirements:
     """vqm_metrics.h:80-87."""
-    spatial: Optional[PipelineMemoryUsage] = None
-    scratch: Optional[PipelineMemoryUsage] = None
+    spatial: Optional[MetricMemoryRequirements] = None
+    scratch: Optional[MetricMemoryRequirements] = None
 
 
 @dataclass
@@ -98,7 +99,7 @@ def _select(config: MetricsConfig) -> Ssimulacra2Config:
     _validate_named(config.original_format, config.size, "original format")
     _validate_named(config.distorted_format, config.size, "distorted format")
     return replace(config.ssimulacra2, size=config.size, streaming_inputs=config.streaming_inputs,
-                   gpu_resident_images=0, input_bytes=0)
+                   memory_policy=config.memory_policy, gpu_resident_images=0, input_bytes=0)
 
 
 def GetMemoryRequirements(config: MetricsConfig) -> MetricsMemoryRequirements:
@@ -106,27 +107,14 @@ def GetMemoryRequirements(config: MetricsConfig) -> MetricsMemoryRequirements:
     return MetricsMemoryRequirements(spatial=_PipelineRequirements(_select(config)))
 
 
-def MaxWorkersForBudgetWorstCase(requirements: MetricsMemoryRequirements,
-                                 heap_budgets: Sequence[int]) -> int:
-    """metric_pipelines.cc:197-212, vulkan_memory_query.cc:259-269,395-424:
-    workers that fit the per-heap budgets (0 device, 1 pinned host)."""
-    parts = [p for p in (requirements.scratch, requirements.spatial) if p is not None]
-    if not parts:
-        raise InvalidArgument("memory: no requirements")
-    charge = [0, 0]
-    for p in parts:
-        for heap, value in ((0, p.total_device_bytes), (1, p.pinned_host_bytes)):
-            if value > _U64_MAX - charge[heap]:
-                raise OverflowError("memory: heap allocation size overflow")
-            charge[heap] += value
-    workers = _U64_MAX
-    for heap in (0, 1):
-        if charge[heap] == 0:
-            continue
-        if heap >= len(heap_budgets):
-            raise InvalidArgument("memory: missing heap budget")
-        workers = min(workers, int(heap_budgets[heap]) // charge[heap])
-    return workers
+def MaxWorkersForBudgetWorstCase(requirements, heap_budgets: Sequence[int]) -> int:
+    """metric_pipelines.cc:197-212 and vqm_memory.h:126-133: workers that fit
+    the per-heap budgets (0 device, 1 pinned host), each charged for its
+    requirements at their largest supported alternative. `requirements` is a
+    MetricsMemoryRequirements or a sequence of MetricMemoryRequirements."""
+    if isinstance(requirements, MetricsMemoryRequirements):
+        requirements = [r for r in (requirements.scratch, requirements.spatial) if r is not None]
+    return _WorstCase(requirements, heap_budgets)
 
 
 class MetricPipelines:

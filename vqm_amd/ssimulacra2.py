@@ -18,7 +18,7 @@ from __future__ import annotations
 import ctypes
 import enum
 import threading
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 from typing import List, Optional, Sequence, Union
 
 import numpy as np
@@ -236,9 +236,10 @@ def required_bytes(fmt: ImageFormat, size: Size) -> int:
 
 @dataclass
 class Image:
-    """vqm_image.h:134-141: packed planes, each on a 4-byte boundary."""
+    """vqm_image.h:134-141: packed planes, each on a 4-byte boundary. The
+    format defaults to ImageFormat() (Float sRGB RGB), as in the reference."""
     size: Size
-    format: ImageFormat = field(default_factory=ImageFormat.srgb8)
+    format: ImageFormat = field(default_factory=ImageFormat)
     data: Union[bytes, bytearray, np.ndarray] = b""
 
     @staticmethod
@@ -288,10 +289,11 @@ class ImageBuffer:
 @dataclass(frozen=True)
 class DeviceImage:
     """A caller-owned device image (replaces VulkanBufferView): a device
-    pointer to packed planes, kept alive until Finish."""
+    pointer to packed planes, kept alive until Finish. Like the reference's
+    view, its format defaults to ImageFormat()."""
     ptr: int
     size: Size
-    format: ImageFormat = field(default_factory=ImageFormat.srgb8)
+    format: ImageFormat = field(default_factory=ImageFormat)
     bytes: int = 0
 
     def to_c(self) -> L.DeviceImageC:
@@ -415,17 +417,19 @@ def srgb_table() -> np.ndarray:
 
 @dataclass
 class Ssimulacra2Config:
-    """ssimulacra2.h:35-64. half_planes selects FP16 planes; plane_format
-    (0 FP32, 1 FP16, 2 BF16) overrides it."""
+    """ssimulacra2.h:35-64: the reference's fields in its order, then those
+    without a reference counterpart. half_planes selects FP16 planes;
+    plane_format (0 FP32, 1 FP16, 2 BF16, 3 fixed point) overrides it."""
     size: Size = Size()
     gpu_resident_images: int = 2
     half_planes: bool = False
+    chroma_filter: int = 0  # ChromaFilter: 0 bilinear, 1 bicubic, 2 lanczos
     input_bytes: int = 0
     streaming_inputs: bool = True
+    memory_policy: int = 0  # MemoryPolicy: 0 automatic, 1 direct, 2 staged (unavailable)
     plane_format: Optional[int] = None
     strip_rows: int = 0
     graph: bool = False  # replay one captured HIP graph per pair (no reference field)
-    chroma_filter: int = 0  # ChromaFilter: 0 bilinear, 1 bicubic, 2 lanczos
     blur: int = 0  # Blur: 0 sampled (the reference), 1 recursive (no reference field)
     map_form: int = 0  # MapForm: 0 default, 1 reference order, 2 difference (no reference field)
 
@@ -448,8 +452,48 @@ class Ssimulacra2Config:
         return c
 
 
+class MemoryPolicy(enum.IntEnum):
+    """vqm_memory_types.h:14. Device memory is the only arena path: Automatic
+    and Direct select it; Staged does not exist here (a pipeline configured
+    with it raises DeviceError, its plan marks every scenario unsupported)."""
+    Automatic = 0
+    Direct = 1
+    Staged = 2
+
+
+DEVICE_HEAP, PINNED_HOST_HEAP = 0, 1  # heap index = memory type
+
+
+@dataclass
+class MemoryAllocation:
+    """vqm_memory_types.h:16-23."""
+    buffer_bytes: int = 0
+    allocation_bytes: int = 0
+    memory_type: int = 0
+    heap: int = 0
+
+
 @dataclass
 class PipelineMemoryUsage:
+    """vqm_memory_types.h:27-35: the device memory a pipeline holds is its
+    arena (heap 0), its pinned host buffers the staging (heap 1)."""
+    uses_staging: bool = False
+    arena: MemoryAllocation = field(default_factory=MemoryAllocation)
+    staging: MemoryAllocation = field(default_factory=MemoryAllocation)
+    auxiliary: List[MemoryAllocation] = field(default_factory=list)
+
+    @staticmethod
+    def from_regions(r: "PipelineMemoryRegions") -> "PipelineMemoryUsage":
+        return PipelineMemoryUsage(
+            uses_staging=r.streaming_bytes > 0,
+            arena=MemoryAllocation(r.total_device_bytes, r.total_device_bytes, DEVICE_HEAP, DEVICE_HEAP),
+            staging=MemoryAllocation(r.pinned_host_bytes, r.pinned_host_bytes, PINNED_HOST_HEAP,
+                                     PINNED_HOST_HEAP))
+
+
+@dataclass
+class PipelineMemoryRegions:
+    """The same bytes by region (vqm_ss2_memory; no reference counterpart)."""
     workspace_bytes: int = 0
     resident_bytes: int = 0
     streaming_bytes: int = 0
@@ -458,29 +502,146 @@ class PipelineMemoryUsage:
     total_device_bytes: int = 0
 
     @staticmethod
-    def from_c(m: L.MemoryC) -> "PipelineMemoryUsage":
-        return PipelineMemoryUsage(m.workspace_bytes, m.resident_bytes, m.streaming_bytes,
-                                   m.input_pool_bytes, m.pinned_host_bytes, m.total_device_bytes)
+    def from_c(m: L.MemoryC) -> "PipelineMemoryRegions":
+        return PipelineMemoryRegions(m.workspace_bytes, m.resident_bytes, m.streaming_bytes,
+                                     m.input_pool_bytes, m.pinned_host_bytes, m.total_device_bytes)
 
 
-def memory_plan(config: Ssimulacra2Config) -> PipelineMemoryUsage:
-    """Device bytes a pipeline with `config` allocates (layout.cc:106-325)."""
+@dataclass
+class MemoryScenario:
+    """vqm_memory.h:26-31."""
+    supported: bool = False
+    arena: MemoryAllocation = field(default_factory=MemoryAllocation)
+    staging: MemoryAllocation = field(default_factory=MemoryAllocation)
+    auxiliary: List[MemoryAllocation] = field(default_factory=list)
+
+
+@dataclass
+class MemoryHeapInfo:
+    """vqm_memory.h:33-42."""
+    size_bytes: int = 0
+    device_local: bool = False
+    budget_available: bool = False
+    budget_bytes: int = 0
+    usage_bytes: int = 0
+
+
+@dataclass
+class MetricScratchRequirements:
+    """vqm_memory_types.h:37-42 (one metric shares nothing)."""
+    arena_bytes: int = 0
+    fallback_staging_bytes: int = 0
+    persistent_bytes: List[int] = field(default_factory=list)
+    reusable: bool = False
+
+
+@dataclass
+class MetricMemoryRequirements:
+    """vqm_memory.h:48-53."""
+    direct: MemoryScenario = field(default_factory=MemoryScenario)
+    staged: MemoryScenario = field(default_factory=MemoryScenario)
+    heaps: List[MemoryHeapInfo] = field(default_factory=list)
+    scratch: MetricScratchRequirements = field(default_factory=MetricScratchRequirements)
+
+
+_U64_MAX = (1 << 64) - 1
+
+
+def memory_plan(config: Ssimulacra2Config) -> PipelineMemoryRegions:
+    """Bytes a pipeline with `config` allocates, by region (layout.cc:106-325)."""
     m = L.MemoryC()
-    _check(L.lib().vqm_ss2_memory_plan(ctypes.byref(config.to_c()), ctypes.byref(m)))
-    return PipelineMemoryUsage.from_c(m)
+    cfg = replace(config, memory_policy=MemoryPolicy.Automatic)
+    _check(L.lib().vqm_ss2_memory_plan(ctypes.byref(cfg.to_c()), ctypes.byref(m)))
+    return PipelineMemoryRegions.from_c(m)
 
 
-def GetMemoryRequirements(config: Ssimulacra2Config) -> PipelineMemoryUsage:
-    """vqm_memory.h:63-69: the plan of memory_plan under the reference's name."""
-    return memory_plan(config)
+def memory_heaps() -> List[MemoryHeapInfo]:
+    """The two heaps (vqm_memory_heaps): 0 device memory, 1 pinned host."""
+    h = (L.MemoryHeapC * 2)()
+    _check(L.lib().vqm_memory_heaps(h))
+    return [MemoryHeapInfo(x.size_bytes, bool(x.device_local), bool(x.budget_available), x.budget_bytes,
+                           x.usage_bytes) for x in h]
 
 
-def MaxWorkersForBudget(per_worker: PipelineMemoryUsage, device_budget: int) -> int:
-    """vqm_memory.h:110-124 for the one device heap (HBM): pipelines of
-    `per_worker` that fit `device_budget` bytes; arithmetic only."""
-    if per_worker.total_device_bytes <= 0:
-        raise InvalidArgument("MaxWorkersForBudget: the requirement holds no device bytes")
-    return int(device_budget) // per_worker.total_device_bytes
+def GetMemoryRequirements(config: Ssimulacra2Config) -> MetricMemoryRequirements:
+    """vqm_memory.h:62-70: the allocations a pipeline with `config` will hold,
+    planned without allocating; InvalidArgument for a configuration the
+    pipeline would refuse."""
+    r = memory_plan(config)
+    usage = PipelineMemoryUsage.from_regions(r)
+    direct = MemoryScenario(config.memory_policy != MemoryPolicy.Staged, usage.arena, usage.staging)
+    return MetricMemoryRequirements(direct=direct, heaps=memory_heaps())
+
+
+def AvailableHeapBudgets(requirements: MetricMemoryRequirements) -> Optional[List[int]]:
+    """vqm_memory.h:55-60: remaining bytes per heap, or None without budgets."""
+    out = []
+    for h in requirements.heaps:
+        if not h.budget_available:
+            return None
+        out.append(max(h.budget_bytes - h.usage_bytes, 0))
+    return out
+
+
+def _charge(per_heap: List[int], scenario: MemoryScenario) -> None:
+    if not scenario.supported:
+        raise InvalidArgument("memory: unsupported or empty scenario")
+    for a in [scenario.arena, scenario.staging] + list(scenario.auxiliary):
+        if a.allocation_bytes == 0:
+            continue
+        if a.heap >= len(per_heap):
+            raise InvalidArgument("memory: missing heap budget")
+        if a.allocation_bytes > _U64_MAX - per_heap[a.heap]:
+            raise OverflowError("memory: heap allocation size overflow")
+        per_heap[a.heap] += a.allocation_bytes
+
+
+def _workers(charge: Sequence[int], budgets: Sequence[int]) -> int:
+    n = _U64_MAX
+    for c, b in zip(charge, budgets):
+        if c:
+            n = min(n, int(b) // c)
+    return n
+
+
+def MaxWorkersForBudget(per_worker, heap_budgets: Sequence[int]) -> int:
+    """vqm_memory.h:110-124: independent pipelines (one MemoryScenario, or a
+    sequence of them per worker) that fit the per-heap budgets; arithmetic
+    only. InvalidArgument for an unsupported scenario or a missing heap."""
+    scenarios = [per_worker] if isinstance(per_worker, MemoryScenario) else list(per_worker)
+    if not scenarios:
+        raise InvalidArgument("memory: no scenarios")
+    charge = [0] * len(heap_budgets)
+    for s in scenarios:
+        _charge(charge, s)
+    return _workers(charge, heap_budgets)
+
+
+def MaxWorkersForBudgetWorstCase(per_worker: Sequence[MetricMemoryRequirements],
+                                 heap_budgets: Sequence[int]) -> int:
+    """vqm_memory.h:126-133: each worker's metrics charged at their largest
+    supported alternative."""
+    per_worker = list(per_worker)
+    if not per_worker:
+        raise InvalidArgument("memory: no requirements")
+    total = [0] * len(heap_budgets)
+    for req in per_worker:
+        largest = [0] * len(heap_budgets)
+        supported = False
+        for s in (req.direct, req.staged):
+            if not s.supported:
+                continue
+            supported = True
+            charge = [0] * len(heap_budgets)
+            _charge(charge, s)
+            largest = [max(x, y) for x, y in zip(largest, charge)]
+        if not supported:
+            raise InvalidArgument("memory: unsupported requirements")
+        for i, v in enumerate(largest):
+            if v > _U64_MAX - total[i]:
+                raise OverflowError("memory: heap allocation size overflow")
+            total[i] += v
+    return _workers(total, heap_budgets)
 
 
 def algorithmic_bytes(size: Size, plane_format: int = L.PLANES_F32) -> int:
@@ -499,6 +660,8 @@ class Ssimulacra2Pipeline:
     def __init__(self, config: Ssimulacra2Config):
         if isinstance(config, Size):
             config = Ssimulacra2Config(size=config)
+        if config.memory_policy == MemoryPolicy.Staged:
+            raise DeviceError("MemoryPolicy.Staged is unavailable: pipelines hold their arena in device memory")
         self.config = config
         handle = ctypes.c_void_p()
         _check(L.lib().vqm_ss2_create(ctypes.byref(config.to_c()), ctypes.byref(handle)))
@@ -600,9 +763,14 @@ class Ssimulacra2Pipeline:
         return self.Finish(self.Submit(original, distorted, background))
 
     def GetMemoryUsage(self) -> PipelineMemoryUsage:
+        """ssimulacra2.h:204-205."""
+        return PipelineMemoryUsage.from_regions(self.GetMemoryRegions())
+
+    def GetMemoryRegions(self) -> PipelineMemoryRegions:
+        """The same bytes by region (no reference counterpart)."""
         m = L.MemoryC()
         _check(L.lib().vqm_ss2_memory_usage(self._h, ctypes.byref(m)))
-        return PipelineMemoryUsage.from_c(m)
+        return PipelineMemoryRegions.from_c(m)
 
     def PreparedImages(self) -> int:
         """Images the last submission prepared: 1 when its resident original
@@ -642,17 +810,43 @@ def ComputeWorstBackground(pipeline: Ssimulacra2Pipeline, original, distorted) -
 class Ssimulacra2Pool:
     """vqm_pool.h:51-86 / pool.cc:24-116: N pipelines round-robin, results in
     input order; after a failure the pending pairs are drained and the error
-    re-raised."""
+    re-raised. When a drain itself fails, the pipelines are dropped and the
+    pool is poisoned: later calls raise LogicError (pool.cc:47-49,94-112)."""
 
     def __init__(self, config: Ssimulacra2Config, workers: int):
         if workers < 1:
             raise InvalidArgument("a pool needs at least one worker")
         self._pipes = [Ssimulacra2Pipeline(config) for _ in range(workers)]
+        self._workers = workers
+        self._poisoned = False
 
     def Workers(self) -> int:
-        return len(self._pipes)
+        return self._workers
+
+    def close(self) -> None:
+        for p in self._pipes:
+            p.close()
+        self._pipes = []
+
+    def _drain_after_failure(self, tokens) -> None:
+        failed = False
+        for slot, token in enumerate(tokens):
+            if token is None:
+                continue
+            try:
+                self._pipes[slot].Finish(token)
+            except Exception:
+                failed = True
+            tokens[slot] = None
+        if failed:
+            self.close()
+            self._poisoned = True
 
     def ComputeMany(self, pairs, background: float = 0.5) -> List[Ssimulacra2Result]:
+        if self._poisoned:
+            raise LogicError("pool cannot be reused after a failed drain")
+        if not self._pipes:
+            raise LogicError("the pool has been closed")
         n = len(self._pipes)
         tokens: List[Optional[int]] = [None] * n
         owner: List[int] = [-1] * n
@@ -670,13 +864,7 @@ class Ssimulacra2Pool:
                     out[owner[slot]] = self._pipes[slot].Finish(tokens[slot])
                     tokens[slot] = None
         except Exception:
-            for slot in range(n):
-                if tokens[slot] is not None:
-                    try:
-                        self._pipes[slot].Finish(tokens[slot])
-                    except Exception:
-                        pass
-                    tokens[slot] = None
+            self._drain_after_failure(tokens)
             raise
         return out  # type: ignore[return-value]
 
