@@ -35,6 +35,8 @@ from .ssimulacra2 import (DeviceImage, Image, ImageFormat, ImageView, InvalidArg
                           MaxWorkersForBudgetWorstCase as _WorstCase, required_bytes,
                           validate_image_format, view_bytes)
 
+_U64_MAX = (1 << 64) - 1
+
 
 @dataclass
 class MetricsConfig:
