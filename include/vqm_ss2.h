@@ -329,6 +329,29 @@ int vqm_ss2_submit_device(vqm_ss2_pipeline* pipeline, const vqm_device_image* or
 int vqm_ss2_finish(vqm_ss2_pipeline* pipeline, uint64_t token, uint32_t* scales,
                    double norms[108]);
 
+/* Allocation-free scoring of one pair: no counterpart in the reference. For
+ * callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc. The variant
+ * of vqm_ss2_create + vqm_ss2_submit_device + vqm_ss2_finish that runs the
+ * same kernels (so its norms are those of a pipeline with this config, bit
+ * for bit) with their workspace in a fixed device array of 128 MiB that the
+ * library holds on each device it runs on until the process ends: pairs up
+ * to 4096x2160 in every plane format. `config` gives the geometry,
+ * plane_format, strip_rows, blur and map_form (the other fields are
+ * ignored); both images are caller-owned 8-bit sRGB RGB device images of
+ * that geometry. The work is queued on `stream` (a hipStream_t, NULL for the
+ * default stream) and the call returns; the 108 norms land in
+ * `norms_device` (device memory, 108 doubles) when the stream reaches them,
+ * and `scales` (host, may be NULL) receives the level count at once. No
+ * hipMalloc, hipFree or other allocation happens on any call. Every call on
+ * a device shares its array: calls that overlap on one device (several
+ * streams or threads) race and must be serialized by the caller.
+ * INVALID_ARGUMENT for a mismatched or non-8-bit-sRGB image or a pair whose
+ * workspace exceeds the array. */
+int vqm_ss2_score_device_noalloc(const vqm_ss2_config* config, const vqm_device_image* original,
+                                 const vqm_device_image* distorted, void* stream, double* norms_device,
+                                 uint32_t* scales);
+
 /* GetMemoryUsage (include/ssimulacra2.h:204-205). */
 int vqm_ss2_memory_usage(const vqm_ss2_pipeline* pipeline, vqm_ss2_memory* usage);
 

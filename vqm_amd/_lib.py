@@ -157,6 +157,9 @@ SIGNATURES = {
     "vqm_ss2_batch_pool_bytes": (ctypes.c_int, [vp, ctypes.POINTER(u64)]),
     "vqm_memory_heaps": (ctypes.c_int, [ctypes.POINTER(MemoryHeapC)]),
     "vqm_ss2_autotune": (ctypes.c_int, [u32, u32, u32, u32, u32, ctypes.POINTER(TuneResultC)]),
+    "vqm_ss2_score_device_noalloc": (ctypes.c_int, [ctypes.POINTER(ConfigC), ctypes.POINTER(DeviceImageC),
+                                                    ctypes.POINTER(DeviceImageC), vp, vp,
+                                                    ctypes.POINTER(u32)]),
     "vqm_ss2_server_config_init": (None, [ctypes.POINTER(ServerConfigC)]),
     "vqm_ss2_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfigC), ctypes.POINTER(vp)]),
     "vqm_ss2_server_destroy": (None, [vp]),

@@ -11,6 +11,7 @@
 #include <string>
 
 #include "ss2_host.h"
+#include "ss2_noalloc.h"
 #include "ss2_runtime.h"
 #include "ss2_server.h"
 #include "ss2_source.h"
@@ -326,6 +327,20 @@ int vqm_ss2_submit_device(vqm_ss2_pipeline* p, const vqm_device_image* a,
         Need(b, "distorted");
         Need(token, "token");
         *token = p->impl.SubmitDevice(*a, *b, static_cast<hipStream_t>(wait_stream), background);
+    });
+}
+
+int vqm_ss2_score_device_noalloc(const vqm_ss2_config* config, const vqm_device_image* original,
+                                 const vqm_device_image* distorted, void* stream, double* norms_device,
+                                 uint32_t* scales) {
+    return Guard([&] {
+        Need(config, "config");
+        Need(original, "original");
+        Need(distorted, "distorted");
+        Need(norms_device, "norms");
+        const uint32_t n = vqm::ss2::ScoreDeviceNoAlloc(*config, *original, *distorted,
+                                                        static_cast<hipStream_t>(stream), norms_device);
+        if (scales) *scales = n;
     });
 }
 
