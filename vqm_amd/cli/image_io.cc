@@ -8,11 +8,15 @@
 
 #include <zlib.h>
 
+#include <algorithm>
 #include <array>
 #include <charconv>
 #include <cstring>
+#include <fstream>
 #include <numeric>
+#include <sstream>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 namespace vqm::image_io {
@@ -387,19 +391,25 @@ std::optional<Image> DecodePng(const std::uint8_t* data, std::size_t size, std::
 }
 
 // ---- YUV4MPEG2 ---------------------------------------------------------------------
+//
+// A stream is "YUV4MPEG2" + parameters up to a newline, then frames, each a
+// "FRAME" line (parameters allowed) followed by the planes. Parameters are a
+// letter and a value: W and H the size, C the chroma layout, I the
+// interlacing, F the frame rate; the rest (A, X) carry nothing the decoder
+// needs.
 
 namespace {
 
-struct Y4mTag {
-    const char* name;
+// What a C value declares: the family, the chroma subsampling and siting,
+// and whether an alpha plane follows. A stream without C is 420jpeg.
+struct ChromaLayout {
+    std::string_view name;
     ColorFamily family;
     std::uint8_t sub_w, sub_h;
     ChromaLocation location;
     bool alpha;
 };
-
-// yuv4mpeg2's C tags and their siting; a missing tag means 420jpeg.
-constexpr Y4mTag kY4mTags[] = {
+constexpr ChromaLayout kChromaLayouts[] = {
     {"mono", ColorFamily::Grey, 0, 0, ChromaLocation::Left, false},
     {"420", ColorFamily::Yuv, 1, 1, ChromaLocation::Left, false},
     {"420mpeg2", ColorFamily::Yuv, 1, 1, ChromaLocation::Left, false},
@@ -410,150 +420,144 @@ constexpr Y4mTag kY4mTags[] = {
     {"411", ColorFamily::Yuv, 2, 0, ChromaLocation::Left, false},
     {"444alpha", ColorFamily::Yuv, 0, 0, ChromaLocation::Left, true},
 };
+// High bit depths are written as a "p<depth>" suffix of the C value.
+constexpr unsigned kSuffixDepths[] = {9, 10, 12, 14, 16};
 
-bool ReadLine(std::FILE* f, std::string& line) {
-    line.clear();
-    for (;;) {
-        const int c = std::fgetc(f);
-        if (c == EOF) return !line.empty();
-        if (c == '\n') return true;
-        if (line.size() >= kMaxHeaderLine) return false;
-        line.push_back(static_cast<char>(c));
-    }
+// A whole decimal number, or nullopt.
+std::optional<std::uint64_t> Decimal(std::string_view s) {
+    std::uint64_t v = 0;
+    const auto [end, ec] = std::from_chars(s.data(), s.data() + s.size(), v);
+    if (s.empty() || ec != std::errc{} || end != s.data() + s.size()) return std::nullopt;
+    return v;
 }
 
-bool ToUnsigned(std::string_view s, std::uint64_t& out) {
-    if (s.empty()) return false;
-    const auto r = std::from_chars(s.data(), s.data() + s.size(), out);
-    return r.ec == std::errc{} && r.ptr == s.data() + s.size();
-}
-
-// "420p10" -> the 420 tag at depth 10. Only the depths 9, 10, 12, 14 and 16
-// split off a suffix; any other keeps the whole text as the tag name.
-const Y4mTag* FindTag(std::string_view text, std::uint8_t& depth) {
+// The layout a C value names and the depth its suffix gives (0: 8 bits).
+const ChromaLayout* ResolveChroma(std::string_view value, std::uint8_t& depth) {
     depth = 0;
-    std::string_view base = text;
-    const std::size_t p = text.rfind('p');
-    std::uint64_t d = 0;
-    if (p != std::string_view::npos && p + 1 < text.size() && ToUnsigned(text.substr(p + 1), d) &&
-        (d == 9 || d == 10 || d == 12 || d == 14 || d == 16)) {
-        base = text.substr(0, p);
-        depth = static_cast<std::uint8_t>(d);
+    for (const unsigned d : kSuffixDepths) {
+        const std::string suffix = "p" + std::to_string(d);
+        if (value.size() > suffix.size() && value.substr(value.size() - suffix.size()) == suffix) {
+            value.remove_suffix(suffix.size());
+            depth = static_cast<std::uint8_t>(d);
+            break;
+        }
     }
-    for (const Y4mTag& t : kY4mTags)
-        if (base == t.name) return &t;
-    return nullptr;
+    const auto it = std::find_if(std::begin(kChromaLayouts), std::end(kChromaLayouts),
+                                 [&](const ChromaLayout& l) { return l.name == value; });
+    return it == std::end(kChromaLayouts) ? nullptr : &*it;
+}
+
+// F<num>:<den>, reduced; 0/0 unless both parts are positive 32-bit numbers.
+FrameRate ParseRate(std::string_view value) {
+    const std::size_t colon = value.find(':');
+    if (colon == std::string_view::npos) return {};
+    const auto num = Decimal(value.substr(0, colon)), den = Decimal(value.substr(colon + 1));
+    constexpr std::uint64_t kMax = 0xffffffffu;
+    if (!num || !den || *num == 0 || *den == 0 || *num > kMax || *den > kMax) return {};
+    const std::uint64_t g = std::gcd(*num, *den);
+    return {static_cast<std::uint32_t>(*num / g), static_cast<std::uint32_t>(*den / g)};
+}
+
+// One line without its newline; false at the end of the stream or when the
+// line exceeds kMaxHeaderLine.
+bool NextLine(std::istream& in, std::string& line) {
+    std::array<char, kMaxHeaderLine + 1> buf{};
+    in.getline(buf.data(), static_cast<std::streamsize>(buf.size()));
+    const std::streamsize got = in.gcount();
+    if (got == 0) return false;
+    if (in.fail() && !in.eof()) return false;  // too long
+    // getline counts the newline it consumed, not the terminator it stores.
+    line.assign(buf.data(), std::strlen(buf.data()));
+    return true;
 }
 
 }  // namespace
 
 bool IsY4m(const std::string& path) {
-    std::FILE* f = std::fopen(path.c_str(), "rb");
-    if (f == nullptr) return false;
-    char magic[9] = {};
-    const bool yes = std::fread(magic, 1, 9, f) == 9 && std::memcmp(magic, kY4mSignature, 9) == 0;
-    std::fclose(f);
-    return yes;
+    std::ifstream in(path, std::ios::binary);
+    char magic[sizeof(kY4mSignature) - 1] = {};
+    return in.read(magic, sizeof(magic)) && std::memcmp(magic, kY4mSignature, sizeof(magic)) == 0;
 }
 
 Y4mReader::Y4mReader(const std::string& path, bool sequence) : path_(path) {
-    file_ = std::fopen(path.c_str(), "rb");
-    if (file_ == nullptr) throw std::runtime_error(path + ": cannot open");
-    auto fail = [&](const char* why) {
-        std::fclose(file_);
-        file_ = nullptr;
-        throw std::runtime_error(path + ": " + why);
-    };
-    char magic[9] = {};
-    if (std::fread(magic, 1, 9, file_) != 9 || std::memcmp(magic, kY4mSignature, 9) != 0) fail("not a YUV4MPEG2 file");
+    auto error = [&](const std::string& why) { return std::runtime_error(path + ": " + why); };
+    in_.open(path, std::ios::binary);
+    if (!in_) throw error("cannot open");
     std::string header;
-    if (!ReadLine(file_, header)) fail("truncated y4m header");
-    std::uint64_t width = 0, height = 0;
-    bool have_w = false, have_h = false, progressive = true;
+    if (!NextLine(in_, header)) throw error("truncated y4m header");
+    std::istringstream fields(header);
+    std::string field;
+    if (!(fields >> field) || field != kY4mSignature) throw error("not a YUV4MPEG2 file");
+
+    std::optional<std::uint64_t> width, height;
     std::string chroma = "420jpeg";
-    std::size_t i = 0;
-    while (i < header.size()) {
-        if (header[i] == ' ') {
-            ++i;
-            continue;
-        }
-        const std::size_t start = i;
-        while (i < header.size() && header[i] != ' ') ++i;
-        const std::string_view token(header.data() + start, i - start);
-        if (token.size() < 2) continue;
-        const std::string_view value = token.substr(1);
-        switch (token[0]) {
-            case 'W':
-                if (!(have_w = ToUnsigned(value, width))) fail("invalid y4m width");
-                break;
-            case 'H':
-                if (!(have_h = ToUnsigned(value, height))) fail("invalid y4m height");
-                break;
-            case 'C': chroma = std::string(value); break;
-            case 'I': progressive = value == "p"; break;
-            case 'F': {
-                const std::size_t colon = value.find(':');
-                std::uint64_t n = 0, d = 0;
-                if (colon != std::string_view::npos && ToUnsigned(value.substr(0, colon), n) &&
-                    ToUnsigned(value.substr(colon + 1), d) && n != 0 && d != 0 && n <= 0xffffffffu &&
-                    d <= 0xffffffffu) {
-                    const std::uint64_t g = std::gcd(n, d);
-                    rate_ = {static_cast<std::uint32_t>(n / g), static_cast<std::uint32_t>(d / g)};
-                }
-                break;
-            }
-            default: break;  // A (aspect ratio) and X (comments) carry no sample metadata
+    bool interlaced = false;
+    while (fields >> field) {
+        const char letter = field[0];
+        const std::string_view value = std::string_view(field).substr(1);
+        if (value.empty()) continue;
+        if (letter == 'W') {
+            if (!(width = Decimal(value))) throw error("invalid y4m width");
+        } else if (letter == 'H') {
+            if (!(height = Decimal(value))) throw error("invalid y4m height");
+        } else if (letter == 'C') {
+            chroma = value;
+        } else if (letter == 'I') {
+            interlaced = value != "p";
+        } else if (letter == 'F') {
+            rate_ = ParseRate(value);
         }
     }
-    if (!have_w || !have_h || width == 0 || height == 0) fail("y4m header has no frame size");
-    if (sequence && rate_.numerator == 0) fail("y4m requires a valid frame rate");
-    if (sequence && !progressive) fail("only progressive y4m is supported");
+    if (!width || !height || *width == 0 || *height == 0) throw error("y4m header has no frame size");
+    if (sequence && rate_.numerator == 0) throw error("y4m requires a valid frame rate");
+    if (sequence && interlaced) throw error("only progressive y4m is supported");
     std::uint8_t depth = 0;
-    const Y4mTag* tag = FindTag(chroma, depth);
-    if (tag == nullptr) fail("unsupported y4m chroma tag");
-    if (width % (1u << tag->sub_w) != 0 || height % (1u << tag->sub_h) != 0)
-        fail("y4m dimensions are not divisible by the chroma subsampling");
-    size_ = {static_cast<std::size_t>(width), static_cast<std::size_t>(height)};
+    const ChromaLayout* layout = ResolveChroma(chroma, depth);
+    if (layout == nullptr) throw error("unsupported y4m chroma tag");
+    if (*width % (1u << layout->sub_w) != 0 || *height % (1u << layout->sub_h) != 0)
+        throw error("y4m dimensions are not divisible by the chroma subsampling");
+
+    size_ = {static_cast<std::size_t>(*width), static_cast<std::size_t>(*height)};
     format_ = ImageFormat{};
     format_.type = depth == 0 ? PixelType::Byte : PixelType::Word;
     format_.depth = depth;
-    format_.family = tag->family;
-    format_.alpha = tag->alpha ? AlphaType::Straight : AlphaType::None;
-    format_.chroma_location = tag->location;
-    format_.subsample_w = tag->sub_w;
-    format_.subsample_h = tag->sub_h;
+    format_.family = layout->family;
+    format_.alpha = layout->alpha ? AlphaType::Straight : AlphaType::None;
+    format_.chroma_location = layout->location;
+    format_.subsample_w = layout->sub_w;
+    format_.subsample_h = layout->sub_h;
+    // Y4M declares no colorimetry: limited range, the BT.470BG (BT.601)
+    // matrix and BT.709 transfer and primaries, overridable by --input-*.
     format_.range = PixelRange::Limited;
-    // y4m declares no colours; BT.601 (BT.470BG) is its customary matrix.
     format_.matrix = MatrixCoefficients::Bt470Bg;
     format_.transfer = TransferCharacteristics::Bt709;
     format_.primaries = ColorPrimaries::Bt709;
 }
 
-Y4mReader::~Y4mReader() {
-    if (file_ != nullptr) std::fclose(file_);
-}
+Y4mReader::~Y4mReader() = default;
 
 bool Y4mReader::Next(Image& image) {
-    const int first = std::fgetc(file_);
-    if (first == EOF) {
-        if (std::ferror(file_)) throw std::runtime_error(path_ + ": could not read y4m frame");
+    if (in_.peek() == std::char_traits<char>::eof()) {
+        if (in_.bad()) throw std::runtime_error(path_ + ": could not read y4m frame");
         return false;
     }
-    std::ungetc(first, file_);
-    std::string line;
-    if (!ReadLine(file_, line) || (line != "FRAME" && line.rfind("FRAME ", 0) != 0))
+    std::string marker;
+    const std::string_view frame = "FRAME";
+    if (!NextLine(in_, marker) || marker.compare(0, frame.size(), frame) != 0 ||
+        (marker.size() > frame.size() && marker[frame.size()] != ' '))
         throw std::runtime_error(path_ + ": y4m frame header missing");
     image.size = size_;
     image.format = format_;
     image.data.assign(RequiredBytes(format_, size_), 0);
-    const std::size_t sample_bytes = format_.type == PixelType::Word ? 2 : 1;
+    // Samples are bytes, or little-endian words (as FFmpeg writes high-depth
+    // y4m), which is the host order.
+    const std::size_t sample = format_.type == PixelType::Word ? 2 : 1;
     std::array<Size, 4> dims{};
-    const auto off = PlaneOffsets(format_, size_, sample_bytes, dims);
-    for (int p = 0; p < 4; ++p) {
-        const std::size_t bytes = dims[p].width * dims[p].height * sample_bytes;
-        if (bytes == 0) continue;
-        // Words are little-endian in the file (FFmpeg's high-depth y4m), as on the host.
-        if (std::fread(image.data.data() + off[p], 1, bytes, file_) != bytes)
+    const std::array<std::size_t, 4> at = PlaneOffsets(format_, size_, sample, dims);
+    for (std::size_t p = 0; p < at.size(); ++p) {
+        const std::size_t bytes = dims[p].width * dims[p].height * sample;
+        if (bytes != 0 && !in_.read(reinterpret_cast<char*>(image.data.data() + at[p]),
+                                    static_cast<std::streamsize>(bytes)))
             throw std::runtime_error(path_ + ": truncated y4m frame");
     }
     return true;
@@ -610,60 +614,67 @@ std::optional<Image> LoadImage(std::string_view path, std::string& error) {
 }
 
 // ---- --input-* option values (io.cc:690-780) -----------------------------------------
+//
+// Each option takes one of a fixed list of names; a bad value is reported
+// with the whole list, '|'-separated.
 
 namespace {
 
+template <typename T>
+struct Named {
+    std::string_view name;
+    T value;
+};
+
 template <typename T, std::size_t N>
-bool Lookup(const std::pair<const char*, T> (&table)[N], std::string_view text, T& out, std::string& accepted) {
-    accepted.clear();
-    for (const auto& [name, value] : table) {
-        if (text == name) {
-            out = value;
-            return true;
-        }
-        if (!accepted.empty()) accepted += '|';
-        accepted += name;
+bool Choose(const std::array<Named<T>, N>& names, std::string_view text, T& out, std::string& accepted) {
+    const auto hit = std::find_if(names.begin(), names.end(), [&](const Named<T>& n) { return n.name == text; });
+    if (hit != names.end()) {
+        out = hit->value;
+        accepted.clear();
+        return true;
     }
+    std::ostringstream list;
+    for (std::size_t i = 0; i < N; ++i) list << (i ? "|" : "") << names[i].name;
+    accepted = list.str();
     return false;
 }
 
-constexpr std::pair<const char*, MatrixCoefficients> kMatrices[] = {
-    {"rgb", MatrixCoefficients::Rgb}, {"bt709", MatrixCoefficients::Bt709}, {"fcc", MatrixCoefficients::Fcc},
-    {"bt470bg", MatrixCoefficients::Bt470Bg}, {"st170m", MatrixCoefficients::St170M},
-    {"st240m", MatrixCoefficients::St240M}, {"ycgco", MatrixCoefficients::YCgCo},
-    {"bt2020ncl", MatrixCoefficients::Bt2020Ncl}, {"bt2020cl", MatrixCoefficients::Bt2020Cl},
-    {"chromaticity-derived-ncl", MatrixCoefficients::ChromaticityDerivedNcl},
-    {"chromaticity-derived-cl", MatrixCoefficients::ChromaticityDerivedCl},
-    {"ictcp", MatrixCoefficients::Bt2100ICtCp},
-};
-constexpr std::pair<const char*, TransferCharacteristics> kTransfers[] = {
-    {"bt709", TransferCharacteristics::Bt709}, {"bt601", TransferCharacteristics::Bt601},
-    {"bt470m", TransferCharacteristics::Bt470M}, {"bt470bg", TransferCharacteristics::Bt470Bg},
-    {"st240m", TransferCharacteristics::St240M}, {"linear", TransferCharacteristics::Linear},
-    {"log100", TransferCharacteristics::Log100}, {"log316", TransferCharacteristics::Log316},
-    {"xvycc", TransferCharacteristics::Iec61966_2_4}, {"bt1361", TransferCharacteristics::Bt1361},
-    {"srgb", TransferCharacteristics::Srgb}, {"bt2020-10", TransferCharacteristics::Bt2020_10},
-    {"bt2020-12", TransferCharacteristics::Bt2020_12}, {"st2084", TransferCharacteristics::St2084},
-    {"st428", TransferCharacteristics::St428}, {"arib-b67", TransferCharacteristics::AribB67},
-};
-constexpr std::pair<const char*, ColorPrimaries> kPrimaries[] = {
-    {"bt709", ColorPrimaries::Bt709}, {"bt470m", ColorPrimaries::Bt470M}, {"bt470bg", ColorPrimaries::Bt470Bg},
-    {"st170m", ColorPrimaries::St170M}, {"st240m", ColorPrimaries::St240M}, {"film", ColorPrimaries::Film},
-    {"bt2020", ColorPrimaries::Bt2020}, {"st428", ColorPrimaries::St428}, {"st431-2", ColorPrimaries::St431_2},
-    {"st432-1", ColorPrimaries::St432_1}, {"ebu3213e", ColorPrimaries::Ebu3213E},
-};
-constexpr std::pair<const char*, PixelRange> kRanges[] = {{"limited", PixelRange::Limited},
-                                                          {"full", PixelRange::Full}};
-constexpr std::pair<const char*, ChromaFilter> kFilters[] = {
-    {"bilinear", ChromaFilter::Bilinear}, {"bicubic", ChromaFilter::Bicubic}, {"lanczos", ChromaFilter::Lanczos}};
+using M = MatrixCoefficients;
+using T = TransferCharacteristics;
+using P = ColorPrimaries;
+constexpr std::array<Named<M>, 12> kMatrixNames{{
+    {"rgb", M::Rgb}, {"bt709", M::Bt709}, {"fcc", M::Fcc}, {"bt470bg", M::Bt470Bg}, {"st170m", M::St170M},
+    {"st240m", M::St240M}, {"ycgco", M::YCgCo}, {"bt2020ncl", M::Bt2020Ncl}, {"bt2020cl", M::Bt2020Cl},
+    {"chromaticity-derived-ncl", M::ChromaticityDerivedNcl},
+    {"chromaticity-derived-cl", M::ChromaticityDerivedCl}, {"ictcp", M::Bt2100ICtCp},
+}};
+constexpr std::array<Named<T>, 16> kTransferNames{{
+    {"bt709", T::Bt709}, {"bt601", T::Bt601}, {"bt470m", T::Bt470M}, {"bt470bg", T::Bt470Bg},
+    {"st240m", T::St240M}, {"linear", T::Linear}, {"log100", T::Log100}, {"log316", T::Log316},
+    {"xvycc", T::Iec61966_2_4}, {"bt1361", T::Bt1361}, {"srgb", T::Srgb}, {"bt2020-10", T::Bt2020_10},
+    {"bt2020-12", T::Bt2020_12}, {"st2084", T::St2084}, {"st428", T::St428}, {"arib-b67", T::AribB67},
+}};
+constexpr std::array<Named<P>, 11> kPrimaryNames{{
+    {"bt709", P::Bt709}, {"bt470m", P::Bt470M}, {"bt470bg", P::Bt470Bg}, {"st170m", P::St170M},
+    {"st240m", P::St240M}, {"film", P::Film}, {"bt2020", P::Bt2020}, {"st428", P::St428},
+    {"st431-2", P::St431_2}, {"st432-1", P::St432_1}, {"ebu3213e", P::Ebu3213E},
+}};
+constexpr std::array<Named<PixelRange>, 2> kRangeNames{{{"limited", PixelRange::Limited},
+                                                        {"full", PixelRange::Full}}};
+constexpr std::array<Named<ChromaFilter>, 3> kFilterNames{{{"bilinear", ChromaFilter::Bilinear},
+                                                           {"bicubic", ChromaFilter::Bicubic},
+                                                           {"lanczos", ChromaFilter::Lanczos}}};
 
 }  // namespace
 
-bool ParseMatrix(std::string_view t, MatrixCoefficients& o, std::string& a) { return Lookup(kMatrices, t, o, a); }
-bool ParseTransfer(std::string_view t, TransferCharacteristics& o, std::string& a) { return Lookup(kTransfers, t, o, a); }
-bool ParsePrimaries(std::string_view t, ColorPrimaries& o, std::string& a) { return Lookup(kPrimaries, t, o, a); }
-bool ParseRange(std::string_view t, PixelRange& o, std::string& a) { return Lookup(kRanges, t, o, a); }
-bool ParseChromaFilter(std::string_view t, ChromaFilter& o, std::string& a) { return Lookup(kFilters, t, o, a); }
+bool ParseMatrix(std::string_view t, MatrixCoefficients& o, std::string& a) { return Choose(kMatrixNames, t, o, a); }
+bool ParseTransfer(std::string_view t, TransferCharacteristics& o, std::string& a) {
+    return Choose(kTransferNames, t, o, a);
+}
+bool ParsePrimaries(std::string_view t, ColorPrimaries& o, std::string& a) { return Choose(kPrimaryNames, t, o, a); }
+bool ParseRange(std::string_view t, PixelRange& o, std::string& a) { return Choose(kRangeNames, t, o, a); }
+bool ParseChromaFilter(std::string_view t, ChromaFilter& o, std::string& a) { return Choose(kFilterNames, t, o, a); }
 
 void InputColorOptions::ApplyTo(ImageFormat& format) const {
     if (matrix) format.matrix = *matrix;

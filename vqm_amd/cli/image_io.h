@@ -17,6 +17,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <fstream>
 #include <memory>
 #include <optional>
 #include <string>
@@ -60,7 +61,7 @@ public:
     [[nodiscard]] FrameRate rate() const { return rate_; }
 
 private:
-    std::FILE* file_ = nullptr;
+    std::ifstream in_;
     std::string path_;
     Size size_;
     ImageFormat format_;
