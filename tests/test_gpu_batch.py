@@ -142,3 +142,52 @@ def test_persistent_grid_matches_one_block_per_item(fmt):
             pers.close()
             assert np.array_equal(got, want), (shapes[0], chunks)
             assert np.array_equal(again, want)
+
+
+def test_ladder_batch_at_workload_sizes(oracle_lib):
+    """The uneven-size batch at its workload's sizes: 40 pairs drawn from the
+    five-size ladder (640x360 to 3840x2160, the synthetic batch generator)
+    in one launch per kernel. Each pair is bit-equal to the same pair alone
+    through a pipeline at the batch's strip height, the default strip height
+    changes only the FP64 summation grouping, and one pair of each size is
+    bit-level with its oracle form (the FP32 ref form: the default map form)
+    and within the contract of FP64."""
+    from conftest import SCORE_ABS, assert_features_close
+    from vqm_amd import DeviceImage, synth
+    picks = [synth.ladder_pair_shape(3, i, None) for i in range(40)]
+    for k, size in enumerate(synth.LADDER):  # every size at least once
+        picks[k] = (size, synth.STRENGTHS[k % 3])
+    host = [synth.pair(3, i, w, h, s) for i, ((w, h), s) in enumerate(picks)]
+    keep, pairs = [], []
+    for ((w, h), _), (a, b) in zip(picks, host):
+        da, db = DeviceBuffer(a.nbytes), DeviceBuffer(b.nbytes)
+        da.upload(a)
+        db.upload(b)
+        keep += [da, db]
+        pairs.append((da.ptr, db.ptr, w, h))
+    print(f"ladder batch: {len(pairs)} pairs", flush=True)
+    batch = Ssimulacra2Batch(max_pairs=64, strip_rows=64)
+    scales, norms = batch.Finish(batch.Submit(pairs), raw=True)
+    batch.close()
+    tall = Ssimulacra2Batch(max_pairs=64)
+    _, tall_norms = tall.Finish(tall.Submit(pairs), raw=True)
+    tall.close()
+    np.testing.assert_allclose(tall_norms, norms, rtol=1e-12, atol=0)
+    pipes = {size: Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(*size))) for size in synth.LADDER}
+    checked = set()
+    for i, (((w, h), _), (a, b)) in enumerate(zip(picks, host)):
+        alone_norms = pipes[(w, h)].Compute(DeviceImage(pairs[i][0], Size(w, h), ImageFormat.srgb8()),
+                                           DeviceImage(pairs[i][1], Size(w, h), ImageFormat.srgb8())).norms()
+        assert np.array_equal(norms[i], alone_norms), f"pair {i} {w}x{h}"
+        assert scales[i] == 6
+        if (w, h) in checked:
+            continue
+        checked.add((w, h))
+        n, ref, _ = oracle_lib.compute(oracle_lib.F32_REF, w, h, a, b)
+        np.testing.assert_allclose(norms[i], ref, rtol=1e-10, atol=0)
+        _, f64, _ = oracle_lib.compute(oracle_lib.F64, w, h, a, b)
+        assert abs(oracle_lib.score(n, norms[i]) - oracle_lib.score(n, f64)) <= SCORE_ABS
+    for p in pipes.values():
+        p.close()
+    for d in keep:
+        d.free()

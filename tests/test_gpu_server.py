@@ -158,3 +158,51 @@ def test_replay_and_misuse():
         server.Wait(t)
         with pytest.raises(InvalidArgument):
             server.Wait(t)  # a ticket is consumed once
+
+
+def test_ladder_requests_are_the_same_alone_and_in_the_mix():
+    """The serving workload's request mix: 60 requests whose sizes follow
+    the ladder weights 0.1/0.2/0.35/0.2/0.15 (640x360 to 3840x2160, the
+    synthetic serving generator), submitted from two threads to a server with
+    three lanes; every result equals the same pair scored alone."""
+    from vqm_amd import DeviceImage, synth
+    reqs = []
+    for i in range(60):
+        (w, h), strength = synth.ladder_pair_shape(4, i, synth.SERVE_WEIGHTS)
+        a, b = synth.pair(4, i, w, h, strength)
+        da, db = DeviceBuffer(a.nbytes), DeviceBuffer(b.nbytes)
+        da.upload(a)
+        db.upload(b)
+        reqs.append({"w": w, "h": h, "da": da, "db": db})
+    assert len({(r["w"], r["h"]) for r in reqs}) == 5
+    print(f"ladder serving: {len(reqs)} requests", flush=True)
+    with Ssimulacra2Server(lanes=3, max_batch=16, hbm_budget=8 << 30) as server:
+        tickets = [None] * len(reqs)
+
+        def submit(idx):
+            for i in idx:
+                r = reqs[i]
+                tickets[i] = server.SubmitDevice(r["da"].ptr, r["db"].ptr, r["w"], r["h"])
+
+        threads = [threading.Thread(target=submit, args=(range(k, len(reqs), 2),)) for k in range(2)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        results = [server.Wait(t) for t in tickets]
+        stats = server.Stats()
+    assert stats["completed"] == len(reqs) and stats["failed"] == 0 and stats["max_batch"] > 1
+    pipes = {}
+    for i, (r, res) in enumerate(zip(reqs, results)):
+        size = (r["w"], r["h"])
+        if size not in pipes:
+            pipes[size] = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(*size)))
+        want = pipes[size].Compute(DeviceImage(r["da"].ptr, Size(*size), ImageFormat.srgb8()),
+                                   DeviceImage(r["db"].ptr, Size(*size), ImageFormat.srgb8())).norms()
+        assert res.status == 0
+        assert np.array_equal(res.norms, want), f"request {i} {size}"
+    for p in pipes.values():
+        p.close()
+    for r in reqs:
+        r["da"].free()
+        r["db"].free()

@@ -4,8 +4,10 @@
 //
 //   cbrt_fast  <= 0.84 ulp on [2^-126, 64); 16 FP32 operations.
 //   cbrt_rn    the cube root correctly rounded to FP32 (round to nearest),
-//              for every normal positive float: cbrt_fast, then two exact
-//              comparisons done in FP64.
+//              for every normal positive float: two FP64 third-order steps
+//              from cbrt_fast's seed, rounded to FP32, with two exact
+//              FP64 comparisons for the one input in 2^20 whose FP64 root
+//              lies too near a midpoint to round safely.
 //
 // Every step is an IEEE conversion, add, multiply or fused multiply-add
 // (written out), with contraction off, so the result does not depend on the
@@ -59,21 +61,47 @@ __host__ __device__ inline int cube_cmp(double x, double m) {
     return r > e ? 1 : (r < e ? -1 : 0);
 }
 
-// The cube root of a positive normal float, correctly rounded. cbrt_fast is
-// within one ulp, so the rounded root is y or a neighbour of y: the sign of
-// x - y^3 names the side the true root t lies on, and one more comparison
-// against the cube of the midpoint between y and that neighbour decides
-// whether t lies beyond it. A midpoint has 25 significant bits and its cube
-// is never a float, so there are no ties. Both midpoints and y are exact in
-// FP64 and their cubes are within a factor of two of x.
-__host__ __device__ inline float cbrt_rn(float x) {
-    const float y = cbrt_fast(x);
+// The correctly rounded root from a float `y` within one ulp of it: the
+// sign of x - y^3 names the side the true root t lies on, and one more
+// comparison against the cube of the midpoint between y and its neighbour
+// on that side decides whether t lies beyond it. A midpoint has 25
+// significant bits and its cube is never a float, so there are no ties.
+// Both midpoints and y are exact in FP64 and their cubes are within a
+// factor of two of x.
+__host__ __device__ inline float cbrt_round_exact(float x, float y) {
     const double xd = x, yd = y;
     const int side = cube_cmp(xd, yd);
     if (side == 0) return y;  // x is the cube of y
     const float n = cbrt_float(cbrt_bits(y) + (side > 0 ? 1u : 0xFFFFFFFFu));
     const double mid = (yd + static_cast<double>(n)) * 0.5;
     return cube_cmp(xd, mid) == side ? n : y;
+}
+
+// The cube root of a positive normal float, correctly rounded. The FP32
+// seed of cbrt_fast (x^(-1/3) within 3.7%), then two third-order steps
+// r <- r (1 + h (1/3 + h (2/9 + h 14/81))), h = 1 - x r^3, in FP64 (the
+// error falls to ~1e-5, then below FP64 rounding), and y = x r^2 in FP64,
+// within 2^-50 of the root. Rounding y to FP32 gives the correctly rounded
+// root unless y lies within 2^-44 of a midpoint between two floats: the 29
+// bits FP32 drops from y's significand (all in its low word) then sit
+// within 2^8 units of 2^28 (one input in 2^20). Those take cbrt_round_exact.
+__host__ __device__ inline float cbrt_rn(float x) {
+    constexpr float kThird = 0x1.555556p-2f, kMagic = 1419910400.0f;
+    constexpr double kThirdD = 1.0 / 3.0, kC2 = 2.0 / 9.0, kC3 = 14.0 / 81.0;
+    const float seed = __builtin_fmaf(static_cast<float>(cbrt_bits(x)), -kThird, kMagic);
+    const double xd = x;
+    double r = cbrt_float(static_cast<uint32_t>(seed));
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double h = __builtin_fma(-xd, (r * r) * r, 1.0);
+        const double p = __builtin_fma(h, __builtin_fma(h, kC3, kC2), kThirdD);
+        r = __builtin_fma(r * h, p, r);
+    }
+    const double y = xd * (r * r);
+    const float rounded = static_cast<float>(y);
+    const uint32_t dropped = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, y)) & 0x1FFFFFFFu;
+    if (dropped - (0x10000000u - 0x100u) > 2 * 0x100u) return rounded;  // far from a midpoint
+    return cbrt_round_exact(x, rounded);
 }
 
 }  // namespace vqm::ss2

@@ -953,15 +953,14 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
             // Vertical pass for output row i - 12 (image row y0 + i - 12),
             // oldest ring entry first: scale.comp:243-298's ascending taps.
             if (i >= 2 * kRadius) {
-                float bl[2][kBlurred];
-                for (int q = 0; q < kBlurred; ++q) {
-                    for (int c = 0; c < 2; ++c) {
+                auto vertical = [&](int c, float* out) {
+                    for (int q = 0; q < kBlurred; ++q) {
                         float acc = TapsT::w[0] * win[(P + 1) % kTaps][q][c];
                         for (int k = 1; k < kTaps; ++k)
                             acc = fma_(TapsT::w[k], win[(P + 1 + k) % kTaps][q][c], acc);
-                        bl[c][q] = acc;
+                        out[q] = acc;
                     }
-                }
+                };
                 const uint32_t y = y0 + static_cast<uint32_t>(i - 2 * kRadius);
                 const int raw_slot = (i - kRadius) & (kRing - 1);
                 const float2 a2 = *reinterpret_cast<const float2*>(&ring_a[raw_slot][jm + kRadius]);
@@ -969,16 +968,23 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
                 const float av[2] = {a2.x, a2.y}, bv[2] = {b2.x, b2.y};
                 if (y < H) {
                     if constexpr (Ref) {
-                        // Each pixel's map values accumulate in FP64.
+                        // One column at a time (its five blurred values
+                        // live only through its map); each pixel's map
+                        // values accumulate in FP64.
                         for (int c = 0; c < 2; ++c) {
+                            float bl[kBlurred];
+                            vertical(c, bl);
                             if (col + static_cast<uint32_t>(c) >= W) continue;
                             float v6[6];
-                            map_values<true>(bl[c], av[c], bv[c], v6);
+                            map_values<true>(bl, av[c], bv[c], v6);
                             for (int k = 0; k < 6; ++k) sums[k] += static_cast<double>(v6[k]);
                         }
                     } else {
                         // The lane's two columns' map values are added in
                         // FP32, then accumulated in FP64.
+                        float bl[2][kBlurred];
+                        vertical(0, bl[0]);
+                        vertical(1, bl[1]);
                         float pair[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
                         for (int c = 0; c < 2; ++c) {
                             if (col + static_cast<uint32_t>(c) >= W) continue;
