@@ -270,12 +270,13 @@ def ladder_batch(args, pool):
     return out
 
 
-def resident_batch(refs, w, h, fmt, steps, warmup, chunks=1, strip_rows=0, persistent=False, map_form=0):
+def resident_batch(refs, w, h, fmt, steps, warmup, chunks=1, strip_rows=0, persistent=False, map_form=0,
+                   strip_cols=0):
     """Pairs/s of the batch engine over resident pairs, with the per-stage
     HIP-event times and the scale kernel's HBM fraction."""
     from vqm_amd import Ssimulacra2Batch
     batch = Ssimulacra2Batch(max_pairs=len(refs), plane_format=fmt, strip_rows=strip_rows, chunks=chunks,
-                             persistent=persistent, map_form=map_form)
+                             persistent=persistent, map_form=map_form, strip_cols=strip_cols)
     for _ in range(warmup):
         batch.Finish(batch.Submit(refs), raw=True)
     stages = []
@@ -401,6 +402,8 @@ def main() -> int:
                     help="SSIM' map form (default: the library's, the reference order for FP32 planes)")
     ap.add_argument("--strip-rows", type=int, default=0,
                     help="output rows per scale work item of the batch (0 = the library default)")
+    ap.add_argument("--strip-cols", type=int, default=0, choices=[0, 64, 128],
+                    help="output columns per scale work item of the batch (0 = the library default)")
     ap.add_argument("--chunks", type=int, default=1,
                     help="runs of pairs per batch whose scale overlaps the next run's prepare")
     ap.add_argument("--persistent", action="store_true",
@@ -461,7 +464,8 @@ def main() -> int:
         dst_buf.free()
 
     batch = Ssimulacra2Batch(max_pairs=len(refs), plane_format=fmt, strip_rows=args.strip_rows,
-                             chunks=args.chunks, persistent=args.persistent, map_form=map_form)
+                             chunks=args.chunks, persistent=args.persistent, map_form=map_form,
+                             strip_cols=args.strip_cols)
     for _ in range(args.warmup):
         scales, norms = batch.Finish(batch.Submit(refs), raw=True)
     step_ms, stages = [], []
@@ -527,7 +531,7 @@ def main() -> int:
                                f"(3 strengths x {args.pairs_per_strength}), resident in HBM, "
                                f"{args.planes} planes, {form} map form, one batched launch per kernel",
                    "pairs_per_step": len(refs), "width": w, "height": h, "map_form": form,
-                   "parallelism": "single GPU"},
+                   "strip_cols": args.strip_cols or "library default", "parallelism": "single GPU"},
         "roofline": roofline,
         "vs_baseline_note": "value / 3836.7 pairs/s (vqm 1080p resident, 3 workers, RTX 5070, BASELINE.md)",
     }
@@ -540,6 +544,14 @@ def main() -> int:
         out["map_forms"] = {map_name(fmt, other): resident_batch(refs, w, h, fmt, sec_steps, 1, args.chunks,
                                                                  persistent=args.persistent, map_form=other)}
         log(f"map forms: {out['map_forms']}")
+        # Both scale strip widths (one or two output columns per lane) in
+        # both map forms, at the batch's default strip height.
+        out["strip_widths"] = {
+            f"{map_name(fmt, m)}_{cols}": resident_batch(refs, w, h, fmt, sec_steps, 1, args.chunks,
+                                                         persistent=args.persistent, map_form=m,
+                                                         strip_cols=cols)
+            for m in (1, 2) for cols in (64, 128)}
+        log(f"strip widths: {out['strip_widths']}")
         forms = {}
         for name, f in sorted(PLANES.items()):
             if f != fmt and name in ("f32", "u16"):

@@ -264,10 +264,11 @@ struct Ssimulacra2Config {
     MemoryPolicy memory_policy = MemoryPolicy::Automatic;
     // No reference fields below.
     PlaneFormat plane_format = PlaneFormat::F32;  // overridden to F16 by half_planes
-    std::uint32_t strip_rows = 0;
+    std::uint32_t strip_rows = 0;         // scale work-item height, a multiple of 8 (0 = 64)
     bool graph = false;  // replay one captured HIP graph per pair
     Blur blur = Blur::Sampled;
     MapForm map_form = MapForm::Default;
+    std::uint32_t strip_cols = 0;         // scale work-item width, 64 or 128 (0 = 128)
 
     [[nodiscard]] vqm_ss2_config ToC() const {
         if (memory_policy == MemoryPolicy::Staged)
@@ -286,6 +287,7 @@ struct Ssimulacra2Config {
         c.chroma_filter = static_cast<std::uint32_t>(chroma_filter);
         c.blur = static_cast<std::uint32_t>(blur);
         c.map_form = static_cast<std::uint32_t>(map_form);
+        c.strip_cols = strip_cols;
         return c;
     }
 };

@@ -159,7 +159,8 @@ typedef struct vqm_ss2_config {
     uint32_t resident_images;   /* gpu_resident_images, default 2 */
     uint32_t plane_format;      /* VQM_PLANES_*; half_planes = VQM_PLANES_F16 */
     uint32_t streaming_inputs;  /* default 1 */
-    uint32_t strip_rows;        /* output rows per scale work item, 0 = default */
+    uint32_t strip_rows;        /* output rows per scale work item, a multiple
+                                   of 8 up to 1024; 0 = default (64) */
     uint64_t input_bytes;       /* caller-owned image pool (AllocateImage), default 0 */
     uint32_t graph;             /* 1: each pair replays one HIP graph of the
                                    descriptor upload, the three kernels and the
@@ -175,6 +176,9 @@ typedef struct vqm_ss2_config {
                                    recursive Gaussian. No reference field. */
     uint32_t map_form;          /* VQM_MAP_*, default VQM_MAP_DEFAULT. No
                                    reference field. */
+    uint32_t strip_cols;        /* output columns per scale work item: 64 or
+                                   128; 0 = default (128). No reference
+                                   field. */
 } vqm_ss2_config;
 
 /* PipelineMemoryUsage, include/vqm_memory_types.h:29-38: device bytes the
@@ -387,20 +391,23 @@ typedef struct vqm_ss2_batch vqm_ss2_batch;
 
 typedef struct vqm_ss2_batch_config {
     uint32_t plane_format;   /* VQM_PLANES_* */
-    uint32_t strip_rows;     /* 0 = default (128; pipelines and servers: 64) */
+    uint32_t strip_rows;     /* multiple of 8; 0 = default (128; pipelines and
+                                servers: 64) */
     uint32_t max_pairs;      /* pairs per submission */
     uint32_t chunks;         /* 0 or 1: prepare, then scale, on one stream; 2..16:
                                 runs of consecutive pairs whose scale launches
                                 (second stream) overlap the next run's prepare */
     uint64_t hbm_budget;     /* device bytes the workspace pool may hold, 0 = unlimited */
-    uint32_t autotune;       /* 1: strip_rows and chunks come from vqm_ss2_autotune
-                                for each geometry the first time it is seen */
+    uint32_t autotune;       /* 1: strip_rows, strip_cols and chunks come from
+                                vqm_ss2_autotune for each geometry the first
+                                time it is seen */
     uint32_t blur;           /* VQM_BLUR_SAMPLED or VQM_BLUR_RECURSIVE */
     uint32_t persistent;     /* 1: each kernel launches only the blocks that stay
                                 resident at once (occupancy x compute units), each
                                 looping over the batch's tiles and strips in a
                                 fixed stride; 0: one block per tile or strip */
     uint32_t map_form;       /* VQM_MAP_* */
+    uint32_t strip_cols;     /* 64 or 128; 0 = default (128) */
 } vqm_ss2_batch_config;
 
 /* One pair of caller-owned device images (packed u8 sRGB RGB). */
@@ -431,6 +438,14 @@ int vqm_ss2_batch_stage_times(const vqm_ss2_batch* batch, float ms[3]);
 /* Device bytes held by the batch's workspace pool. */
 int vqm_ss2_batch_pool_bytes(const vqm_ss2_batch* batch, uint64_t* bytes);
 
+/* Fixes the scale strip shape (rows: a multiple of 8, 0 = the batch's
+ * default; cols: 64 or 128, 0 = the batch's default) of one geometry before
+ * its first submission, for example a shape vqm_ss2_autotune chose in an
+ * earlier process. LOGIC_ERROR once the geometry has been submitted. One
+ * launch may mix strip widths (one scale kernel launch per width). */
+int vqm_ss2_batch_set_shape(vqm_ss2_batch* batch, uint32_t width, uint32_t height, uint32_t strip_rows,
+                            uint32_t strip_cols);
+
 /* ---- launch-shape autotuner (no reference counterpart) ------------------ */
 
 typedef struct vqm_ss2_tune_result {
@@ -439,14 +454,20 @@ typedef struct vqm_ss2_tune_result {
     uint32_t candidates;     /* shapes timed */
     uint32_t cached;         /* 1 when the choice came from the process cache */
     float ms_per_launch;     /* measured time of the chosen shape */
-    float default_ms;        /* measured time of 64 rows, 1 chunk */
+    float default_ms;        /* measured time of 64 rows, 128 columns, 1 chunk */
+    uint32_t strip_cols;     /* scale work-item width chosen */
 } vqm_ss2_tune_result;
 
-/* Times every (strip height 16/32/64/128) x (chunks 1/2/4/8) shape for a
- * launch of `pairs` pairs of this geometry and format (rounded up to a power
- * of two, at most 512) on the device and returns the fastest; the choice is
- * cached for the process. Shapes do not change results, only time. Holds
- * up to 16 GiB of temporary device memory while it runs. */
+/* Times every (strip height 16/32/64/128/256) x (strip width 64/128) x
+ * (chunks 1/2/4/8) shape for a launch of `pairs` pairs of this geometry and
+ * format (rounded up to a power of two, at most 512) on the device and
+ * returns the fastest; the choice is cached for the process. The strip width
+ * sets the scale kernel's registers per lane and so its waves per SIMD (two
+ * columns per lane: 2 waves in the reference form, 3 in the difference
+ * form; one column: more). Shapes change only the order in which the FP64
+ * partial sums of the strips are added (every pixel's map value and every
+ * FP32 block sum is the same), so results agree to about 1e-15 relative.
+ * Holds up to 16 GiB of temporary device memory while it runs. */
 int vqm_ss2_autotune(uint32_t width, uint32_t height, uint32_t plane_format, uint32_t map_form,
                      uint32_t pairs, vqm_ss2_tune_result* result);
 
@@ -469,11 +490,12 @@ typedef struct vqm_ss2_server_config {
     uint32_t plane_format;   /* VQM_PLANES_* */
     uint32_t lanes;          /* batches in flight at once, 1..16; 0 = 3 */
     uint32_t max_batch;      /* requests per launch, 1..4096; 0 = 64 */
-    uint32_t strip_rows;     /* 0 = default */
+    uint32_t strip_rows;     /* multiple of 8; 0 = default */
     uint64_t hbm_budget;     /* device bytes for workspaces and host-input
                                 staging; 0 = unlimited */
     uint32_t blur;           /* VQM_BLUR_SAMPLED (default) or VQM_BLUR_RECURSIVE */
     uint32_t map_form;       /* VQM_MAP_* */
+    uint32_t strip_cols;     /* 64 or 128; 0 = default (128) */
 } vqm_ss2_server_config;
 
 /* The outcome of one request. Times are vqm_now_ns() readings. */

@@ -20,21 +20,21 @@ LIB_PATH = os.path.join(HERE, "libss2_oracle.so")
 class Mode(ctypes.Structure):
     _fields_ = [("arith", ctypes.c_int32), ("blur_order", ctypes.c_int32),
                 ("cbrt_kind", ctypes.c_int32), ("plane_format", ctypes.c_int32),
-                ("plane_round_levels", ctypes.c_uint32), ("sum_quads", ctypes.c_int32),
+                ("plane_round_levels", ctypes.c_uint32), ("sum_blocks", ctypes.c_int32),
                 ("round_linear", ctypes.c_int32), ("blur_kind", ctypes.c_int32),
                 ("sum_variances", ctypes.c_int32)]
 
 
 # The three forms (ss2_oracle.h).
 F64 = Mode(0, 0, 0, 0, 0, 0)
-F32_REF = Mode(1, 0, 0, 0, 0, 0)
-F32_DEV = Mode(1, 0, 1, 0, 0, 0, 0, 0, 1)
+F32_REF = Mode(1, 0, 0, 0, 0, 1)
+F32_DEV = Mode(1, 0, 1, 0, 0, 1, 0, 0, 1)
 
 
 def f32_dev_planes(plane_format: int, levels_mask: int = 0x3F) -> Mode:
     """The device form with the stored planes of the levels in `levels_mask`
     rounded to FP16 (1), BF16 (2) or unsigned 16-bit fixed point (3)."""
-    return Mode(1, 0, 1, plane_format, levels_mask, 0, 0, 0, 1)
+    return Mode(1, 0, 1, plane_format, levels_mask, 1, 0, 0, 1)
 
 
 def f32_dev_for(plane_format: int) -> Mode:
@@ -64,7 +64,7 @@ def form_for(plane_format: int = 0, map_form: int = MAP_DEFAULT, blur: int = 0) 
     if map_form == MAP_DIFFERENCE:
         mode = Mode.from_buffer_copy(f32_dev_for(plane_format))
     else:
-        mode = Mode(1, 0, 0, plane_format, 0x3F if plane_format else 0, 0,
+        mode = Mode(1, 0, 0, plane_format, 0x3F if plane_format else 0, 1,
                     1 if plane_format == 1 else 0, 0, 0)
     mode.blur_kind = blur
     return mode

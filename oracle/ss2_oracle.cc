@@ -399,7 +399,7 @@ struct Pyramid {
 // The six map sums of one (level, channel), scale.comp:305-333.
 template <typename T>
 void map_sums(const T* a, const T* b, uint32_t w, uint32_t h, int order,
-              int sum_quads, int sum_variances, const T* wt, double out[6]) {
+              int sum_blocks, int sum_variances, const T* wt, double out[6]) {
     const size_t n = static_cast<size_t>(w) * h;
     std::vector<T> q(n), mu1(n), mu2(n), s11(n), s22(n), s12(n);
     blur(a, mu1.data(), w, h, order, wt);
@@ -427,10 +427,11 @@ void map_sums(const T* a, const T* b, uint32_t w, uint32_t h, int order,
     const T c2 = static_cast<T>(kC2);
     const T one = T(1), two = T(2), zero = T(0);
     for (int k = 0; k < 6; ++k) out[k] = 0.0;
+    // sum_blocks: FP32 sums over blocks of one column by eight rows (rows
+    // 8k .. 8k+7), in row order, then the block sums in FP64 -- the order of
+    // the HIP kernels.
+    std::vector<T> block(sum_blocks ? static_cast<size_t>(w) * 6 : 0, T(0));
     for (uint32_t y = 0; y < h; ++y) {
-        T quad[6] = {0, 0, 0, 0, 0, 0};
-        T pair[6] = {0, 0, 0, 0, 0, 0};
-        int in_quad = 0;
         for (uint32_t x = 0; x < w; ++x) {
             const size_t i = static_cast<size_t>(y) * w + x;
             const T m1 = mu1[i], m2 = mu2[i];
@@ -458,24 +459,19 @@ void map_sums(const T* a, const T* b, uint32_t w, uint32_t h, int order,
             const T blurv = std::max(-d1, zero);
             const T rr = ring * ring, bb = blurv * blurv;
             const T v[6] = {d, dd * dd, ring, rr * rr, blurv, bb * bb};
-            if (sum_quads) {
-                for (int k = 0; k < 6; ++k) quad[k] = quad[k] + v[k];
-                if (++in_quad == 4 || x + 1 == w) {
-                    for (int k = 0; k < 6; ++k) {
-                        out[k] += static_cast<double>(quad[k]);
-                        quad[k] = 0;
-                    }
-                    in_quad = 0;
-                }
-            } else if (sum_variances) {
-                // The device form adds columns 2j and 2j+1 (one lane's two
-                // columns) in FP32, then accumulates in FP64.
-                for (int k = 0; k < 6; ++k) pair[k] = (x & 1u) == 0 ? v[k] : pair[k] + v[k];
-                if ((x & 1u) == 1 || x + 1 == w)
-                    for (int k = 0; k < 6; ++k) out[k] += static_cast<double>(pair[k]);
+            if (sum_blocks) {
+                T* blk = block.data() + static_cast<size_t>(x) * 6;
+                for (int k = 0; k < 6; ++k) blk[k] = blk[k] + v[k];
             } else {
                 for (int k = 0; k < 6; ++k) out[k] += static_cast<double>(v[k]);
             }
+        }
+        if (sum_blocks && ((y & 7u) == 7u || y + 1 == h)) {
+            for (size_t j = 0; j < block.size(); j += 6)
+                for (int k = 0; k < 6; ++k) {
+                    out[k] += static_cast<double>(block[j + k]);
+                    block[j + k] = T(0);
+                }
         }
     }
 }
@@ -635,7 +631,7 @@ int32_t compute(const ss2o_mode& mode, uint32_t width, uint32_t height,
             double s[6];
             map_sums<T>(xyb[0][l][c].data(), xyb[1][l][c].data(), w, h,
                         sizeof(T) == 8 ? 0 : mode.blur_order,
-                        sizeof(T) == 8 ? 0 : mode.sum_quads,
+                        sizeof(T) == 8 ? 0 : mode.sum_blocks,
                         sizeof(T) == 8 ? 0 : mode.sum_variances, wt, s);
             // vulkan.cc:545-583: mean in double, fourth root of the 4-norms.
             double* out = norms + l * 18;

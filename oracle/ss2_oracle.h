@@ -16,10 +16,13 @@
  *   FP32 dev  the HIP kernels' FP32 operations: the reference's blur order
  *             (horizontal pass first), the kernels' own reproducible cube
  *             root, (a - b)^2 blurred in place of a^2 and b^2 (the SSIM'
- *             map needs only s11 + s22 = blur((a - b)^2) + 2 s12), per-pixel map values
- *             summed in FP64, optional FP16/BF16/fixed-point rounding of
- *             stored planes. The HIP path matches this form per pixel bit
- *             for bit; only the order of the FP64 sums differs.
+ *             map needs only s11 + s22 = blur((a - b)^2) + 2 s12), optional
+ *             FP16/BF16/fixed-point rounding of stored planes. The HIP path
+ *             matches this form per pixel bit for bit.
+ *
+ * Both FP32 forms sum each map in FP32 over 1 x 8 pixel blocks, then in FP64
+ * (sum_blocks), as the HIP kernels do: the kernels then match every FP32
+ * block sum, and only the order of the FP64 sums differs.
  *
  * Parity pins (checked in tests/test_oracle.py):
  *   - end to end: the reference's 32x32 goldens 93.17505696 / 92.12553218
@@ -47,7 +50,10 @@ typedef struct ss2o_mode {
     int32_t cbrt_kind;    /* 0 = libm cbrt (correctly rounded to FP32), 1 = device Newton */
     int32_t plane_format; /* FP32 arith only: 0 = FP32, 1 = FP16, 2 = BF16, 3 = unorm16 stored planes */
     uint32_t plane_round_levels; /* levels whose stored planes use plane_format */
-    int32_t sum_quads;    /* 1 = FP32 sums of 4 adjacent pixels, then FP64; 0 = FP64 per pixel */
+    int32_t sum_blocks;   /* FP32 only: 1 = each map's values summed in FP32 over blocks of
+                             one column by eight rows (rows 8k .. 8k+7, in row order), then
+                             the block sums in FP64 (the HIP kernels' order); 0 = FP64 per
+                             pixel */
     int32_t round_linear; /* 1 = the linear box averages of levels >= 1 are rounded to
                              plane_format as well, as the reference's half-plane build
                              does (prepare.comp:100-111, ssimu2_common.glsl:84-96) */

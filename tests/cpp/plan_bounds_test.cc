@@ -1,7 +1,8 @@
 // Host-only check of the work plans the kernels consume (vqm_amd/csrc/
-// ss2_host.cc): for every geometry and strip height, every address the scale
-// and prepare kernels compute lies inside its plane, planes do not overlap,
-// and the work items cover each (level, channel) exactly once.
+// ss2_host.cc): for every geometry, strip height and strip width, every
+// address the scale and prepare kernels compute lies inside its plane,
+// planes do not overlap, the work items cover each (level, channel) exactly
+// once, and strips start on the map-sum block rows.
 //
 // Built and run by tests/test_plan_bounds.py (g++, no GPU).
 
@@ -28,10 +29,12 @@ static int failures = 0;
         }                                                      \
     } while (0)
 
-static void check(uint32_t width, uint32_t height, PlaneFormat fmt, uint32_t strip_rows) {
-    const GeometryPlan g = BuildPlan(width, height, fmt, strip_rows);
+static void check(uint32_t width, uint32_t height, PlaneFormat fmt, uint32_t strip_rows, uint32_t strip_cols) {
+    const GeometryPlan g = BuildPlan(width, height, fmt, strip_rows, strip_cols);
     const Plan& p = g.plan;
-    const uint32_t sh = p.strip_h;
+    const uint32_t sh = p.strip_h, sw = p.strip_w;
+    EXPECT(sw == (strip_cols ? strip_cols : 128u), "strip width %u", sw);
+    EXPECT(sh % kSumBlockRows == 0, "strip height %u", sh);
     // Planes: aligned, disjoint, inside image_bytes (offsets in bytes, the
     // element size per level).
     uint64_t prev_end = 0;
@@ -58,8 +61,9 @@ static void check(uint32_t width, uint32_t height, PlaneFormat fmt, uint32_t str
     EXPECT(p.image_bytes % 256 == 0, "image planes not 256-byte aligned");
     EXPECT(g.plane_bytes_per_image == p.image_bytes, "plane bytes");
 
-    // Scale items: reads of rows y0 .. y0+sh+11 and columns x0+2 .. x0+141 of
-    // the padded plane; the map covers each output pixel once.
+    // Scale items: reads of rows y0 .. y0+sh+11 and columns x0+2 .. x0+sw+13
+    // of the padded plane; the map covers each output pixel once; strips
+    // start on even columns and on rows that are multiples of 8.
     std::vector<std::vector<std::vector<uint8_t>>> cover(p.levels);
     for (uint32_t l = 0; l < p.levels; ++l) cover[l].assign(3, std::vector<uint8_t>(static_cast<size_t>(p.w[l]) * p.h[l], 0));
     for (uint32_t i = 0; i < p.items; ++i) {
@@ -70,11 +74,12 @@ static void check(uint32_t width, uint32_t height, PlaneFormat fmt, uint32_t str
                    i < p.item_begin[it.level][it.channel] + p.item_count[it.level][it.channel],
                "item outside its range");
         const uint64_t last_row = it.y0 + sh + 2 * kRadius - 1;
-        const uint64_t last_col = it.x0 + 2 + kSrcCols - 1;
+        const uint64_t last_col = it.x0 + 2 + sw + 2 * kRadius - 1;
         EXPECT(last_row < p.rows[it.level], "scale reads past the last row");
         EXPECT(last_col < p.pitch[it.level], "scale reads past the pitch");
+        EXPECT(it.x0 % 2 == 0 && it.y0 % kSumBlockRows == 0, "strip origin off the block grid");
         for (uint32_t y = it.y0; y < it.y0 + sh && y < p.h[it.level]; ++y)
-            for (uint32_t x = it.x0; x < it.x0 + kStripCols && x < p.w[it.level]; ++x)
+            for (uint32_t x = it.x0; x < it.x0 + sw && x < p.w[it.level]; ++x)
                 cover[it.level][it.channel][static_cast<size_t>(y) * p.w[it.level] + x]++;
     }
     for (uint32_t l = 0; l < p.levels; ++l)
@@ -106,7 +111,8 @@ int main() {
         for (uint32_t h = 8; h < 80; h += 9) sizes.push_back({w, h});
     for (auto [w, h] : sizes)
         for (PlaneFormat f : fmts)
-            for (uint32_t sh : {0u, 4u, 16u, 64u, 100u, 128u}) check(w, h, f, sh);
+            for (uint32_t sh : {0u, 8u, 16u, 64u, 104u, 128u, 256u})
+                for (uint32_t sw : {0u, 64u, 128u}) check(w, h, f, sh, sw);
     // Invalid geometry and strip heights are rejected.
     bool threw = false;
     try {
@@ -122,6 +128,20 @@ int main() {
         threw = e.status == VQM_INVALID_ARGUMENT;
     }
     EXPECT(threw, "strip_rows 6 accepted");
+    threw = false;
+    try {
+        BuildPlan(64, 64, kPlaneF32, 4);
+    } catch (const Error& e) {
+        threw = e.status == VQM_INVALID_ARGUMENT;
+    }
+    EXPECT(threw, "strip_rows 4 accepted");
+    threw = false;
+    try {
+        BuildPlan(64, 64, kPlaneF32, 64, 96);
+    } catch (const Error& e) {
+        threw = e.status == VQM_INVALID_ARGUMENT;
+    }
+    EXPECT(threw, "strip_cols 96 accepted");
     if (failures) {
         std::fprintf(stderr, "%d failures\n", failures);
         return 1;

@@ -27,8 +27,9 @@ def upload_pairs(shapes, seed=0):
     return pairs, keep, host
 
 
-def alone(w, h, a, b, fmt=0):
-    p = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(w, h), plane_format=fmt))
+def alone(w, h, a, b, fmt=0, strip_rows=0, strip_cols=0):
+    p = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(w, h), plane_format=fmt, strip_rows=strip_rows,
+                                              strip_cols=strip_cols))
     r = p.Compute(Image(Size(w, h), ImageFormat.srgb8(), a), Image(Size(w, h), ImageFormat.srgb8(), b))
     p.close()
     return r.norms()
@@ -104,15 +105,18 @@ def test_chunked_batch_equals_each_pair_alone(chunks):
 
 
 def test_autotune_picks_a_shape_and_keeps_results():
-    """The tuner times every (strip height, chunks) shape and caches the
-    fastest; a tuned shape keeps every pixel's arithmetic, so the norms agree
-    to the order of the FP64 sums over strips (relative 1e-10)."""
+    """The tuner times every (strip height, strip width, chunks) shape and
+    caches the fastest; a tuned shape keeps every pixel's arithmetic and FP32
+    block sum, so the norms agree to the order of the FP64 sums over strips
+    (relative 1e-10)."""
     first = vqm_amd.autotune(Size(320, 180), vqm_amd.PLANES_F32, pairs=6)
-    assert first["strip_rows"] in (16, 32, 64, 128) and first["chunks"] in (1, 2, 4, 8)
-    assert first["candidates"] == 4 * 4 and not first["cached"]
+    assert first["strip_rows"] in (16, 32, 64, 128, 256) and first["chunks"] in (1, 2, 4, 8)
+    assert first["strip_cols"] in (64, 128)
+    assert first["candidates"] == 5 * 2 * 4 and not first["cached"]
     assert 0.0 < first["ms_per_launch"] <= first["default_ms"] * 1.0001
     again = vqm_amd.autotune(Size(320, 180), vqm_amd.PLANES_F32, pairs=5)  # same size class
     assert again["cached"] and again["strip_rows"] == first["strip_rows"]
+    assert again["strip_cols"] == first["strip_cols"]
     shapes = [(320, 180)] * 6 + [(64, 48), (200, 120)]
     pairs, keep, host = upload_pairs(shapes, seed=61)
     plain = Ssimulacra2Batch(max_pairs=8)
@@ -122,6 +126,30 @@ def test_autotune_picks_a_shape_and_keeps_results():
     np.testing.assert_allclose(b, a, rtol=1e-10, atol=0)
     with pytest.raises(InvalidArgument):
         vqm_amd.autotune(Size(4, 4))
+
+
+@pytest.mark.parametrize("chunks", [1, 2])
+def test_mixed_strip_widths_in_one_launch(chunks):
+    """Geometries pinned to different strip shapes share one submission (one
+    scale launch per strip width); each pair equals a pipeline of its shape
+    bit for bit. A used geometry's shape cannot change."""
+    shapes = [(320, 180), (200, 130), (320, 180), (64, 48), (200, 130)]
+    pinned = {(320, 180): (32, 64), (200, 130): (64, 128), (64, 48): (16, 64)}
+    pairs, keep, host = upload_pairs(shapes, seed=71)
+    batch = Ssimulacra2Batch(max_pairs=8, chunks=chunks)
+    for (w, h), (rows, cols) in pinned.items():
+        batch.SetShape(Size(w, h), rows, cols)
+    _, norms = batch.Finish(batch.Submit(pairs), raw=True)
+    for i, ((w, h), (a, b)) in enumerate(zip(shapes, host)):
+        rows, cols = pinned[(w, h)]
+        assert np.array_equal(norms[i], alone(w, h, a, b, strip_rows=rows, strip_cols=cols)), f"pair {i}"
+    with pytest.raises(LogicError):
+        batch.SetShape(Size(320, 180), 64, 128)
+    with pytest.raises(InvalidArgument):
+        batch.SetShape(Size(100, 100), 12, 128)
+    with pytest.raises(InvalidArgument):
+        batch.SetShape(Size(100, 100), 64, 32)
+    batch.close()
 
 
 @pytest.mark.parametrize("fmt", [vqm_amd.PLANES_F32, vqm_amd.PLANES_U16])

@@ -204,10 +204,24 @@ def test_mixed_form_keeps_the_fp32_contract_at_1080p(oracle_lib, strength):
 
 
 @pytest.mark.parametrize("map_form", MAPS)
-@pytest.mark.parametrize("strip_rows", [4, 16, 128])
-def test_strip_height_does_not_change_results(oracle_lib, strip_rows, map_form):
+@pytest.mark.parametrize("strip_cols", [64, 128])
+@pytest.mark.parametrize("strip_rows", [8, 16, 128])
+def test_strip_shape_does_not_change_results(oracle_lib, strip_rows, strip_cols, map_form):
+    """Every strip shape (height, and width: one or two output columns per
+    lane) is bit-level with the oracle form."""
     a, b = random_pair(300, 200, seed=9)
-    check_against_oracle(oracle_lib, 300, 200, a, b, strip_rows=strip_rows, map_form=map_form)
+    check_against_oracle(oracle_lib, 300, 200, a, b, strip_rows=strip_rows, strip_cols=strip_cols,
+                         map_form=map_form)
+
+
+@pytest.mark.parametrize("map_form", MAPS)
+def test_narrow_strips_at_1080p(oracle_lib, map_form):
+    """64-column strips (one output column per lane) at 1920x1080, FP32 and
+    fixed-point planes: bit-level with the oracle form."""
+    a, b = synth.pair(0, 21, 1920, 1080, synth.STRENGTHS[1])
+    check_against_oracle(oracle_lib, 1920, 1080, a, b, strip_cols=64, map_form=map_form)
+    check_against_oracle(oracle_lib, 1920, 1080, a, b, strip_cols=64, map_form=map_form,
+                         plane_format=vqm_amd.PLANES_U16)
 
 
 def test_repeatable_and_interleaved():

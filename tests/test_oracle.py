@@ -319,6 +319,29 @@ def test_fp32_forms_within_contract_of_fp64(oracle_lib, strength):
         assert_features_close(f32, f64, what="fp32 vs fp64")
 
 
+@pytest.mark.parametrize("form", ["F32_REF", "F32_DEV"])
+def test_block_sums_within_the_fp32_bound(oracle_lib, form):
+    """The FP32 block sums (one column by eight rows) against the same form
+    summed in FP64 per pixel: every map value is >= 0, so an FP32 sum of at
+    most 8 of them is within 7 u (u = 2^-24) of its exact sum, and so is
+    every level sum; the 4-norms (fourth roots) within a quarter of that.
+    The blocks must also change the result (the sums really are FP32). The
+    block shape itself is a choice of this path, not of the reference; the
+    GPU parity tests hold the kernels to it bit for bit."""
+    from vqm_amd import synth
+    a, b = synth.pair(0, 4, 300, 200, (12, 3.0))
+    mode = getattr(oracle_lib, form)
+    per_pixel = oracle_lib.Mode.from_buffer_copy(mode)
+    per_pixel.sum_blocks = 0
+    n, blocks, _ = oracle_lib.compute(mode, 300, 200, a, b)
+    _, exact, _ = oracle_lib.compute(per_pixel, 300, 200, a, b)
+    assert not np.array_equal(blocks, exact)
+    bound = 7 * 2.0 ** -24 * 1.001
+    for k in range(18 * n):
+        rel = bound / 4 if k % 18 in (1, 3, 5, 7, 9, 11, 13, 15, 17) else bound
+        assert abs(blocks[k] - exact[k]) <= rel * abs(exact[k]) + 1e-300, (k, blocks[k], exact[k])
+
+
 def _bf16_reference(v: np.ndarray) -> np.ndarray:
     """Round float32 values to the nearest bfloat16 (ties to even) through
     float64: keep 8 significant bits of the float32 significand."""

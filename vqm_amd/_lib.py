@@ -55,7 +55,7 @@ class ConfigC(ctypes.Structure):
     _fields_ = [("width", u32), ("height", u32), ("resident_images", u32),
                 ("plane_format", u32), ("streaming_inputs", u32), ("strip_rows", u32),
                 ("input_bytes", u64), ("graph", u32), ("chroma_filter", u32), ("blur", u32),
-                ("map_form", u32)]
+                ("map_form", u32), ("strip_cols", u32)]
 
 
 class MemoryC(ctypes.Structure):
@@ -72,12 +72,12 @@ class MemoryHeapC(ctypes.Structure):
 class BatchConfigC(ctypes.Structure):
     _fields_ = [("plane_format", u32), ("strip_rows", u32), ("max_pairs", u32),
                 ("chunks", u32), ("hbm_budget", u64), ("autotune", u32), ("blur", u32),
-                ("persistent", u32), ("map_form", u32)]
+                ("persistent", u32), ("map_form", u32), ("strip_cols", u32)]
 
 
 class TuneResultC(ctypes.Structure):
     _fields_ = [("strip_rows", u32), ("chunks", u32), ("candidates", u32), ("cached", u32),
-                ("ms_per_launch", f32), ("default_ms", f32)]
+                ("ms_per_launch", f32), ("default_ms", f32), ("strip_cols", u32)]
 
 
 class DevicePairC(ctypes.Structure):
@@ -86,7 +86,7 @@ class DevicePairC(ctypes.Structure):
 
 class ServerConfigC(ctypes.Structure):
     _fields_ = [("plane_format", u32), ("lanes", u32), ("max_batch", u32), ("strip_rows", u32),
-                ("hbm_budget", u64), ("blur", u32), ("map_form", u32)]
+                ("hbm_budget", u64), ("blur", u32), ("map_form", u32), ("strip_cols", u32)]
 
 
 class RequestResultC(ctypes.Structure):
@@ -156,6 +156,7 @@ SIGNATURES = {
     "vqm_ss2_batch_stage_times": (ctypes.c_int, [vp, ctypes.POINTER(f32)]),
     "vqm_ss2_batch_pool_bytes": (ctypes.c_int, [vp, ctypes.POINTER(u64)]),
     "vqm_memory_heaps": (ctypes.c_int, [ctypes.POINTER(MemoryHeapC)]),
+    "vqm_ss2_batch_set_shape": (ctypes.c_int, [vp, u32, u32, u32, u32]),
     "vqm_ss2_autotune": (ctypes.c_int, [u32, u32, u32, u32, u32, ctypes.POINTER(TuneResultC)]),
     "vqm_ss2_score_device_noalloc": (ctypes.c_int, [ctypes.POINTER(ConfigC), ctypes.POINTER(DeviceImageC),
                                                     ctypes.POINTER(DeviceImageC), vp, vp,

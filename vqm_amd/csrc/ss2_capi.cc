@@ -217,7 +217,7 @@ int vqm_ss2_memory_plan(const vqm_ss2_config* config, vqm_ss2_memory* plan) {
         if (config->plane_format > 3) vqm::ss2::Fail(VQM_INVALID_ARGUMENT, "invalid plane_format");
         const auto g = vqm::ss2::BuildPlan(config->width, config->height,
                                            static_cast<vqm::ss2::PlaneFormat>(config->plane_format),
-                                           config->strip_rows);
+                                           config->strip_rows, config->strip_cols);
         const auto layout = vqm::ss2::LayoutWorkspace(g);
         vqm_image_format f;
         vqm_image_format_srgb8(&f);
@@ -457,6 +457,14 @@ int vqm_ss2_batch_pool_bytes(const vqm_ss2_batch* batch, uint64_t* bytes) {
     });
 }
 
+int vqm_ss2_batch_set_shape(vqm_ss2_batch* batch, uint32_t width, uint32_t height, uint32_t strip_rows,
+                            uint32_t strip_cols) {
+    return Guard([&] {
+        Need(batch, "batch");
+        batch->impl.SetShape(width, height, strip_rows, strip_cols);
+    });
+}
+
 int vqm_ss2_autotune(uint32_t width, uint32_t height, uint32_t plane_format, uint32_t map_form,
                      uint32_t pairs, vqm_ss2_tune_result* result) {
     return Guard([&] {
@@ -465,6 +473,7 @@ int vqm_ss2_autotune(uint32_t width, uint32_t height, uint32_t plane_format, uin
         const vqm::ss2::TuneChoice c = vqm::ss2::Tuner::Get().Choose(
             width, height, format, vqm::ss2::ToMapForm(map_form, format), pairs);
         result->strip_rows = c.strip_rows;
+        result->strip_cols = c.strip_cols;
         result->chunks = c.chunks;
         result->candidates = c.candidates;
         result->cached = c.cached ? 1u : 0u;

@@ -25,10 +25,14 @@ inline constexpr int kNorms = 108;          // 18 per level
 inline constexpr int kRadius = 6;           // ceil(3.5 * 1.5), pipeline.cc:276
 inline constexpr int kTaps = 13;
 
-// Scale kernel geometry: one 64-lane wave per work item, two output columns
-// per lane.
+// Scale kernel geometry: one 64-lane wave per work item, one or two output
+// columns per lane (strips of 64 or 128 columns, Plan::strip_w). Planes are
+// padded for the widest strip.
 inline constexpr int kStripCols = 128;
-inline constexpr int kSrcCols = kStripCols + 2 * kRadius;  // 140
+inline constexpr uint32_t kDefaultStripCols = 128;
+constexpr bool ValidStripCols(uint32_t cols) { return cols == 64 || cols == 128; }
+// Strip heights are multiples of the map-sum block height (8 rows).
+inline constexpr uint32_t kSumBlockRows = 8;
 // Plane padding: 6 zero rows above, 8 zero columns on the left (so rows of
 // four samples stay 16-byte aligned), and enough on the right and below to
 // cover the last strip's halo.
@@ -59,6 +63,7 @@ struct Plan {
     uint32_t height;
     uint32_t levels;  // 1..6
     uint32_t strip_h; // output rows per scale work item
+    uint32_t strip_w; // output columns per scale work item: 64 or 128
     uint32_t w[kMaxScales], h[kMaxScales];
     uint32_t pitch[kMaxScales], rows[kMaxScales];  // padded plane geometry
     uint32_t mask[kMaxScales];                      // active channels, bit c
@@ -147,16 +152,16 @@ struct SourceDesc {
 
 // Scale work item i of a plan: the (level, channel) ranges follow one
 // another in level-major order, and within a range the items run row-major
-// over strip_h-row bands of 128-column strips. constexpr, so host code and
-// device code share it.
+// over strip_h-row bands of strip_w-column strips. constexpr, so host code
+// and device code share it.
 constexpr ScaleItem ItemAt(const Plan& p, uint32_t i) {
     for (uint32_t l = 0; l < p.levels; ++l) {
         for (uint32_t c = 0; c < static_cast<uint32_t>(kChannels); ++c) {
             const uint32_t begin = p.item_begin[l][c], count = p.item_count[l][c];
             if (i >= begin && i - begin < count) {
                 const uint32_t k = i - begin;
-                const uint32_t strips = (p.w[l] + kStripCols - 1) / kStripCols;
-                return ScaleItem{l, c, (k % strips) * kStripCols, (k / strips) * p.strip_h};
+                const uint32_t strips = (p.w[l] + p.strip_w - 1) / p.strip_w;
+                return ScaleItem{l, c, (k % strips) * p.strip_w, (k / strips) * p.strip_h};
             }
         }
     }

@@ -222,11 +222,13 @@ void PackView(const vqm_host_view& view, uint8_t* dst) {
     }
 }
 
-Plan FillPlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows) {
+Plan FillPlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows, uint32_t strip_cols) {
     const ScalePlan scales = PlanScales(width, height);
     if (strip_rows == 0) strip_rows = 64;
-    if (strip_rows % 4 != 0 || strip_rows > 1024)
-        Fail(VQM_INVALID_ARGUMENT, "strip_rows must be a multiple of 4 up to 1024");
+    if (strip_cols == 0) strip_cols = kDefaultStripCols;
+    if (strip_rows % kSumBlockRows != 0 || strip_rows > 1024)
+        Fail(VQM_INVALID_ARGUMENT, "strip_rows must be a multiple of 8 up to 1024");
+    if (!ValidStripCols(strip_cols)) Fail(VQM_INVALID_ARGUMENT, "strip_cols must be 64 or 128");
     if (width > (1u << 20) || height > (1u << 20))
         Fail(VQM_INVALID_ARGUMENT, "image dimensions above 1048576 are not supported");
     Plan p{};
@@ -234,6 +236,7 @@ Plan FillPlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t stri
     p.height = height;
     p.levels = scales.count;
     p.strip_h = strip_rows;
+    p.strip_w = strip_cols;
     p.plane_stride = static_cast<uint32_t>(Align(static_cast<uint64_t>(width) * height, 4));
     uint64_t cursor = 0;  // bytes
     for (uint32_t l = 0; l < static_cast<uint32_t>(kMaxScales); ++l) {
@@ -255,14 +258,14 @@ Plan FillPlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t stri
     }
     p.image_bytes = cursor;
     // Scale work: per level and active channel, bands of strip_rows rows by
-    // 128-column strips, row-major so that consecutive items are neighbours
-    // (ItemAt enumerates them).
+    // strip_cols-column strips, row-major so that consecutive items are
+    // neighbours (ItemAt enumerates them).
     uint32_t items = 0;
     for (uint32_t l = 0; l < scales.count; ++l) {
         for (uint32_t c = 0; c < static_cast<uint32_t>(kChannels); ++c) {
             p.item_begin[l][c] = items;
             if (!(p.mask[l] & (1u << c))) continue;
-            const uint32_t strips = (p.w[l] + kStripCols - 1) / kStripCols;
+            const uint32_t strips = (p.w[l] + strip_cols - 1) / strip_cols;
             const uint32_t bands = (p.h[l] + strip_rows - 1) / strip_rows;
             p.item_count[l][c] = strips * bands;
             items += strips * bands;
@@ -274,9 +277,10 @@ Plan FillPlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t stri
     return p;
 }
 
-GeometryPlan BuildPlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows) {
+GeometryPlan BuildPlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows,
+                       uint32_t strip_cols) {
     GeometryPlan g;
-    g.plan = FillPlan(width, height, format, strip_rows);
+    g.plan = FillPlan(width, height, format, strip_rows, strip_cols);
     g.format = format;
     g.plane_bytes_per_image = g.plan.image_bytes;
     g.prep_tiles = g.plan.prep_tiles_x * g.plan.prep_tiles_y;

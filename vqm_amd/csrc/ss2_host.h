@@ -77,12 +77,15 @@ struct GeometryPlan {
     uint32_t prep_tiles = 0;             // prepare blocks per image
 };
 
+// strip_rows: output rows per scale work item, a multiple of 8 up to 1024
+// (0 = 64); strip_cols: its width, 64 or 128 columns (0 = 128).
 GeometryPlan BuildPlan(uint32_t width, uint32_t height, PlaneFormat format,
-                       uint32_t strip_rows);
+                       uint32_t strip_rows, uint32_t strip_cols = 0);
 
 // The plan alone, without the item table (ItemAt enumerates the items):
 // nothing is allocated.
-Plan FillPlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows);
+Plan FillPlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows,
+              uint32_t strip_cols = 0);
 
 // Bytes of one pair's workspace: planes of both images, partials, norms,
 // each rounded to 256 bytes.

@@ -33,14 +33,16 @@ hipError_t LaunchPrepare(PlaneFormat format, MapForm map, bool general, const Pa
                          const uint8_t* src0, const uint8_t* src1, hipStream_t stream,
                          bool persistent = false);
 
-// One 64-lane wave per scale work item (level, channel, 128-column strip).
-// Uniform geometry: blocks = pairs * items_per_pair, work = nullptr. `blur`
-// picks the kernel: kBlurSampled (13-tap sampled Gaussian) or kBlurRecursive
-// (the recursive Gaussian's 9-tap truncated-cosine kernel); `map` the
-// SSIM' map's form, which must be the one the planes were prepared with.
-hipError_t LaunchScale(PlaneFormat format, uint32_t blur, MapForm map, const PairDesc* pairs,
-                       const WorkRef* work, uint32_t blocks, uint32_t items_per_pair, hipStream_t stream,
-                       bool persistent = false);
+// One 64-lane wave per scale work item (level, channel, strip of 64 or 128
+// columns). Uniform geometry: blocks = pairs * items_per_pair, work =
+// nullptr. `blur` picks the kernel: kBlurSampled (13-tap sampled Gaussian)
+// or kBlurRecursive (the recursive Gaussian's 9-tap truncated-cosine
+// kernel); `map` the SSIM' map's form, which must be the one the planes were
+// prepared with; `strip_cols` the strip width of every plan in the launch
+// (Plan::strip_w).
+hipError_t LaunchScale(PlaneFormat format, uint32_t blur, MapForm map, uint32_t strip_cols,
+                       const PairDesc* pairs, const WorkRef* work, uint32_t blocks, uint32_t items_per_pair,
+                       hipStream_t stream, bool persistent = false);
 
 // Sums each (level, channel) norm's partials in FP64 and writes the 108 norms.
 hipError_t LaunchReduce(const PairDesc* pairs, uint32_t pairs_count, hipStream_t stream);

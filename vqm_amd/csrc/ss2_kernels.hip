@@ -93,6 +93,23 @@ __device__ __forceinline__ float rcp_dev(float x) {
     return r;
 }
 
+// a / b correctly rounded (the IEEE quotient), for the map's quotients: b a
+// positive normal float and a either 0.0f (never -0.0f) or |a| >= 2^-100,
+// with |a / b| inside [2^-100, 2^100]. This is the division sequence the
+// compiler emits for `a / b` (hardware reciprocal, one Newton step, two
+// residual corrections) without its operand scaling and special-value
+// fix-up, which leave such operands unchanged: 8 instructions instead of 11.
+// The map's operands stay in that domain: denominators are at least
+// c2 - 2^-22 or 1, and numerators are 0.0f (an exact difference of equal
+// values rounds to +0) or multiples of 2^-82.
+__device__ __forceinline__ float div_rn(float a, float b) {
+    float r = __builtin_amdgcn_rcpf(b);
+    r = fma_(fma_(-b, r, 1.0f), r, r);
+    float q = a * r;
+    q = fma_(fma_(-b, q, a), r, q);
+    return fma_(fma_(-b, q, a), r, q);
+}
+
 // The two arithmetic forms of the path (MapForm):
 //
 //   Reference form (Ref = true): prepare.comp:81-98 and scale.comp:130-142,
@@ -146,8 +163,8 @@ __device__ __forceinline__ void xyb(float r, float g, float b, uint32_t mask, fl
 
 // ---- plane element formats ------------------------------------------------
 
-// A format loads two adjacent elements as Raw bits and unpacks them to FP32
-// only where they are consumed, so that the wait for a prefetched row lands
+// A format loads two adjacent elements as Raw bits (or one element) and
+// unpacks them to FP32 only where they are consumed, so that the wait for a prefetched row lands
 // one row later instead of right behind the load.
 struct F32 {
     using E = float;
@@ -156,6 +173,7 @@ struct F32 {
         return *reinterpret_cast<GlobalAS<const f32x2>*>(p);
     }
     static __device__ __forceinline__ float2 unpack(Raw v) { return make_float2(v.x, v.y); }
+    static __device__ __forceinline__ float unpack1(E v) { return v; }
     static __device__ __forceinline__ void store1(GlobalAS<E>* p, float v) { *p = v; }
     static __device__ __forceinline__ void store2(GlobalAS<E>* p, float a, float b) {
         *reinterpret_cast<GlobalAS<f32x2>*>(p) = f32x2{a, b};
@@ -203,6 +221,7 @@ struct Narrow {
         return make_float2(From(static_cast<uint16_t>(v & 0xFFFFu)),
                            From(static_cast<uint16_t>(v >> 16)));
     }
+    static __device__ __forceinline__ float unpack1(E v) { return From(v); }
     static __device__ __forceinline__ void store1(GlobalAS<E>* p, float v) { *p = To(v); }
     static __device__ __forceinline__ void store2(GlobalAS<E>* p, float a, float b) {
         *reinterpret_cast<GlobalAS<uint32_t>*>(p) =
@@ -795,18 +814,75 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// The products of one lane's two samples of a and b that the form blurs.
-template <bool Ref>
-__device__ __forceinline__ void stage_products(float (*prod)[kSrcCols], int j, float2 a, float2 b) {
-    if constexpr (Ref) {
-        *reinterpret_cast<float2*>(&prod[0][j]) = make_float2(a.x * a.x, a.y * a.y);
-        *reinterpret_cast<float2*>(&prod[1][j]) = make_float2(b.x * b.x, b.y * b.y);
-        *reinterpret_cast<float2*>(&prod[2][j]) = make_float2(a.x * b.x, a.y * b.y);
-    } else {
-        const float dx = a.x - b.x, dy = a.y - b.y;
-        *reinterpret_cast<float2*>(&prod[0][j]) = make_float2(dx * dx, dy * dy);
-        *reinterpret_cast<float2*>(&prod[1][j]) = make_float2(a.x * b.x, a.y * b.y);
+// Strip geometry of a scale work item: 64 lanes of Cols output columns
+// each (64- or 128-column strips), the source row wider by the blur radius
+// on both sides; the 12 halo columns are loaded by the first 12 / Cols lanes.
+template <int Cols>
+struct StripGeom {
+    static_assert(Cols == 1 || Cols == 2, "one or two output columns per lane");
+    static constexpr int kCols = 64 * Cols;
+    static constexpr int kSrc = kCols + 2 * kRadius;
+    static constexpr int kHaloLanes = 2 * kRadius / Cols;
+};
+
+// A lane's Cols adjacent plane samples of one row: loaded as raw bits,
+// unpacked to FP32 where they are consumed.
+template <typename Fmt, int Cols> struct LaneSamples;
+template <typename Fmt> struct LaneSamples<Fmt, 2> {
+    using Raw = typename Fmt::Raw;
+    static __device__ __forceinline__ Raw load(GlobalAS<const typename Fmt::E>* p) { return Fmt::load2(p); }
+    static __device__ __forceinline__ void unpack(Raw v, float out[2]) {
+        const float2 t = Fmt::unpack(v);
+        out[0] = t.x;
+        out[1] = t.y;
     }
+};
+template <typename Fmt> struct LaneSamples<Fmt, 1> {
+    using Raw = typename Fmt::E;
+    static __device__ __forceinline__ Raw load(GlobalAS<const typename Fmt::E>* p) { return *p; }
+    static __device__ __forceinline__ void unpack(Raw v, float out[1]) { out[0] = Fmt::unpack1(v); }
+};
+
+// Cols adjacent floats to and from LDS (one 8-byte access for two).
+template <int Cols>
+__device__ __forceinline__ void lds_put(float* p, const float* v) {
+    if constexpr (Cols == 2) {
+        *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+    } else {
+        *p = v[0];
+    }
+}
+template <int Cols>
+__device__ __forceinline__ void lds_get(const float* p, float* v) {
+    if constexpr (Cols == 2) {
+        const float2 t = *reinterpret_cast<const float2*>(p);
+        v[0] = t.x;
+        v[1] = t.y;
+    } else {
+        v[0] = *p;
+    }
+}
+
+// The products of one lane's samples of a and b that the form blurs, staged
+// at source column j of each product row.
+template <bool Ref, int Cols>
+__device__ __forceinline__ void stage_products(float (*prod)[StripGeom<Cols>::kSrc], int j, const float* a,
+                                               const float* b) {
+    float p0[Cols], p1[Cols], p2[Cols];
+    for (int c = 0; c < Cols; ++c) {
+        if constexpr (Ref) {
+            p0[c] = a[c] * a[c];
+            p1[c] = b[c] * b[c];
+            p2[c] = a[c] * b[c];
+        } else {
+            const float d = a[c] - b[c];
+            p0[c] = d * d;
+            p1[c] = a[c] * b[c];
+        }
+    }
+    lds_put<Cols>(&prod[0][j], p0);
+    lds_put<Cols>(&prod[1][j], p1);
+    if constexpr (Ref) lds_put<Cols>(&prod[2][j], p2);
 }
 
 // The six map values of one pixel from its blurred quantities `bl` (in the
@@ -823,13 +899,13 @@ __device__ __forceinline__ void map_values(const float* bl, float a, float b, fl
         // num_m = 1 - (mu1 - mu2)^2, num_s = 2 (s12 - mu1 mu2) + c2,
         // den = (s11 - mu1^2) + (s22 - mu2^2) + c2,
         // d = max((den - num_m num_s) / den, 0), d1 = (det_dist - det_orig) / det_orig,
-        // with IEEE divisions.
+        // with correctly rounded divisions.
         const float t = mu1 - mu2;
         const float num_m = fma_(-t, t, 1.0f);
         const float num_s = fma_(2.0f, fma_(-mu1, mu2, bl[4]), kC2);
         const float den = (fma_(-mu1, mu1, bl[2]) + fma_(-mu2, mu2, bl[3])) + kC2;
-        d = fmaxf(fma_(-num_m, num_s, den) / den, 0.0f);
-        d1 = (det_dist - det_orig) / det_orig;
+        d = fmaxf(div_rn(fma_(-num_m, num_s, den), den), 0.0f);
+        d1 = div_rn(det_dist - det_orig, det_orig);
     } else {
         // scale.comp:305-333 rearranged: with e the blurred (a - b)^2 and
         // t = mu1 - mu2, s11 + s22 - mu1^2 - mu2^2 = num_s - c2 + v where
@@ -860,12 +936,15 @@ __device__ __forceinline__ void map_values(const float* bl, float a, float b, fl
 }
 
 // One scale work item: the strip of `level`, `channel` starting at (x0, y0),
-// with the plane elements in format Fmt.
-template <typename Fmt, typename TapsT, bool Ref>
+// with the plane elements in format Fmt and Cols output columns per lane.
+template <typename Fmt, typename TapsT, bool Ref, int Cols>
 __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>* plan, uint32_t item_idx,
                                            uint32_t level, uint32_t channel, uint32_t x0, uint32_t y0,
-                                           float (*ring_a)[kSrcCols], float (*ring_b)[kSrcCols],
-                                           float (*prod)[kSrcCols]) {
+                                           float (*ring_a)[StripGeom<Cols>::kSrc],
+                                           float (*ring_b)[StripGeom<Cols>::kSrc],
+                                           float (*prod)[StripGeom<Cols>::kSrc]) {
+    using Geo = StripGeom<Cols>;
+    using IO = LaneSamples<Fmt, Cols>;
     constexpr int kBlurred = MapQuantities<Ref>::kBlurred;
     const uint32_t W = plan->w[level], H = plan->h[level], pitch = plan->pitch[level];
     // The strip's output rows stop at the level's last row (y0 < H for every
@@ -875,7 +954,8 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
 
     using E = typename Fmt::E;
     // Source row r (0 .. total-1) is image row y0 - 6 + r, padded row y0 + r;
-    // source column j (0 .. 139) is image column x0 - 6 + j, padded x0 + 2 + j.
+    // source column j (0 .. kSrc-1) is image column x0 - 6 + j, padded
+    // x0 + 2 + j.
     GlobalAS<const uint8_t>* base = as_global(static_cast<const uint8_t*>(pdp->planes));
     GlobalAS<const E>* pa = reinterpret_cast<GlobalAS<const E>*>(base + plan->plane_off[level][channel]) +
                             (static_cast<uint64_t>(y0) * pitch + x0 + 2u);
@@ -883,20 +963,27 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
         reinterpret_cast<GlobalAS<const uint8_t>*>(pa) + plan->image_bytes);
 
     const int lane = static_cast<int>(threadIdx.x);
-    const bool halo = lane < (kSrcCols - kStripCols) / 2;  // lanes 0..5
-    const int jm = 2 * lane, jh = kStripCols + 2 * lane;
+    const bool halo = lane < Geo::kHaloLanes;
+    const int jm = Cols * lane, jh = Geo::kCols + Cols * lane;
 
-    using Raw = typename Fmt::Raw;
-    Raw cur_a = Fmt::load2(pa + jm), cur_b = Fmt::load2(pb + jm);
-    Raw cur_ha{}, cur_hb{};
+    typename IO::Raw cur_a = IO::load(pa + jm), cur_b = IO::load(pb + jm);
+    typename IO::Raw cur_ha{}, cur_hb{};
     if (halo) {
-        cur_ha = Fmt::load2(pa + jh);
-        cur_hb = Fmt::load2(pb + jh);
+        cur_ha = IO::load(pa + jh);
+        cur_hb = IO::load(pb + jh);
     }
 
-    float win[kTaps][kBlurred][2];
+    float win[kTaps][kBlurred][Cols];
+    // Map sums: each lane adds each of its columns' values over a block of
+    // eight rows (rows 8k .. 8k+7 of the level, in row order) in FP32, then
+    // the block's sum into FP64. Strips start on rows that are multiples of
+    // eight, so the blocks, and every FP32 sum, do not depend on the strip
+    // geometry.
+    float block[Cols][6];
+    for (int c = 0; c < Cols; ++c)
+        for (int k = 0; k < 6; ++k) block[c][k] = 0.0f;
     double sums[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    const uint32_t col = x0 + 2u * static_cast<uint32_t>(lane);
+    const uint32_t col = x0 + static_cast<uint32_t>(jm);
 
     for (int base_row = 0; base_row < total; base_row += kTaps) {
         unroll(std::make_integer_sequence<int, kTaps>{}, [&](auto pc) {
@@ -907,47 +994,44 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
             float* ra = ring_a[i & (kRing - 1)];
             float* rb = ring_b[i & (kRing - 1)];
             {
-                const float2 a = Fmt::unpack(cur_a), b = Fmt::unpack(cur_b);
-                *reinterpret_cast<float2*>(ra + jm) = a;
-                *reinterpret_cast<float2*>(rb + jm) = b;
-                stage_products<Ref>(prod, jm, a, b);
+                float a[Cols], b[Cols];
+                IO::unpack(cur_a, a);
+                IO::unpack(cur_b, b);
+                lds_put<Cols>(ra + jm, a);
+                lds_put<Cols>(rb + jm, b);
+                stage_products<Ref, Cols>(prod, jm, a, b);
             }
             if (halo) {
-                const float2 a = Fmt::unpack(cur_ha), b = Fmt::unpack(cur_hb);
-                *reinterpret_cast<float2*>(ra + jh) = a;
-                *reinterpret_cast<float2*>(rb + jh) = b;
-                stage_products<Ref>(prod, jh, a, b);
+                float a[Cols], b[Cols];
+                IO::unpack(cur_ha, a);
+                IO::unpack(cur_hb, b);
+                lds_put<Cols>(ra + jh, a);
+                lds_put<Cols>(rb + jh, b);
+                stage_products<Ref, Cols>(prod, jh, a, b);
             }
             // Prefetch input row i + 1.
             if (i + 1 < total) {
                 const uint64_t off = static_cast<uint64_t>(i + 1) * pitch;
-                cur_a = Fmt::load2(pa + off + jm);
-                cur_b = Fmt::load2(pb + off + jm);
+                cur_a = IO::load(pa + off + jm);
+                cur_b = IO::load(pb + off + jm);
                 if (halo) {
-                    cur_ha = Fmt::load2(pa + off + jh);
-                    cur_hb = Fmt::load2(pb + off + jh);
+                    cur_ha = IO::load(pa + off + jh);
+                    cur_hb = IO::load(pb + off + jh);
                 }
             }
             __syncthreads();
 
             // Horizontal 13-tap pass of each blurred quantity at this lane's
-            // two output columns (source columns jm + 6 and jm + 7).
+            // output columns (source columns jm + 6 .. jm + 5 + Cols).
             for (int q = 0; q < kBlurred; ++q) {
                 const float* s = q == 0 ? ra : q == 1 ? rb : prod[q - 2];
-                float v[14];
-                for (int k = 0; k < 7; ++k) {
-                    const float2 t = *reinterpret_cast<const float2*>(s + jm + 2 * k);
-                    v[2 * k] = t.x;
-                    v[2 * k + 1] = t.y;
+                float v[kTaps - 1 + Cols];
+                for (int k = 0; k < kTaps - 1 + Cols; k += Cols) lds_get<Cols>(s + jm + k, v + k);
+                for (int c = 0; c < Cols; ++c) {
+                    float h = TapsT::w[0] * v[c];
+                    for (int k = 1; k < kTaps; ++k) h = fma_(TapsT::w[k], v[k + c], h);
+                    win[P][q][c] = h;
                 }
-                float h0 = TapsT::w[0] * v[0];
-                float h1 = TapsT::w[0] * v[1];
-                for (int k = 1; k < kTaps; ++k) {
-                    h0 = fma_(TapsT::w[k], v[k], h0);
-                    h1 = fma_(TapsT::w[k], v[k + 1], h1);
-                }
-                win[P][q][0] = h0;
-                win[P][q][1] = h1;
             }
 
             // Vertical pass for output row i - 12 (image row y0 + i - 12),
@@ -963,36 +1047,39 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
                 };
                 const uint32_t y = y0 + static_cast<uint32_t>(i - 2 * kRadius);
                 const int raw_slot = (i - kRadius) & (kRing - 1);
-                const float2 a2 = *reinterpret_cast<const float2*>(&ring_a[raw_slot][jm + kRadius]);
-                const float2 b2 = *reinterpret_cast<const float2*>(&ring_b[raw_slot][jm + kRadius]);
-                const float av[2] = {a2.x, a2.y}, bv[2] = {b2.x, b2.y};
+                float av[Cols], bv[Cols];
+                lds_get<Cols>(&ring_a[raw_slot][jm + kRadius], av);
+                lds_get<Cols>(&ring_b[raw_slot][jm + kRadius], bv);
                 if (y < H) {
                     if constexpr (Ref) {
-                        // One column at a time (its five blurred values
-                        // live only through its map); each pixel's map
-                        // values accumulate in FP64.
-                        for (int c = 0; c < 2; ++c) {
+                        // One column at a time: its five blurred values
+                        // live only through its map.
+                        for (int c = 0; c < Cols; ++c) {
                             float bl[kBlurred];
                             vertical(c, bl);
                             if (col + static_cast<uint32_t>(c) >= W) continue;
                             float v6[6];
                             map_values<true>(bl, av[c], bv[c], v6);
-                            for (int k = 0; k < 6; ++k) sums[k] += static_cast<double>(v6[k]);
+                            for (int k = 0; k < 6; ++k) block[c][k] = block[c][k] + v6[k];
                         }
                     } else {
-                        // The lane's two columns' map values are added in
-                        // FP32, then accumulated in FP64.
-                        float bl[2][kBlurred];
-                        vertical(0, bl[0]);
-                        vertical(1, bl[1]);
-                        float pair[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-                        for (int c = 0; c < 2; ++c) {
+                        float bl[Cols][kBlurred];
+                        for (int c = 0; c < Cols; ++c) vertical(c, bl[c]);
+                        for (int c = 0; c < Cols; ++c) {
                             if (col + static_cast<uint32_t>(c) >= W) continue;
                             float v6[6];
                             map_values<false>(bl[c], av[c], bv[c], v6);
-                            for (int k = 0; k < 6; ++k) pair[k] = c == 0 ? v6[k] : pair[k] + v6[k];
+                            for (int k = 0; k < 6; ++k) block[c][k] = block[c][k] + v6[k];
                         }
-                        for (int k = 0; k < 6; ++k) sums[k] += static_cast<double>(pair[k]);
+                    }
+                    // A block ends on rows 7, 15, 23, ... and on the level's
+                    // last row (a uniform branch: y is the wave's row).
+                    if ((y & 7u) == 7u || y + 1u == H) {
+                        for (int c = 0; c < Cols; ++c)
+                            for (int k = 0; k < 6; ++k) {
+                                sums[k] += static_cast<double>(block[c][k]);
+                                block[c][k] = 0.0f;
+                            }
                     }
                 }
             }
@@ -1007,14 +1094,21 @@ __device__ __forceinline__ void scale_item(ConstAS<PairDesc>* pdp, ConstAS<Plan>
 }
 
 // Work items blockIdx.x, blockIdx.x + gridDim.x, ... of `blocks`, as in
-// prepare_kernel.
-template <typename Lay, typename TapsT, bool Ref>
-__global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ pairs,
+// prepare_kernel. Two columns per lane hold a 130-register ring in the
+// reference form (two waves per SIMD) and a 104-register one in the
+// difference form, held to 168 registers (three waves); one column per lane
+// halves the ring (four waves).
+template <int Cols, bool Ref>
+constexpr int kScaleWaves = (Cols == 2 && !Ref) ? 3 : 1;
+
+template <typename Lay, typename TapsT, bool Ref, int Cols>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kScaleWaves<Cols, Ref>))) void scale_kernel(const PairDesc* __restrict__ pairs,
                                                    const WorkRef* __restrict__ work,
                                                    uint32_t blocks, uint32_t items_per_pair) {
-    __shared__ __align__(16) float ring_a[kRing][kSrcCols];
-    __shared__ __align__(16) float ring_b[kRing][kSrcCols];
-    __shared__ __align__(16) float prod[MapQuantities<Ref>::kProducts][kSrcCols];
+    constexpr int kSrc = StripGeom<Cols>::kSrc;
+    __shared__ __align__(16) float ring_a[kRing][kSrc];
+    __shared__ __align__(16) float ring_b[kRing][kSrc];
+    __shared__ __align__(16) float prod[MapQuantities<Ref>::kProducts][kSrc];
 
     for (uint32_t b = blockIdx.x; b < blocks; b += gridDim.x) {
         uint32_t pair_idx, item_idx;
@@ -1032,8 +1126,8 @@ __global__ __launch_bounds__(64) void scale_kernel(const PairDesc* __restrict__ 
         ConstAS<ScaleItem>* item = as_const(pdp->items) + item_idx;
         const uint32_t level = item->level, channel = item->channel;
         const uint32_t x0 = item->x0, y0 = item->y0;
-        scale_item<typename Lay::Fmt, TapsT, Ref>(pdp, plan, item_idx, level, channel, x0, y0, ring_a, ring_b,
-                                                  prod);
+        scale_item<typename Lay::Fmt, TapsT, Ref, Cols>(pdp, plan, item_idx, level, channel, x0, y0, ring_a,
+                                                        ring_b, prod);
         __syncthreads();  // the next item reuses the row ring
     }
 }
@@ -1138,28 +1232,37 @@ hipError_t launch_prepare_t(MapForm map, bool general, const PairDesc* pairs, co
                : launch_prepare_m<Lay, false>(general, pairs, work, blocks, tiles, lut, src0, src1, persistent, stream);
 }
 
-template <typename Lay, typename TapsT, bool Ref>
+template <typename Lay, typename TapsT, bool Ref, int Cols>
 hipError_t launch_scale_g(const PairDesc* pairs, const WorkRef* work, uint32_t blocks, uint32_t items,
                           bool persistent, hipStream_t stream) {
-    const uint32_t grid = GridOf(&scale_kernel<Lay, TapsT, Ref>, 64, blocks, persistent);
-    hipLaunchKernelGGL((scale_kernel<Lay, TapsT, Ref>), dim3(grid), dim3(64), 0, stream, pairs, work, blocks,
-                       items);
+    const uint32_t grid = GridOf(&scale_kernel<Lay, TapsT, Ref, Cols>, 64, blocks, persistent);
+    hipLaunchKernelGGL((scale_kernel<Lay, TapsT, Ref, Cols>), dim3(grid), dim3(64), 0, stream, pairs, work,
+                       blocks, items);
     return hipGetLastError();
 }
 
-template <typename Lay, bool Ref>
+template <typename Lay, bool Ref, int Cols>
 hipError_t launch_scale_m(uint32_t blur, const PairDesc* pairs, const WorkRef* work, uint32_t blocks,
                           uint32_t items, bool persistent, hipStream_t stream) {
     return blur == kBlurRecursive
-               ? launch_scale_g<Lay, CosineTaps, Ref>(pairs, work, blocks, items, persistent, stream)
-               : launch_scale_g<Lay, SampledTaps, Ref>(pairs, work, blocks, items, persistent, stream);
+               ? launch_scale_g<Lay, CosineTaps, Ref, Cols>(pairs, work, blocks, items, persistent, stream)
+               : launch_scale_g<Lay, SampledTaps, Ref, Cols>(pairs, work, blocks, items, persistent, stream);
+}
+
+template <typename Lay, int Cols>
+hipError_t launch_scale_c(uint32_t blur, MapForm map, const PairDesc* pairs, const WorkRef* work,
+                          uint32_t blocks, uint32_t items, bool persistent, hipStream_t stream) {
+    return map == kMapReference
+               ? launch_scale_m<Lay, true, Cols>(blur, pairs, work, blocks, items, persistent, stream)
+               : launch_scale_m<Lay, false, Cols>(blur, pairs, work, blocks, items, persistent, stream);
 }
 
 template <typename Lay>
-hipError_t launch_scale_t(uint32_t blur, MapForm map, const PairDesc* pairs, const WorkRef* work,
-                          uint32_t blocks, uint32_t items, bool persistent, hipStream_t stream) {
-    return map == kMapReference ? launch_scale_m<Lay, true>(blur, pairs, work, blocks, items, persistent, stream)
-                                : launch_scale_m<Lay, false>(blur, pairs, work, blocks, items, persistent, stream);
+hipError_t launch_scale_t(uint32_t strip_cols, uint32_t blur, MapForm map, const PairDesc* pairs,
+                          const WorkRef* work, uint32_t blocks, uint32_t items, bool persistent,
+                          hipStream_t stream) {
+    return strip_cols == 64 ? launch_scale_c<Lay, 1>(blur, map, pairs, work, blocks, items, persistent, stream)
+                            : launch_scale_c<Lay, 2>(blur, map, pairs, work, blocks, items, persistent, stream);
 }
 
 }  // namespace
@@ -1196,17 +1299,18 @@ hipError_t LaunchPrepare(PlaneFormat format, MapForm map, bool general, const Pa
     return hipErrorInvalidValue;
 }
 
-hipError_t LaunchScale(PlaneFormat format, uint32_t blur, MapForm map, const PairDesc* pairs,
-                       const WorkRef* work, uint32_t blocks, uint32_t items_per_pair, hipStream_t stream,
-                       bool persistent) {
+hipError_t LaunchScale(PlaneFormat format, uint32_t blur, MapForm map, uint32_t strip_cols,
+                       const PairDesc* pairs, const WorkRef* work, uint32_t blocks, uint32_t items_per_pair,
+                       hipStream_t stream, bool persistent) {
     if (blocks == 0) return hipSuccess;
-    if (blur > kBlurRecursive || map > kMapDifference) return hipErrorInvalidValue;
-    const uint32_t n = items_per_pair;
+    if (blur > kBlurRecursive || map > kMapDifference || !ValidStripCols(strip_cols)) return hipErrorInvalidValue;
+    const uint32_t n = items_per_pair, sc = strip_cols;
     switch (format) {
-        case kPlaneF32: return launch_scale_t<LayoutF32>(blur, map, pairs, work, blocks, n, persistent, stream);
-        case kPlaneF16: return launch_scale_t<LayoutF16>(blur, map, pairs, work, blocks, n, persistent, stream);
-        case kPlaneBF16: return launch_scale_t<LayoutBF16>(blur, map, pairs, work, blocks, n, persistent, stream);
-        case kPlaneU16: return launch_scale_t<LayoutU16>(blur, map, pairs, work, blocks, n, persistent, stream);
+        case kPlaneF32: return launch_scale_t<LayoutF32>(sc, blur, map, pairs, work, blocks, n, persistent, stream);
+        case kPlaneF16: return launch_scale_t<LayoutF16>(sc, blur, map, pairs, work, blocks, n, persistent, stream);
+        case kPlaneBF16:
+            return launch_scale_t<LayoutBF16>(sc, blur, map, pairs, work, blocks, n, persistent, stream);
+        case kPlaneU16: return launch_scale_t<LayoutU16>(sc, blur, map, pairs, work, blocks, n, persistent, stream);
     }
     return hipErrorInvalidValue;
 }

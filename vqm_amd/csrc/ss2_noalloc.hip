@@ -122,7 +122,7 @@ uint32_t ScoreDeviceNoAlloc(const vqm_ss2_config& config, const vqm_device_image
              "(score other formats through a pipeline)");
 
     SetupArgs args{};
-    args.plan = FillPlan(config.width, config.height, format, config.strip_rows);
+    args.plan = FillPlan(config.width, config.height, format, config.strip_rows, config.strip_cols);
     const Plan& p = args.plan;
     const uint64_t partials_at = Align256(kItemsAt + static_cast<uint64_t>(p.items) * sizeof(ScaleItem));
     const uint64_t planes_at = Align256(partials_at + static_cast<uint64_t>(p.items) * 6 * sizeof(double));
@@ -156,7 +156,8 @@ uint32_t ScoreDeviceNoAlloc(const vqm_ss2_config& config, const vqm_device_image
     const uint32_t tiles = p.prep_tiles_x * p.prep_tiles_y;
     Check(LaunchPrepare(format, map, false, dpair, nullptr, 2 * tiles, tiles, dlut, nullptr, nullptr, stream),
           "launch prepare");
-    Check(LaunchScale(format, config.blur, map, dpair, nullptr, p.items, p.items, stream), "launch scale");
+    Check(LaunchScale(format, config.blur, map, p.strip_w, dpair, nullptr, p.items, p.items, stream),
+          "launch scale");
     Check(LaunchReduce(dpair, 1, stream), "launch reduce");
     return p.levels;
 }

@@ -146,8 +146,9 @@ DeviceContext& DeviceContext::Get() {
     return *ctx;
 }
 
-DevicePlan::DevicePlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows)
-    : host(BuildPlan(width, height, format, strip_rows)),
+DevicePlan::DevicePlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows,
+                       uint32_t strip_cols)
+    : host(BuildPlan(width, height, format, strip_rows, strip_cols)),
       layout(LayoutWorkspace(host)),
       plan(sizeof(Plan)),
       items(std::max<uint64_t>(host.items.size(), 1) * sizeof(ScaleItem)) {
@@ -189,8 +190,8 @@ Workspace::Workspace(const WorkspaceLayout& layout, uint64_t input_image_bytes, 
 
 // ---- workspace pools ------------------------------------------------------------
 
-WorkspacePools::WorkspacePools(PlaneFormat format, uint32_t strip_rows, uint64_t budget)
-    : format_(format), strip_rows_(strip_rows), budget_(budget) {}
+WorkspacePools::WorkspacePools(PlaneFormat format, uint32_t strip_rows, uint32_t strip_cols, uint64_t budget)
+    : format_(format), strip_rows_(strip_rows), strip_cols_(strip_cols), budget_(budget) {}
 
 void WorkspacePools::Drop(std::unique_ptr<Workspace> ws) {
     bytes_ -= ws->bytes;
@@ -200,10 +201,11 @@ void WorkspacePools::Drop(std::unique_ptr<Workspace> ws) {
 const DevicePlan& WorkspacePools::PlanFor(uint32_t width, uint32_t height) {
     auto& slot = plans_[std::make_pair(width, height)];
     if (!slot) {
-        const auto rows = strip_rows_for_.find(std::make_pair(width, height));
+        const auto shape = shape_for_.find(std::make_pair(width, height));
+        const bool set = shape != shape_for_.end();
         try {
-            slot = std::make_unique<DevicePlan>(width, height, format_,
-                                                rows != strip_rows_for_.end() ? rows->second : strip_rows_);
+            slot = std::make_unique<DevicePlan>(width, height, format_, set ? shape->second.first : strip_rows_,
+                                                set ? shape->second.second : strip_cols_);
         } catch (...) {
             plans_.erase(std::make_pair(width, height));
             throw;
@@ -321,6 +323,9 @@ void Lane::Launch(PlaneFormat fmt, MapForm map, const float* lut, const LaunchEn
           "upload pair descriptors");
     const WorkRef* wprep = nullptr;
     const WorkRef* wscale = nullptr;
+    auto chunk_begin = [&](uint32_t c) { return static_cast<uint32_t>(uint64_t(c) * count / chunks); };
+    constexpr uint32_t kStripWidths[2] = {64, 128};
+    std::vector<uint64_t> scale_seg(2 * chunks + 1, 0);  // work-list runs by chunk and strip width
     if (!uniform) {
         if (h_work_prep_.bytes() < prep_work * sizeof(WorkRef)) {
             h_work_prep_ = PinnedBuffer(prep_work * sizeof(WorkRef));
@@ -337,8 +342,20 @@ void Lane::Launch(PlaneFormat fmt, MapForm map, const float* lut, const LaunchEn
             const GeometryPlan& g = entries[i].plan->host;
             for (uint32_t image = 0; image < 2; ++image)
                 for (uint32_t k = 0; k < g.prep_tiles; ++k) wp[np++] = WorkRef{i, k | (image << 31)};
-            for (uint32_t k = 0; k < g.plan.items; ++k) ws[ns++] = WorkRef{i, k};
         }
+        // Scale work, per chunk, in one run per strip width (each run is one
+        // launch of that width's kernel).
+        for (uint32_t c = 0; c < chunks; ++c) {
+            for (uint32_t g = 0; g < 2; ++g) {
+                scale_seg[2 * c + g] = ns;
+                for (uint32_t i = chunk_begin(c); i < chunk_begin(c + 1); ++i) {
+                    const Plan& p = entries[i].plan->host.plan;
+                    if (p.strip_w != kStripWidths[g]) continue;
+                    for (uint32_t k = 0; k < p.items; ++k) ws[ns++] = WorkRef{i, k};
+                }
+            }
+        }
+        scale_seg[2 * chunks] = ns;
         Check(hipMemcpyAsync(work_prep_.get(), wp, np * sizeof(WorkRef), hipMemcpyHostToDevice, s),
               "upload prepare work");
         Check(hipMemcpyAsync(work_scale_.get(), ws, ns * sizeof(WorkRef), hipMemcpyHostToDevice, s),
@@ -350,8 +367,7 @@ void Lane::Launch(PlaneFormat fmt, MapForm map, const float* lut, const LaunchEn
     const PairDesc* dpairs = pairs_.as<PairDesc>();
     chunks_ = chunks;
     for (uint32_t c = 0; c < chunks; ++c) {
-        const uint32_t i0 = static_cast<uint32_t>(uint64_t(c) * count / chunks);
-        const uint32_t i1 = static_cast<uint32_t>(uint64_t(c + 1) * count / chunks);
+        const uint32_t i0 = chunk_begin(c), i1 = chunk_begin(c + 1);
         // Uniform runs index pairs from their first descriptor; work lists
         // hold absolute pair indices and are cut at the run's offsets.
         const PairDesc* base = uniform ? dpairs + i0 : dpairs;
@@ -363,14 +379,23 @@ void Lane::Launch(PlaneFormat fmt, MapForm map, const float* lut, const LaunchEn
         Check(hipEventRecord(Ev(4 * c + 1).get(), s), "hipEventRecord");
     }
     for (uint32_t c = 0; c < chunks; ++c) {
-        const uint32_t i0 = static_cast<uint32_t>(uint64_t(c) * count / chunks);
-        const uint32_t i1 = static_cast<uint32_t>(uint64_t(c + 1) * count / chunks);
-        const PairDesc* base = uniform ? dpairs + i0 : dpairs;
-        const WorkRef* ws = uniform ? nullptr : wscale + scale_at[i0];
-        const auto ns = static_cast<uint32_t>(scale_at[i1] - scale_at[i0]);
+        const uint32_t i0 = chunk_begin(c), i1 = chunk_begin(c + 1);
         if (t != s) Check(hipStreamWaitEvent(t, Ev(4 * c + 1).get(), 0), "order scale after prepare");
         Check(hipEventRecord(Ev(4 * c + 2).get(), t), "hipEventRecord");
-        Check(LaunchScale(fmt, blur, map, base, ws, ns, g0.plan.items, t, persistent), "launch scale");
+        if (uniform) {
+            const auto ns = static_cast<uint32_t>(scale_at[i1] - scale_at[i0]);
+            Check(LaunchScale(fmt, blur, map, g0.plan.strip_w, dpairs + i0, nullptr, ns, g0.plan.items, t,
+                              persistent),
+                  "launch scale");
+        } else {
+            for (uint32_t g = 0; g < 2; ++g) {
+                const auto ns = static_cast<uint32_t>(scale_seg[2 * c + g + 1] - scale_seg[2 * c + g]);
+                if (ns == 0) continue;
+                Check(LaunchScale(fmt, blur, map, kStripWidths[g], dpairs, wscale + scale_seg[2 * c + g], ns,
+                                  g0.plan.items, t, persistent),
+                      "launch scale");
+            }
+        }
         Check(hipEventRecord(Ev(4 * c + 3).get(), t), "hipEventRecord");
     }
     const uint32_t r = 4 * chunks;
@@ -440,7 +465,7 @@ Pipeline::Pipeline(const vqm_ss2_config& config)
       map_(ToMapForm(config.map_form, static_cast<PlaneFormat>(config.plane_format))),
       ctx_(DeviceContext::Get()),
       plan_(config.width, config.height, static_cast<PlaneFormat>(config.plane_format),
-            config.strip_rows),
+            config.strip_rows, config.strip_cols),
       ws_(plan_.layout),
       pair_(sizeof(PairDesc)),
       sources_(2 * sizeof(SourceDesc)),
@@ -526,7 +551,8 @@ void Pipeline::BuildGraph() {
     hipError_t e = hipMemcpyAsync(pair_.get(), h_pair_.get(), sizeof(PairDesc), hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
         e = LaunchPrepare(fmt, map_, false, pd, nullptr, 2 * g.prep_tiles, g.prep_tiles, ctx_.lut, nullptr, nullptr, s);
-    if (e == hipSuccess) e = LaunchScale(fmt, config_.blur, map_, pd, nullptr, g.plan.items, g.plan.items, s);
+    if (e == hipSuccess)
+        e = LaunchScale(fmt, config_.blur, map_, g.plan.strip_w, pd, nullptr, g.plan.items, g.plan.items, s);
     if (e == hipSuccess) e = LaunchReduce(pd, 1, s);
     if (e == hipSuccess)
         e = hipMemcpyAsync(host_norms_.get(), ws_.norms, kNorms * sizeof(double), hipMemcpyDeviceToHost, s);
@@ -619,7 +645,8 @@ uint64_t Pipeline::Launch(const uint8_t* src0, const uint8_t* src1, bool reuse_o
                             s),
               "launch prepare");
     Check(hipEventRecord(ev_[1].get(), s), "hipEventRecord");
-    Check(LaunchScale(fmt, config_.blur, map_, pd, nullptr, g.plan.items, g.plan.items, s), "launch scale");
+    Check(LaunchScale(fmt, config_.blur, map_, g.plan.strip_w, pd, nullptr, g.plan.items, g.plan.items, s),
+          "launch scale");
     Check(hipEventRecord(ev_[2].get(), s), "hipEventRecord");
     Check(LaunchReduce(pd, 1, s), "launch reduce");
     Check(hipEventRecord(ev_[3].get(), s), "hipEventRecord");
@@ -815,6 +842,8 @@ const vqm_ss2_batch_config& ValidatedBatch(const vqm_ss2_batch_config& config) {
     if (config.blur > kBlurRecursive) Fail(VQM_INVALID_ARGUMENT, "blur must be sampled (0) or recursive (1)");
     if (config.persistent > 1) Fail(VQM_INVALID_ARGUMENT, "persistent must be 0 or 1");
     ToMapForm(config.map_form, static_cast<PlaneFormat>(config.plane_format));
+    // Validates strip_rows and strip_cols.
+    BuildPlan(8, 8, static_cast<PlaneFormat>(config.plane_format), config.strip_rows, config.strip_cols);
     return config;
 }
 }  // namespace
@@ -829,10 +858,18 @@ Batch::Batch(const vqm_ss2_batch_config& config)
       map_(ToMapForm(config.map_form, static_cast<PlaneFormat>(config.plane_format))),
       ctx_(DeviceContext::Get()),
       pools_(static_cast<PlaneFormat>(config.plane_format),
-             config.strip_rows != 0 ? config.strip_rows : kBatchStripRows, config.hbm_budget),
+             config.strip_rows != 0 ? config.strip_rows : kBatchStripRows, config.strip_cols, config.hbm_budget),
       lane_(config.max_pairs) {}
 
 Batch::~Batch() = default;
+
+void Batch::SetShape(uint32_t width, uint32_t height, uint32_t strip_rows, uint32_t strip_cols) {
+    BuildPlan(width, height, static_cast<PlaneFormat>(config_.plane_format), strip_rows, strip_cols);  // validates
+    if (pools_.HasPlan(width, height))
+        Fail(VQM_LOGIC_ERROR, "the strip shape of a geometry already used cannot change");
+    pools_.SetStripShape(width, height, strip_rows != 0 ? strip_rows : kBatchStripRows,
+                         strip_cols != 0 ? strip_cols : config_.strip_cols);
+}
 
 uint64_t Batch::Submit(const vqm_ss2_device_pair* pairs, uint32_t count) {
     if (in_flight_ != 0) Fail(VQM_LOGIC_ERROR, "a batch is in flight: call Finish first");
@@ -848,14 +885,14 @@ uint64_t Batch::Submit(const vqm_ss2_device_pair* pairs, uint32_t count) {
     }
     uint32_t chunks = config_.chunks;
     if (config_.autotune) {
-        // Each new geometry's strip height comes from the tuner, timed for
+        // Each new geometry's strip shape comes from the tuner, timed for
         // this batch's count of it; the chunk count follows the geometry
         // with the most pairs.
         uint32_t most = 0;
         for (const auto& [geometry, n] : per_geometry) {
             const auto [w, h] = geometry;
             const TuneChoice c = Tuner::Get().Choose(w, h, static_cast<PlaneFormat>(config_.plane_format), map_, n);
-            if (!pools_.HasPlan(w, h)) pools_.SetStripRows(w, h, c.strip_rows);
+            if (!pools_.HasPlan(w, h)) pools_.SetStripShape(w, h, c.strip_rows, c.strip_cols);
             if (n > most) {
                 most = n;
                 chunks = c.chunks;

@@ -118,7 +118,7 @@ struct DevicePlan {
     WorkspaceLayout layout;
     DeviceBuffer plan;   // one Plan
     DeviceBuffer items;  // ScaleItem[items]
-    DevicePlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows);
+    DevicePlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows, uint32_t strip_cols);
 };
 
 // One pair's device workspace: planes (zero borders), partials, norms and,
@@ -165,14 +165,14 @@ private:
 // is freed. Not thread-safe.
 class WorkspacePools {
 public:
-    WorkspacePools(PlaneFormat format, uint32_t strip_rows, uint64_t budget);
+    WorkspacePools(PlaneFormat format, uint32_t strip_rows, uint32_t strip_cols, uint64_t budget);
     const DevicePlan& PlanFor(uint32_t width, uint32_t height);
     bool HasPlan(uint32_t width, uint32_t height) const {
         return plans_.count(std::make_pair(width, height)) != 0;
     }
-    // The strip height of a geometry whose plan does not exist yet.
-    void SetStripRows(uint32_t width, uint32_t height, uint32_t strip_rows) {
-        strip_rows_for_[std::make_pair(width, height)] = strip_rows;
+    // The strip height and width of a geometry whose plan does not exist yet.
+    void SetStripShape(uint32_t width, uint32_t height, uint32_t strip_rows, uint32_t strip_cols) {
+        shape_for_[std::make_pair(width, height)] = std::make_pair(strip_rows, strip_cols);
     }
     // Device bytes one workspace of this geometry holds.
     uint64_t Need(uint32_t width, uint32_t height, bool inputs);
@@ -198,11 +198,11 @@ private:
     };
     void Drop(std::unique_ptr<Workspace> ws);
     PlaneFormat format_;
-    uint32_t strip_rows_;
+    uint32_t strip_rows_, strip_cols_;
     uint64_t budget_;
     std::unique_ptr<Arena> arena_;
     std::map<std::pair<uint32_t, uint32_t>, std::unique_ptr<DevicePlan>> plans_;
-    std::map<std::pair<uint32_t, uint32_t>, uint32_t> strip_rows_for_;
+    std::map<std::pair<uint32_t, uint32_t>, std::pair<uint32_t, uint32_t>> shape_for_;
     std::map<Key, Pool> pools_;
     uint64_t bytes_ = 0, peak_ = 0;
 };
@@ -367,6 +367,9 @@ public:
     void Finish(uint64_t token, uint32_t* scales, double* norms);
     std::array<float, 3> StageTimes() const { return stage_ms_; }
     uint64_t PoolBytes() const { return pools_.bytes(); }
+    // Fixes the strip shape of a geometry not used yet (a shape tuned
+    // earlier, or by another process).
+    void SetShape(uint32_t width, uint32_t height, uint32_t strip_rows, uint32_t strip_cols);
 
 private:
     struct Taken {

@@ -25,7 +25,8 @@ const vqm_ss2_server_config& ValidatedServer(const vqm_ss2_server_config& c) {
     if (c.max_batch > 4096) Fail(VQM_INVALID_ARGUMENT, "max_batch must be between 1 and 4096");
     if (c.blur > kBlurRecursive) Fail(VQM_INVALID_ARGUMENT, "blur must be sampled (0) or recursive (1)");
     ToMapForm(c.map_form, static_cast<PlaneFormat>(c.plane_format));
-    BuildPlan(8, 8, static_cast<PlaneFormat>(c.plane_format), c.strip_rows);  // validates strip_rows
+    // Validates strip_rows and strip_cols.
+    BuildPlan(8, 8, static_cast<PlaneFormat>(c.plane_format), c.strip_rows, c.strip_cols);
     return c;
 }
 
@@ -42,7 +43,8 @@ Server::Server(const vqm_ss2_server_config& config)
       ctx_(DeviceContext::Get()),
       format_(static_cast<PlaneFormat>(config.plane_format)),
       map_(ToMapForm(config.map_form, static_cast<PlaneFormat>(config.plane_format))),
-      pools_(static_cast<PlaneFormat>(config.plane_format), config.strip_rows, config.hbm_budget) {
+      pools_(static_cast<PlaneFormat>(config.plane_format), config.strip_rows, config.strip_cols,
+             config.hbm_budget) {
     for (uint32_t i = 0; i < config_.lanes; ++i) {
         lanes_.push_back(std::make_unique<Lane>(config_.max_batch));
         free_lanes_.push_back(config_.lanes - 1 - i);

@@ -52,39 +52,44 @@ TuneChoice Tuner::Choose(uint32_t width, uint32_t height, PlaneFormat format, Ma
     TuneChoice best;
     float best_per_pair = std::numeric_limits<float>::infinity();
     for (uint32_t strip : kStripRows) {
-        WorkspacePools pools(format, strip, 0);
-        const DevicePlan& plan = pools.PlanFor(width, height);
-        const uint64_t each = pools.Need(width, height, false);
-        const uint32_t n = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(cls, kTuneBytes / each)));
-        Lane lane(n);
-        std::vector<std::unique_ptr<Workspace>> ws;
-        std::vector<LaunchEntry> entries(n);
-        for (uint32_t i = 0; i < n; ++i) {
-            ws.push_back(pools.Acquire(width, height, false));
-            entries[i].plan = &plan;
-            entries[i].ws = ws.back().get();
-            entries[i].src[0] = src[0];
-            entries[i].src[1] = src[1];
-        }
-        for (uint32_t chunks : kChunks) {
-            if (chunks > n) continue;
-            lane.Launch(format, map, ctx.lut, entries.data(), n, chunks);  // warm-up
-            lane.Wait();
-            float ms = std::numeric_limits<float>::infinity();
-            for (int r = 0; r < kRepeats; ++r) {
-                const auto t0 = std::chrono::steady_clock::now();
-                lane.Launch(format, map, ctx.lut, entries.data(), n, chunks);
-                lane.Wait();
-                const auto t1 = std::chrono::steady_clock::now();
-                ms = std::min(ms, std::chrono::duration<float, std::milli>(t1 - t0).count());
+        for (uint32_t cols : kStripWidths) {
+            WorkspacePools pools(format, strip, cols, 0);
+            const DevicePlan& plan = pools.PlanFor(width, height);
+            const uint64_t each = pools.Need(width, height, false);
+            const uint32_t n =
+                static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(cls, kTuneBytes / each)));
+            Lane lane(n);
+            std::vector<std::unique_ptr<Workspace>> ws;
+            std::vector<LaunchEntry> entries(n);
+            for (uint32_t i = 0; i < n; ++i) {
+                ws.push_back(pools.Acquire(width, height, false));
+                entries[i].plan = &plan;
+                entries[i].ws = ws.back().get();
+                entries[i].src[0] = src[0];
+                entries[i].src[1] = src[1];
             }
-            const float per_pair = ms / static_cast<float>(n);
-            ++best.candidates;
-            if (strip == 64 && chunks == 1) best.default_ms = per_pair * static_cast<float>(cls);
-            if (per_pair < best_per_pair) {
-                best_per_pair = per_pair;
-                best.strip_rows = strip;
-                best.chunks = chunks;
+            for (uint32_t chunks : kChunks) {
+                if (chunks > n) continue;
+                lane.Launch(format, map, ctx.lut, entries.data(), n, chunks);  // warm-up
+                lane.Wait();
+                float ms = std::numeric_limits<float>::infinity();
+                for (int r = 0; r < kRepeats; ++r) {
+                    const auto t0 = std::chrono::steady_clock::now();
+                    lane.Launch(format, map, ctx.lut, entries.data(), n, chunks);
+                    lane.Wait();
+                    const auto t1 = std::chrono::steady_clock::now();
+                    ms = std::min(ms, std::chrono::duration<float, std::milli>(t1 - t0).count());
+                }
+                const float per_pair = ms / static_cast<float>(n);
+                ++best.candidates;
+                if (strip == 64 && cols == kDefaultStripCols && chunks == 1)
+                    best.default_ms = per_pair * static_cast<float>(cls);
+                if (per_pair < best_per_pair) {
+                    best_per_pair = per_pair;
+                    best.strip_rows = strip;
+                    best.strip_cols = cols;
+                    best.chunks = chunks;
+                }
             }
         }
     }

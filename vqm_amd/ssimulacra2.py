@@ -428,10 +428,11 @@ class Ssimulacra2Config:
     streaming_inputs: bool = True
     memory_policy: int = 0  # MemoryPolicy: 0 automatic, 1 direct, 2 staged (unavailable)
     plane_format: Optional[int] = None
-    strip_rows: int = 0
+    strip_rows: int = 0  # scale work-item height, a multiple of 8 (0 = 64)
     graph: bool = False  # replay one captured HIP graph per pair (no reference field)
     blur: int = 0  # Blur: 0 sampled (the reference), 1 recursive (no reference field)
     map_form: int = 0  # MapForm: 0 default, 1 reference order, 2 difference (no reference field)
+    strip_cols: int = 0  # scale work-item width, 64 or 128 (0 = 128; no reference field)
 
     def to_c(self) -> L.ConfigC:
         c = L.ConfigC()
@@ -448,6 +449,7 @@ class Ssimulacra2Config:
         c.plane_format = fmt
         c.streaming_inputs = 1 if self.streaming_inputs else 0
         c.strip_rows = self.strip_rows
+        c.strip_cols = self.strip_cols
         c.input_bytes = self.input_bytes
         return c
 
@@ -940,8 +942,9 @@ class Ssimulacra2Batch:
 
     def __init__(self, max_pairs: int, plane_format: int = L.PLANES_F32, hbm_budget: int = 0,
                  strip_rows: int = 0, chunks: int = 1, autotune: bool = False, blur: int = 0,
-                 persistent: bool = False, map_form: int = 0):
+                 persistent: bool = False, map_form: int = 0, strip_cols: int = 0):
         cfg = L.BatchConfigC()
+        cfg.strip_cols = int(strip_cols)
         cfg.blur = int(blur)
         cfg.map_form = int(map_form)
         cfg.persistent = 1 if persistent else 0
@@ -967,6 +970,12 @@ class Ssimulacra2Batch:
             self.close()
         except Exception:
             pass
+
+    def SetShape(self, size: Size, strip_rows: int = 0, strip_cols: int = 0) -> None:
+        """Fixes the scale strip shape of one geometry before its first
+        submission (vqm_ss2_batch_set_shape)."""
+        _check(L.lib().vqm_ss2_batch_set_shape(self._h, size.width, size.height, int(strip_rows),
+                                               int(strip_cols)))
 
     def Submit(self, pairs: Sequence[tuple]) -> int:
         """pairs: (original_ptr, distorted_ptr, width, height) tuples."""
@@ -1077,9 +1086,11 @@ class Ssimulacra2Server:
     against `hbm_budget`, and overlap on the device. Thread-safe."""
 
     def __init__(self, plane_format: int = L.PLANES_F32, lanes: int = 3, max_batch: int = 64,
-                 hbm_budget: int = 0, strip_rows: int = 0, blur: int = 0, map_form: int = 0):
+                 hbm_budget: int = 0, strip_rows: int = 0, blur: int = 0, map_form: int = 0,
+                 strip_cols: int = 0):
         cfg = L.ServerConfigC()
         L.lib().vqm_ss2_server_config_init(ctypes.byref(cfg))
+        cfg.strip_cols = int(strip_cols)
         cfg.blur = int(blur)
         cfg.map_form = int(map_form)
         cfg.plane_format = plane_format
