@@ -270,6 +270,63 @@ def ladder_batch(args, pool):
     return out
 
 
+def general_decode(pairs, w, h, map_form, count=60):
+    """Pairs/s and per-stage times of one pipeline scoring resident device
+    images one pair at a time, in 8-bit 4:2:0 limited-range BT.709 YUV (the
+    general per-pixel decode of the prepare kernel) beside the same content
+    as 8-bit sRGB RGB (the table decode). The YUV planes are the RGB pair's
+    R plane as luma and its G and B planes subsampled 2x2 as chroma."""
+    from vqm_amd import (ColorFamily, DeviceBuffer, DeviceImage, ImageFormat, MatrixCoefficients, PixelRange,
+                         PixelType, Size, Ssimulacra2Config, Ssimulacra2Pipeline, TransferCharacteristics,
+                         required_bytes)
+    size = Size(w, h)
+    yuv = ImageFormat(type=PixelType.Byte, family=ColorFamily.Yuv, range=PixelRange.Limited,
+                      matrix=MatrixCoefficients.Bt709, transfer=TransferCharacteristics.Bt709,
+                      subsample_w=1, subsample_h=1)
+    plane = (w * h + 3) // 4 * 4
+
+    def to_yuv(img):
+        rgb = [img[k * plane:k * plane + w * h].reshape(h, w) for k in range(3)]
+        out = np.zeros(required_bytes(yuv, size), np.uint8)
+        out[:w * h] = rgb[0].ravel()
+        cw, ch = (w + 1) // 2, (h + 1) // 2
+        off = plane
+        for k in (1, 2):
+            out[off:off + cw * ch] = rgb[k][::2, ::2].ravel()
+            off += (cw * ch + 3) // 4 * 4
+        return out
+
+    out = {"workload": f"{w}x{h}, {count} resident pairs scored one at a time by one pipeline"}
+    for name, fmt, conv in (("yuv420p8_bt709_limited", yuv, to_yuv), ("srgb8", ImageFormat.srgb8(), None)):
+        bufs = []
+        for a, b in pairs[:count]:
+            pair = []
+            for img in (a, b):
+                data = conv(img) if conv else img
+                buf = DeviceBuffer(data.nbytes)
+                buf.upload(data)
+                pair.append(buf)
+            bufs.append(pair)
+        pipe = Ssimulacra2Pipeline(Ssimulacra2Config(size=size, map_form=map_form))
+        for da, db in bufs[:3]:
+            pipe.Compute(DeviceImage(da.ptr, size, fmt), DeviceImage(db.ptr, size, fmt))
+        stages = []
+        t0 = time.perf_counter()
+        for da, db in bufs:
+            pipe.Compute(DeviceImage(da.ptr, size, fmt), DeviceImage(db.ptr, size, fmt))
+            stages.append(pipe.StageTimes())
+        dt = time.perf_counter() - t0
+        pipe.close()
+        for pair in bufs:
+            for buf in pair:
+                buf.free()
+        out[name] = {"pairs_per_s": round(len(bufs) / dt, 1),
+                     "stage_ms": {k: round(statistics.mean(st[i] for st in stages), 4)
+                                  for i, k in enumerate(("prepare", "scale", "reduce"))}}
+    log(f"general decode: {out}")
+    return out
+
+
 def resident_batch(refs, w, h, fmt, steps, warmup, chunks=1, strip_rows=0, persistent=False, map_form=0,
                    strip_cols=0):
     """Pairs/s of the batch engine over resident pairs, with the per-stage
@@ -424,6 +481,8 @@ def main() -> int:
     ap.add_argument("--no-modes", action="store_true")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the 4K batch and the other plane forms at the main workload")
+    ap.add_argument("--general-decode-only", action="store_true",
+                    help="run only the general-decode leg (8-bit 4:2:0 YUV beside 8-bit sRGB) and print it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run the plain copy kernel over MIB MiB (traffic counter reference)")
     args = ap.parse_args()
@@ -443,12 +502,6 @@ def main() -> int:
     info = device_info()
     log(f"device {info['name']} {info['arch']} cus={info['cus']}")
 
-    t0 = time.perf_counter()
-    pairs = make_pairs(w, h, cfg, args.pairs_per_strength)
-    log(f"generated {len(pairs)} pairs of {w}x{h} in {time.perf_counter() - t0:.1f} s")
-    dev, refs = upload_pairs(pairs, w, h)
-    log(f"uploaded {2 * len(pairs)} images ({dev.nbytes / 1e9:.2f} GB)")
-
     if args.copy_calibration:
         import ctypes
         from vqm_amd import _lib
@@ -462,6 +515,15 @@ def main() -> int:
         log(f"copy calibration: 3 copies of {args.copy_calibration} MiB")
         src_buf.free()
         dst_buf.free()
+
+    t0 = time.perf_counter()
+    pairs = make_pairs(w, h, cfg, args.pairs_per_strength)
+    log(f"generated {len(pairs)} pairs of {w}x{h} in {time.perf_counter() - t0:.1f} s")
+    if args.general_decode_only:
+        print(json.dumps({"general_decode": general_decode(pairs, w, h, map_form)}), flush=True)
+        return 0
+    dev, refs = upload_pairs(pairs, w, h)
+    log(f"uploaded {2 * len(pairs)} images ({dev.nbytes / 1e9:.2f} GB)")
 
     batch = Ssimulacra2Batch(max_pairs=len(refs), plane_format=fmt, strip_rows=args.strip_rows,
                              chunks=args.chunks, persistent=args.persistent, map_form=map_form,
@@ -570,6 +632,9 @@ def main() -> int:
             dev4.free()
             del pairs4
             log(f"4k: {out['workload_4k']}")
+
+    if not args.no_secondary:
+        out["general_decode"] = general_decode(pairs, w, h, map_form)
 
     if not args.no_streaming:
         out["streaming_pool"] = {"1920x1080": streaming_pool(1920, 1080, 1, map_form, count=90),

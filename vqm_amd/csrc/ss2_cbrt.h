@@ -4,10 +4,10 @@
 //
 //   cbrt_fast  <= 0.84 ulp on [2^-126, 64); 16 FP32 operations.
 //   cbrt_rn    the cube root correctly rounded to FP32 (round to nearest),
-//              for every normal positive float: two FP64 third-order steps
-//              from cbrt_fast's seed, rounded to FP32, with two exact
-//              FP64 comparisons for the one input in 2^20 whose FP64 root
-//              lies too near a midpoint to round safely.
+//              for every normal positive float: a third-order and a
+//              second-order FP64 step from cbrt_fast's seed, rounded to
+//              FP32, with two exact FP64 comparisons for the one input in
+//              2^19 whose FP64 root lies too near a midpoint to round safely.
 //
 // Every step is an IEEE conversion, add, multiply or fused multiply-add
 // (written out), with contraction off, so the result does not depend on the
@@ -78,29 +78,30 @@ __host__ __device__ inline float cbrt_round_exact(float x, float y) {
 }
 
 // The cube root of a positive normal float, correctly rounded. The FP32
-// seed of cbrt_fast (x^(-1/3) within 3.7%), then two third-order steps
-// r <- r (1 + h (1/3 + h (2/9 + h 14/81))), h = 1 - x r^3, in FP64 (the
-// error falls to ~1e-5, then below FP64 rounding), and y = x r^2 in FP64,
-// within 2^-50 of the root. Rounding y to FP32 gives the correctly rounded
-// root unless y lies within 2^-44 of a midpoint between two floats: the 29
-// bits FP32 drops from y's significand (all in its low word) then sit
-// within 2^8 units of 2^28 (one input in 2^20). Those take cbrt_round_exact.
+// seed of cbrt_fast (x^(-1/3) within 3.7%), then in FP64 one third-order
+// step r <- r (1 + h (1/3 + h (2/9 + h 14/81))), h = 1 - x r^3 (the error
+// falls to |h| < 5e-5), and one second-order step r <- r (1 + h (1/3 +
+// h 2/9)) (the dropped term 14/81 h^3 is below 2.2e-14), and y = x r^2 in
+// FP64: within 2^-44.4 of the root, relatively (the largest error over every
+// float in [2^-126, 2^24); the seed's bit pattern, and with it the error,
+// repeats every three binades). Rounding y to FP32 gives the correctly
+// rounded root unless y lies within 2^-44 of a midpoint between two floats:
+// the 29 bits FP32 drops from y's significand (all in its low word) then sit
+// within 2^9 units of 2^28 (one input in 2^19). Those take cbrt_round_exact.
 __host__ __device__ inline float cbrt_rn(float x) {
     constexpr float kThird = 0x1.555556p-2f, kMagic = 1419910400.0f;
     constexpr double kThirdD = 1.0 / 3.0, kC2 = 2.0 / 9.0, kC3 = 14.0 / 81.0;
     const float seed = __builtin_fmaf(static_cast<float>(cbrt_bits(x)), -kThird, kMagic);
     const double xd = x;
     double r = cbrt_float(static_cast<uint32_t>(seed));
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const double h = __builtin_fma(-xd, (r * r) * r, 1.0);
-        const double p = __builtin_fma(h, __builtin_fma(h, kC3, kC2), kThirdD);
-        r = __builtin_fma(r * h, p, r);
-    }
+    double h = __builtin_fma(-xd, (r * r) * r, 1.0);
+    r = __builtin_fma(r * h, __builtin_fma(h, __builtin_fma(h, kC3, kC2), kThirdD), r);
+    h = __builtin_fma(-xd, (r * r) * r, 1.0);
+    r = __builtin_fma(r * h, __builtin_fma(h, kC2, kThirdD), r);
     const double y = xd * (r * r);
     const float rounded = static_cast<float>(y);
     const uint32_t dropped = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, y)) & 0x1FFFFFFFu;
-    if (dropped - (0x10000000u - 0x100u) > 2 * 0x100u) return rounded;  // far from a midpoint
+    if (dropped - (0x10000000u - 0x200u) > 2 * 0x200u) return rounded;  // far from a midpoint
     return cbrt_round_exact(x, rounded);
 }
 

@@ -515,14 +515,20 @@ __device__ __forceinline__ void prepare_level(ConstAS<Plan>* plan, GlobalAS<uint
     if (gx < plan->w[L] && gy < plan->h[L]) {
         const uint32_t ox = X0 >> (L - 1u), oy = Y0 >> (L - 1u);
         const uint32_t cx0 = 2u * jx, cy0 = 2u * jy;
-        const uint32_t cx1 = min(2u * gx + 1u, plan->w[L - 1] - 1u) - ox;
+        // The block's right column is cx0 + 1 unless the level's last
+        // column clamps it onto cx0 (odd widths); each row's pair is one
+        // 8-byte read (cx0 is even, cx0 + 1 stays inside the ladder row), so
+        // a wave reads whole contiguous rows without bank conflicts.
+        const bool clamp_x = min(2u * gx + 1u, plan->w[L - 1] - 1u) - ox == cx0;
         const uint32_t cy1 = min(2u * gy + 1u, plan->h[L - 1] - 1u) - oy;
 #pragma unroll
         for (uint32_t c = 0; c < 3; ++c) {
-            float s = prev[c][cy0 * prev_x + cx0];
-            s = s + prev[c][cy0 * prev_x + cx1];
-            s = s + prev[c][cy1 * prev_x + cx0];
-            s = s + prev[c][cy1 * prev_x + cx1];
+            const float2 top = *reinterpret_cast<const float2*>(&prev[c][cy0 * prev_x + cx0]);
+            const float2 bottom = *reinterpret_cast<const float2*>(&prev[c][cy1 * prev_x + cx0]);
+            float s = top.x;
+            s = s + (clamp_x ? top.x : top.y);
+            s = s + bottom.x;
+            s = s + (clamp_x ? bottom.x : bottom.y);
             v[c] = linear_sample<Lay>(s * 0.25f);
         }
         const uint32_t mask = plan->mask[L];
