@@ -297,10 +297,25 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
 // SourceDesc, in the stage order of color.glsl:447-541: raw sample with the
 // range/depth scale and offset, chroma upsampling, the matrix (or the
 // constant-luminance or ICtCp op), alpha, the transfer curve, and the
-// primaries matrix to linear BT.709. These stages use the device's powf/expf
-// and are held to the A/B tolerances of the input tests, not bit parity.
+// primaries matrix to linear BT.709. Powers and exponentials go through the
+// hardware base-2 logarithm and exponential (as a shader's pow does), and
+// the stages are held to the A/B tolerances of the input tests, not bit
+// parity.
 
 namespace decode {
+
+// x^y for x >= 0 from v_log_f32 and v_exp_f32 (about one ulp each): relative
+// error about |y log2 x| 2^-23 plus one ulp, some 150 instructions fewer than
+// the library powf. Inputs below the smallest normal float are taken as it
+// (their powers are 0 to FP32 precision for y >= 1; the curves with y < 1
+// map them to 0 after their own offsets).
+__device__ __forceinline__ float pow_hw(float x, float y) {
+    if (x <= 0.0f) return 0.0f;
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(fmaxf(x, 0x1p-126f)));
+}
+// e^x and 10^x through v_exp_f32.
+__device__ __forceinline__ float exp_hw(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+__device__ __forceinline__ float exp10_hw(float x) { return __builtin_amdgcn_exp2f(x * 3.32192809488736235f); }
 
 constexpr float kRec709Alpha = 1.09929682680944f;
 constexpr float kRec709Beta = 0.018053968510807f;
@@ -325,46 +340,46 @@ __device__ float raw_sample(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* src
 __device__ float rec709_inverse_oetf(float x) {
     x = fmaxf(x, 0.0f);
     return x < 4.5f * kRec709Beta ? x / 4.5f
-                                  : powf((x + (kRec709Alpha - 1.0f)) / kRec709Alpha, 1.0f / 0.45f);
+                                  : pow_hw((x + (kRec709Alpha - 1.0f)) / kRec709Alpha, 1.0f / 0.45f);
 }
 
 // IEC 61966-2-1.
 __device__ float srgb_eotf(float v) {
-    return v <= 0.04045f ? v / 12.92f : powf((v + 0.055f) / 1.055f, 2.4f);
+    return v <= 0.04045f ? v / 12.92f : pow_hw((v + 0.055f) / 1.055f, 2.4f);
 }
 
 // SMPTE ST 2084, normalised to 10000 cd/m^2.
 __device__ float pq_eotf(float x) {
     if (x <= 0.0f) return 0.0f;
-    const float p = powf(x, 1.0f / 78.84375f);
+    const float p = pow_hw(x, 1.0f / 78.84375f);
     const float num = fmaxf(p - 0.8359375f, 0.0f);
     const float den = fmaxf(18.8515625f - 18.6875f * p, 1.17549435e-38f);
-    return powf(num / den, 1.0f / 0.1593017578125f);
+    return pow_hw(num / den, 1.0f / 0.1593017578125f);
 }
 
 // ARIB STD-B67 inverse OETF (scene light).
 __device__ float hlg_inverse_oetf(float x) {
     x = fmaxf(x, 0.0f);
-    return x <= 0.5f ? (x * x) * (1.0f / 3.0f) : (expf((x - 0.55991073f) / 0.17883277f) + 0.28466892f) / 12.0f;
+    return x <= 0.5f ? (x * x) * (1.0f / 3.0f) : (exp_hw((x - 0.55991073f) / 0.17883277f) + 0.28466892f) / 12.0f;
 }
 
 __device__ float curve(uint32_t id, float v, float scale) {
     switch (id) {
         case kCurveLinear: return v;
-        case kCurvePow24: return v < 0.0f ? 0.0f : powf(v, 2.4f);
-        case kCurvePow22: return v < 0.0f ? 0.0f : powf(v, 2.2f);
-        case kCurvePow28: return v < 0.0f ? 0.0f : powf(v, 2.8f);
+        case kCurvePow24: return v < 0.0f ? 0.0f : pow_hw(v, 2.4f);
+        case kCurvePow22: return v < 0.0f ? 0.0f : pow_hw(v, 2.2f);
+        case kCurvePow28: return v < 0.0f ? 0.0f : pow_hw(v, 2.8f);
         case kCurveSrgb: return srgb_eotf(v);
-        case kCurveLog100: return v <= 0.0f ? 0.01f : powf(10.0f, 2.0f * (v - 1.0f));
-        case kCurveLog316: return v <= 0.0f ? 0.00316227766f : powf(10.0f, 2.5f * (v - 1.0f));
+        case kCurveLog100: return v <= 0.0f ? 0.01f : exp10_hw(2.0f * (v - 1.0f));
+        case kCurveLog316: return v <= 0.0f ? 0.00316227766f : exp10_hw(2.5f * (v - 1.0f));
         case kCurveXvycc:
-            if (v >= 0.0f && v <= 1.0f) return powf(v, 2.4f);
+            if (v >= 0.0f && v <= 1.0f) return pow_hw(v, 2.4f);
             return v < 0.0f ? -rec709_inverse_oetf(-v) : rec709_inverse_oetf(v);
         case kCurveBt1361:
-            if (v >= 0.0f && v <= 1.0f) return powf(v, 2.4f);
+            if (v >= 0.0f && v <= 1.0f) return pow_hw(v, 2.4f);
             return v >= 0.0f ? rec709_inverse_oetf(v) : -rec709_inverse_oetf(-4.0f * v) / 4.0f;
         case kCurvePq: return scale * pq_eotf(v);
-        case kCurveSt428: return v < 0.0f ? 0.0f : (52.37f / 48.0f) * powf(v, 2.6f);
+        case kCurveSt428: return v < 0.0f ? 0.0f : (52.37f / 48.0f) * pow_hw(v, 2.6f);
         default: return v;
     }
 }
@@ -420,7 +435,7 @@ constexpr int kMaxTaps = 6;
 __device__ int axis_taps(uint32_t filter, uint32_t out, uint32_t sub, float shift, uint32_t dim,
                          float w[kMaxTaps], uint32_t idx[kMaxTaps]) {
     const int taps = filter == 0 ? 2 : filter == 1 ? 4 : 6;
-    const float pos = (static_cast<float>(out) + 0.5f) / static_cast<float>(1u << sub) + shift;
+    const float pos = (static_cast<float>(out) + 0.5f) * __builtin_ldexpf(1.0f, -static_cast<int>(sub)) + shift;
     const float base = round_halfup(pos - 0.5f * static_cast<float>(taps)) + 0.5f;
     if (filter == 0) {
         const float t = pos - base;
@@ -435,29 +450,37 @@ __device__ int axis_taps(uint32_t filter, uint32_t out, uint32_t sub, float shif
     return taps;
 }
 
-__device__ float chroma_sample(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* src, uint32_t plane,
-                               uint32_t x, uint32_t y) {
+// Both chroma samples of pixel (x, y): the two chroma planes share their
+// size and siting, so one set of taps serves both.
+__device__ float2 chroma_samples(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* src, uint32_t x, uint32_t y) {
     float wx[kMaxTaps], wy[kMaxTaps];
     uint32_t ix[kMaxTaps], iy[kMaxTaps];
-    const int tx = axis_taps(d->filter, x, d->sub_w, d->shift_x, d->plane_w[plane], wx, ix);
-    const int ty = axis_taps(d->filter, y, d->sub_h, d->shift_y, d->plane_h[plane], wy, iy);
-    float sum = 0.0f;
+    const int tx = axis_taps(d->filter, x, d->sub_w, d->shift_x, d->plane_w[1], wx, ix);
+    const int ty = axis_taps(d->filter, y, d->sub_h, d->shift_y, d->plane_h[1], wy, iy);
+    float sum_u = 0.0f, sum_v = 0.0f;
     for (int j = 0; j < ty; ++j) {
-        float row = 0.0f;
-        for (int k = 0; k < tx; ++k) row += wx[k] * raw_sample(d, src, plane, ix[k], iy[j]);
-        sum += wy[j] * row;
+        float row_u = 0.0f, row_v = 0.0f;
+        for (int k = 0; k < tx; ++k) {
+            row_u += wx[k] * raw_sample(d, src, 1, ix[k], iy[j]);
+            row_v += wx[k] * raw_sample(d, src, 2, ix[k], iy[j]);
+        }
+        sum_u += wy[j] * row_u;
+        sum_v += wy[j] * row_v;
     }
-    return sum;
+    return make_float2(sum_u, sum_v);
 }
 
-// Linear BT.709 RGB of pixel (x, y).
-__device__ float3 pixel(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* src, uint32_t x, uint32_t y) {
+// Linear BT.709 RGB of pixel (x, y). Inlined into its eight call sites (a
+// called function measured 12% slower on 8-bit 4:2:0 frames).
+__device__ __forceinline__ float3 pixel(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* src, uint32_t x,
+                                       uint32_t y) {
     const float alpha = d->alpha != 0 ? raw_sample(d, src, 3, x, y) : 1.0f;
     const float y0 = raw_sample(d, src, 0, x, y);
     float3 e = make_float3(y0, y0, y0);
     if (d->family == 2 && (d->sub_w != 0 || d->sub_h != 0)) {
-        e.y = chroma_sample(d, src, 1, x, y);
-        e.z = chroma_sample(d, src, 2, x, y);
+        const float2 uv = chroma_samples(d, src, x, y);
+        e.y = uv.x;
+        e.z = uv.y;
     } else if (d->family != 0) {
         e.y = raw_sample(d, src, 1, x, y);
         e.z = raw_sample(d, src, 2, x, y);
@@ -477,7 +500,7 @@ __device__ float3 pixel(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* src, ui
             // Per-channel inverse OETF, then the luminance OOTF and the peak.
             const float3 c = make_float3(hlg_inverse_oetf(rgb.x), hlg_inverse_oetf(rgb.y), hlg_inverse_oetf(rgb.z));
             const float luma = fmaxf(d->kr * c.x + d->kg * c.y + d->kb * c.z, 1.17549435e-38f);
-            const float k = powf(luma, 0.2f) * d->to_linear_scale;
+            const float k = pow_hw(luma, 0.2f) * d->to_linear_scale;
             lin = make_float3(c.x * k, c.y * k, c.z * k);
         } else {
             lin = make_float3(curve(d->transfer, rgb.x, d->to_linear_scale),
