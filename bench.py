@@ -674,6 +674,25 @@ def main() -> int:
                                           "p50_ms": round(float(np.percentile(lat, 50)), 4),
                                           "p99_ms": round(float(np.percentile(lat, 99)), 4)}
         pipe.close()
+        # Small batches of resident pairs, eager and replayed from captured
+        # graphs: where launch overhead is a visible share of a batch.
+        from vqm_amd import Ssimulacra2Batch as _Batch
+        small = {}
+        for n in (1, 4, 16):
+            for graph in (False, True):
+                b = _Batch(max_pairs=n, plane_format=fmt, map_form=map_form, graph=graph)
+                chunk = refs[:n]
+                for _ in range(3):
+                    b.Finish(b.Submit(chunk), raw=True)
+                reps = max(10, 240 // n)
+                t = time.perf_counter()
+                for _ in range(reps):
+                    b.Finish(b.Submit(chunk), raw=True)
+                dt = time.perf_counter() - t
+                b.close()
+                small[f"batch{n}_{'graph' if graph else 'eager'}"] = {
+                    "pairs_per_s": round(n * reps / dt, 1), "ms_per_batch": round(dt / reps * 1e3, 4)}
+        modes["small_batches"] = small
         out["modes"] = modes
         log(f"modes: {modes}")
 

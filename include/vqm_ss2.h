@@ -408,6 +408,12 @@ typedef struct vqm_ss2_batch_config {
                                 fixed stride; 0: one block per tile or strip */
     uint32_t map_form;       /* VQM_MAP_* */
     uint32_t strip_cols;     /* 64 or 128; 0 = default (128) */
+    uint32_t graph;          /* 1: a submission of one geometry in one run
+                                (chunks 1), 8-bit sRGB device inputs, is
+                                replayed from a HIP graph captured the first
+                                time its shape (geometry, pair count) is seen;
+                                its stage times then report the whole graph
+                                as the prepare stage. 0 (default): eager. */
 } vqm_ss2_batch_config;
 
 /* One pair of caller-owned device images (packed u8 sRGB RGB). */
@@ -496,6 +502,9 @@ typedef struct vqm_ss2_server_config {
     uint32_t blur;           /* VQM_BLUR_SAMPLED (default) or VQM_BLUR_RECURSIVE */
     uint32_t map_form;       /* VQM_MAP_* */
     uint32_t strip_cols;     /* 64 or 128; 0 = default (128) */
+    uint32_t graph;          /* 1: batches of one geometry with device inputs
+                                are replayed from HIP graphs, one per (geometry,
+                                batch size) a lane has seen (at most 32 a lane) */
 } vqm_ss2_server_config;
 
 /* The outcome of one request. Times are vqm_now_ns() readings. */

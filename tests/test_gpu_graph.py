@@ -47,3 +47,66 @@ def test_graph_replays_equal_eager_launches(w, h, fmt):
     # A second round over the same pairs in reverse order replays the same graph.
     for i, (a, b) in reversed(list(enumerate(pairs))):
         assert np.array_equal(graph.Compute(img(a), img(b)).norms(), eager.Compute(img(a), img(b)).norms())
+
+
+def _device_pairs(shapes, seed):
+    pairs, keep = [], []
+    for i, (w, h) in enumerate(shapes):
+        a, b = random_pair(w, h, seed=seed + i)
+        da, db = DeviceBuffer(a.nbytes), DeviceBuffer(b.nbytes)
+        da.upload(a)
+        db.upload(b)
+        keep += [da, db]
+        pairs.append((da.ptr, db.ptr, w, h))
+    return pairs, keep
+
+
+@pytest.mark.parametrize("map_form", [vqm_amd.MapForm.Reference, vqm_amd.MapForm.Difference])
+def test_batch_graph_replays_equal_eager_batches(map_form):
+    """A batch with graphs replays one captured graph per (geometry, pair
+    count): every replay equals the eager batch bit for bit, with other eager
+    batches and host synchronisations between the replays, over fresh
+    workspaces and inputs at new addresses; a mixed batch and a chunked one
+    run eagerly and still agree."""
+    eager = vqm_amd.Ssimulacra2Batch(max_pairs=8, map_form=map_form)
+    graph = vqm_amd.Ssimulacra2Batch(max_pairs=8, map_form=map_form, graph=True)
+    other = vqm_amd.Ssimulacra2Batch(max_pairs=4)
+    opairs, okeep = _device_pairs([(96, 64)] * 4, seed=7)
+    for rnd in range(4):
+        pairs, keep = _device_pairs([(320, 180)] * (5 + rnd % 2), seed=100 + 10 * rnd)
+        _, want = eager.Finish(eager.Submit(pairs), raw=True)
+        for rep in range(3):
+            _, got = graph.Finish(graph.Submit(pairs), raw=True)
+            assert np.array_equal(got, want), f"round {rnd} replay {rep}"
+            assert graph.StageTimes()[0] > 0.0
+            other.Finish(other.Submit(opairs), raw=True)
+            vqm_amd.ssimulacra2.L.lib().vqm_device_synchronize()
+    mixed, mkeep = _device_pairs([(320, 180), (64, 48), (200, 130)], seed=300)
+    _, want = eager.Finish(eager.Submit(mixed), raw=True)
+    _, got = graph.Finish(graph.Submit(mixed), raw=True)
+    assert np.array_equal(got, want)
+    chunked = vqm_amd.Ssimulacra2Batch(max_pairs=8, map_form=map_form, graph=True, chunks=2)
+    pairs, keep = _device_pairs([(320, 180)] * 6, seed=400)
+    _, want = eager.Finish(eager.Submit(pairs), raw=True)
+    _, got = chunked.Finish(chunked.Submit(pairs), raw=True)
+    assert np.array_equal(got, want)
+    for b in (eager, graph, other, chunked):
+        b.close()
+
+
+def test_server_graph_results_equal_eager_server():
+    """A server whose lanes replay graphs returns, for each request, the
+    norms of the same request on an eager server."""
+    shapes = [(320, 180)] * 12 + [(200, 130)] * 6
+    pairs, keep = _device_pairs(shapes, seed=500)
+    results = []
+    for graph in (False, True):
+        with vqm_amd.Ssimulacra2Server(lanes=2, max_batch=4, graph=graph) as server:
+            got = []
+            for _ in range(3):
+                tickets = [server.SubmitDevice(a, b, w, h) for a, b, w, h in pairs]
+                got.append([server.Wait(t).norms for t in tickets])
+            results.append(got)
+    for rnd in range(3):
+        for i in range(len(pairs)):
+            assert np.array_equal(results[1][rnd][i], results[0][0][i]), f"round {rnd} request {i}"

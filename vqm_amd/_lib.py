@@ -72,7 +72,7 @@ class MemoryHeapC(ctypes.Structure):
 class BatchConfigC(ctypes.Structure):
     _fields_ = [("plane_format", u32), ("strip_rows", u32), ("max_pairs", u32),
                 ("chunks", u32), ("hbm_budget", u64), ("autotune", u32), ("blur", u32),
-                ("persistent", u32), ("map_form", u32), ("strip_cols", u32)]
+                ("persistent", u32), ("map_form", u32), ("strip_cols", u32), ("graph", u32)]
 
 
 class TuneResultC(ctypes.Structure):
@@ -86,7 +86,8 @@ class DevicePairC(ctypes.Structure):
 
 class ServerConfigC(ctypes.Structure):
     _fields_ = [("plane_format", u32), ("lanes", u32), ("max_batch", u32), ("strip_rows", u32),
-                ("hbm_budget", u64), ("blur", u32), ("map_form", u32), ("strip_cols", u32)]
+                ("hbm_budget", u64), ("blur", u32), ("map_form", u32), ("strip_cols", u32),
+                ("graph", u32)]
 
 
 class RequestResultC(ctypes.Structure):

@@ -267,17 +267,52 @@ void WorkspacePools::Return(uint32_t width, uint32_t height, bool inputs, std::u
 
 // ---- lane -----------------------------------------------------------------------
 
-Lane::Lane(uint32_t max_pairs)
+Lane::Lane(uint32_t max_pairs, bool graphs)
     : max_pairs_(max_pairs),
       pairs_(max_pairs * sizeof(PairDesc)),
       norms_(static_cast<uint64_t>(max_pairs) * kNorms * sizeof(double)),
       h_pairs_(max_pairs * sizeof(PairDesc)),
-      h_norms_(static_cast<uint64_t>(max_pairs) * kNorms * sizeof(double)) {
+      h_norms_(static_cast<uint64_t>(max_pairs) * kNorms * sizeof(double)),
+      graphs_(graphs) {
     for (uint32_t i = 0; i < 4 * kMaxChunks + 3; ++i) ev_.push_back(std::make_unique<Event>());
 }
 
 Lane::~Lane() {
     if (pending_) (void)hipEventSynchronize(Ev(DoneEvent()).get());
+    for (auto& [key, exec] : graph_cache_) (void)hipGraphExecDestroy(exec);
+}
+
+// Captures one launch shape on the lane's first stream: the upload of the
+// pinned pair descriptors (rewritten before every replay: workspaces and
+// inputs change), the prepare, scale and reduce kernels over `count` pairs
+// of `plan`, and the read-back of their norms.
+hipGraphExec_t Lane::CaptureGraph(PlaneFormat fmt, MapForm map, const float* lut, const DevicePlan& plan,
+                                  uint32_t count, uint32_t blur, bool persistent) {
+    const hipStream_t s = stream_.get();
+    const GeometryPlan& g = plan.host;
+    const PairDesc* dpairs = pairs_.as<PairDesc>();
+    const uint64_t prep = 2ull * g.prep_tiles * count, scale = static_cast<uint64_t>(g.plan.items) * count;
+    Check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin graph capture");
+    hipError_t e = hipMemcpyAsync(pairs_.get(), h_pairs_.get(), count * sizeof(PairDesc), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = LaunchPrepare(fmt, map, false, dpairs, nullptr, static_cast<uint32_t>(prep), g.prep_tiles, lut, nullptr,
+                          nullptr, s, persistent);
+    if (e == hipSuccess)
+        e = LaunchScale(fmt, blur, map, g.plan.strip_w, dpairs, nullptr, static_cast<uint32_t>(scale), g.plan.items, s,
+                        persistent);
+    if (e == hipSuccess) e = LaunchReduce(dpairs, count, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h_norms_.get(), norms_.get(), static_cast<uint64_t>(count) * kNorms * sizeof(double),
+                           hipMemcpyDeviceToHost, s);
+    hipGraph_t graph = nullptr;
+    const hipError_t end = hipStreamEndCapture(s, &graph);
+    Check(e, "capture a batch launch");
+    Check(end, "end graph capture");
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    Check(e, "instantiate a batch launch's graph");
+    return exec;
 }
 
 void Lane::Launch(PlaneFormat fmt, MapForm map, const float* lut, const LaunchEntry* entries,
@@ -312,6 +347,27 @@ void Lane::Launch(PlaneFormat fmt, MapForm map, const float* lut, const LaunchEn
         Fail(VQM_INVALID_ARGUMENT, "the submission has too many work items");
 
     const hipStream_t s = stream_.get();
+    bool host_inputs = false;
+    for (uint32_t i = 0; i < count; ++i) host_inputs = host_inputs || entries[i].host[0] || entries[i].host[1];
+    replayed_ = graphs_ && uniform && chunks == 1 && !general && !host_inputs;
+    if (replayed_) {
+        const GraphKey key{entries[0].plan, count, static_cast<uint32_t>(fmt), static_cast<uint32_t>(map), blur,
+                           persistent};
+        auto it = graph_cache_.find(key);
+        if (it == graph_cache_.end()) {
+            if (graph_cache_.size() >= kMaxGraphs) {
+                for (auto& [k, exec] : graph_cache_) (void)hipGraphExecDestroy(exec);
+                graph_cache_.clear();
+            }
+            it = graph_cache_.emplace(key, CaptureGraph(fmt, map, lut, *entries[0].plan, count, blur, persistent)).first;
+        }
+        chunks_ = 1;
+        Check(hipEventRecord(Ev(0).get(), s), "hipEventRecord");
+        Check(hipGraphLaunch(it->second, s), "launch a batch graph");
+        Check(hipEventRecord(Ev(DoneEvent()).get(), s), "hipEventRecord");
+        pending_ = true;
+        return;
+    }
     const hipStream_t t = chunks > 1 ? scale_stream_.get() : s;
     for (uint32_t i = 0; i < count; ++i)
         for (int k = 0; k < 2; ++k)
@@ -430,6 +486,10 @@ void Lane::Synchronize() {
 
 std::array<float, 3> Lane::StageTimes() const {
     std::array<float, 3> t{};
+    if (replayed_) {
+        t[0] = Elapsed(Ev(0), Ev(DoneEvent()));
+        return t;
+    }
     for (uint32_t c = 0; c < chunks_; ++c) {
         t[0] += Elapsed(Ev(4 * c), Ev(4 * c + 1));
         t[1] += Elapsed(Ev(4 * c + 2), Ev(4 * c + 3));
@@ -841,6 +901,7 @@ const vqm_ss2_batch_config& ValidatedBatch(const vqm_ss2_batch_config& config) {
     if (config.chunks > Lane::kMaxChunks) Fail(VQM_INVALID_ARGUMENT, "chunks must be at most 16");
     if (config.blur > kBlurRecursive) Fail(VQM_INVALID_ARGUMENT, "blur must be sampled (0) or recursive (1)");
     if (config.persistent > 1) Fail(VQM_INVALID_ARGUMENT, "persistent must be 0 or 1");
+    if (config.graph > 1) Fail(VQM_INVALID_ARGUMENT, "graph must be 0 or 1");
     ToMapForm(config.map_form, static_cast<PlaneFormat>(config.plane_format));
     // Validates strip_rows and strip_cols.
     BuildPlan(8, 8, static_cast<PlaneFormat>(config.plane_format), config.strip_rows, config.strip_cols);
@@ -859,7 +920,7 @@ Batch::Batch(const vqm_ss2_batch_config& config)
       ctx_(DeviceContext::Get()),
       pools_(static_cast<PlaneFormat>(config.plane_format),
              config.strip_rows != 0 ? config.strip_rows : kBatchStripRows, config.strip_cols, config.hbm_budget),
-      lane_(config.max_pairs) {}
+      lane_(config.max_pairs, config.graph != 0) {}
 
 Batch::~Batch() = default;
 

@@ -227,7 +227,11 @@ struct LaunchEntry {
 class Lane {
 public:
     static constexpr uint32_t kMaxChunks = 16;
-    explicit Lane(uint32_t max_pairs);
+    // Instantiated graphs a lane keeps (one per launch shape).
+    static constexpr size_t kMaxGraphs = 32;
+    // graphs: replay launches that qualify from captured HIP graphs (see
+    // Launch).
+    explicit Lane(uint32_t max_pairs, bool graphs = false);
     ~Lane();
     // Queues the uploads, the prepare, scale and reduce launches and the norm
     // read-back, then returns. With chunks > 1 the pairs are split into that
@@ -237,7 +241,12 @@ public:
     // scale of one run overlaps the prepare of the next.
     // `blur` is the scale kernel's BlurKind, `map` both kernels' MapForm;
     // `persistent` sizes both kernels' grids to the resident occupancy
-    // (LaunchPrepare, LaunchScale).
+    // (LaunchPrepare, LaunchScale). With graphs on, a launch of one
+    // geometry, one run, 8-bit sRGB device inputs is replayed from a HIP
+    // graph captured the first time its shape (plan, pair count, format,
+    // map form, blur, persistent) is seen: the descriptor upload from the
+    // pinned descriptors written for this launch, the three kernels and the
+    // norm read-back, as one submission. It holds no memset node.
     void Launch(PlaneFormat format, MapForm map, const float* lut, const LaunchEntry* entries,
                 uint32_t count, uint32_t chunks = 1, uint32_t blur = kBlurSampled,
                 bool persistent = false);
@@ -249,8 +258,11 @@ public:
     // Norms of entry i of the finished launch (pinned host memory).
     const double* Norms(uint32_t i) const { return h_norms_.as<double>() + static_cast<uint64_t>(i) * kNorms; }
     // Milliseconds the prepare, scale and reduce launches of the finished
-    // launch ran, each summed over its chunks (chunks may overlap).
+    // launch ran, each summed over its chunks (chunks may overlap). A
+    // replayed graph reports its whole time as the first stage.
     std::array<float, 3> StageTimes() const;
+    // Whether the last launch was a graph replay.
+    bool replayed() const { return replayed_; }
     hipStream_t stream() const { return stream_.get(); }
     uint32_t max_pairs() const { return max_pairs_; }
 
@@ -264,6 +276,19 @@ private:
     std::vector<std::unique_ptr<Event>> ev_;
     uint32_t chunks_ = 1;
     bool pending_ = false;
+    struct GraphKey {
+        const DevicePlan* plan;
+        uint32_t count, format, map, blur;
+        bool persistent;
+        bool operator<(const GraphKey& o) const {
+            return std::tie(plan, count, format, map, blur, persistent) <
+                   std::tie(o.plan, o.count, o.format, o.map, o.blur, o.persistent);
+        }
+    };
+    bool graphs_ = false, replayed_ = false;
+    std::map<GraphKey, hipGraphExec_t> graph_cache_;
+    hipGraphExec_t CaptureGraph(PlaneFormat fmt, MapForm map, const float* lut, const DevicePlan& plan,
+                                uint32_t count, uint32_t blur, bool persistent);
     Event& Ev(uint32_t i) { return *ev_[i]; }
     const Event& Ev(uint32_t i) const { return *ev_[i]; }
     uint32_t DoneEvent() const { return 4 * chunks_ + 2; }

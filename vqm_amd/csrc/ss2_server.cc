@@ -24,6 +24,7 @@ const vqm_ss2_server_config& ValidatedServer(const vqm_ss2_server_config& c) {
     if (c.lanes > 16) Fail(VQM_INVALID_ARGUMENT, "lanes must be between 1 and 16");
     if (c.max_batch > 4096) Fail(VQM_INVALID_ARGUMENT, "max_batch must be between 1 and 4096");
     if (c.blur > kBlurRecursive) Fail(VQM_INVALID_ARGUMENT, "blur must be sampled (0) or recursive (1)");
+    if (c.graph > 1) Fail(VQM_INVALID_ARGUMENT, "graph must be 0 or 1");
     ToMapForm(c.map_form, static_cast<PlaneFormat>(c.plane_format));
     // Validates strip_rows and strip_cols.
     BuildPlan(8, 8, static_cast<PlaneFormat>(c.plane_format), c.strip_rows, c.strip_cols);
@@ -46,7 +47,7 @@ Server::Server(const vqm_ss2_server_config& config)
       pools_(static_cast<PlaneFormat>(config.plane_format), config.strip_rows, config.strip_cols,
              config.hbm_budget) {
     for (uint32_t i = 0; i < config_.lanes; ++i) {
-        lanes_.push_back(std::make_unique<Lane>(config_.max_batch));
+        lanes_.push_back(std::make_unique<Lane>(config_.max_batch, config_.graph != 0));
         free_lanes_.push_back(config_.lanes - 1 - i);
     }
     dispatcher_ = std::thread([this] { DispatchLoop(); });

@@ -942,9 +942,10 @@ class Ssimulacra2Batch:
 
     def __init__(self, max_pairs: int, plane_format: int = L.PLANES_F32, hbm_budget: int = 0,
                  strip_rows: int = 0, chunks: int = 1, autotune: bool = False, blur: int = 0,
-                 persistent: bool = False, map_form: int = 0, strip_cols: int = 0):
+                 persistent: bool = False, map_form: int = 0, strip_cols: int = 0, graph: bool = False):
         cfg = L.BatchConfigC()
         cfg.strip_cols = int(strip_cols)
+        cfg.graph = 1 if graph else 0
         cfg.blur = int(blur)
         cfg.map_form = int(map_form)
         cfg.persistent = 1 if persistent else 0
@@ -1087,10 +1088,11 @@ class Ssimulacra2Server:
 
     def __init__(self, plane_format: int = L.PLANES_F32, lanes: int = 3, max_batch: int = 64,
                  hbm_budget: int = 0, strip_rows: int = 0, blur: int = 0, map_form: int = 0,
-                 strip_cols: int = 0):
+                 strip_cols: int = 0, graph: bool = False):
         cfg = L.ServerConfigC()
         L.lib().vqm_ss2_server_config_init(ctypes.byref(cfg))
         cfg.strip_cols = int(strip_cols)
+        cfg.graph = 1 if graph else 0
         cfg.blur = int(blur)
         cfg.map_form = int(map_form)
         cfg.plane_format = plane_format
