@@ -3,8 +3,9 @@
 Both map forms run bit for bit per pixel with their oracle form
 (``oracle.form_for``): the reference form with the FP32 ref form (the
 reference shaders' operation order, correctly rounded cube roots), the
-difference form with the FP32 dev form. Their 108 norms therefore agree to
-the order of the FP64 sums (relative 1e-10).
+difference form with the FP32 dev form, each pixel's map values and each
+FP32 block sum (one column by eight rows) alike. Their 108 norms therefore
+agree to the order of the FP64 sums (relative 1e-10).
 
 Bars of the reference form (the default):
   * the reference's 32x32 goldens within 1e-3

@@ -3,13 +3,14 @@
 //   prepare  u8 sRGB -> linear (256-entry table) -> XYB, plus the 2x2 box
 //            pyramid of the linear planes, writing only the (level, channel)
 //            planes whose maps survive pruning (prepare.comp:170-232).
-//   scale    one wave per 128-column strip of one (level, channel): the
-//            13-tap blurs of the map's quantities, the SSIM' and edge maps,
-//            and six FP64 map sums (scale.comp:335-437). Rows stream through
-//            the wave: each input row is staged in LDS, blurred horizontally
-//            (each lane owns two output columns), and the horizontal results
-//            enter a 13-row register ring from which the vertical blur of the
-//            row six above is taken. No map ever reaches HBM.
+//   scale    one wave per strip (128 or 64 columns) of one (level,
+//            channel): the 13-tap blurs of the map's quantities, the SSIM'
+//            and edge maps, and six map sums, FP32 over 8-row blocks then
+//            FP64 (scale.comp:335-437). Rows stream through the wave: each
+//            input row is staged in LDS, blurred horizontally (each lane owns
+//            two output columns, or one), and the horizontal results enter a
+//            13-row register ring from which the vertical blur of the row six
+//            above is taken. No map ever reaches HBM.
 //   reduce   FP64 sum of each norm's per-strip partials, 1-norm means and
 //            4-norm fourth roots (reduce.comp:21-63, vulkan.cc:545-583).
 //
