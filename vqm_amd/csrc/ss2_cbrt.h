@@ -77,18 +77,12 @@ __host__ __device__ inline float cbrt_round_exact(float x, float y) {
     return cube_cmp(xd, mid) == side ? n : y;
 }
 
-// The cube root of a positive normal float, correctly rounded. The FP32
-// seed of cbrt_fast (x^(-1/3) within 3.7%), then in FP64 one third-order
-// step r <- r (1 + h (1/3 + h (2/9 + h 14/81))), h = 1 - x r^3 (the error
-// falls to |h| < 5e-5), and one second-order step r <- r (1 + h (1/3 +
-// h 2/9)) (the dropped term 14/81 h^3 is below 2.2e-14), and y = x r^2 in
-// FP64: within 2^-44.4 of the root, relatively (the largest error over every
-// float in [2^-126, 2^24); the seed's bit pattern, and with it the error,
-// repeats every three binades). Rounding y to FP32 gives the correctly
-// rounded root unless y lies within 2^-44 of a midpoint between two floats:
-// the 29 bits FP32 drops from y's significand (all in its low word) then sit
-// within 2^9 units of 2^28 (one input in 2^19). Those take cbrt_round_exact.
-__host__ __device__ inline float cbrt_rn(float x) {
+// cbrt_rn (below) without its rare exact step: the FP64 root rounded to
+// FP32, and in `near` whether that rounding is unsafe (then
+// cbrt_round_exact(x, result) is the correctly rounded root). Kernels
+// evaluate several roots this way and settle the near ones in one branch, so
+// the roots' instruction chains stay in one basic block and interleave.
+__host__ __device__ inline float cbrt_rn_near(float x, bool* near) {
     constexpr float kThird = 0x1.555556p-2f, kMagic = 1419910400.0f;
     constexpr double kThirdD = 1.0 / 3.0, kC2 = 2.0 / 9.0, kC3 = 14.0 / 81.0;
     const float seed = __builtin_fmaf(static_cast<float>(cbrt_bits(x)), -kThird, kMagic);
@@ -101,8 +95,25 @@ __host__ __device__ inline float cbrt_rn(float x) {
     const double y = xd * (r * r);
     const float rounded = static_cast<float>(y);
     const uint32_t dropped = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, y)) & 0x1FFFFFFFu;
-    if (dropped - (0x10000000u - 0x200u) > 2 * 0x200u) return rounded;  // far from a midpoint
-    return cbrt_round_exact(x, rounded);
+    *near = dropped - (0x10000000u - 0x200u) <= 2 * 0x200u;  // within 2^9 units of a midpoint
+    return rounded;
+}
+
+// The cube root of a positive normal float, correctly rounded. The FP32
+// seed of cbrt_fast (x^(-1/3) within 3.7%), then in FP64 one third-order
+// step r <- r (1 + h (1/3 + h (2/9 + h 14/81))), h = 1 - x r^3 (the error
+// falls to |h| < 5e-5), and one second-order step r <- r (1 + h (1/3 +
+// h 2/9)) (the dropped term 14/81 h^3 is below 2.2e-14), and y = x r^2 in
+// FP64: within 2^-44.4 of the root, relatively (the largest error over every
+// float in [2^-126, 2^24); the seed's bit pattern, and with it the error,
+// repeats every three binades). Rounding y to FP32 gives the correctly
+// rounded root unless y lies within 2^-44 of a midpoint between two floats:
+// the 29 bits FP32 drops from y's significand (all in its low word) then sit
+// within 2^9 units of 2^28 (one input in 2^19). Those take cbrt_round_exact.
+__host__ __device__ inline float cbrt_rn(float x) {
+    bool near;
+    const float rounded = cbrt_rn_near(x, &near);
+    return near ? cbrt_round_exact(x, rounded) : rounded;
 }
 
 }  // namespace vqm::ss2

@@ -162,6 +162,57 @@ __device__ __forceinline__ void xyb(float r, float g, float b, uint32_t mask, fl
     }
 }
 
+// XYB of N pixels (px[j] = linear r, g, b), only the channels in `mask`. The
+// reference form takes every cube root's fast part first and settles the
+// rare near-midpoint roots in one branch for the group, so the roots'
+// instruction chains share a basic block and interleave; the arithmetic of
+// each pixel is xyb<true>'s.
+template <bool Ref, int N>
+__device__ __forceinline__ void xyb_group(const float (*px)[3], uint32_t mask, float (*out)[3]) {
+    if constexpr (!Ref) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) xyb<false>(px[j][0], px[j][1], px[j][2], mask, out[j]);
+    } else {
+        float m[N][3], c[N][3];
+        uint32_t near_bits = 0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const float r = px[j][0], g = px[j][1], b = px[j][2];
+            m[j][0] = fmaxf(fma_(kM00, r, fma_(kM01, g, fma_(kM02, b, kOpsinBias))), kMinNormal);
+            m[j][1] = fmaxf(fma_(kM10, r, fma_(kM11, g, fma_(kM12, b, kOpsinBias))), kMinNormal);
+            bool n0, n1;
+            c[j][0] = cbrt_rn_near(m[j][0], &n0);
+            c[j][1] = cbrt_rn_near(m[j][1], &n1);
+            near_bits |= (static_cast<uint32_t>(n0) | static_cast<uint32_t>(n1) << 1) << (3 * j);
+            m[j][2] = 1.0f;
+            c[j][2] = 0.0f;
+        }
+        if (mask & 4u) {  // B: a block of its own (a uniform branch)
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                const float r = px[j][0], g = px[j][1], b = px[j][2];
+                m[j][2] = fmaxf(fma_(kM20, r, fma_(kM21, g, fma_(kM22, b, kOpsinBias))), kMinNormal);
+                bool n2;
+                c[j][2] = cbrt_rn_near(m[j][2], &n2);
+                near_bits |= static_cast<uint32_t>(n2) << (3 * j + 2);
+            }
+        }
+        if (near_bits != 0u) {
+#pragma unroll
+            for (int k = 0; k < 3 * N; ++k)
+                if ((near_bits >> k) & 1u) c[k / 3][k % 3] = cbrt_round_exact(m[k / 3][k % 3], c[k / 3][k % 3]);
+        }
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const float c0 = c[j][0] - kCbrtBias, c1 = c[j][1] - kCbrtBias;
+            const float half_sum = 0.5f * (c0 + c1);
+            out[j][0] = fma_(0.5f * (c0 - c1), 14.0f, 0.42f);
+            out[j][1] = half_sum + 0.01f;
+            out[j][2] = (mask & 4u) ? (c[j][2] - kCbrtBias - half_sum) + 0.55f : 0.0f;
+        }
+    }
+}
+
 // ---- plane element formats ------------------------------------------------
 
 // A format loads two adjacent elements as Raw bits (or one element) and
@@ -710,9 +761,7 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
             const uint32_t y = Y0 + 2u * ty + r;
             if (y >= H) continue;
             float v[4][3];
-#pragma unroll
-            for (uint32_t j = 0; j < 4; ++j)
-                xyb<Ref>(lin[r][j][0], lin[r][j][1], lin[r][j][2], mask0, v[j]);
+            xyb_group<Ref, 4>(lin[r], mask0, v);
             const uint64_t at = static_cast<uint64_t>(y + kPadTop) * pitch0 + bx + kPadLeft;
 #pragma unroll
             for (uint32_t c = 0; c < 3; ++c) {
@@ -753,9 +802,8 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
         const uint32_t mask1 = plan->mask[1];
         if (mask1 != 0u && gy < H1) {
             const uint32_t gx = X0 / 2u + 2u * tx;
-            float x0[3], x1[3];
-            xyb<Ref>(v1[0][0], v1[0][1], v1[0][2], mask1, x0);
-            xyb<Ref>(v1[1][0], v1[1][1], v1[1][2], mask1, x1);
+            float xo[2][3];
+            xyb_group<Ref, 2>(v1, mask1, xo);
             const uint64_t at = static_cast<uint64_t>(gy + kPadTop) * plan->pitch[1] + gx + kPadLeft;
 #pragma unroll
             for (uint32_t c = 0; c < 3; ++c) {
@@ -763,9 +811,9 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
                 using Fmt = typename Lay::Fmt;
                 auto* p = plane_at<Fmt>(planes, plan->plane_off[1][c]) + at;
                 if (gx + 1u < W1) {
-                    Fmt::store2(p, x0[c], x1[c]);
+                    Fmt::store2(p, xo[0][c], xo[1][c]);
                 } else if (gx < W1) {
-                    Fmt::store1(p, x0[c]);
+                    Fmt::store1(p, xo[0][c]);
                 }
             }
         }
@@ -783,11 +831,12 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
 // sized to the resident occupancy. The grid is a multiple of 8, so item b
 // stays on XCD b % 8 and xcd_remap keeps neighbouring tiles on one L2.
 // The table-decode instantiations are held to 80 VGPRs (six waves per SIMD)
-// in the difference form and 96 (five) in the reference form, whose
-// correctly rounded cube roots hold FP64 temporaries, both without spilling;
-// the per-pixel decode of general formats needs ~130.
+// in the difference form and 128 (four) in the reference form, whose
+// interleaved correctly rounded cube roots hold FP64 temporaries (117
+// registers; at five waves they spill), both without spilling; the
+// per-pixel decode of general formats needs ~110.
 template <typename Lay, bool General, bool Ref>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(General ? 1 : (Ref ? 5 : 6)))) void prepare_kernel(const PairDesc* __restrict__ pairs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(General ? 1 : (Ref ? 4 : 6)))) void prepare_kernel(const PairDesc* __restrict__ pairs,
                                                       const WorkRef* __restrict__ work,
                                                       uint32_t blocks, uint32_t tiles_per_image,
                                                       const float* __restrict__ lut_g,
