@@ -391,7 +391,7 @@ typedef struct vqm_ss2_batch vqm_ss2_batch;
 
 typedef struct vqm_ss2_batch_config {
     uint32_t plane_format;   /* VQM_PLANES_* */
-    uint32_t strip_rows;     /* multiple of 8; 0 = default (128; pipelines and
+    uint32_t strip_rows;     /* multiple of 8; 0 = default (192; pipelines and
                                 servers: 64) */
     uint32_t max_pairs;      /* pairs per submission */
     uint32_t chunks;         /* 0 or 1: prepare, then scale, on one stream; 2..16:
@@ -464,7 +464,7 @@ typedef struct vqm_ss2_tune_result {
     uint32_t strip_cols;     /* scale work-item width chosen */
 } vqm_ss2_tune_result;
 
-/* Times every (strip height 16/32/64/128/256) x (strip width 64/128) x
+/* Times every (strip height 16/32/64/128/192/256) x (strip width 64/128) x
  * (chunks 1/2/4/8) shape for a launch of `pairs` pairs of this geometry and
  * format (rounded up to a power of two, at most 512) on the device and
  * returns the fastest; the choice is cached for the process. The strip width

@@ -911,8 +911,10 @@ const vqm_ss2_batch_config& ValidatedBatch(const vqm_ss2_batch_config& config) {
 
 // Batches default to taller strips than single pairs: with many pairs in a
 // launch there are blocks enough, and each strip's 12 halo rows are blurred
-// over 128 output rows instead of 64.
-constexpr uint32_t kBatchStripRows = 128;
+// over 192 output rows instead of 64 (scale 2.5% faster than 128 rows on the
+// 1080p batch, as fast as 256 rows with less row padding: 1080 rows pad to
+// 1152 as with 128).
+constexpr uint32_t kBatchStripRows = 192;
 
 Batch::Batch(const vqm_ss2_batch_config& config)
     : config_(ValidatedBatch(config)),

@@ -32,7 +32,7 @@ class Tuner {
 public:
     static Tuner& Get();
     // Strip heights, widths and chunk counts tried.
-    static constexpr uint32_t kStripRows[] = {16, 32, 64, 128, 256};
+    static constexpr uint32_t kStripRows[] = {16, 32, 64, 128, 192, 256};
     static constexpr uint32_t kStripWidths[] = {64, 128};
     static constexpr uint32_t kChunks[] = {1, 2, 4, 8};
     // The batch-size class a pair count falls in: the next power of two,
