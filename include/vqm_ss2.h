@@ -447,8 +447,9 @@ int vqm_ss2_batch_pool_bytes(const vqm_ss2_batch* batch, uint64_t* bytes);
 /* Fixes the scale strip shape (rows: a multiple of 8, 0 = the batch's
  * default; cols: 64 or 128, 0 = the batch's default) of one geometry before
  * its first submission, for example a shape vqm_ss2_autotune chose in an
- * earlier process. LOGIC_ERROR once the geometry has been submitted. One
- * launch may mix strip widths (one scale kernel launch per width). */
+ * earlier process; it takes precedence over `autotune`. LOGIC_ERROR once
+ * the geometry has been submitted. One launch may mix strip widths (one
+ * scale kernel launch per width). */
 int vqm_ss2_batch_set_shape(vqm_ss2_batch* batch, uint32_t width, uint32_t height, uint32_t strip_rows,
                             uint32_t strip_cols);
 

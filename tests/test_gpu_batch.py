@@ -145,6 +145,14 @@ def test_mixed_strip_widths_in_one_launch(chunks):
         assert np.array_equal(norms[i], alone(w, h, a, b, strip_rows=rows, strip_cols=cols)), f"pair {i}"
     with pytest.raises(LogicError):
         batch.SetShape(Size(320, 180), 64, 128)
+    # A pinned shape takes precedence over the autotuner.
+    tuned = Ssimulacra2Batch(max_pairs=8, chunks=chunks, autotune=True)
+    tuned.SetShape(Size(200, 130), 16, 64)
+    _, got = tuned.Finish(tuned.Submit(pairs), raw=True)
+    for i, ((w, h), (a, b)) in enumerate(zip(shapes, host)):
+        if (w, h) == (200, 130):
+            assert np.array_equal(got[i], alone(w, h, a, b, strip_rows=16, strip_cols=64)), f"pinned pair {i}"
+    tuned.close()
     with pytest.raises(InvalidArgument):
         batch.SetShape(Size(100, 100), 12, 128)
     with pytest.raises(InvalidArgument):

@@ -638,18 +638,21 @@ __device__ __forceinline__ TileRef resolve_tile(uint32_t b, const PairDesc* __re
         image = w->index >> 31;
         tile = w->index & 0x7FFFFFFFu;
     } else {
+        // One division: l = (pair * 2 + image) * tiles_per_image + tile.
         const uint32_t l = xcd_remap(b, blocks);
-        tile = l % tiles_per_image;
-        image = (l / tiles_per_image) & 1u;
-        pair_idx = l / (2 * tiles_per_image);
+        const uint32_t q = l / tiles_per_image;
+        tile = l - q * tiles_per_image;
+        image = q & 1u;
+        pair_idx = q >> 1;
     }
     TileRef t;
     t.pdp = as_const(pairs) + pair_idx;
     t.plan = as_const(t.pdp->plan);
     t.image = image;
     const uint32_t tiles_x = t.plan->prep_tiles_x;
-    t.X0 = (tile % tiles_x) * kPrepTileW;
-    t.Y0 = (tile / tiles_x) * kPrepTileH;
+    const uint32_t ty = tile / tiles_x;
+    t.X0 = (tile - ty * tiles_x) * kPrepTileW;
+    t.Y0 = ty * kPrepTileH;
     return t;
 }
 

@@ -949,13 +949,13 @@ uint64_t Batch::Submit(const vqm_ss2_device_pair* pairs, uint32_t count) {
     uint32_t chunks = config_.chunks;
     if (config_.autotune) {
         // Each new geometry's strip shape comes from the tuner, timed for
-        // this batch's count of it; the chunk count follows the geometry
-        // with the most pairs.
+        // this batch's count of it, unless SetShape pinned one; the chunk
+        // count follows the geometry with the most pairs.
         uint32_t most = 0;
         for (const auto& [geometry, n] : per_geometry) {
             const auto [w, h] = geometry;
             const TuneChoice c = Tuner::Get().Choose(w, h, static_cast<PlaneFormat>(config_.plane_format), map_, n);
-            if (!pools_.HasPlan(w, h)) pools_.SetStripShape(w, h, c.strip_rows, c.strip_cols);
+            if (!pools_.HasPlan(w, h) && !pools_.HasShape(w, h)) pools_.SetStripShape(w, h, c.strip_rows, c.strip_cols);
             if (n > most) {
                 most = n;
                 chunks = c.chunks;

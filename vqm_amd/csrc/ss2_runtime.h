@@ -174,6 +174,9 @@ public:
     void SetStripShape(uint32_t width, uint32_t height, uint32_t strip_rows, uint32_t strip_cols) {
         shape_for_[std::make_pair(width, height)] = std::make_pair(strip_rows, strip_cols);
     }
+    bool HasShape(uint32_t width, uint32_t height) const {
+        return shape_for_.count(std::make_pair(width, height)) != 0;
+    }
     // Device bytes one workspace of this geometry holds.
     uint64_t Need(uint32_t width, uint32_t height, bool inputs);
     // A workspace, or null when it cannot fit the budget until busy
