@@ -391,7 +391,10 @@ typedef struct vqm_ss2_batch vqm_ss2_batch;
 
 typedef struct vqm_ss2_batch_config {
     uint32_t plane_format;   /* VQM_PLANES_* */
-    uint32_t strip_rows;     /* multiple of 8; 0 = default (192; pipelines and
+    uint32_t strip_rows;     /* multiple of 8; 0 = chosen per geometry when it
+                                is first submitted: 192, 128 or 64 rows, the
+                                tallest that still gives the launch a work item
+                                per wave slot of the device (pipelines and
                                 servers: 64) */
     uint32_t max_pairs;      /* pairs per submission */
     uint32_t chunks;         /* 0 or 1: prepare, then scale, on one stream; 2..16:

@@ -40,7 +40,8 @@ def test_mixed_batch_equals_each_pair_alone(fmt):
     shapes = [(64, 64), (200, 130), (33, 17), (640, 360), (64, 64), (129, 97), (8, 8)]
     pairs, keep, host = upload_pairs(shapes, seed=11)
     # Bit for bit with a pipeline at the pipeline's strip height (64 rows);
-    # the batch default (128) groups the FP64 strip sums differently.
+    # the batch default (chosen per geometry) groups the FP64 strip sums
+    # differently.
     batch = Ssimulacra2Batch(max_pairs=16, plane_format=fmt, strip_rows=64)
     scales, norms = batch.Finish(batch.Submit(pairs), raw=True)
     tall = Ssimulacra2Batch(max_pairs=16, plane_format=fmt)

@@ -909,19 +909,30 @@ const vqm_ss2_batch_config& ValidatedBatch(const vqm_ss2_batch_config& config) {
 }
 }  // namespace
 
-// Batches default to taller strips than single pairs: with many pairs in a
-// launch there are blocks enough, and each strip's 12 halo rows are blurred
-// over 192 output rows instead of 64 (scale 2.5% faster than 128 rows on the
-// 1080p batch, as fast as 256 rows with less row padding: 1080 rows pad to
-// 1152 as with 128).
-constexpr uint32_t kBatchStripRows = 192;
+// A batch's default strip height for a geometry, chosen when the geometry
+// is first submitted with `pairs` pairs of it: the tallest of 192, 128 and
+// 64 rows that still gives the scale launch one work item per wave slot of
+// the device (8 per CU), else 64. Taller strips blur each strip's 12 halo
+// rows over more output rows (192 rows: scale 2.5% faster than 128 on 300
+// 1080p pairs, as fast as 256 with less padding), shorter ones keep small
+// batches' launches wide (one 1080p pair: 6.5k pairs/s at 64 rows, 3.6k at
+// 192).
+uint32_t Batch::DefaultStripRows(uint32_t width, uint32_t height, uint32_t pairs) const {
+    const uint64_t slots = 8ull * static_cast<uint64_t>(std::max(ctx_.cus, 1));
+    for (uint32_t rows : {192u, 128u}) {
+        const Plan p = FillPlan(width, height, static_cast<PlaneFormat>(config_.plane_format), rows,
+                                config_.strip_cols);
+        if (static_cast<uint64_t>(p.items) * pairs >= slots) return rows;
+    }
+    return 64;
+}
 
 Batch::Batch(const vqm_ss2_batch_config& config)
     : config_(ValidatedBatch(config)),
       map_(ToMapForm(config.map_form, static_cast<PlaneFormat>(config.plane_format))),
       ctx_(DeviceContext::Get()),
-      pools_(static_cast<PlaneFormat>(config.plane_format),
-             config.strip_rows != 0 ? config.strip_rows : kBatchStripRows, config.strip_cols, config.hbm_budget),
+      pools_(static_cast<PlaneFormat>(config.plane_format), config.strip_rows != 0 ? config.strip_rows : 64,
+             config.strip_cols, config.hbm_budget),
       lane_(config.max_pairs, config.graph != 0) {}
 
 Batch::~Batch() = default;
@@ -930,7 +941,8 @@ void Batch::SetShape(uint32_t width, uint32_t height, uint32_t strip_rows, uint3
     BuildPlan(width, height, static_cast<PlaneFormat>(config_.plane_format), strip_rows, strip_cols);  // validates
     if (pools_.HasPlan(width, height))
         Fail(VQM_LOGIC_ERROR, "the strip shape of a geometry already used cannot change");
-    pools_.SetStripShape(width, height, strip_rows != 0 ? strip_rows : kBatchStripRows,
+    pools_.SetStripShape(width, height,
+                         strip_rows != 0 ? strip_rows : (config_.strip_rows != 0 ? config_.strip_rows : 128),
                          strip_cols != 0 ? strip_cols : config_.strip_cols);
 }
 
@@ -962,7 +974,12 @@ uint64_t Batch::Submit(const vqm_ss2_device_pair* pairs, uint32_t count) {
             }
         }
     }
-    for (const auto& [geometry, n] : per_geometry) pools_.PlanFor(geometry.first, geometry.second);
+    for (const auto& [geometry, n] : per_geometry) {
+        const auto [w, h] = geometry;
+        if (config_.strip_rows == 0 && !pools_.HasPlan(w, h) && !pools_.HasShape(w, h))
+            pools_.SetStripShape(w, h, DefaultStripRows(w, h, n), config_.strip_cols);
+        pools_.PlanFor(w, h);
+    }
     std::vector<Taken> taken;
     std::vector<LaunchEntry> entries(count);
     try {

@@ -398,6 +398,7 @@ public:
     // Fixes the strip shape of a geometry not used yet (a shape tuned
     // earlier, or by another process).
     void SetShape(uint32_t width, uint32_t height, uint32_t strip_rows, uint32_t strip_cols);
+    uint32_t DefaultStripRows(uint32_t width, uint32_t height, uint32_t pairs) const;
 
 private:
     struct Taken {
