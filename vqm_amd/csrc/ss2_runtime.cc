@@ -1010,9 +1010,21 @@ void Batch::Finish(uint64_t token, uint32_t* scales, double* norms) {
     if (token == 0 || token != in_flight_)
         Fail(VQM_INVALID_ARGUMENT, "unknown or already finished submission token");
     in_flight_ = 0;
-    for (Taken& t : busy_) pools_.Return(t.width, t.height, false, std::move(t.ws));
-    busy_.clear();
-    lane_.Wait();
+    auto give_back = [&] {
+        for (Taken& t : busy_) pools_.Return(t.width, t.height, false, std::move(t.ws));
+        busy_.clear();
+    };
+    try {
+        lane_.Wait();
+    } catch (...) {
+        // The launch's work may still touch its workspaces: whatever was
+        // queued ends before they go back to the pools.
+        lane_.Synchronize();
+        lane_.Abandon();
+        give_back();
+        throw;
+    }
+    give_back();
     stage_ms_ = lane_.StageTimes();
     const auto count = static_cast<uint32_t>(levels_.size());
     if (norms) std::memcpy(norms, lane_.Norms(0), static_cast<uint64_t>(count) * kNorms * sizeof(double));

@@ -283,7 +283,11 @@ void Server::CompleteLoop() {
                 status = e.status;
                 message = e.message;
             }
-        } else {
+        }
+        if (status != VQM_OK) {
+            // The batch's work may still touch its workspaces: whatever was
+            // queued ends before they go back to the pools.
+            lanes_[lane]->Synchronize();
             lanes_[lane]->Abandon();
         }
         for (uint32_t i = 0; i < done.requests.size(); ++i) {
