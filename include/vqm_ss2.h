@@ -606,6 +606,26 @@ int vqm_device_synchronize(void);
  * streaming kernel and waits: the reference point for HBM traffic counters. */
 int vqm_device_copy(void* dst, const void* src, uint64_t bytes);
 
+/* ---- self-check (no reference counterpart) ------------------------------ */
+
+/* Result of vqm_ss2_check_cbrt. */
+typedef struct vqm_ss2_cbrt_check {
+    uint64_t mismatches;     /* inputs whose two roots differ */
+    uint64_t exact_steps;    /* inputs that took the exact midpoint comparison */
+    double max_seed_error;   /* largest |1 - x r^3| of the hardware seed r */
+    uint32_t first_mismatch; /* bits of the smallest differing input, or 0xFFFFFFFF */
+    uint32_t reserved;
+} vqm_ss2_cbrt_check;
+
+/* Checks, on the current device, the correctly rounded cube root the
+ * reference form's XYB takes (seeded by the hardware log2 and exp2, whose
+ * rounding no document fixes) against the same root computed in IEEE FP64
+ * steps only (which the CPU tests check against exact arithmetic), for
+ * every float whose bits lie in [first_bits, end_bits), a range of positive
+ * normal floats (0x00800000 to 0x7F800000 covers them all). Waits for the
+ * device. */
+int vqm_ss2_check_cbrt(uint32_t first_bits, uint32_t end_bits, vqm_ss2_cbrt_check* result);
+
 #ifdef __cplusplus
 }
 #endif

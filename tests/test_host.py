@@ -35,6 +35,16 @@ def test_library_exports_every_declared_symbol():
     assert set(declared) == set(L.SIGNATURES), "ctypes signatures and the header disagree"
 
 
+def test_cbrt_check_rejects_ranges_outside_the_normal_floats():
+    """The device cube-root check validates its range before touching the GPU."""
+    r = L.CbrtCheckC()
+    for first, end in [(0, 0x00900000), (0x00800000, 0x7F800001), (0x3F800000, 0x3F800000),
+                       (0x40000000, 0x3F800000)]:
+        assert L.lib().vqm_ss2_check_cbrt(first, end, ctypes.byref(r)) == L.VQM_INVALID_ARGUMENT
+        assert b"positive normal floats" in L.lib().vqm_last_error()
+    assert L.lib().vqm_ss2_check_cbrt(0x3F800000, 0x40000000, None) == L.VQM_INVALID_ARGUMENT
+
+
 def test_scale_plan_matches_oracle(oracle_lib):
     for size in [(8, 8), (13, 9), (32, 32), (64, 64), (65, 65), (129, 97), (1920, 1080),
                  (3840, 2160), (640, 360), (1280, 720), (2560, 1440), (512, 512), (4096, 17)]:

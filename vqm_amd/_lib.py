@@ -106,6 +106,11 @@ class ReplayRequestC(ctypes.Structure):
                 ("host", u32), ("reserved", u32), ("arrival_ns", u64)]
 
 
+class CbrtCheckC(ctypes.Structure):
+    _fields_ = [("mismatches", u64), ("exact_steps", u64), ("max_seed_error", ctypes.c_double),
+                ("first_mismatch", u32), ("reserved", u32)]
+
+
 # Every function include/vqm_ss2.h declares: name -> (restype, argtypes).
 SIGNATURES = {
     "vqm_last_error": (ctypes.c_char_p, []),
@@ -187,6 +192,7 @@ SIGNATURES = {
     "vqm_memcpy_dtoh": (ctypes.c_int, [vp, vp, u64]),
     "vqm_device_synchronize": (ctypes.c_int, []),
     "vqm_device_copy": (ctypes.c_int, [vp, vp, u64]),
+    "vqm_ss2_check_cbrt": (ctypes.c_int, [u32, u32, ctypes.POINTER(CbrtCheckC)]),
 }
 
 _lock = threading.Lock()

@@ -11,6 +11,7 @@
 #include <string>
 
 #include "ss2_host.h"
+#include "ss2_kernels.h"
 #include "ss2_noalloc.h"
 #include "ss2_runtime.h"
 #include "ss2_server.h"
@@ -621,6 +622,26 @@ int vqm_device_copy(void* dst, const void* src, uint64_t bytes) {
         if (bytes % 16 != 0) vqm::ss2::Fail(VQM_INVALID_ARGUMENT, "copy size must be a multiple of 16");
         vqm::ss2::Check(vqm::ss2::LaunchCopy(dst, src, bytes, nullptr), "copy kernel");
         vqm::ss2::Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    });
+}
+
+int vqm_ss2_check_cbrt(uint32_t first_bits, uint32_t end_bits, vqm_ss2_cbrt_check* result) {
+    return Guard([&] {
+        Need(result, "result");
+        if (first_bits < 0x00800000u || end_bits > 0x7F800000u || first_bits >= end_bits)
+            vqm::ss2::Fail(VQM_INVALID_ARGUMENT,
+                           "the range must lie in the positive normal floats [0x00800000, 0x7F800000) and not be empty");
+        vqm::ss2::DeviceBuffer counters(sizeof(vqm::ss2::CbrtCheck));
+        vqm::ss2::CbrtCheck c{};
+        c.first_mismatch = 0xFFFFFFFFu;
+        vqm::ss2::Check(hipMemcpy(counters.get(), &c, sizeof(c), hipMemcpyHostToDevice), "hipMemcpy");
+        vqm::ss2::Check(vqm::ss2::LaunchCbrtCheck(first_bits, end_bits, counters.as<vqm::ss2::CbrtCheck>(), nullptr),
+                        "cube-root check kernel");
+        vqm::ss2::Check(hipMemcpy(&c, counters.get(), sizeof(c), hipMemcpyDeviceToHost), "hipMemcpy");
+        result->mismatches = c.mismatches;
+        result->exact_steps = c.near;
+        std::memcpy(&result->max_seed_error, &c.max_h_bits, sizeof(double));
+        result->first_mismatch = c.first_mismatch;
     });
 }
 

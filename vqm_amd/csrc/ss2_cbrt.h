@@ -8,12 +8,16 @@
 //              second-order FP64 step from cbrt_fast's seed, rounded to
 //              FP32, with two exact FP64 comparisons for the one input in
 //              2^19 whose FP64 root lies too near a midpoint to round safely.
+//   cbrt_rn_hw the same root on the device from a hardware log2/exp2 seed
+//              and one FP64 step (the kernels' reference form).
 //
-// Every step is an IEEE conversion, add, multiply or fused multiply-add
-// (written out), with contraction off, so the result does not depend on the
-// compiler or on where it runs. tests/cpp/cbrt_exhaustive.hip checks
-// cbrt_rn against an exact integer comparison for every float in
-// [2^-126, 2^24).
+// Every step of cbrt_fast and cbrt_rn is an IEEE conversion, add, multiply
+// or fused multiply-add (written out), with contraction off, so the result
+// does not depend on the compiler or on where it runs.
+// tests/cpp/cbrt_exhaustive.hip checks cbrt_rn against an exact integer
+// comparison for every float in [2^-126, 2^24). cbrt_rn_hw's seed comes from
+// hardware instructions whose rounding no document fixes, so it is checked
+// on the device instead: equal to cbrt_rn for every positive normal float.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -97,6 +101,44 @@ __host__ __device__ inline float cbrt_rn_near(float x, bool* near) {
     const uint32_t dropped = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, y)) & 0x1FFFFFFFu;
     *near = dropped - (0x10000000u - 0x200u) <= 2 * 0x200u;  // within 2^9 units of a midpoint
     return rounded;
+}
+
+// The seed of cbrt_rn_near_hw: x^(-1/3) as 2^(-log2(x) / 3) from the
+// hardware base-2 logarithm and exponential (v_log_f32, v_exp_f32; one
+// transcendental instruction each).
+__device__ inline float cbrt_seed_hw(float x) {
+    constexpr float kThird = 0x1.555556p-2f;
+    return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * -kThird);
+}
+
+// cbrt_rn_near for the device, from the hardware seed: |h| = |1 - x r^3|
+// is at most 1.09e-5 (2^-16.5) over every positive normal float (measured
+// by vqm_ss2_check_cbrt; the instruction set documents no bound), so one
+// second-order FP64 step r <- r (1 + h (1/3 + h 2/9)) (dropped term
+// 14/81 h^3 < 2^-52) and y = x r^2 leave y within about 2^-50 of the root,
+// inside the 2^9-unit band (2^-44) of the midpoint test. 17 instructions,
+// two of them transcendental, against cbrt_rn_near's 25. For every positive
+// normal float, cbrt_rn_near_hw followed by cbrt_round_exact where `near`
+// returns cbrt_rn's root (checked on the device for all 2^31 - 2^24 of them
+// by vqm_ss2_check_cbrt; cbrt_rn is checked against exact arithmetic on the
+// CPU).
+__device__ inline float cbrt_rn_near_hw(float x, bool* near) {
+    constexpr double kThirdD = 1.0 / 3.0, kC2 = 2.0 / 9.0;
+    const double xd = x;
+    double r = cbrt_seed_hw(x);
+    const double h = __builtin_fma(-xd, (r * r) * r, 1.0);
+    r = __builtin_fma(r * h, __builtin_fma(h, kC2, kThirdD), r);
+    const double y = xd * (r * r);
+    const float rounded = static_cast<float>(y);
+    const uint32_t dropped = static_cast<uint32_t>(__builtin_bit_cast(uint64_t, y)) & 0x1FFFFFFFu;
+    *near = dropped - (0x10000000u - 0x200u) <= 2 * 0x200u;  // within 2^9 units of a midpoint
+    return rounded;
+}
+
+__device__ inline float cbrt_rn_hw(float x) {
+    bool near;
+    const float rounded = cbrt_rn_near_hw(x, &near);
+    return near ? cbrt_round_exact(x, rounded) : rounded;
 }
 
 // The cube root of a positive normal float, correctly rounded. The FP32

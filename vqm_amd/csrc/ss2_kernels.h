@@ -50,4 +50,18 @@ hipError_t LaunchReduce(const PairDesc* pairs, uint32_t pairs_count, hipStream_t
 // Device-to-device copy of `bytes` (a multiple of 16) by a plain kernel.
 hipError_t LaunchCopy(void* dst, const void* src, uint64_t bytes, hipStream_t stream);
 
+// Counters of the cube-root self-check; `first_mismatch` starts at
+// 0xFFFFFFFF and the others at 0.
+struct CbrtCheck {
+    unsigned long long mismatches;
+    unsigned long long near;
+    unsigned long long max_h_bits;
+    uint32_t first_mismatch;
+    uint32_t pad;
+};
+
+// Compares the reference form's cube root (hardware-seeded) with the one of
+// IEEE steps only for every float with bits in [first, end), adding to `out`.
+hipError_t LaunchCbrtCheck(uint32_t first, uint32_t end, CbrtCheck* out, hipStream_t stream);
+
 }  // namespace vqm::ss2

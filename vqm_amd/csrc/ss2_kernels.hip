@@ -115,7 +115,7 @@ __device__ __forceinline__ float div_rn(float a, float b) {
 //
 //   Reference form (Ref = true): prepare.comp:81-98 and scale.comp:130-142,
 //   305-333 in their own operation order. The cube roots are correctly
-//   rounded (cbrt_rn), the five quantities a, b, a*a, b*b and a*b are
+//   rounded (cbrt_rn_hw), the five quantities a, b, a*a, b*b and a*b are
 //   blurred, and the SSIM' map subtracts the squared means from the blurred
 //   second moments directly, with IEEE divisions.
 //
@@ -140,14 +140,14 @@ __device__ __forceinline__ void xyb(float r, float g, float b, uint32_t mask, fl
     m1 = fmaxf(m1, kMinNormal);
     out[2] = 0.0f;
     if constexpr (Ref) {
-        const float c0 = cbrt_rn(m0) - kCbrtBias, c1 = cbrt_rn(m1) - kCbrtBias;
+        const float c0 = cbrt_rn_hw(m0) - kCbrtBias, c1 = cbrt_rn_hw(m1) - kCbrtBias;
         const float half_sum = 0.5f * (c0 + c1);
         out[0] = fma_(0.5f * (c0 - c1), 14.0f, 0.42f);
         out[1] = half_sum + 0.01f;
         if (mask & 4u) {
             float m2 = fma_(kM20, r, fma_(kM21, g, fma_(kM22, b, kOpsinBias)));
             m2 = fmaxf(m2, kMinNormal);
-            out[2] = (cbrt_rn(m2) - kCbrtBias - half_sum) + 0.55f;
+            out[2] = (cbrt_rn_hw(m2) - kCbrtBias - half_sum) + 0.55f;
         }
     } else {
         const float r0 = cbrt_fast(m0), r1 = cbrt_fast(m1);
@@ -181,8 +181,8 @@ __device__ __forceinline__ void xyb_group(const float (*px)[3], uint32_t mask, f
             m[j][0] = fmaxf(fma_(kM00, r, fma_(kM01, g, fma_(kM02, b, kOpsinBias))), kMinNormal);
             m[j][1] = fmaxf(fma_(kM10, r, fma_(kM11, g, fma_(kM12, b, kOpsinBias))), kMinNormal);
             bool n0, n1;
-            c[j][0] = cbrt_rn_near(m[j][0], &n0);
-            c[j][1] = cbrt_rn_near(m[j][1], &n1);
+            c[j][0] = cbrt_rn_near_hw(m[j][0], &n0);
+            c[j][1] = cbrt_rn_near_hw(m[j][1], &n1);
             near_bits |= (static_cast<uint32_t>(n0) | static_cast<uint32_t>(n1) << 1) << (3 * j);
             m[j][2] = 1.0f;
             c[j][2] = 0.0f;
@@ -193,7 +193,7 @@ __device__ __forceinline__ void xyb_group(const float (*px)[3], uint32_t mask, f
                 const float r = px[j][0], g = px[j][1], b = px[j][2];
                 m[j][2] = fmaxf(fma_(kM20, r, fma_(kM21, g, fma_(kM22, b, kOpsinBias))), kMinNormal);
                 bool n2;
-                c[j][2] = cbrt_rn_near(m[j][2], &n2);
+                c[j][2] = cbrt_rn_near_hw(m[j][2], &n2);
                 near_bits |= static_cast<uint32_t>(n2) << (3 * j + 2);
             }
         }
@@ -1265,6 +1265,44 @@ __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src
         dst[i] = src[i];
 }
 
+// Self-check of the reference form's cube root over the floats with bits in
+// [first, end): cbrt_rn_hw (what the kernels take) against cbrt_rn (IEEE
+// steps only). Counts the inputs where they differ and those that took the
+// exact step, and keeps the smallest differing input and the largest
+// |1 - x r^3| of the hardware seed r (as FP64 bits: non-negative doubles
+// order as their bits). One atomic per wave and counter.
+__global__ __launch_bounds__(256) void cbrt_check_kernel(uint32_t first, uint32_t end, CbrtCheck* out) {
+    unsigned long long bad = 0, near = 0, h_bits = 0;
+    uint32_t first_bad = 0xFFFFFFFFu;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t u = first + static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; u < end; u += stride) {
+        const float x = __uint_as_float(static_cast<uint32_t>(u));
+        bool n;
+        const float rounded = cbrt_rn_near_hw(x, &n);
+        const float got = n ? cbrt_round_exact(x, rounded) : rounded;
+        near += n ? 1u : 0u;
+        if (got != cbrt_rn(x)) {
+            ++bad;
+            first_bad = min(first_bad, static_cast<uint32_t>(u));
+        }
+        const double r = cbrt_seed_hw(x);
+        const double h = fabs(fma_(-static_cast<double>(x), (r * r) * r, 1.0));
+        h_bits = max(h_bits, static_cast<unsigned long long>(__double_as_longlong(h)));
+    }
+    for (int offset = 32; offset > 0; offset >>= 1) {
+        bad += __shfl_xor(bad, offset, 64);
+        near += __shfl_xor(near, offset, 64);
+        h_bits = max(h_bits, __shfl_xor(h_bits, offset, 64));
+        first_bad = min(first_bad, __shfl_xor(first_bad, offset, 64));
+    }
+    if (threadIdx.x % 64u == 0u) {
+        atomicAdd(&out->mismatches, bad);
+        atomicAdd(&out->near, near);
+        atomicMax(&out->max_h_bits, h_bits);
+        atomicMin(&out->first_mismatch, first_bad);
+    }
+}
+
 // Blocks of `kernel` the device holds resident at once: occupancy per
 // compute unit x compute units, rounded down to a multiple of 8 (at least
 // 8). A persistent launch uses min(work items, this) blocks, each looping
@@ -1355,6 +1393,14 @@ hipError_t LaunchCopy(void* dst, const void* src, uint64_t bytes, hipStream_t st
     const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 256 * 64);
     hipLaunchKernelGGL(copy_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, stream,
                        static_cast<const uint4*>(src), static_cast<uint4*>(dst), n);
+    return hipGetLastError();
+}
+
+hipError_t LaunchCbrtCheck(uint32_t first, uint32_t end, CbrtCheck* out, hipStream_t stream) {
+    if (end <= first) return hipSuccess;
+    const uint64_t blocks = std::min<uint64_t>((static_cast<uint64_t>(end - first) + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(cbrt_check_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, stream, first, end,
+                       out);
     return hipGetLastError();
 }
 
