@@ -764,7 +764,9 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
             const uint32_t y = Y0 + 2u * ty + r;
             if (y >= H) continue;
             float v[4][3];
-            xyb_group<Ref, 4>(lin[r], mask0, v);
+            // Two pixels' roots at a time (see prepare_kernel's registers).
+            xyb_group<Ref, 2>(lin[r], mask0, v);
+            xyb_group<Ref, 2>(lin[r] + 2, mask0, v + 2);
             const uint64_t at = static_cast<uint64_t>(y + kPadTop) * pitch0 + bx + kPadLeft;
 #pragma unroll
             for (uint32_t c = 0; c < 3; ++c) {
@@ -834,12 +836,13 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
 // sized to the resident occupancy. The grid is a multiple of 8, so item b
 // stays on XCD b % 8 and xcd_remap keeps neighbouring tiles on one L2.
 // The table-decode instantiations are held to 80 VGPRs (six waves per SIMD)
-// in the difference form and 128 (four) in the reference form, whose
-// interleaved correctly rounded cube roots hold FP64 temporaries (117
-// registers; at five waves they spill), both without spilling; the
-// per-pixel decode of general formats needs ~110.
+// in the difference form and 96 (five) in the reference form, whose
+// interleaved correctly rounded cube roots hold FP64 temporaries (level 0
+// takes its four pixels' roots two pixels at a time: four at once need 117
+// registers and spill at five waves; at six waves pairs spill too), both
+// without spilling; the per-pixel decode of general formats needs ~110.
 template <typename Lay, bool General, bool Ref>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(General ? 1 : (Ref ? 4 : 6)))) void prepare_kernel(const PairDesc* __restrict__ pairs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(General ? 1 : (Ref ? 5 : 6)))) void prepare_kernel(const PairDesc* __restrict__ pairs,
                                                       const WorkRef* __restrict__ work,
                                                       uint32_t blocks, uint32_t tiles_per_image,
                                                       const float* __restrict__ lut_g,
