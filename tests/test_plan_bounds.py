@@ -36,3 +36,18 @@ def test_range_allocator():
                         "-o", exe], check=True)
         r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
         assert r.returncode == 0, r.stderr
+
+
+def test_host_copy():
+    """The threaded host copy of streamed images equals std::memcpy at the
+    chunking thresholds, with concurrent callers and in a forked child."""
+    with tempfile.TemporaryDirectory() as tmp:
+        exe = os.path.join(tmp, "hostcopy_test")
+        subprocess.run(["g++", "-std=c++20", "-O2", "-Wall", "-pthread",
+                        "-I", os.path.join(ROOT, "vqm_amd", "csrc"),
+                        os.path.join(ROOT, "tests", "cpp", "hostcopy_test.cc"),
+                        os.path.join(ROOT, "vqm_amd", "csrc", "ss2_hostcopy.cc"),
+                        "-o", exe], check=True)
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert "failures: 0" in r.stdout

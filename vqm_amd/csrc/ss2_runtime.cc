@@ -2,6 +2,7 @@
 
 #include "ss2_runtime.h"
 
+#include "ss2_hostcopy.h"
 #include "ss2_source.h"
 #include "ss2_tune.h"
 
@@ -729,12 +730,16 @@ uint64_t Pipeline::SubmitHost(const vqm_image_desc& a, const void* da, const vqm
     CheckImage(b, config_.width, config_.height, "distorted");
     if (da == nullptr || db == nullptr) Fail(VQM_INVALID_ARGUMENT, "image data is null");
     EnsureStreaming(std::max(a.bytes, b.bytes));
+    // Each image goes up as soon as it is staged, so the original's upload
+    // overlaps the distorted image's copy; the copies use several threads.
     auto* staging = staging_.as<uint8_t>();
-    std::memcpy(staging, da, a.bytes);
-    std::memcpy(staging + stream_image_bytes_, db, b.bytes);
-    Check(hipMemcpyAsync(stream_inputs_.get(), staging, 2 * stream_image_bytes_, hipMemcpyHostToDevice,
-                         stream_.get()),
-          "upload pair");
+    auto* inputs = stream_inputs_.as<uint8_t>();
+    HostCopy(staging, da, a.bytes);
+    Check(hipMemcpyAsync(inputs, staging, a.bytes, hipMemcpyHostToDevice, stream_.get()), "upload original");
+    HostCopy(staging + stream_image_bytes_, db, b.bytes);
+    Check(hipMemcpyAsync(inputs + stream_image_bytes_, staging + stream_image_bytes_, b.bytes,
+                         hipMemcpyHostToDevice, stream_.get()),
+          "upload distorted");
     const auto* base = stream_inputs_.as<uint8_t>();
     SetSources(a.format, b.format, background);
     cached_original_ = 0;  // the workspace now holds a host image's planes
@@ -787,7 +792,7 @@ uint64_t Pipeline::SubmitResidentHost(uint64_t a, const vqm_image_desc& b, const
     if (db == nullptr) Fail(VQM_INVALID_ARGUMENT, "image data is null");
     EnsureStreaming(b.bytes);
     auto* staging = staging_.as<uint8_t>() + stream_image_bytes_;
-    std::memcpy(staging, db, b.bytes);
+    HostCopy(staging, db, b.bytes);
     auto* dst = stream_inputs_.as<uint8_t>() + stream_image_bytes_;
     Check(hipMemcpyAsync(dst, staging, b.bytes, hipMemcpyHostToDevice, stream_.get()), "upload distorted");
     const bool reuse = a == cached_original_ && background == cached_background_;
