@@ -20,8 +20,9 @@ baseline (the oracle's FP32 ref form on the host's cores, one pair per
 thread, median of rounds), and extras: the other map form and plane form at
 the same workload (`map_forms`, `plane_forms`), the 4K batch in FP32 and the
 mixed form (`workload_4k`), streaming through Ssimulacra2Pool with 1 and 3
-workers (`streaming_pool`), single-pair latency (`modes`), the uneven-size
-batch (`ladder_batch`) and the serving loop (`serving`).
+workers (`streaming_pool`), resident pairs through 1 and 3 pipelines by vqm's
+own protocol (`resident_workers`), single-pair latency (`modes`), the
+uneven-size batch (`ladder_batch`) and the serving loop (`serving`).
 """
 from __future__ import annotations
 
@@ -44,6 +45,9 @@ PLANES = {"f32": 0, "f16": 1, "bf16": 2, "u16": 3}  # u16: the mixed form
 MAPS = {"default": 0, "reference": 1, "difference": 2}
 # vqm's published streaming rates with 3 workers (RTX 5070).
 RTX5070_STREAMING = {"1920x1080": 1816.4, "3840x2160": 455.2}
+# vqm's resident rates on an RTX 5070 with 1 and 3 workers (bench/README.md:32,162,169).
+RTX5070_RESIDENT = {"1920x1080": {"workers_1": 3043.0, "workers_3": 3836.7},
+                    "3840x2160": {"workers_1": 845.7, "workers_3": 915.0}}
 
 
 def map_name(fmt: int, map_form: int) -> str:
@@ -447,6 +451,55 @@ def streaming_pool(w, h, cfg, map_form, workers_list=(1, 3), count=60, rounds=3)
     return out
 
 
+def resident_workers(w, h, cfg, map_form, workers_list=(1, 3), iterations=300, rounds=5, strip_rows=0,
+                     strip_cols=0):
+    """vqm's resident protocol (bench/driver/ssimulacra2.cc:175-285), one
+    pipeline per worker: each uploads the original twice and the distorted
+    image once and alternates the two originals, so no prepared original is
+    reused; the workers' pairs overlap round-robin (submit on every worker,
+    then finish one and submit it again). Median pairs/s of `rounds` rounds
+    of `iterations` pairs."""
+    from vqm_amd import Image, ImageFormat, Size, Ssimulacra2Config, Ssimulacra2Pipeline
+    a, b = make_pairs_cycled(w, h, cfg, 1)[0]
+    size = Size(w, h)
+    ia, ib = Image(size, ImageFormat.srgb8(), a), Image(size, ImageFormat.srgb8(), b)
+    out = {"map_form": map_name(0, map_form)}
+    for workers in workers_list:
+        pipes = [Ssimulacra2Pipeline(Ssimulacra2Config(size=size, map_form=map_form, gpu_resident_images=3,
+                                                       strip_rows=strip_rows, strip_cols=strip_cols))
+                 for _ in range(workers)]
+        handles = [(p.Upload(ia), p.Upload(ia), p.Upload(ib)) for p in pipes]
+        flip = [0] * workers
+
+        def submit(k):
+            original = handles[k][flip[k]]
+            flip[k] ^= 1
+            return pipes[k].Submit(original, handles[k][2])
+
+        for _ in range(3):
+            for k in range(workers):
+                pipes[k].Finish(submit(k))
+        rates, score = [], None
+        for _ in range(rounds):
+            t0 = time.perf_counter()
+            tokens = [submit(k) for k in range(min(workers, iterations))]
+            for i in range(iterations):
+                k = i % len(tokens)
+                score = pipes[k].Finish(tokens[k]).Score()
+                if i + len(tokens) < iterations:
+                    tokens[k] = submit(k)
+            rates.append(iterations / (time.perf_counter() - t0))
+        if not np.isfinite(score):
+            raise RuntimeError("non-finite resident score")
+        for p in pipes:
+            p.close()
+        out[f"workers_{workers}"] = {"pairs_per_s": round(statistics.median(rates), 1)}
+    key = f"{w}x{h}"
+    if key in RTX5070_RESIDENT:
+        out["vqm_rtx5070"] = RTX5070_RESIDENT[key]
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -640,6 +693,11 @@ def main() -> int:
         out["streaming_pool"] = {"1920x1080": streaming_pool(1920, 1080, 1, map_form, count=90),
                                  "3840x2160": streaming_pool(3840, 2160, 2, map_form, count=30)}
         log(f"streaming pool: {out['streaming_pool']}")
+
+    if not args.no_streaming:
+        out["resident_workers"] = {"1920x1080": resident_workers(1920, 1080, 1, map_form, iterations=300),
+                                   "3840x2160": resident_workers(3840, 2160, 2, map_form, iterations=100)}
+        log(f"resident workers: {out['resident_workers']}")
 
     if not args.no_modes:
         modes = {}

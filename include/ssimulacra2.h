@@ -264,7 +264,7 @@ struct Ssimulacra2Config {
     MemoryPolicy memory_policy = MemoryPolicy::Automatic;
     // No reference fields below.
     PlaneFormat plane_format = PlaneFormat::F32;  // overridden to F16 by half_planes
-    std::uint32_t strip_rows = 0;         // scale work-item height, a multiple of 8 (0 = 64)
+    std::uint32_t strip_rows = 0;         // scale work-item height, a multiple of 8 (0 = chosen per geometry)
     bool graph = false;  // replay one captured HIP graph per pair
     Blur blur = Blur::Sampled;
     MapForm map_form = MapForm::Default;

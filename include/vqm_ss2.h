@@ -160,7 +160,9 @@ typedef struct vqm_ss2_config {
     uint32_t plane_format;      /* VQM_PLANES_*; half_planes = VQM_PLANES_F16 */
     uint32_t streaming_inputs;  /* default 1 */
     uint32_t strip_rows;        /* output rows per scale work item, a multiple
-                                   of 8 up to 1024; 0 = default (64) */
+                                   of 8 up to 1024; 0 = chosen for the geometry
+                                   as a one-pair batch's on a whole MI355X
+                                   (below): 16 rows at 1080p, 64 at 4K */
     uint64_t input_bytes;       /* caller-owned image pool (AllocateImage), default 0 */
     uint32_t graph;             /* 1: each pair replays one HIP graph of the
                                    descriptor upload, the three kernels and the
@@ -392,10 +394,11 @@ typedef struct vqm_ss2_batch vqm_ss2_batch;
 typedef struct vqm_ss2_batch_config {
     uint32_t plane_format;   /* VQM_PLANES_* */
     uint32_t strip_rows;     /* multiple of 8; 0 = chosen per geometry when it
-                                is first submitted: 192, 128 or 64 rows, the
+                                is first submitted: 192 or 128 rows, the
                                 tallest that still gives the launch a work item
-                                per wave slot of the device (pipelines and
-                                servers: 64) */
+                                per wave slot of the device, else 64 or 32,
+                                the tallest that gives one per SIMD, else 16
+                                (servers: 64) */
     uint32_t max_pairs;      /* pairs per submission */
     uint32_t chunks;         /* 0 or 1: prepare, then scale, on one stream; 2..16:
                                 runs of consecutive pairs whose scale launches

@@ -27,7 +27,10 @@ def upload_pairs(shapes, seed=0):
     return pairs, keep, host
 
 
-def alone(w, h, a, b, fmt=0, strip_rows=0, strip_cols=0):
+def alone(w, h, a, b, fmt=0, strip_rows=64, strip_cols=0):
+    """The pair through a pipeline; at 64-row strips by default, the height
+    the batches below are pinned to (a pipeline's own default follows the
+    geometry)."""
     p = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(w, h), plane_format=fmt, strip_rows=strip_rows,
                                               strip_cols=strip_cols))
     r = p.Compute(Image(Size(w, h), ImageFormat.srgb8(), a), Image(Size(w, h), ImageFormat.srgb8(), b))
@@ -39,8 +42,8 @@ def alone(w, h, a, b, fmt=0, strip_rows=0, strip_cols=0):
 def test_mixed_batch_equals_each_pair_alone(fmt):
     shapes = [(64, 64), (200, 130), (33, 17), (640, 360), (64, 64), (129, 97), (8, 8)]
     pairs, keep, host = upload_pairs(shapes, seed=11)
-    # Bit for bit with a pipeline at the pipeline's strip height (64 rows);
-    # the batch default (chosen per geometry) groups the FP64 strip sums
+    # Bit for bit with a pipeline at the same strip height (64 rows); the
+    # batch default (chosen per geometry) groups the FP64 strip sums
     # differently.
     batch = Ssimulacra2Batch(max_pairs=16, plane_format=fmt, strip_rows=64)
     scales, norms = batch.Finish(batch.Submit(pairs), raw=True)
@@ -210,7 +213,7 @@ def test_ladder_batch_at_workload_sizes(oracle_lib):
     _, tall_norms = tall.Finish(tall.Submit(pairs), raw=True)
     tall.close()
     np.testing.assert_allclose(tall_norms, norms, rtol=1e-12, atol=0)
-    pipes = {size: Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(*size))) for size in synth.LADDER}
+    pipes = {size: Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(*size), strip_rows=64)) for size in synth.LADDER}
     checked = set()
     for i, (((w, h), _), (a, b)) in enumerate(zip(picks, host)):
         alone_norms = pipes[(w, h)].Compute(DeviceImage(pairs[i][0], Size(w, h), ImageFormat.srgb8()),

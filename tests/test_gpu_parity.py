@@ -252,7 +252,8 @@ def test_recursive_blur_matches_oracle(oracle_lib, fmt, map_form):
         _, sampled, _ = gpu(32, 32, orig, dist, plane_format=fmt, map_form=map_form)
         assert not np.array_equal(got, sampled)
     a, b = synth.pair(0, 3, 512, 512, synth.STRENGTHS[1])
-    n, got, _ = check_against_oracle(oracle_lib, 512, 512, a, b, plane_format=fmt, blur=rec, map_form=map_form)
+    n, got, _ = check_against_oracle(oracle_lib, 512, 512, a, b, plane_format=fmt, blur=rec, map_form=map_form,
+                                     strip_rows=64)
     dev = [vqm_amd.DeviceBuffer(a.nbytes), vqm_amd.DeviceBuffer(b.nbytes)]
     dev[0].upload(a)
     dev[1].upload(b)

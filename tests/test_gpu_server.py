@@ -33,8 +33,9 @@ def make_requests(n, seed):
 
 
 def alone(r, fmt):
+    """The request through a pipeline at the server's strip height (64 rows)."""
     w, h = r["w"], r["h"]
-    p = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(w, h), plane_format=fmt))
+    p = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(w, h), plane_format=fmt, strip_rows=64))
     res = p.Compute(Image(Size(w, h), ImageFormat.srgb8(), r["a"]),
                     Image(Size(w, h), ImageFormat.srgb8(), r["b"]))
     p.close()
@@ -196,7 +197,7 @@ def test_ladder_requests_are_the_same_alone_and_in_the_mix():
     for i, (r, res) in enumerate(zip(reqs, results)):
         size = (r["w"], r["h"])
         if size not in pipes:
-            pipes[size] = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(*size)))
+            pipes[size] = Ssimulacra2Pipeline(Ssimulacra2Config(size=Size(*size), strip_rows=64))
         want = pipes[size].Compute(DeviceImage(r["da"].ptr, Size(*size), ImageFormat.srgb8()),
                                    DeviceImage(r["db"].ptr, Size(*size), ImageFormat.srgb8())).norms()
         assert res.status == 0

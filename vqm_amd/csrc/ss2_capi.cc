@@ -216,9 +216,11 @@ int vqm_ss2_memory_plan(const vqm_ss2_config* config, vqm_ss2_memory* plan) {
         Need(config, "config");
         Need(plan, "plan");
         if (config->plane_format > 3) vqm::ss2::Fail(VQM_INVALID_ARGUMENT, "invalid plane_format");
-        const auto g = vqm::ss2::BuildPlan(config->width, config->height,
-                                           static_cast<vqm::ss2::PlaneFormat>(config->plane_format),
-                                           config->strip_rows, config->strip_cols);
+        const auto fmt = static_cast<vqm::ss2::PlaneFormat>(config->plane_format);
+        const auto g = vqm::ss2::BuildPlan(
+            config->width, config->height, fmt,
+            vqm::ss2::SinglePairStripRows(config->width, config->height, fmt, config->strip_rows, config->strip_cols),
+            config->strip_cols);
         const auto layout = vqm::ss2::LayoutWorkspace(g);
         vqm_image_format f;
         vqm_image_format_srgb8(&f);

@@ -277,6 +277,25 @@ Plan FillPlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t stri
     return p;
 }
 
+uint32_t DefaultStripRows(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_cols,
+                          uint64_t pairs, uint32_t cus) {
+    const uint64_t per_cu = std::max<uint64_t>(cus, 1);
+    auto items = [&](uint32_t rows) {
+        return static_cast<uint64_t>(FillPlan(width, height, format, rows, strip_cols).items) * pairs;
+    };
+    for (uint32_t rows : {192u, 128u})
+        if (items(rows) >= 8 * per_cu) return rows;
+    for (uint32_t rows : {64u, 32u})
+        if (items(rows) >= 4 * per_cu) return rows;
+    return 16;
+}
+
+uint32_t SinglePairStripRows(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows,
+                             uint32_t strip_cols) {
+    constexpr uint32_t kWholeDeviceCUs = 256;
+    return strip_rows != 0 ? strip_rows : DefaultStripRows(width, height, format, strip_cols, 1, kWholeDeviceCUs);
+}
+
 GeometryPlan BuildPlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows,
                        uint32_t strip_cols) {
     GeometryPlan g;

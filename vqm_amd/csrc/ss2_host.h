@@ -78,7 +78,8 @@ struct GeometryPlan {
 };
 
 // strip_rows: output rows per scale work item, a multiple of 8 up to 1024
-// (0 = 64); strip_cols: its width, 64 or 128 columns (0 = 128).
+// (0 = 64; launches pick theirs with DefaultStripRows); strip_cols: its
+// width, 64 or 128 columns (0 = 128).
 GeometryPlan BuildPlan(uint32_t width, uint32_t height, PlaneFormat format,
                        uint32_t strip_rows, uint32_t strip_cols = 0);
 
@@ -86,6 +87,23 @@ GeometryPlan BuildPlan(uint32_t width, uint32_t height, PlaneFormat format,
 // nothing is allocated.
 Plan FillPlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows,
               uint32_t strip_cols = 0);
+
+// The default strip height of a scale launch over `pairs` pairs of this
+// geometry on a device of `cus` compute units: the tallest of 192 and 128
+// rows whose work items fill every wave slot of the device (8 per CU), else
+// the tallest of 64 and 32 rows whose items give each SIMD one (4 per CU),
+// else 16. Taller strips blur each strip's 12 halo rows over more output
+// rows; shorter ones keep a narrow launch (one pair, a small batch) wide.
+uint32_t DefaultStripRows(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_cols,
+                          uint64_t pairs, uint32_t cus);
+
+// The strip height of a single-pair pipeline (and of the allocation-free
+// entry point): `strip_rows` when set, else DefaultStripRows for one pair on
+// a whole MI355X (256 CUs), a function of the geometry alone so that the
+// memory plan, which needs no device, predicts it (16 rows at 1080p, 64 at
+// 4K).
+uint32_t SinglePairStripRows(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_rows,
+                             uint32_t strip_cols);
 
 // Bytes of one pair's workspace: planes of both images, partials, norms,
 // each rounded to 256 bytes.

@@ -121,8 +121,13 @@ uint32_t ScoreDeviceNoAlloc(const vqm_ss2_config& config, const vqm_device_image
              "the allocation-free entry point takes 8-bit full-range sRGB RGB images only "
              "(score other formats through a pipeline)");
 
+    // The strip height a pipeline of this geometry takes, so both give the
+    // same sums.
     SetupArgs args{};
-    args.plan = FillPlan(config.width, config.height, format, config.strip_rows, config.strip_cols);
+    args.plan = FillPlan(config.width, config.height, format,
+                         SinglePairStripRows(config.width, config.height, format, config.strip_rows,
+                                             config.strip_cols),
+                         config.strip_cols);
     const Plan& p = args.plan;
     const uint64_t partials_at = Align256(kItemsAt + static_cast<uint64_t>(p.items) * sizeof(ScaleItem));
     const uint64_t planes_at = Align256(partials_at + static_cast<uint64_t>(p.items) * 6 * sizeof(double));

@@ -526,7 +526,9 @@ Pipeline::Pipeline(const vqm_ss2_config& config)
       map_(ToMapForm(config.map_form, static_cast<PlaneFormat>(config.plane_format))),
       ctx_(DeviceContext::Get()),
       plan_(config.width, config.height, static_cast<PlaneFormat>(config.plane_format),
-            config.strip_rows, config.strip_cols),
+            SinglePairStripRows(config.width, config.height, static_cast<PlaneFormat>(config.plane_format),
+                                config.strip_rows, config.strip_cols),
+            config.strip_cols),
       ws_(plan_.layout),
       pair_(sizeof(PairDesc)),
       sources_(2 * sizeof(SourceDesc)),
@@ -915,21 +917,13 @@ const vqm_ss2_batch_config& ValidatedBatch(const vqm_ss2_batch_config& config) {
 }  // namespace
 
 // A batch's default strip height for a geometry, chosen when the geometry
-// is first submitted with `pairs` pairs of it: the tallest of 192, 128 and
-// 64 rows that still gives the scale launch one work item per wave slot of
-// the device (8 per CU), else 64. Taller strips blur each strip's 12 halo
-// rows over more output rows (192 rows: scale 2.5% faster than 128 on 300
-// 1080p pairs, as fast as 256 with less padding), shorter ones keep small
-// batches' launches wide (one 1080p pair: 6.5k pairs/s at 64 rows, 3.6k at
-// 192).
+// is first submitted with `pairs` pairs of it (DefaultStripRows). Measured on
+// 300 1080p pairs, 192 rows run the scale launch 2.5% faster than 128 (as
+// fast as 256, with less padding); one 1080p pair takes 6.5k pairs/s at 64
+// rows and 3.6k at 192.
 uint32_t Batch::DefaultStripRows(uint32_t width, uint32_t height, uint32_t pairs) const {
-    const uint64_t slots = 8ull * static_cast<uint64_t>(std::max(ctx_.cus, 1));
-    for (uint32_t rows : {192u, 128u}) {
-        const Plan p = FillPlan(width, height, static_cast<PlaneFormat>(config_.plane_format), rows,
-                                config_.strip_cols);
-        if (static_cast<uint64_t>(p.items) * pairs >= slots) return rows;
-    }
-    return 64;
+    return ss2::DefaultStripRows(width, height, static_cast<PlaneFormat>(config_.plane_format), config_.strip_cols,
+                                 pairs, static_cast<uint32_t>(std::max(ctx_.cus, 1)));
 }
 
 Batch::Batch(const vqm_ss2_batch_config& config)

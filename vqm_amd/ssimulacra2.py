@@ -428,7 +428,7 @@ class Ssimulacra2Config:
     streaming_inputs: bool = True
     memory_policy: int = 0  # MemoryPolicy: 0 automatic, 1 direct, 2 staged (unavailable)
     plane_format: Optional[int] = None
-    strip_rows: int = 0  # scale work-item height, a multiple of 8 (0 = 64)
+    strip_rows: int = 0  # scale work-item height, a multiple of 8 (0 = chosen per geometry)
     graph: bool = False  # replay one captured HIP graph per pair (no reference field)
     blur: int = 0  # Blur: 0 sampled (the reference), 1 recursive (no reference field)
     map_form: int = 0  # MapForm: 0 default, 1 reference order, 2 difference (no reference field)
