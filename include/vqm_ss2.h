@@ -394,11 +394,11 @@ typedef struct vqm_ss2_batch vqm_ss2_batch;
 typedef struct vqm_ss2_batch_config {
     uint32_t plane_format;   /* VQM_PLANES_* */
     uint32_t strip_rows;     /* multiple of 8; 0 = chosen per geometry when it
-                                is first submitted: 192 or 128 rows, the
-                                tallest that still gives the launch a work item
-                                per wave slot of the device, else 64 or 32,
-                                the tallest that gives one per SIMD, else 16
-                                (servers: 64) */
+                                is first submitted: 192 rows when the launch
+                                has four work items per wave slot of the
+                                device, else 128 when it has one, else 64 or
+                                32, the tallest that gives one per SIMD, else
+                                16 (servers: 64) */
     uint32_t max_pairs;      /* pairs per submission */
     uint32_t chunks;         /* 0 or 1: prepare, then scale, on one stream; 2..16:
                                 runs of consecutive pairs whose scale launches

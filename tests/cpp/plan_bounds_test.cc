@@ -142,6 +142,25 @@ int main() {
         threw = e.status == VQM_INVALID_ARGUMENT;
     }
     EXPECT(threw, "strip_cols 96 accepted");
+
+    // Default strip heights on a 256-CU device: 1080p has 163 / 254 / 463 /
+    // 893 / 1769 work items per pair at 192 / 128 / 64 / 32 / 16 rows, 4K
+    // 1752 at 64 rows.
+    struct Case {
+        uint32_t w, h;
+        uint64_t pairs;
+        uint32_t rows;
+    };
+    for (const Case& c : {Case{1920, 1080, 300, 192}, Case{1920, 1080, 16, 128}, Case{1920, 1080, 64, 192}, Case{1920, 1080, 4, 64},
+                          Case{1920, 1080, 2, 32}, Case{1920, 1080, 1, 16}, Case{3840, 2160, 1, 64},
+                          Case{3840, 2160, 50, 192}, Case{64, 64, 1, 16}}) {
+        const uint32_t got = DefaultStripRows(c.w, c.h, kPlaneF32, 0, c.pairs, 256);
+        EXPECT(got == c.rows, "%ux%u x %llu pairs: %u rows, want %u", c.w, c.h,
+               static_cast<unsigned long long>(c.pairs), got, c.rows);
+    }
+    EXPECT(SinglePairStripRows(1920, 1080, kPlaneF32, 0, 0) == 16, "1080p pipeline default");
+    EXPECT(SinglePairStripRows(3840, 2160, kPlaneF32, 0, 0) == 64, "4K pipeline default");
+    EXPECT(SinglePairStripRows(1920, 1080, kPlaneF32, 40, 0) == 40, "an explicit height is kept");
     if (failures) {
         std::fprintf(stderr, "%d failures\n", failures);
         return 1;

@@ -283,8 +283,8 @@ uint32_t DefaultStripRows(uint32_t width, uint32_t height, PlaneFormat format, u
     auto items = [&](uint32_t rows) {
         return static_cast<uint64_t>(FillPlan(width, height, format, rows, strip_cols).items) * pairs;
     };
-    for (uint32_t rows : {192u, 128u})
-        if (items(rows) >= 8 * per_cu) return rows;
+    if (items(192) >= 32 * per_cu) return 192;
+    if (items(128) >= 8 * per_cu) return 128;
     for (uint32_t rows : {64u, 32u})
         if (items(rows) >= 4 * per_cu) return rows;
     return 16;

@@ -89,11 +89,12 @@ Plan FillPlan(uint32_t width, uint32_t height, PlaneFormat format, uint32_t stri
               uint32_t strip_cols = 0);
 
 // The default strip height of a scale launch over `pairs` pairs of this
-// geometry on a device of `cus` compute units: the tallest of 192 and 128
-// rows whose work items fill every wave slot of the device (8 per CU), else
-// the tallest of 64 and 32 rows whose items give each SIMD one (4 per CU),
-// else 16. Taller strips blur each strip's 12 halo rows over more output
-// rows; shorter ones keep a narrow launch (one pair, a small batch) wide.
+// geometry on a device of `cus` compute units: 192 rows when their work
+// items fill every wave slot of the device (8 per CU) four times over, else
+// 128 when theirs fill every slot once, else the tallest of 64 and 32 rows
+// whose items give each SIMD one (4 per CU), else 16. Taller strips blur
+// each strip's 12 halo rows over more output rows; shorter ones keep a
+// narrow launch (one pair, a small batch) wide.
 uint32_t DefaultStripRows(uint32_t width, uint32_t height, PlaneFormat format, uint32_t strip_cols,
                           uint64_t pairs, uint32_t cus);
 
