@@ -419,7 +419,11 @@ typedef struct vqm_ss2_batch_config {
                                 replayed from a HIP graph captured the first
                                 time its shape (geometry, pair count) is seen;
                                 its stage times then report the whole graph
-                                as the prepare stage. 0 (default): eager. */
+                                as the prepare stage. 2: the same for
+                                submissions of up to 8 pairs, where the launch
+                                overhead shows (+16% for one 1080p pair, +6%
+                                for four, nothing from 16 on; the Python
+                                binding's default). 0: eager. */
 } vqm_ss2_batch_config;
 
 /* One pair of caller-owned device images (packed u8 sRGB RGB). */

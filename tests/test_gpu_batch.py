@@ -66,12 +66,21 @@ def test_mixed_batch_equals_each_pair_alone(fmt):
 def test_uniform_batch_equals_each_pair_alone():
     shapes = [(320, 180)] * 5
     pairs, keep, host = upload_pairs(shapes, seed=21)
-    batch = Ssimulacra2Batch(max_pairs=8, strip_rows=64)
+    batch = Ssimulacra2Batch(max_pairs=8, strip_rows=64, graph=False)
     _, norms = batch.Finish(batch.Submit(pairs), raw=True)
     for i, (a, b) in enumerate(host):
         assert np.array_equal(norms[i], alone(320, 180, a, b))
     t = batch.StageTimes()
     assert all(v > 0 for v in t)
+    # The default replays submissions of up to 8 pairs from a graph: the
+    # same norms, and the whole graph timed as the prepare stage.
+    auto = Ssimulacra2Batch(max_pairs=8, strip_rows=64)
+    for _ in range(3):
+        _, again = auto.Finish(auto.Submit(pairs), raw=True)
+        assert np.array_equal(again, norms)
+        t = auto.StageTimes()
+        assert t[0] > 0 and t[1] == 0 and t[2] == 0
+    auto.close()
 
 
 def test_batch_budget_and_misuse():

@@ -35,6 +35,21 @@ def test_library_exports_every_declared_symbol():
     assert set(declared) == set(L.SIGNATURES), "ctypes signatures and the header disagree"
 
 
+def test_batch_graph_modes_are_validated_before_any_device_work():
+    """graph is 0 (eager), 1 (every qualifying submission) or 2 (submissions
+    of up to 8 pairs); other values and other Python spellings are refused
+    before the device is touched."""
+    for graph in (3, 7):
+        cfg = L.BatchConfigC()
+        cfg.max_pairs = 4
+        cfg.graph = graph
+        h = ctypes.c_void_p()
+        assert L.lib().vqm_ss2_batch_create(ctypes.byref(cfg), ctypes.byref(h)) == L.VQM_INVALID_ARGUMENT
+        assert b"graph must be 0, 1 or 2" in L.lib().vqm_last_error()
+    with pytest.raises(InvalidArgument, match="graph must be"):
+        vqm_amd.Ssimulacra2Batch(max_pairs=4, graph="always")
+
+
 def test_cbrt_check_rejects_ranges_outside_the_normal_floats():
     """The device cube-root check validates its range before touching the GPU."""
     r = L.CbrtCheckC()

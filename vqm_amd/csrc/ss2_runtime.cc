@@ -268,7 +268,7 @@ void WorkspacePools::Return(uint32_t width, uint32_t height, bool inputs, std::u
 
 // ---- lane -----------------------------------------------------------------------
 
-Lane::Lane(uint32_t max_pairs, bool graphs)
+Lane::Lane(uint32_t max_pairs, uint32_t graphs)
     : max_pairs_(max_pairs),
       pairs_(max_pairs * sizeof(PairDesc)),
       norms_(static_cast<uint64_t>(max_pairs) * kNorms * sizeof(double)),
@@ -350,7 +350,8 @@ void Lane::Launch(PlaneFormat fmt, MapForm map, const float* lut, const LaunchEn
     const hipStream_t s = stream_.get();
     bool host_inputs = false;
     for (uint32_t i = 0; i < count; ++i) host_inputs = host_inputs || entries[i].host[0] || entries[i].host[1];
-    replayed_ = graphs_ && uniform && chunks == 1 && !general && !host_inputs;
+    replayed_ = (graphs_ == 1 || (graphs_ == 2 && count <= kAutoGraphPairs)) && uniform && chunks == 1 &&
+                !general && !host_inputs;
     if (replayed_) {
         const GraphKey key{entries[0].plan, count, static_cast<uint32_t>(fmt), static_cast<uint32_t>(map), blur,
                            persistent};
@@ -908,7 +909,7 @@ const vqm_ss2_batch_config& ValidatedBatch(const vqm_ss2_batch_config& config) {
     if (config.chunks > Lane::kMaxChunks) Fail(VQM_INVALID_ARGUMENT, "chunks must be at most 16");
     if (config.blur > kBlurRecursive) Fail(VQM_INVALID_ARGUMENT, "blur must be sampled (0) or recursive (1)");
     if (config.persistent > 1) Fail(VQM_INVALID_ARGUMENT, "persistent must be 0 or 1");
-    if (config.graph > 1) Fail(VQM_INVALID_ARGUMENT, "graph must be 0 or 1");
+    if (config.graph > 2) Fail(VQM_INVALID_ARGUMENT, "graph must be 0, 1 or 2");
     ToMapForm(config.map_form, static_cast<PlaneFormat>(config.plane_format));
     // Validates strip_rows and strip_cols.
     BuildPlan(8, 8, static_cast<PlaneFormat>(config.plane_format), config.strip_rows, config.strip_cols);
@@ -932,7 +933,7 @@ Batch::Batch(const vqm_ss2_batch_config& config)
       ctx_(DeviceContext::Get()),
       pools_(static_cast<PlaneFormat>(config.plane_format), config.strip_rows != 0 ? config.strip_rows : 64,
              config.strip_cols, config.hbm_budget),
-      lane_(config.max_pairs, config.graph != 0) {}
+      lane_(config.max_pairs, config.graph) {}
 
 Batch::~Batch() = default;
 

@@ -232,9 +232,12 @@ public:
     static constexpr uint32_t kMaxChunks = 16;
     // Instantiated graphs a lane keeps (one per launch shape).
     static constexpr size_t kMaxGraphs = 32;
-    // graphs: replay launches that qualify from captured HIP graphs (see
-    // Launch).
-    explicit Lane(uint32_t max_pairs, bool graphs = false);
+    // Launches of at most this many pairs qualify for graph replay in the
+    // automatic mode (where the launch overhead is a visible share).
+    static constexpr uint32_t kAutoGraphPairs = 8;
+    // graphs: 0 eager; 1 replay every launch that qualifies from captured
+    // HIP graphs (see Launch); 2 only those of up to kAutoGraphPairs pairs.
+    explicit Lane(uint32_t max_pairs, uint32_t graphs = 0);
     ~Lane();
     // Queues the uploads, the prepare, scale and reduce launches and the norm
     // read-back, then returns. With chunks > 1 the pairs are split into that
@@ -288,7 +291,8 @@ private:
                    std::tie(o.plan, o.count, o.format, o.map, o.blur, o.persistent);
         }
     };
-    bool graphs_ = false, replayed_ = false;
+    uint32_t graphs_ = 0;
+    bool replayed_ = false;
     std::map<GraphKey, hipGraphExec_t> graph_cache_;
     hipGraphExec_t CaptureGraph(PlaneFormat fmt, MapForm map, const float* lut, const DevicePlan& plan,
                                 uint32_t count, uint32_t blur, bool persistent);
