@@ -43,7 +43,7 @@ def test_hardware_seeded_root_equals_the_ieee_root_for_every_normal_float():
 
 def test_check_counts_a_partial_range():
     """A range the check splits across blocks unevenly: its counters add up
-    to the same verdict (no input skipped, none counted twice)."""
+    to the same result (no input skipped, none counted twice)."""
     one = check(0x3F800000, 0x40000001)  # [1, 2]
     assert one.mismatches == 0
     halves = [check(0x3F800000, 0x3FC00123), check(0x3FC00123, 0x40000001)]
