@@ -137,6 +137,37 @@ def test_random_pairs_match_oracle(oracle_lib, size, map_form):
         assert abs(oracle_lib.score(n, got) - s64) <= bound
 
 
+def test_seeded_random_geometries_match_oracle(oracle_lib):
+    """24 geometries drawn from a fixed seed (8 to 400 on each side), each
+    pair through a pipeline at its default strip height and all 24 in one
+    batch launch at the batch's default heights: every feature within
+    DEV_REL of the reference form's oracle (bit-level per pixel; only the
+    FP64 grouping of strip sums differs)."""
+    rng = np.random.default_rng(2026)
+    sizes = [(int(rng.integers(8, 401)), int(rng.integers(8, 401))) for _ in range(24)]
+    want, dev = [], []
+    for i, (w, h) in enumerate(sizes):
+        print(f"random geometry {i}: {w}x{h}", flush=True)
+        a, b = random_pair(w, h, seed=1000 + i)
+        n, got, _ = check_against_oracle(oracle_lib, w, h, a, b)
+        want.append((n, got))
+        da, db = vqm_amd.DeviceBuffer(a.nbytes), vqm_amd.DeviceBuffer(b.nbytes)
+        da.upload(a)
+        db.upload(b)
+        dev.append((da, db))
+    batch = vqm_amd.Ssimulacra2Batch(max_pairs=len(sizes))
+    scales, norms = batch.Finish(batch.Submit([(da.ptr, db.ptr, w, h) for (da, db), (w, h) in zip(dev, sizes)]),
+                                 raw=True)
+    batch.close()
+    for i, ((n, got), (w, h)) in enumerate(zip(want, sizes)):
+        assert scales[i] == n
+        bad = np.nonzero(np.abs(norms[i] - got) > DEV_REL * np.abs(got))[0]
+        assert bad.size == 0, f"{w}x{h}: batch vs pipeline features {bad.tolist()}"
+    for da, db in dev:
+        da.free()
+        db.free()
+
+
 def check_workload(oracle_lib, w, h, a, b, map_form):
     """The workload bars of each form against the FP32 ref form (and FP64)."""
     n, got, res = check_against_oracle(oracle_lib, w, h, a, b, map_form=map_form)
