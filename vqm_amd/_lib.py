@@ -111,6 +111,11 @@ class CbrtCheckC(ctypes.Structure):
                 ("first_mismatch", u32), ("reserved", u32)]
 
 
+class DivCheckC(ctypes.Structure):
+    _fields_ = [("mismatches", u64), ("reciprocal_mismatches", u64), ("first_mismatch", u32),
+                ("reserved", u32)]
+
+
 # Every function include/vqm_ss2.h declares: name -> (restype, argtypes).
 SIGNATURES = {
     "vqm_last_error": (ctypes.c_char_p, []),
@@ -193,6 +198,7 @@ SIGNATURES = {
     "vqm_device_synchronize": (ctypes.c_int, []),
     "vqm_device_copy": (ctypes.c_int, [vp, vp, u64]),
     "vqm_ss2_check_cbrt": (ctypes.c_int, [u32, u32, ctypes.POINTER(CbrtCheckC)]),
+    "vqm_ss2_check_div": (ctypes.c_int, [u32, u32, ctypes.POINTER(DivCheckC)]),
 }
 
 _lock = threading.Lock()

@@ -647,6 +647,30 @@ int vqm_ss2_check_cbrt(uint32_t first_bits, uint32_t end_bits, vqm_ss2_cbrt_chec
     });
 }
 
+int vqm_ss2_check_div(uint32_t first_significand, uint32_t end_significand, vqm_ss2_div_check* result) {
+    return Guard([&] {
+        Need(result, "result");
+        if (first_significand >= end_significand || end_significand > (1u << 23))
+            vqm::ss2::Fail(VQM_INVALID_ARGUMENT, "the significand range must lie in [0, 2^23) and not be empty");
+        vqm::ss2::DeviceBuffer counters(sizeof(vqm::ss2::DivCheck));
+        vqm::ss2::DivCheck c{};
+        c.first_mismatch = 0xFFFFFFFFu;
+        vqm::ss2::Check(hipMemcpy(counters.get(), &c, sizeof(c), hipMemcpyHostToDevice), "hipMemcpy");
+        // Launches of at most 4096 rows (about 0.25 s each), waited for one
+        // by one.
+        for (uint32_t mb = first_significand; mb < end_significand; mb += 4096u) {
+            const uint32_t rows = std::min(4096u, end_significand - mb);
+            vqm::ss2::Check(vqm::ss2::LaunchDivCheck(mb, rows, counters.as<vqm::ss2::DivCheck>(), nullptr),
+                            "division check kernel");
+            vqm::ss2::Check(hipDeviceSynchronize(), "division check");
+        }
+        vqm::ss2::Check(hipMemcpy(&c, counters.get(), sizeof(c), hipMemcpyDeviceToHost), "hipMemcpy");
+        result->mismatches = c.mismatches;
+        result->reciprocal_mismatches = c.rcp_mismatches;
+        result->first_mismatch = c.first_mismatch;
+    });
+}
+
 int vqm_device_synchronize(void) {
     return Guard([&] { vqm::ss2::Check(hipDeviceSynchronize(), "hipDeviceSynchronize"); });
 }

@@ -64,4 +64,18 @@ struct CbrtCheck {
 // IEEE steps only for every float with bits in [first, end), adding to `out`.
 hipError_t LaunchCbrtCheck(uint32_t first, uint32_t end, CbrtCheck* out, hipStream_t stream);
 
+// Counters of the division self-check; `first_mismatch` starts at
+// 0xFFFFFFFF and the others at 0.
+struct DivCheck {
+    unsigned long long mismatches;
+    unsigned long long rcp_mismatches;
+    uint32_t first_mismatch;
+    uint32_t pad;
+};
+
+// Compares the map's division with the IEEE quotient for b significands
+// [mb_first, mb_first + rows) against every a significand, and the hardware
+// reciprocal's exponent invariance for those b, adding to `out`.
+hipError_t LaunchDivCheck(uint32_t mb_first, uint32_t rows, DivCheck* out, hipStream_t stream);
+
 }  // namespace vqm::ss2

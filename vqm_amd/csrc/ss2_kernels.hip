@@ -95,19 +95,23 @@ __device__ __forceinline__ float rcp_dev(float x) {
 }
 
 // a / b correctly rounded (the IEEE quotient), for the map's quotients: b a
-// positive normal float and a either 0.0f (never -0.0f) or |a| >= 2^-100,
-// with |a / b| inside [2^-100, 2^100]. This is the division sequence the
-// compiler emits for `a / b` (hardware reciprocal, one Newton step, two
-// residual corrections) without its operand scaling and special-value
-// fix-up, which leave such operands unchanged: 8 instructions instead of 11.
-// The map's operands stay in that domain: denominators are at least
-// c2 - 2^-22 or 1, and numerators are 0.0f (an exact difference of equal
-// values rounds to +0) or multiples of 2^-82.
+// positive normal float in [2^-24, 2^24) and a either 0.0f (never -0.0f) or
+// |a| >= 2^-100, with |a / b| inside [2^-100, 2^100]. The hardware
+// reciprocal, one Newton step (the reciprocal correctly rounded), the
+// product, and one residual correction (Markstein's): 6 instructions, where
+// the compiler's `a / b` takes 11 (two corrections, operand scaling and a
+// special-value fix-up). vqm_ss2_check_div compares it with the IEEE
+// quotient on the device: over every pair of significands (a, b in [1, 2),
+// 2^46 pairs, about 30 s) it differs nowhere, and the hardware reciprocal of
+// m 2^e is that of m scaled by 2^-e for every e in [-24, 24); the fused
+// steps scale exactly, so the whole domain follows. The map's operands stay
+// in it: denominators are at least c2 - 2^-22 or 1 and at most a few units,
+// and numerators are 0.0f (an exact difference of equal values rounds to
+// +0) or multiples of 2^-82.
 __device__ __forceinline__ float div_rn(float a, float b) {
     float r = __builtin_amdgcn_rcpf(b);
     r = fma_(fma_(-b, r, 1.0f), r, r);
-    float q = a * r;
-    q = fma_(fma_(-b, q, a), r, q);
+    const float q = a * r;
     return fma_(fma_(-b, q, a), r, q);
 }
 
@@ -1306,6 +1310,43 @@ __global__ __launch_bounds__(256) void cbrt_check_kernel(uint32_t first, uint32_
     }
 }
 
+// Self-check of div_rn over the significand pairs a = 1.ma, b = 1.mb with
+// mb in [mb_first, mb_first + rows) and every ma, against the IEEE quotient,
+// and of the hardware reciprocal's exponent invariance for those b over
+// 2^e, e in [-24, 24). One atomic per wave and counter.
+__global__ __launch_bounds__(256) void div_check_kernel(uint32_t mb_first, uint32_t rows, DivCheck* out) {
+    unsigned long long bad = 0, rcp_bad = 0;
+    uint32_t first_bad = 0xFFFFFFFFu;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t lane0 = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t k = 0; k < rows; ++k) {
+        const uint32_t mb = mb_first + k;
+        const float b = __uint_as_float(0x3F800000u | mb);
+        for (uint32_t ma = lane0; ma < (1u << 23); ma += stride) {
+            const float a = __uint_as_float(0x3F800000u | ma);
+            if (__float_as_uint(div_rn(a, b)) != __float_as_uint(a / b)) {
+                ++bad;
+                first_bad = min(first_bad, mb);
+            }
+        }
+        if (lane0 < 48u) {
+            const int e = static_cast<int>(lane0) - 24;
+            const float want = __builtin_ldexpf(__builtin_amdgcn_rcpf(b), -e);
+            if (__float_as_uint(__builtin_amdgcn_rcpf(__builtin_ldexpf(b, e))) != __float_as_uint(want)) ++rcp_bad;
+        }
+    }
+    for (int offset = 32; offset > 0; offset >>= 1) {
+        bad += __shfl_xor(bad, offset, 64);
+        rcp_bad += __shfl_xor(rcp_bad, offset, 64);
+        first_bad = min(first_bad, __shfl_xor(first_bad, offset, 64));
+    }
+    if (threadIdx.x % 64u == 0u) {
+        atomicAdd(&out->mismatches, bad);
+        atomicAdd(&out->rcp_mismatches, rcp_bad);
+        atomicMin(&out->first_mismatch, first_bad);
+    }
+}
+
 // Blocks of `kernel` the device holds resident at once: occupancy per
 // compute unit x compute units, rounded down to a multiple of 8 (at least
 // 8). A persistent launch uses min(work items, this) blocks, each looping
@@ -1396,6 +1437,12 @@ hipError_t LaunchCopy(void* dst, const void* src, uint64_t bytes, hipStream_t st
     const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 256 * 64);
     hipLaunchKernelGGL(copy_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, stream,
                        static_cast<const uint4*>(src), static_cast<uint4*>(dst), n);
+    return hipGetLastError();
+}
+
+hipError_t LaunchDivCheck(uint32_t mb_first, uint32_t rows, DivCheck* out, hipStream_t stream) {
+    if (rows == 0) return hipSuccess;
+    hipLaunchKernelGGL(div_check_kernel, dim3(2048), dim3(256), 0, stream, mb_first, rows, out);
     return hipGetLastError();
 }
 
