@@ -144,13 +144,11 @@ float cbrt_dev(float x) {
     return std::fma(e, rr * third, y);
 }
 
-// Device reciprocal of a positive normal float: a bit-trick seed (within
-// 5%) and three Newton steps r += r (1 - x r), fma only (<= 0.501 ulp).
-float rcp_dev(float x) {
-    float r = bits_to_float(0x7ef311c7u - float_to_bits(x));
-    for (int k = 0; k < 3; ++k) r = std::fma(r, std::fma(-x, r, 1.0f), r);
-    return r;
-}
+// Device reciprocal of a positive normal float: the correctly rounded 1 / x
+// (the kernels take the hardware reciprocal and one Newton step, which the
+// library's device self-check compares with the IEEE quotient for every
+// significand).
+float rcp_dev(float x) { return 1.0f / x; }
 double rcp_dev(double x) { return 1.0 / x; }  // the FP64 forms divide
 
 // A finite positive float or a midpoint of two adjacent ones as M * 2^k

@@ -103,8 +103,8 @@ float ss2o_cbrt_dev(float x);
  * form's root. 0 for x <= 0. */
 float ss2o_cbrt_rn(float x);
 
-/* The kernels' reciprocal of a positive normal float (bit-trick seed and
- * three Newton steps, <= 0.501 ulp). */
+/* The kernels' reciprocal of a positive normal float: the correctly rounded
+ * 1 / x. */
 float ss2o_rcp_dev(float x);
 
 /* linear RGB -> XYB in FP32 (fused multiply-adds as the kernels issue them)

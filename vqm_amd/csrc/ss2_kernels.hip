@@ -83,15 +83,14 @@ __device__ __forceinline__ float fma_(float a, float b, float c) {
 // 2^-42 minus the cube-root bias rounds to minus the bias.
 constexpr float kMinNormal = 0x1p-126f;
 
-// Reciprocal of a positive normal float: a bit-trick seed (within 5%) and
-// three Newton steps r += r (1 - x r). <= 0.501 ulp; fma only, so the oracle
-// reproduces it (the correctly rounded division takes ten instructions and a
-// quarter-rate reciprocal).
-__device__ __forceinline__ float rcp_dev(float x) {
-    float r = __uint_as_float(0x7ef311c7u - __float_as_uint(x));
-#pragma unroll
-    for (int k = 0; k < 3; ++k) r = fma_(r, fma_(-x, r, 1.0f), r);
-    return r;
+// 1 / x correctly rounded, for x a positive normal float in [2^-24, 2^24):
+// the hardware reciprocal and one Newton step r += r (1 - x r), 3
+// instructions. vqm_ss2_check_div checks it against the IEEE quotient for
+// every significand and the hardware reciprocal's invariance under scaling
+// by 2^e, e in [-24, 24) (the Newton step scales exactly).
+__device__ __forceinline__ float rcp_rn(float x) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    return fma_(fma_(-x, r, 1.0f), r, r);
 }
 
 // a / b correctly rounded (the IEEE quotient), for the map's quotients: b a
@@ -109,8 +108,7 @@ __device__ __forceinline__ float rcp_dev(float x) {
 // and numerators are 0.0f (an exact difference of equal values rounds to
 // +0) or multiples of 2^-82.
 __device__ __forceinline__ float div_rn(float a, float b) {
-    float r = __builtin_amdgcn_rcpf(b);
-    r = fma_(fma_(-b, r, 1.0f), r, r);
+    const float r = rcp_rn(b);
     const float q = a * r;
     return fma_(fma_(-b, q, a), r, q);
 }
@@ -1008,7 +1006,7 @@ __device__ __forceinline__ void map_values(const float* bl, float a, float b, fl
         const float num_s = fma_(2.0f, fma_(-mu1, mu2, bl[3]), kC2);
         const float v = fma_(-t, t, bl[2]);
         const float den = num_s + v;
-        const float q = rcp_dev(den * det_orig);
+        const float q = rcp_rn(den * det_orig);
         d = fmaxf((fma_(t * t, num_s, v) * det_orig) * q, 0.0f);
         d1 = ((det_dist - det_orig) * den) * q;
     }
@@ -1333,6 +1331,9 @@ __global__ __launch_bounds__(256) void div_check_kernel(uint32_t mb_first, uint3
             const int e = static_cast<int>(lane0) - 24;
             const float want = __builtin_ldexpf(__builtin_amdgcn_rcpf(b), -e);
             if (__float_as_uint(__builtin_amdgcn_rcpf(__builtin_ldexpf(b, e))) != __float_as_uint(want)) ++rcp_bad;
+        } else if (lane0 == 48u) {
+            // The refined reciprocal is the correctly rounded 1 / b.
+            if (__float_as_uint(rcp_rn(b)) != __float_as_uint(1.0f / b)) ++rcp_bad;
         }
     }
     for (int offset = 32; offset > 0; offset >>= 1) {
