@@ -636,7 +636,9 @@ int vqm_ss2_check_cbrt(uint32_t first_bits, uint32_t end_bits, vqm_ss2_cbrt_chec
 /* Result of vqm_ss2_check_div. */
 typedef struct vqm_ss2_div_check {
     uint64_t mismatches;             /* (a, b) pairs whose quotients differ */
-    uint64_t reciprocal_mismatches;  /* (b, e) where rcp(b 2^e) != rcp(b) 2^-e */
+    uint64_t reciprocal_mismatches;  /* b whose refined reciprocal is not the
+                                        correctly rounded 1/b, and (b, e) where
+                                        rcp(b 2^e) != rcp(b) 2^-e */
     uint32_t first_mismatch;         /* smallest b significand that differs, or 0xFFFFFFFF */
     uint32_t reserved;
 } vqm_ss2_div_check;
@@ -644,9 +646,11 @@ typedef struct vqm_ss2_div_check {
 /* Checks, on the current device, the division the reference form's SSIM'
  * map takes (hardware reciprocal, one Newton step, one correction) against
  * the IEEE quotient: for b = 1.mb with mb in [first_significand,
- * end_significand) and every a = 1.ma, and the hardware reciprocal's
- * invariance under scaling b by 2^e, e in [-24, 24). [0, 2^23) covers every
- * pair of significands (about 30 s). Waits for the device. */
+ * end_significand) and every a = 1.ma; that the refined reciprocal (the
+ * difference form's) is the correctly rounded 1/b; and the hardware
+ * reciprocal's invariance under scaling b by 2^e, e in [-24, 24).
+ * [0, 2^23) covers every pair of significands (about 30 s). Waits for the
+ * device. */
 int vqm_ss2_check_div(uint32_t first_significand, uint32_t end_significand, vqm_ss2_div_check* result);
 
 #ifdef __cplusplus

@@ -63,10 +63,10 @@ def check_div(first: int, end: int) -> L.DivCheckC:
 def test_map_division_equals_the_ieee_quotient():
     """div_rn (hardware reciprocal, one Newton step, one correction) against
     the IEEE quotient for 8 runs of 8192 b significands spread over [1, 2),
-    each against all 2^23 a significands (2^39 pairs; all 2^46 take ~30 s),
-    and the hardware reciprocal's invariance under scaling by 2^e for those
-    b. Fused multiply-adds scale exactly, so together these cover the map's
-    operand domain."""
+    each against all 2^23 a significands (2^39 pairs; all 2^46 take ~30 s);
+    rcp_rn(b) against 1 / b; and the hardware reciprocal's invariance under
+    scaling by 2^e for those b. Fused multiply-adds scale exactly, so
+    together these cover the map's operand domain."""
     total_bad = total_rcp = 0
     for k in range(8):
         first = k * (1 << 20) + (k * 7919) % (1 << 19)

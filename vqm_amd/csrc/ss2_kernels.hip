@@ -1309,9 +1309,10 @@ __global__ __launch_bounds__(256) void cbrt_check_kernel(uint32_t first, uint32_
 }
 
 // Self-check of div_rn over the significand pairs a = 1.ma, b = 1.mb with
-// mb in [mb_first, mb_first + rows) and every ma, against the IEEE quotient,
-// and of the hardware reciprocal's exponent invariance for those b over
-// 2^e, e in [-24, 24). One atomic per wave and counter.
+// mb in [mb_first, mb_first + rows) and every ma, against the IEEE quotient;
+// of rcp_rn(b) against 1 / b; and of the hardware reciprocal's exponent
+// invariance for those b over 2^e, e in [-24, 24). One atomic per wave and
+// counter.
 __global__ __launch_bounds__(256) void div_check_kernel(uint32_t mb_first, uint32_t rows, DivCheck* out) {
     unsigned long long bad = 0, rcp_bad = 0;
     uint32_t first_bad = 0xFFFFFFFFu;
