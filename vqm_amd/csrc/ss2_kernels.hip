@@ -830,7 +830,12 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
     if (levels > 2) { __syncthreads(); prepare_level<Lay, Ref, 2>(plan, planes, X0, Y0, tid, ladder); }
     if (levels > 3) { __syncthreads(); prepare_level<Lay, Ref, 3>(plan, planes, X0, Y0, tid, ladder); }
     if (levels > 4) { __syncthreads(); prepare_level<Lay, Ref, 4>(plan, planes, X0, Y0, tid, ladder); }
-    if (levels > 5) { __syncthreads(); prepare_level<Lay, Ref, 5>(plan, planes, X0, Y0, tid, ladder); }
+    // Nothing reads level 5's samples but its own stored channels, and with
+    // six scales the pruning leaves it none.
+    if (levels > 5 && plan->mask[5] != 0u) {
+        __syncthreads();
+        prepare_level<Lay, Ref, 5>(plan, planes, X0, Y0, tid, ladder);
+    }
 }
 
 // Work items blockIdx.x, blockIdx.x + gridDim.x, ... of `blocks`: one each
