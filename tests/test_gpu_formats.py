@@ -349,6 +349,32 @@ def test_subsampling_matches_reference(sub_w, sub_h, flt, location):
                  f"subsampling {sub_w}x{sub_h} filter {int(flt)} siting {int(location)}")
 
 
+@pytest.mark.parametrize("sub_w,sub_h,flt,location", SUBSAMPLING, ids=lambda v: str(int(v)))
+def test_subsampling_across_tiles(sub_w, sub_h, flt, location):
+    """The same A/B check on a 136 x 72 frame: 3 x 3 prepare tiles of 64 x 32
+    pixels, the last row and column ragged, each tile upsampling from its own
+    window of chroma taps. The chroma samples are independent random values,
+    so a tap read from the wrong sample moves the norms far past the
+    tolerance."""
+    w, h = 136, 72
+    size = Size(w, h)
+    rng = np.random.default_rng(100 * sub_w + 10 * sub_h + int(flt) + 1000 * int(location))
+    cw, ch = w >> sub_w, h >> sub_h
+    y, x = np.mgrid[0:h, 0:w]
+    planes = {0: (0.2 + 0.6 * (x + y) / (w + h)).astype(np.float32).astype(np.float64),
+              1: rng.uniform(-0.2, 0.2, (ch, cw)).astype(np.float32).astype(np.float64),
+              2: rng.uniform(-0.2, 0.2, (ch, cw)).astype(np.float32).astype(np.float64)}
+    up = {0: planes[0], 1: R.upsample(planes[1], sub_w, sub_h, flt, location, w, h),
+          2: R.upsample(planes[2], sub_w, sub_h, flt, location, w, h)}
+    with Ssimulacra2Pipeline(Ssimulacra2Config(size=size, chroma_filter=int(flt))) as p:
+        a = R.pack(yuv_format(sub_w, sub_h, location), size, planes)
+        b = R.pack(yuv_format(0, 0, location), size, up)
+        pair, identity = p.Compute(a, b), p.Compute(b, b)
+        delta = max_norm_delta(pair, identity)
+        assert delta <= ROUNDING, (f"subsampling {sub_w}x{sub_h} filter {int(flt)} siting {int(location)}: "
+                                   f"{delta:.3e}")
+
+
 def test_chroma_filter_and_siting_change_the_score():
     """input_format_chroma_test.cc:185-266: the three filters give different
     scores on a chroma-shifted pair; an edge scores differently left- and

@@ -482,15 +482,30 @@ __device__ uint32_t mirror(float pos, uint32_t dim) {
 
 constexpr int kMaxTaps = 6;
 
-// The taps of one axis for output sample `out`: chroma position
-// (out + 0.5) / ratio + shift, the first tap half a sample past
-// round(pos - taps / 2). Bilinear weights are 1 - t and t (their sum is
-// exactly one); the others are normalised by their sum.
-__device__ int axis_taps(uint32_t filter, uint32_t out, uint32_t sub, float shift, uint32_t dim,
-                         float w[kMaxTaps], uint32_t idx[kMaxTaps]) {
-    const int taps = filter == 0 ? 2 : filter == 1 ? 4 : 6;
-    const float pos = (static_cast<float>(out) + 0.5f) * __builtin_ldexpf(1.0f, -static_cast<int>(sub)) + shift;
-    const float base = round_halfup(pos - 0.5f * static_cast<float>(taps)) + 0.5f;
+__device__ __forceinline__ int tap_count(uint32_t filter) { return filter == 0 ? 2 : filter == 1 ? 4 : 6; }
+
+// Chroma position of output sample `out` on one axis: (out + 0.5) / ratio
+// + shift.
+__device__ __forceinline__ float chroma_pos(uint32_t out, uint32_t sub, float shift) {
+    return (static_cast<float>(out) + 0.5f) * __builtin_ldexpf(1.0f, -static_cast<int>(sub)) + shift;
+}
+
+// The first tap's sample index before mirroring, round(pos - taps / 2): an
+// integer-valued float, non-decreasing in pos.
+__device__ __forceinline__ float first_tap(int taps, float pos) {
+    return round_halfup(pos - 0.5f * static_cast<float>(taps));
+}
+
+// The weights of one axis for output sample `out`, taps centred half a
+// sample past the first tap. Bilinear weights are 1 - t and t (their sum is
+// exactly one); the others are normalised by their sum. Returns the tap
+// count; *first is the first tap's unmirrored index.
+__device__ int axis_weights(uint32_t filter, uint32_t out, uint32_t sub, float shift, float w[kMaxTaps],
+                            int* first) {
+    const int taps = tap_count(filter);
+    const float pos = chroma_pos(out, sub, shift);
+    const float n0 = first_tap(taps, pos);
+    const float base = n0 + 0.5f;
     if (filter == 0) {
         const float t = pos - base;
         w[0] = 1.0f - t;
@@ -500,39 +515,109 @@ __device__ int axis_taps(uint32_t filter, uint32_t out, uint32_t sub, float shif
         for (int j = 0; j < taps; ++j) total += filter_weight(filter, base + static_cast<float>(j) - pos);
         for (int j = 0; j < taps; ++j) w[j] = filter_weight(filter, base + static_cast<float>(j) - pos) / total;
     }
-    for (int j = 0; j < taps; ++j) idx[j] = mirror(base + static_cast<float>(j), dim);
+    *first = static_cast<int>(n0);
     return taps;
 }
 
+// Sample index of unmirrored tap index n: the tap's centre n + 0.5 mirrored
+// once at the plane edges, then clamped.
+__device__ __forceinline__ uint32_t tap_index(int n, uint32_t dim) {
+    return mirror(static_cast<float>(n) + 0.5f, dim);
+}
+
+// The chroma samples a tile's pixels read, staged in LDS: both chroma planes
+// (scaled and offset) interleaved as (U, V), rows of `cols` entries from
+// unmirrored tap index (x0, y0). Over every subsampling (1x, 2x, 4x per
+// axis), siting and filter, a 64 x 32 tile needs at most 69 x 22 = 1518
+// entries (Lanczos-3, 2x vertical subsampling only); the bound was found by
+// enumerating tile positions on frames up to 3840 x 2160.
+constexpr uint32_t kChromaWindow = 1664;
+struct ChromaWindow {
+    int x0, y0;
+    uint32_t cols, rows;
+};
+
+// The window of pixels x_first .. x_last by y_first .. y_last; false if it
+// does not fit in kChromaWindow entries.
+__device__ bool chroma_window(ConstAS<SourceDesc>* d, uint32_t x_first, uint32_t x_last, uint32_t y_first,
+                              uint32_t y_last, ChromaWindow* w) {
+    const int taps = tap_count(d->filter);
+    const int x0 = static_cast<int>(first_tap(taps, chroma_pos(x_first, d->sub_w, d->shift_x)));
+    const int x1 = static_cast<int>(first_tap(taps, chroma_pos(x_last, d->sub_w, d->shift_x)));
+    const int y0 = static_cast<int>(first_tap(taps, chroma_pos(y_first, d->sub_h, d->shift_y)));
+    const int y1 = static_cast<int>(first_tap(taps, chroma_pos(y_last, d->sub_h, d->shift_y)));
+    w->x0 = x0;
+    w->y0 = y0;
+    w->cols = static_cast<uint32_t>(x1 - x0 + taps);
+    w->rows = static_cast<uint32_t>(y1 - y0 + taps);
+    return w->cols * w->rows <= kChromaWindow;
+}
+
+// Block-wide: thread `tid` of 256 loads every 256th entry of the window.
+__device__ void stage_chroma(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* src, const ChromaWindow& w,
+                             uint32_t tid, float2* window) {
+    const uint32_t n = w.cols * w.rows;
+    for (uint32_t i = tid; i < n; i += 256u) {
+        const uint32_t r = i / w.cols, c = i - r * w.cols;
+        const uint32_t ix = tap_index(w.x0 + static_cast<int>(c), d->plane_w[1]);
+        const uint32_t iy = tap_index(w.y0 + static_cast<int>(r), d->plane_h[1]);
+        window[i] = make_float2(raw_sample(d, src, 1, ix, iy), raw_sample(d, src, 2, ix, iy));
+    }
+}
+
 // Both chroma samples of pixel (x, y): the two chroma planes share their
-// size and siting, so one set of taps serves both.
-__device__ float2 chroma_samples(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* src, uint32_t x, uint32_t y) {
+// size and siting, so one set of taps serves both. Rows are summed first,
+// then weighted by the vertical taps. With a staged window the samples come
+// from LDS (the same values, in the same order), else from the planes.
+__device__ float2 chroma_samples(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* src, const float2* window,
+                                 const ChromaWindow& cw, uint32_t x, uint32_t y) {
     float wx[kMaxTaps], wy[kMaxTaps];
-    uint32_t ix[kMaxTaps], iy[kMaxTaps];
-    const int tx = axis_taps(d->filter, x, d->sub_w, d->shift_x, d->plane_w[1], wx, ix);
-    const int ty = axis_taps(d->filter, y, d->sub_h, d->shift_y, d->plane_h[1], wy, iy);
+    int fx, fy;
+    const int tx = axis_weights(d->filter, x, d->sub_w, d->shift_x, wx, &fx);
+    const int ty = axis_weights(d->filter, y, d->sub_h, d->shift_y, wy, &fy);
     float sum_u = 0.0f, sum_v = 0.0f;
-    for (int j = 0; j < ty; ++j) {
-        float row_u = 0.0f, row_v = 0.0f;
-        for (int k = 0; k < tx; ++k) {
-            row_u += wx[k] * raw_sample(d, src, 1, ix[k], iy[j]);
-            row_v += wx[k] * raw_sample(d, src, 2, ix[k], iy[j]);
+    if (window != nullptr) {
+        // The first taps are non-decreasing in x and y, so they lie within
+        // the window; the clamp only guards the LDS bounds.
+        const uint32_t cx = min(static_cast<uint32_t>(fx - cw.x0), cw.cols - static_cast<uint32_t>(tx));
+        const uint32_t cy = min(static_cast<uint32_t>(fy - cw.y0), cw.rows - static_cast<uint32_t>(ty));
+        for (int j = 0; j < ty; ++j) {
+            const float2* row = window + (cy + static_cast<uint32_t>(j)) * cw.cols + cx;
+            float row_u = 0.0f, row_v = 0.0f;
+            for (int k = 0; k < tx; ++k) {
+                const float2 s = row[k];
+                row_u += wx[k] * s.x;
+                row_v += wx[k] * s.y;
+            }
+            sum_u += wy[j] * row_u;
+            sum_v += wy[j] * row_v;
         }
-        sum_u += wy[j] * row_u;
-        sum_v += wy[j] * row_v;
+    } else {
+        for (int j = 0; j < ty; ++j) {
+            const uint32_t iy = tap_index(fy + j, d->plane_h[1]);
+            float row_u = 0.0f, row_v = 0.0f;
+            for (int k = 0; k < tx; ++k) {
+                const uint32_t ix = tap_index(fx + k, d->plane_w[1]);
+                row_u += wx[k] * raw_sample(d, src, 1, ix, iy);
+                row_v += wx[k] * raw_sample(d, src, 2, ix, iy);
+            }
+            sum_u += wy[j] * row_u;
+            sum_v += wy[j] * row_v;
+        }
     }
     return make_float2(sum_u, sum_v);
 }
 
 // Linear BT.709 RGB of pixel (x, y). Inlined into its eight call sites (a
-// called function measured 12% slower on 8-bit 4:2:0 frames).
-__device__ __forceinline__ float3 pixel(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* src, uint32_t x,
-                                       uint32_t y) {
+// called function measured 12% slower on 8-bit 4:2:0 frames). `window` is
+// the tile's staged chroma, or null to read the chroma planes directly.
+__device__ __forceinline__ float3 pixel(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* src, const float2* window,
+                                       const ChromaWindow& cw, uint32_t x, uint32_t y) {
     const float alpha = d->alpha != 0 ? raw_sample(d, src, 3, x, y) : 1.0f;
     const float y0 = raw_sample(d, src, 0, x, y);
     float3 e = make_float3(y0, y0, y0);
     if (d->family == 2 && (d->sub_w != 0 || d->sub_h != 0)) {
-        const float2 uv = chroma_samples(d, src, x, y);
+        const float2 uv = chroma_samples(d, src, window, cw, x, y);
         e.y = uv.x;
         e.z = uv.y;
     } else if (d->family != 0) {
@@ -728,10 +813,23 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
         if (sdp != nullptr && as_const(sdp)->kind != 0u) {
             table = false;
             ConstAS<SourceDesc>* sd = as_const(sdp);
+            // Subsampled chroma: the block stages the tile's chroma samples
+            // in LDS once, instead of each pixel loading its taps from the
+            // planes (8 loads a pixel for bilinear 4:2:0, 72 for Lanczos-3).
+            __shared__ float2 chroma[decode::kChromaWindow];
+            decode::ChromaWindow cw{0, 0, 0, 0};
+            const float2* window = nullptr;
+            if (sd->family == 2 && (sd->sub_w != 0 || sd->sub_h != 0) &&
+                decode::chroma_window(sd, X0, min(X0 + kPrepTileW - 1u, W - 1u), Y0,
+                                      min(Y0 + kPrepTileH - 1u, H - 1u), &cw)) {
+                decode::stage_chroma(sd, src, cw, tid, chroma);
+                window = chroma;
+                __syncthreads();
+            }
             for (uint32_t r = 0; r < 2; ++r) {
                 const uint32_t y = min(Y0 + 2u * ty + r, H - 1u);
                 for (uint32_t j = 0; j < 4; ++j) {
-                    const float3 c = decode::pixel(sd, src, min(bx + j, W - 1u), y);
+                    const float3 c = decode::pixel(sd, src, window, cw, min(bx + j, W - 1u), y);
                     lin[r][j][0] = c.x;
                     lin[r][j][1] = c.y;
                     lin[r][j][2] = c.z;
@@ -847,7 +945,8 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
 // interleaved correctly rounded cube roots hold FP64 temporaries (level 0
 // takes its four pixels' roots two pixels at a time: four at once need 117
 // registers and spill at five waves; at six waves pairs spill too), both
-// without spilling; the per-pixel decode of general formats needs ~110.
+// without spilling. The per-pixel decode of general formats is left
+// unconstrained; it compiles to 88 (five waves) with 26 KB of LDS a block.
 template <typename Lay, bool General, bool Ref>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(General ? 1 : (Ref ? 5 : 6)))) void prepare_kernel(const PairDesc* __restrict__ pairs,
                                                       const WorkRef* __restrict__ work,
