@@ -499,14 +499,17 @@ __device__ __forceinline__ float first_tap(int taps, float pos) {
 // The weights of one axis for output sample `out`, taps centred half a
 // sample past the first tap. Bilinear weights are 1 - t and t (their sum is
 // exactly one); the others are normalised by their sum. Returns the tap
-// count; *first is the first tap's unmirrored index.
-__device__ int axis_weights(uint32_t filter, uint32_t out, uint32_t sub, float shift, float w[kMaxTaps],
-                            int* first) {
-    const int taps = tap_count(filter);
+// count; *first is the first tap's unmirrored index. Bilinear (filter 0)
+// callers set Bilinear, so the tap loops unroll and the weights stay in
+// registers.
+template <bool Bilinear>
+__device__ __forceinline__ int axis_weights(uint32_t filter, uint32_t out, uint32_t sub, float shift,
+                                            float (&w)[Bilinear ? 2 : kMaxTaps], int* first) {
+    const int taps = Bilinear ? 2 : tap_count(filter);
     const float pos = chroma_pos(out, sub, shift);
     const float n0 = first_tap(taps, pos);
     const float base = n0 + 0.5f;
-    if (filter == 0) {
+    if (Bilinear || filter == 0) {
         const float t = pos - base;
         w[0] = 1.0f - t;
         w[1] = t;
@@ -569,12 +572,15 @@ __device__ void stage_chroma(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* sr
 // size and siting, so one set of taps serves both. Rows are summed first,
 // then weighted by the vertical taps. With a staged window the samples come
 // from LDS (the same values, in the same order), else from the planes.
-__device__ float2 chroma_samples(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* src, const float2* window,
-                                 const ChromaWindow& cw, uint32_t x, uint32_t y) {
-    float wx[kMaxTaps], wy[kMaxTaps];
+// Bilinear: the descriptor's filter is 0.
+template <bool Bilinear>
+__device__ __forceinline__ float2 chroma_samples(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* src,
+                                                 const float2* window, const ChromaWindow& cw, uint32_t x,
+                                                 uint32_t y) {
+    float wx[Bilinear ? 2 : kMaxTaps], wy[Bilinear ? 2 : kMaxTaps];
     int fx, fy;
-    const int tx = axis_weights(d->filter, x, d->sub_w, d->shift_x, wx, &fx);
-    const int ty = axis_weights(d->filter, y, d->sub_h, d->shift_y, wy, &fy);
+    const int tx = axis_weights<Bilinear>(d->filter, x, d->sub_w, d->shift_x, wx, &fx);
+    const int ty = axis_weights<Bilinear>(d->filter, y, d->sub_h, d->shift_y, wy, &fy);
     float sum_u = 0.0f, sum_v = 0.0f;
     if (window != nullptr) {
         // The first taps are non-decreasing in x and y, so they lie within
@@ -617,7 +623,8 @@ __device__ __forceinline__ float3 pixel(ConstAS<SourceDesc>* d, GlobalAS<const u
     const float y0 = raw_sample(d, src, 0, x, y);
     float3 e = make_float3(y0, y0, y0);
     if (d->family == 2 && (d->sub_w != 0 || d->sub_h != 0)) {
-        const float2 uv = chroma_samples(d, src, window, cw, x, y);
+        const float2 uv = d->filter == 0 ? chroma_samples<true>(d, src, window, cw, x, y)
+                                         : chroma_samples<false>(d, src, window, cw, x, y);
         e.y = uv.x;
         e.z = uv.y;
     } else if (d->family != 0) {
