@@ -614,13 +614,41 @@ __device__ __forceinline__ float2 chroma_samples(ConstAS<SourceDesc>* d, GlobalA
     return make_float2(sum_u, sum_v);
 }
 
-// Linear BT.709 RGB of pixel (x, y). Inlined into its eight call sites (a
-// called function measured 12% slower on 8-bit 4:2:0 frames). `window` is
-// the tile's staged chroma, or null to read the chroma planes directly.
+// Plane 0's samples (luma, R or grey) of pixels bx .. bx + 3 of row y,
+// clamped to the image: one 4- or 8-byte load for byte and word samples on
+// rows of a multiple of 4 samples, else one load a pixel.
+__device__ __forceinline__ void first_plane4(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* src, uint32_t bx,
+                                             uint32_t y, uint32_t W, float (&out)[4]) {
+    const uint32_t pw = d->plane_w[0];
+    if (d->type <= 1u && (pw & 3u) == 0u && bx + 3u < W) {
+        const uint64_t index = static_cast<uint64_t>(y) * pw + bx;
+        GlobalAS<const uint8_t>* base = src + d->plane_off[0];
+        const float scale = d->scale[0], offset = d->offset[0];
+        if (d->type == 0u) {
+            const uint32_t w = *reinterpret_cast<GlobalAS<const uint32_t>*>(base + index);
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) out[j] = static_cast<float>((w >> (8u * j)) & 255u) * scale + offset;
+        } else {
+            const u32x2 w = *reinterpret_cast<GlobalAS<const u32x2>*>(base + 2u * index);
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) {
+                const uint32_t word = (j < 2u ? w[0] : w[1]) >> (16u * (j & 1u));
+                out[j] = static_cast<float>(word & 0xFFFFu) * scale + offset;
+            }
+        }
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) out[j] = raw_sample(d, src, 0, min(bx + j, W - 1u), y);
+    }
+}
+
+// Linear BT.709 RGB of pixel (x, y), whose plane 0 sample is y0. Inlined into
+// its eight call sites (a called function measured 12% slower on 8-bit 4:2:0
+// frames). `window` is the tile's staged chroma, or null to read the chroma
+// planes directly.
 __device__ __forceinline__ float3 pixel(ConstAS<SourceDesc>* d, GlobalAS<const uint8_t>* src, const float2* window,
-                                       const ChromaWindow& cw, uint32_t x, uint32_t y) {
+                                       const ChromaWindow& cw, uint32_t x, uint32_t y, float y0) {
     const float alpha = d->alpha != 0 ? raw_sample(d, src, 3, x, y) : 1.0f;
-    const float y0 = raw_sample(d, src, 0, x, y);
     float3 e = make_float3(y0, y0, y0);
     if (d->family == 2 && (d->sub_w != 0 || d->sub_h != 0)) {
         const float2 uv = d->filter == 0 ? chroma_samples<true>(d, src, window, cw, x, y)
@@ -835,8 +863,10 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
             }
             for (uint32_t r = 0; r < 2; ++r) {
                 const uint32_t y = min(Y0 + 2u * ty + r, H - 1u);
+                float first[4];
+                decode::first_plane4(sd, src, bx, y, W, first);
                 for (uint32_t j = 0; j < 4; ++j) {
-                    const float3 c = decode::pixel(sd, src, window, cw, min(bx + j, W - 1u), y);
+                    const float3 c = decode::pixel(sd, src, window, cw, min(bx + j, W - 1u), y, first[j]);
                     lin[r][j][0] = c.x;
                     lin[r][j][1] = c.y;
                     lin[r][j][2] = c.z;
@@ -952,10 +982,11 @@ __device__ __forceinline__ void prepare_tile(uint32_t b, const PairDesc* __restr
 // interleaved correctly rounded cube roots hold FP64 temporaries (level 0
 // takes its four pixels' roots two pixels at a time: four at once need 117
 // registers and spill at five waves; at six waves pairs spill too), both
-// without spilling. The per-pixel decode of general formats is left
-// unconstrained; it compiles to 88 (five waves) with 26 KB of LDS a block.
+// without spilling. The per-pixel decode of general formats is held to 96
+// (five waves, 26 KB of LDS a block) without spilling vector registers; it
+// would take 98 and four waves unconstrained.
 template <typename Lay, bool General, bool Ref>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(General ? 1 : (Ref ? 5 : 6)))) void prepare_kernel(const PairDesc* __restrict__ pairs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(General ? 5 : (Ref ? 5 : 6)))) void prepare_kernel(const PairDesc* __restrict__ pairs,
                                                       const WorkRef* __restrict__ work,
                                                       uint32_t blocks, uint32_t tiles_per_image,
                                                       const float* __restrict__ lut_g,
