@@ -155,3 +155,41 @@ def test_plane_layout_and_classes(dump):
     assert grey["identity1"] == 1 and grey["identity2"] == 1  # grey carries no gamut
     assert dump(ImageFormat.srgb8())["srgb8"] == 1
     assert dump(ImageFormat())["srgb8"] == 0
+
+
+def _first_taps(n: int, sub: int, shift: float, taps: int) -> np.ndarray:
+    """The prepare kernel's first chroma tap of output samples 0 .. n-1 in
+    FP32, operation by operation: pos = (out + 0.5) * 2^-sub + shift, then
+    round-half-up of pos - taps / 2."""
+    f = np.float32
+    pos = (np.arange(n, dtype=f) + f(0.5)) * f(2.0 ** -sub) + f(shift)
+    v = pos - f(0.5 * taps)
+    return np.where(v < 0, np.floor(v + f(0.5)), np.floor(v + f(0.49999997))).astype(np.int64)
+
+
+def test_chroma_window_fits_every_tile(dump):
+    """Each 64 x 32 prepare tile stages the chroma taps its pixels read in an
+    LDS window of kChromaWindow entries. For every subsampling, siting and
+    filter, on 3840 x 2160 and a ragged 136 x 72, the window fits, and the
+    first taps are non-decreasing, so every pixel's taps lie in its tile's
+    window."""
+    import re
+    src = open(os.path.join(ROOT, "vqm_amd", "csrc", "ss2_kernels.hip")).read()
+    capacity = int(re.search(r"kChromaWindow = (\d+);", src).group(1))
+    worst = 0
+    for sw, sh in [(1, 1), (1, 0), (2, 0), (0, 1), (2, 2), (0, 2), (1, 2), (2, 1)]:
+        for location in ChromaLocation:
+            d = dump(yuv(MC.Bt709, chroma_location=location, subsample_w=sw, subsample_h=sh))
+            sx, sy = d["shift"]
+            for taps in (2, 4, 6):
+                for w, h in ((3840, 2160), (136, 72)):
+                    fx = _first_taps(w, sw, sx, taps)
+                    fy = _first_taps(h, sh, sy, taps)
+                    assert np.all(np.diff(fx) >= 0) and np.all(np.diff(fy) >= 0)
+                    x0 = np.arange(0, w, 64)
+                    y0 = np.arange(0, h, 32)
+                    cols = fx[np.minimum(x0 + 63, w - 1)] - fx[x0] + taps
+                    rows = fy[np.minimum(y0 + 31, h - 1)] - fy[y0] + taps
+                    worst = max(worst, int(cols.max() * rows.max()))
+    assert worst <= capacity
+    assert worst == 1518  # Lanczos-3 with 2x vertical subsampling only: 69 x 22
