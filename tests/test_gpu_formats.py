@@ -349,7 +349,15 @@ def test_subsampling_matches_reference(sub_w, sub_h, flt, location):
                  f"subsampling {sub_w}x{sub_h} filter {int(flt)} siting {int(location)}")
 
 
-@pytest.mark.parametrize("sub_w,sub_h,flt,location", SUBSAMPLING, ids=lambda v: str(int(v)))
+# 4x on one axis with 1x or 2x on the other, beyond the reference's cases.
+MIXED_RATIOS = [
+    (0, 2, ChromaFilter.Lanczos, ChromaLocation.Top),
+    (1, 2, ChromaFilter.Bicubic, ChromaLocation.BottomLeft),
+    (2, 1, ChromaFilter.Bilinear, ChromaLocation.TopLeft),
+]
+
+
+@pytest.mark.parametrize("sub_w,sub_h,flt,location", SUBSAMPLING + MIXED_RATIOS, ids=lambda v: str(int(v)))
 def test_subsampling_across_tiles(sub_w, sub_h, flt, location):
     """The same A/B check on a 136 x 72 frame: 3 x 3 prepare tiles of 64 x 32
     pixels, the last row and column ragged, each tile upsampling from its own
