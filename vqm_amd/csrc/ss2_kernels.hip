@@ -1363,12 +1363,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kScaleWaves<
 
 // ---- reduce -----------------------------------------------------------------
 
-// One block per pair. Wave w takes the (level, channel) ranges w, w+4, ...
-// (18 at most); its lanes stride over the range's strips, each adding the
-// six partial sums of its strips in FP64, and the wave then sums its lanes.
+// One block per pair. Wave w takes the (level, channel) ranges w, w+16, ...
+// (18 at most, so nearly every range has a wave of its own and a single
+// pair's reduction is as long as its longest range, not the sum of four);
+// its lanes stride over the range's strips, each adding the six partial sums
+// of its strips in FP64 (four strips' loads in flight at a time, added in
+// strip order), and the wave then sums its lanes.
 // Norm level*18 + k: k < 6 is channel k/2's SSIM' slot k%2; k >= 6 is
 // channel (k-6)/4's edge slot 2 + (k-6)%4 (ring 1-, 4-norm, blur 1-, 4-norm).
-constexpr uint32_t kReduceWaves = 4;
+constexpr uint32_t kReduceWaves = 16;
 
 __global__ __launch_bounds__(64 * kReduceWaves) void reduce_kernel(const PairDesc* __restrict__ pairs) {
     ConstAS<PairDesc>* pdp = as_const(pairs) + blockIdx.x;
@@ -1382,6 +1385,7 @@ __global__ __launch_bounds__(64 * kReduceWaves) void reduce_kernel(const PairDes
         double sum[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         if (active) {
             const uint32_t begin = plan->item_begin[level][c], count = plan->item_count[level][c];
+#pragma unroll 4
             for (uint32_t it = lane; it < count; it += 64u) {
                 GlobalAS<const double>* p = partials + static_cast<uint64_t>(begin + it) * 6u;
 #pragma unroll
